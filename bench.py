@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark: converged solves/s of the segregated small-strain solid solver on
+one MI355X, on the large configuration: unit cube 200^3 = 8M hex cells,
+interior vertices jittered +/-15% (seed 2), E=200 GPa, nu=0.3, z-min fixed,
+z-max normal traction 1e7 Pa, other faces traction-free.
+
+A step is one converged solve (outer defect-correction loop of
+linearGeometryTotalDisplacement to solution/alternative tolerance 1e-8,
+PCG tolerance 1e-9, relTol 0.1).  `value` is the mixed-precision path (FP32
+PCG sweeps inside the FP64 defect loop); the plain FP64 path is timed beside
+it and serves as the FP64 reference for the displacement / von Mises error.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def algorithmic_bytes(kernel, C, F, B, T):
+    """Compulsory HBM bytes per launch: each array a kernel touches counted
+    once; neighbour gathers assumed cache-resident.  C cells, F internal faces,
+    B boundary faces, T = bytes of the PCG scalar (8 FP64, 4 FP32)."""
+    if kernel == "spmv":            # diag, p, q per cell; a + col per ELL entry (2 per face)
+        return C * 7 * T + F * 2 * (T + 4)
+    if kernel == "pcg_update":      # x r/w, p, r r/w, q, 1/diag
+        return C * 19 * T
+    if kernel == "pcg_direction":   # r, 1/diag, p r/w
+        return C * 10 * T
+    if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
+        return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
+    if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w
+        return C * (24 + 3 * T + 24 + 48 + 72 + 24) + F * 2 * 4 + B * (4 + 24 + 8 + 24 + 24 + 24 + 8 + 24 * 6 + 72 + 48)
+    raise KeyError(kernel)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(n, jitter, seed, nOuter, iters):
+    """Oracle (single-threaded C restatement, PCG + DIC relTol 0.1) on the same
+    mesh: seconds for `iters` outer iterations after the first, scaled by the
+    GPU FP64 outer-iteration count to solves/s."""
+    import oracle
+    t0 = time.time()
+    om = oracle.Mesh.box(n, n, n, (1.0, 1.0, 1.0), jitter, seed)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(5)] = (0.0, 0.0, 1e7)
+    oc = oracle.Case(om, 200e9, 0.3, [1, 1, 1, 1, 0, 1], bv)
+    setup = time.time() - t0
+    oc.time_iterations(1)  # iteration 0 (zero residual) and warm-up
+    sec = oc.time_iterations(iters)
+    per_iter = sec / iters
+    cell_iter_s = om.nCells / per_iter
+    return dict(value=1.0 / (per_iter * nOuter), unit="solves/s", cores=1, kind="port",
+                cell_iter_per_s=cell_iter_s,
+                sample=f"oracle (C restatement, 1 thread, PCG+DIC relTol 0.1) on the same {n}^3 mesh: "
+                       f"{iters} outer iteration(s) timed ({sec:.1f} s, setup {setup:.1f} s untimed); "
+                       f"solves/s = 1/(s_per_outer_iter x {nOuter} outer iterations of the GPU FP64 solve)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--n", type=int, default=200)
+    ap.add_argument("--jitter", type=float, default=0.15)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--profile-every", type=int, default=16)
+    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
+              "a design (see DESIGN.md), not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+
+    import solids4foam_amd as s4f
+    from solids4foam_amd import _abi, cases
+
+    ctx = s4f.Context(0)
+    t0 = time.time()
+    pm, mesh = cases.cube_tension(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
+    setup_s = time.time() - t0
+    info = mesh.info()
+    C, B = info["nCells"], info["nBoundaryFaces"]
+    F = info["nInternalFaces"]
+    print(f"[bench] mesh {args.n}^3: C={C} F={F} B={B} ELL entries={info['ellEntries']} "
+          f"device={info['deviceBytes'] / 2**30:.2f} GiB setup={setup_s:.1f}s", file=sys.stderr, flush=True)
+    opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=0.1,
+                nCorrectors=100000)
+    L = _abi.lib()
+
+    def run(precision, steps, profile):
+        if profile:
+            _abi.check(L.s4f_profile(ctx.h, args.profile_every))
+        lat, outs = [], []
+        for _ in range(steps):
+            t = time.perf_counter()
+            out = mesh.solve(want_sigma=True, precision=precision, **opts)
+            lat.append(time.perf_counter() - t)
+            outs.append(out)
+            print(f"[bench] precision={precision} nOuter={out['nOuter']} inner={out['innerTotal']} "
+                  f"converged={out['converged']} {lat[-1]:.2f}s", file=sys.stderr, flush=True)
+        prof = {}
+        if profile:
+            for k, kid in _abi.KERNELS.items():
+                avg, ns, nl = ctypes_read(L, ctx, kid)
+                prof[k] = dict(avg_ms=avg, samples=ns, launches=nl)
+            _abi.check(L.s4f_profile(ctx.h, 0))
+        return lat, outs, prof
+
+    for _ in range(args.warmup):
+        mesh.solve(want_sigma=False, precision=1, **opts)
+    res = {}
+    if args.only in ("both", "mixed"):
+        res["mixed"] = run(1, args.steps, True)
+    if args.only in ("both", "fp64"):
+        res["fp64"] = run(0, args.steps, args.only == "fp64")
+
+    head = "mixed" if "mixed" in res else "fp64"
+    lat, outs, prof = res[head]
+    T = 4 if head == "mixed" else 8
+    total = sum(lat)
+    value = args.steps / total
+    nOuter = outs[-1]["nOuter"]
+    kern = {}
+    for k, p in prof.items():
+        if p["samples"] == 0:
+            continue
+        b = algorithmic_bytes(k, C, F, B, T)
+        kern[k] = dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"], samples=p["samples"],
+                       alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1),
+                       total_ms_est=round(p["avg_ms"] * p["launches"], 1))
+    dom = max(kern, key=lambda k: kern[k]["total_ms_est"]) if kern else None
+    roofline = None
+    if dom:
+        a = kern[dom]["gbps"]
+        roofline = dict(bound="hbm", achieved=a, peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(a / HBM_PEAK_GBPS, 4),
+                        traffic=None, kernel=dom, alg_bytes_per_launch=kern[dom]["alg_bytes"])
+    extra = {}
+    if "mixed" in res and "fp64" in res:
+        lat64, outs64, _ = res["fp64"]
+        Dm, D64 = outs[-1]["D"], outs64[-1]["D"]
+        extra = dict(fp64_solves_per_s=round(args.steps / sum(lat64), 5),
+                     mixed_over_fp64=round((args.steps / total) / (args.steps / sum(lat64)), 3),
+                     err_D_vs_fp64=rel_l2(Dm, D64),
+                     err_vonMises_vs_fp64=rel_l2(s4f.sigma_eq(outs[-1]["sigma"]), s4f.sigma_eq(outs64[-1]["sigma"])),
+                     fp64_nOuter=outs64[-1]["nOuter"], fp64_ms_per_step=round(1e3 * sum(lat64) / args.steps, 1))
+    cpu = None
+    if not args.no_cpu_baseline:
+        n64 = extra.get("fp64_nOuter", nOuter)
+        cpu = cpu_baseline(args.n, args.jitter, args.seed, n64, args.cpu_iters)
+    line = {
+        "metric": "converged solves/s & cell-iter/s at 1e-6 err vs FP64; p50/p99 solve latency",
+        "value": round(value, 6),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cube {args.n}^3 hex, jitter {args.jitter} seed {args.seed}, z-min fixed, "
+                               f"z-max traction 1e7 Pa, linearElastic E=200GPa nu=0.3",
+                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": "diagonal, relTol 0.1"},
+        "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
+        "nOuter": nOuter,
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "roofline": roofline,
+        "kernels": kern,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+def ctypes_read(L, ctx, kid):
+    import ctypes
+    avg = ctypes.c_double()
+    ns = ctypes.c_int64()
+    nl = ctypes.c_int64()
+    L.s4f_profile_read(ctx.h, kid, ctypes.byref(avg), ctypes.byref(ns), ctypes.byref(nl))
+    return avg.value, ns.value, nl.value
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,252 @@
+/*
+ * s4f_abi.h -- C ABI of the MI355X implementation of solids4foam's segregated
+ * small-strain solid solver (solidModel "linearGeometryTotalDisplacement").
+ *
+ * Everything crossing this boundary is plain C: opaque handles, int32/int64
+ * sizes, FP64 host arrays in OpenFOAM's layouts (vector = xyz per cell,
+ * tensor = XX XY XZ YX YY YZ ZX ZY ZZ per cell, symmTensor = XX XY XZ YY YZ
+ * ZZ per cell).  Precision modes are internal; every value at the ABI is FP64.
+ *
+ * Conventions
+ *   - every function returns s4f_status (0 = OK); s4f_last_error() returns a
+ *     thread-local message for the last failure on the calling thread;
+ *   - no C++ exception or abort crosses the ABI;
+ *   - input host buffers are borrowed for the duration of the call only;
+ *     output buffers are caller-owned; device memory is owned by the handles;
+ *   - calls on one context are serialised by the caller (a context is not
+ *     thread-safe); separate contexts are independent.
+ *
+ * Reference interfaces replaced (paths relative to the solids4foam tree):
+ *   solidModel run-time selection      src/solids4FoamModels/solidModels/solidModel/solidModel.H:376-386
+ *   linGeomTotalDispSolid::evolve      src/solids4FoamModels/solidModels/linGeomTotalDispSolid/linGeomTotalDispSolid.C:111-232
+ *   tractionBoundarySnGrad             linGeomTotalDispSolid.C:235-271
+ *   lduMatrix::solver::solve (PCG)     OpenFOAM lduMatrix solver table, selected by fvSolution "solver"
+ */
+#ifndef S4F_ABI_H
+#define S4F_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    S4F_OK = 0,
+    S4F_ERR_INVALID = 1,      /* bad argument, mesh or dictionary value */
+    S4F_ERR_HIP = 2,          /* HIP runtime failure (device missing, launch failure) */
+    S4F_ERR_OOM = 3,          /* device allocation failed */
+    S4F_ERR_DIVERGED = 4,     /* NaN/Inf in the outer residual norms */
+    S4F_WARN_MAXITER = 5,     /* nCorrectors exhausted before convergence (result is valid) */
+    S4F_ERR_ADMISSION = 6,    /* serving: request larger than the HBM budget */
+    S4F_ERR_NOT_READY = 7     /* serving: no finished result to return yet */
+} s4f_status;
+
+typedef struct s4f_ctx s4f_ctx;
+typedef struct s4f_polymesh s4f_polymesh;
+typedef struct s4f_mesh s4f_mesh;
+typedef struct s4f_batch s4f_batch;
+typedef struct s4f_server s4f_server;
+
+/* boundary patch types (solidModels/fvPatchFields): */
+enum {
+    S4F_PATCH_FIXED_DISPLACEMENT = 0, /* fixedDisplacementFvPatchVectorField.C:258-356 */
+    S4F_PATCH_TRACTION = 1            /* solidTractionFvPatchVectorField.C:321-463 */
+};
+
+/* precision of the inner linear solve */
+enum {
+    S4F_PRECISION_FP64 = 0,  /* plain path: everything FP64 */
+    S4F_PRECISION_MIXED = 1  /* FP32 PCG sweeps inside the FP64 defect-correction loop */
+};
+
+/* inner preconditioner (fvSolution "preconditioner") */
+enum {
+    S4F_PRECOND_DIAGONAL = 0,  /* OpenFOAM "diagonal" (Jacobi) */
+    S4F_PRECOND_CHEBYSHEV = 1  /* Chebyshev polynomial of the Jacobi-scaled matrix */
+};
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+/* Creates a context on HIP device `device` with its own stream. */
+s4f_status s4f_ctx_create(int device, s4f_ctx **out);
+s4f_status s4f_ctx_destroy(s4f_ctx *ctx);
+/* Thread-local message describing the last non-OK status on this thread. */
+const char *s4f_last_error(void);
+/* Library version string. */
+const char *s4f_version(void);
+
+/* ------------------------------------------------------------------ */
+/* polyMesh generation and geometry (host)                             */
+/* ------------------------------------------------------------------ */
+/* Single-block box mesh in blockMesh numbering (points and cells i-fastest,
+   internal faces in upper-triangular owner/neighbour order, patches x-min,
+   x-max, y-min, y-max, z-min, z-max).  jitter > 0 moves every interior vertex
+   by a seeded uniform +/- jitter*spacing per axis.  tets != 0 splits every
+   hex into 6 tetrahedra around its (0,0,0)-(1,1,1) diagonal. */
+s4f_status s4f_polymesh_box(int nx, int ny, int nz, const double L[3], double jitter,
+                            uint64_t seed, int tets, s4f_polymesh **out);
+/* Borrows external polyMesh arrays (copied): faceOffsets has nFaces+1 entries. */
+s4f_status s4f_polymesh_from_arrays(int32_t nPoints, const double *points, int32_t nCells,
+                                    int32_t nFaces, int32_t nInternalFaces,
+                                    const int32_t *faceOffsets, const int32_t *faceVerts,
+                                    const int32_t *owner, const int32_t *neighbour,
+                                    int32_t nPatches, const int32_t *patchStart,
+                                    const int32_t *patchSize, s4f_polymesh **out);
+s4f_status s4f_polymesh_free(s4f_polymesh *pm);
+/* sizes: nPoints, nCells, nFaces, nInternalFaces, nPatches, nFaceVerts */
+s4f_status s4f_polymesh_sizes(const s4f_polymesh *pm, int64_t out[6]);
+/* copies addressing out (any pointer may be NULL) */
+s4f_status s4f_polymesh_addressing(const s4f_polymesh *pm, double *points, int32_t *faceOffsets,
+                                   int32_t *faceVerts, int32_t *owner, int32_t *neighbour,
+                                   int32_t *patchStart, int32_t *patchSize);
+/* OpenFOAM primitiveMesh geometry: face centres Cf, area vectors Sf (nFaces*3),
+   cell centres C (nCells*3), volumes V (nCells). */
+s4f_status s4f_polymesh_geometry(const s4f_polymesh *pm, double *Cf, double *Sf, double *C, double *V);
+
+/* ------------------------------------------------------------------ */
+/* solver mesh: addressing + geometry on the device                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t nCells;
+    int32_t nFaces;          /* internal + boundary */
+    int32_t nInternalFaces;
+    const int32_t *owner;    /* nFaces */
+    const int32_t *neighbour;/* nInternalFaces */
+    const double *C;         /* nCells*3 cell centres */
+    const double *V;         /* nCells volumes */
+    const double *Sf;        /* nFaces*3 area vectors (owner -> neighbour / outward) */
+    const double *Cf;        /* nFaces*3 face centres */
+    int32_t nPatches;
+    const int32_t *patchStart; /* absolute face index of each patch's first face */
+    const int32_t *patchSize;
+    const int32_t *patchType;  /* S4F_PATCH_* */
+    int32_t lsFullDelta;       /* 0: OpenFOAM leastSquares (normal patch delta), 1: extendedLeastSquares 0 */
+} s4f_mesh_desc;
+
+/* Uploads addressing and geometry, builds the sliced-ELL cell->face lists,
+   interpolation weights, delta coefficients, correction vectors and the
+   FP64 least-squares inverse matrices.  Host buffers may be freed on return.
+   Replaces the fvMesh / surfaceInterpolation / leastSquaresVectors set-up the
+   linGeomTotalDispSolid constructor triggers (linGeomTotalDispSolid.C:64-105). */
+s4f_status s4f_mesh_create(s4f_ctx *ctx, const s4f_mesh_desc *desc, s4f_mesh **out);
+/* Convenience: computes the geometry of `pm` on the host and creates the mesh. */
+s4f_status s4f_mesh_create_from_polymesh(s4f_ctx *ctx, const s4f_polymesh *pm,
+                                         const int32_t *patchType, int32_t lsFullDelta,
+                                         s4f_mesh **out);
+s4f_status s4f_mesh_destroy(s4f_mesh *mesh);
+/* nCells, nInternalFaces, nBoundaryFaces, ELL entries, ELL slices, device bytes */
+s4f_status s4f_mesh_info(const s4f_mesh *mesh, int64_t out[6]);
+
+/* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
+   linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
+s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);
+/* Per-face boundary data of patch `patch`: fixedDisplacement -> displacement
+   (size*3); solidTraction -> traction (size*3) and pressure (size, or NULL = 0). */
+s4f_status s4f_bc_set(s4f_mesh *mesh, int32_t patch, const double *value, const double *pressure);
+
+/* ------------------------------------------------------------------ */
+/* solve                                                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    /* solidProperties linearGeometryTotalDisplacementCoeffs (solidModel.C:1147-1177) */
+    int32_t nCorrectors;           /* default 10000 */
+    double solutionTolerance;      /* default 1e-6 */
+    double alternativeTolerance;   /* default 1e-7 */
+    /* stabilisation { type RhieChow; scaleFactor; } (solidModel.C:1307-1315) */
+    double rhieChowScale;          /* default 0.1 */
+    double interfaceScale;         /* default 0.01 (momentumStabilisation.C:122-125) */
+    /* fvSolution solvers { D { solver PCG; tolerance; relTol; maxIter; } } */
+    double pcgTolerance;           /* default 1e-9 */
+    double pcgRelTol;              /* default 0.1 */
+    int32_t pcgMaxIter;            /* default 1000 */
+    int32_t preconditioner;        /* S4F_PRECOND_* */
+    int32_t chebyshevDegree;       /* polynomial degree when preconditioner = CHEBYSHEV */
+    int32_t precision;             /* S4F_PRECISION_* */
+    int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
+} s4f_solve_opts;
+
+typedef struct {
+    int32_t nOuter;          /* outer iterations run (iCorr + 1) */
+    int32_t converged;       /* solidModel::converged became true */
+    int64_t innerTotal;      /* PCG iterations summed over components and outer iterations */
+    double initialResidual;  /* |initRes| of the last outer iteration */
+    double residualvf;       /* relative displacement change of the last outer iteration */
+    double seconds;          /* wall time of the solve on the device stream */
+    int32_t mixedFallback;   /* mixed precision fell back to FP64 inner sweeps */
+} s4f_solve_report;
+
+/* Fills opts with the reference defaults. */
+void s4f_solve_opts_default(s4f_solve_opts *opts);
+
+/* Runs the momentum loop of linGeomTotalDispSolid::evolve (:135-209) to
+   convergence.  D_inout (nCells*3): initial field when opts->warmStart, the
+   converged displacement on return.  sigma_out (nCells*6), gradD_out
+   (nCells*9) and Db_out (nBoundaryFaces*3) may be NULL.  Returns S4F_OK,
+   S4F_WARN_MAXITER or S4F_ERR_DIVERGED. */
+s4f_status s4f_solve(s4f_mesh *mesh, const s4f_solve_opts *opts, double *D_inout,
+                     double *sigma_out, double *gradD_out, double *Db_out,
+                     s4f_solve_report *report);
+/* Per-outer-iteration history of the last solve: initRes, rvf (doubles) and
+   inner iterations (int32) for the first `cap` iterations. */
+s4f_status s4f_solve_history(s4f_mesh *mesh, int32_t cap, double *initRes, double *rvf,
+                             int32_t *inner, int32_t *count);
+
+/* ------------------------------------------------------------------ */
+/* fine-grained operators (host buffers in, host buffers out)          */
+/* ------------------------------------------------------------------ */
+/* least-squares grad(D) with the boundary correction: D (C*3), Db (B*3),
+   snGrad_b (B*3) -> gradD (C*9), gradD_b (B*9, may be NULL).
+   [OF leastSquaresGrad + gaussGrad::correctBoundaryConditions] */
+s4f_status s4f_grad(s4f_mesh *mesh, const double *D, const double *Db, const double *snb,
+                    double *gradD, double *gradDb);
+/* linearElastic::correct: gradD (C*9) -> sigma (C*6) */
+s4f_status s4f_sigma(s4f_mesh *mesh, const double *gradD, double *sigma);
+/* explicit force imbalance fvc::div(sigma) + Rhie-Chow, integrated per cell:
+   D (C*3), gradD (C*9), sigma_b (B*6) -> R (C*3) */
+s4f_status s4f_residual(s4f_mesh *mesh, const double *D, const double *gradD,
+                        const double *sigmab, double *R);
+/* y = M x for three right-hand sides (x, y: C*3, vector layout); precision
+   S4F_PRECISION_MIXED runs the FP32 kernel */
+s4f_status s4f_spmv3(s4f_mesh *mesh, const double *x, double *y, int32_t precision);
+/* PCG on M x = b for three right-hand sides from x = 0 with the given
+   tolerance/relTol/maxIter; normFactor per component supplied by the caller
+   (OpenFOAM lduMatrix::solver::normFactor); iters (3) may be NULL. */
+s4f_status s4f_pcg3(s4f_mesh *mesh, const s4f_solve_opts *opts, const double *b,
+                    const double normFactor[3], double *x, int32_t iters[3]);
+/* M's diagonal (C) and per-face coefficient a_f (F, upper = -a_f) */
+s4f_status s4f_matrix(s4f_mesh *mesh, double *diag, double *faceCoeff);
+
+/* ------------------------------------------------------------------ */
+/* measurement (no counterpart in the reference: its timing is the      */
+/* per-time-step ExecutionTime/ClockTime print, solids4Foam.C:68-72)    */
+/* ------------------------------------------------------------------ */
+/* kernel ids for s4f_profile_read */
+enum { S4F_KERNEL_RESIDUAL = 0, S4F_KERNEL_SPMV = 1, S4F_KERNEL_PCG_UPDATE = 2, S4F_KERNEL_PCG_DIRECTION = 3,
+       S4F_KERNEL_GRAD = 4 };
+/* Enables (sampleEvery > 0) or disables (0) sampled hipEvent timing of the
+   solver kernels on the context's stream and resets the counters: every
+   sampleEvery-th launch of each kernel is bracketed by an event pair; launches
+   that found their solve finished are excluded. */
+s4f_status s4f_profile(s4f_ctx *ctx, int32_t sampleEvery);
+/* average duration (ms) of the sampled working launches of `kernel` */
+s4f_status s4f_profile_read(s4f_ctx *ctx, int32_t kernel, double *avgMs, int64_t *samples, int64_t *launches);
+
+/* ------------------------------------------------------------------ */
+/* batching: many meshes packed into one sliced-ELL structure          */
+/* ------------------------------------------------------------------ */
+s4f_status s4f_batch_create(s4f_ctx *ctx, s4f_batch **out);
+/* adds a mesh (with its material and BCs set) to the batch; returns its index */
+s4f_status s4f_batch_add(s4f_batch *batch, s4f_mesh *mesh, int32_t *index);
+/* Solves every instance to its own convergence with shared launches;
+   D_inout[i] (nCells_i*3) per instance; reports (count entries). */
+s4f_status s4f_batch_solve(s4f_batch *batch, const s4f_solve_opts *opts, double **D_inout,
+                           double **sigma_out, s4f_solve_report *reports);
+s4f_status s4f_batch_destroy(s4f_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S4F_ABI_H */

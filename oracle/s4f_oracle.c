@@ -939,6 +939,7 @@ static double maxmag(const double *a, const double *b, int n)
     for (int i = 0; i < n; ++i) {
         double d[3] = { a[3*i] - (b ? b[3*i] : 0.0), a[3*i+1] - (b ? b[3*i+1] : 0.0), a[3*i+2] - (b ? b[3*i+2] : 0.0) };
         double v = mag3(d);
+        if (isnan(v)) return NAN;
         if (v > mx) mx = v;
     }
     return mx;

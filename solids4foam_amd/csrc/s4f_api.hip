@@ -1,0 +1,1475 @@
+// s4f_api.hip -- host side of libs4f: device mesh set-up, material and
+// boundary data, the outer defect-correction loop of
+// linGeomTotalDispSolid::evolve (linGeomTotalDispSolid.C:111-232) driving the
+// HIP kernels, the fine-grained operator entry points and batching.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/s4f_abi.h"
+#include "s4f_device.h"
+#include "s4f_internal.h"
+#include "s4f_kernels.h"
+
+namespace s4f {
+
+thread_local std::string gLastError;
+void setError(const std::string& msg) { gLastError = msg; }
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct OomError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidError : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+#define HIP_CHECK(x)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess) {                                                                         \
+            if (e_ == hipErrorOutOfMemory) throw OomError(std::string("out of device memory: ") + #x);   \
+            throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                             \
+        }                                                                                               \
+    } while (0)
+
+// Device allocation owned by a handle.
+template <typename T>
+struct DArr {
+    T* p = nullptr;
+    size_t n = 0;
+    DArr() = default;
+    DArr(const DArr&) = delete;
+    DArr& operator=(const DArr&) = delete;
+    ~DArr() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) HIP_CHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace s4f
+
+using namespace s4f;
+
+// Sampled per-kernel timing: every `every`-th launch of a kernel is bracketed
+// by a hipEvent pair on the library stream.  Samples are resolved after the
+// host synchronises; launches that found their solve already finished
+// (early-exit no-ops) are discarded so the average is that of working launches.
+namespace s4f {
+enum KernelId { K_RESIDUAL = 0, K_SPMV = 1, K_UPDATE = 2, K_PDIR = 3, K_GRAD = 4, K_COUNT = 5 };
+struct Profiler {
+    int every = 0;
+    struct Sample { int kernel; hipEvent_t a, b; int inner; int outer; };
+    std::vector<Sample> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    long long launches[K_COUNT] = {0, 0, 0, 0, 0};
+    long long samples[K_COUNT] = {0, 0, 0, 0, 0};
+    double ms[K_COUNT] = {0, 0, 0, 0, 0};
+    std::pair<hipEvent_t, hipEvent_t> take() {
+        if (!pool.empty()) { auto e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t a, b;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        return {a, b};
+    }
+    // returns the sample index or -1
+    int begin(int k, hipStream_t s, int inner, int outer) {
+        const long long n = launches[k]++;
+        if (every <= 0 || n % every) return -1;
+        auto e = take();
+        HIP_CHECK(hipEventRecord(e.first, s));
+        pending.push_back({k, e.first, e.second, inner, outer});
+        return (int)pending.size() - 1;
+    }
+    void end(int idx, hipStream_t s) {
+        if (idx >= 0) HIP_CHECK(hipEventRecord(pending[idx].b, s));
+    }
+    // after a stream synchronisation: keep samples of launches that did work
+    template <typename Pred>
+    void resolve(Pred didWork) {
+        for (auto& x : pending) {
+            if (didWork(x)) {
+                float t = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&t, x.a, x.b));
+                ms[x.kernel] += t;
+                samples[x.kernel]++;
+            }
+            pool.push_back({x.a, x.b});
+        }
+        pending.clear();
+    }
+    void reset(int ev) {
+        every = ev;
+        for (int k = 0; k < K_COUNT; ++k) { launches[k] = 0; samples[k] = 0; ms[k] = 0; }
+    }
+    ~Profiler() {
+        for (auto& x : pending) { (void)hipEventDestroy(x.a); (void)hipEventDestroy(x.b); }
+        for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    }
+};
+}  // namespace s4f
+
+struct s4f_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nCU = 0;
+    s4f::Profiler prof;
+};
+
+struct s4f_polymesh {
+    std::unique_ptr<PolyMesh> pm;
+};
+
+// Per-mesh host staging (kept for batching and for re-uploading material data)
+struct HostMesh {
+    int C = 0, F = 0, B = 0, nPatches = 0;
+    std::vector<int> owner, neighbour, patchStart, patchSize, patchType;
+    std::vector<double> w, nonOrthDelta, magSf, bMagSf, bDelta;  // F, F, F, B, B
+    // ELL
+    int nSlices = 0;
+    std::vector<int> sliceStart, col;
+    std::vector<int> entOwn, entNei, entB;  // ELL entry of each face side
+    std::vector<double> eS, eW;             // 3E, E
+    std::vector<double> cC, minv;           // SoA 3C, 6C
+    std::vector<int> bCell, bType;
+    std::vector<double> bN, bKv, bLs;       // SoA 3B
+    // material / boundary values
+    bool uniformMaterial = true;
+    std::vector<double> mu, K, impK;        // C (or 1)
+    std::vector<double> eA, eC;             // E
+    std::vector<double> diag;               // C
+    std::vector<double> bVal, bPres;        // SoA 3B, B
+    // instances packed in this structure (one for a plain mesh): cells start
+    // on a 64-row slice boundary, padding rows carry no faces
+    std::vector<int> instCell0, instNCell, instB0, instNB;
+};
+
+struct SolveState {
+    DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
+    DArr<double> r64, p64, q64, x64;
+    DArr<float> r32, p32, q32, x32;
+    DArr<Ctl> ctl;
+    DArr<double> part, hist;
+    int histCap = 0;
+    std::vector<double> hHist;
+    int lastNOuter = 0;
+};
+
+struct s4f_mesh {
+    s4f_ctx* ctx = nullptr;
+    HostMesh h;
+    // device geometry / addressing
+    DArr<int> sliceStart, col, bCell, bType;
+    DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
+    DArr<float> eA32, diag32, invDiag32;
+    // block map (single instance)
+    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell;
+    int nBlocks = 0, slicesPerBlock = 0;
+    bool materialSet = false;
+    double rcScale = 0.1, ifScale = 0.01;  // stabilisation scales baked into eC
+    std::vector<char> bcSet;
+    SolveState st;
+    DevMesh dm() const;
+    BlockMap bmap() const;
+};
+
+DevMesh s4f_mesh::dm() const {
+    DevMesh m{};
+    m.nCells = h.C;
+    m.nBoundary = h.B;
+    m.nSlices = h.nSlices;
+    m.slicesPerBlock = slicesPerBlock;
+    m.nBlocks = nBlocks;
+    m.nEntries = h.col.size();
+    m.sliceStart = sliceStart.p;
+    m.col = col.p;
+    m.eS = eS.p;
+    m.eW = eW.p;
+    m.eA = eA.p;
+    m.eC = eC.p;
+    m.eA32 = eA32.p;
+    m.cC = cC.p;
+    m.minv = minv.p;
+    m.diag = diag.p;
+    m.invDiag = invDiag.p;
+    m.diag32 = diag32.p;
+    m.invDiag32 = invDiag32.p;
+    if (h.uniformMaterial && !h.mu.empty()) {
+        m.mu0 = h.mu[0];
+        m.K0 = h.K[0];
+        m.impK0 = h.impK[0];
+        m.mu = m.K = m.impK = nullptr;
+    } else {
+        m.mu = mu.p;
+        m.K = K.p;
+        m.impK = impK.p;
+    }
+    m.bCell = bCell.p;
+    m.bType = bType.p;
+    m.bN = bN.p;
+    m.bKv = bKv.p;
+    m.bDelta = bDelta.p;
+    m.bLs = bLs.p;
+    m.bVal = bVal.p;
+    m.bPres = bPres.p;
+    m.cellInst = nullptr;
+    m.nInst = 1;
+    return m;
+}
+
+BlockMap s4f_mesh::bmap() const {
+    BlockMap b{};
+    b.inst = bmInst.p;
+    b.s0 = bmS0.p;
+    b.s1 = bmS1.p;
+    b.iBlk0 = iBlk0.p;
+    b.iNBlk = iNBlk.p;
+    b.iCell0 = iCell0.p;
+    b.iNCell = iNCell.p;
+    b.nBlocks = nBlocks;
+    return b;
+}
+
+namespace {
+
+s4f_status fail(s4f_status s, const std::string& msg) {
+    setError(msg);
+    return s;
+}
+
+template <typename F>
+s4f_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const OomError& e) {
+        return fail(S4F_ERR_OOM, e.what());
+    } catch (const HipError& e) {
+        return fail(S4F_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(S4F_ERR_INVALID, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(S4F_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(S4F_ERR_HIP, e.what());
+    }
+}
+
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Number of workgroups for the reducing kernels: enough to fill the chip
+// (8 per CU), a multiple of 8 so that consecutive slices stay on one XCD.
+int gMaxBlocks = 2048;
+
+void buildBlockMap(int nSlices, int maxBlocks, int& spb, int& nb) {
+    const int want = std::max(1, std::min(maxBlocks, (nSlices + 3) / 4));
+    spb = (nSlices + want - 1) / want;
+    if (spb < 1) spb = 1;
+    nb = (nSlices + spb - 1) / spb;
+    if (nb >= 8 && nb % 8) nb += 8 - nb % 8;
+}
+
+// Host-side precomputation shared by every entry point: OpenFOAM
+// surfaceInterpolation weights and deltas, boundary normals/correction
+// vectors, least-squares matrices (inverted in FP64) and the sliced-ELL
+// cell->face lists.
+void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
+    const int C = d.nCells, F = d.nInternalFaces, NF = d.nFaces, B = NF - F;
+    if (C <= 0 || F < 0 || NF < F || !d.owner || (F && !d.neighbour) || !d.C || !d.V || !d.Sf || !d.Cf)
+        throw InvalidError("invalid mesh descriptor");
+    if (d.nPatches <= 0 || !d.patchStart || !d.patchSize || !d.patchType) throw InvalidError("mesh needs patches");
+    h.C = C; h.F = F; h.B = B; h.nPatches = d.nPatches;
+    h.owner.assign(d.owner, d.owner + NF);
+    h.neighbour.assign(d.neighbour, d.neighbour + F);
+    h.patchStart.assign(d.patchStart, d.patchStart + d.nPatches);
+    h.patchSize.assign(d.patchSize, d.patchSize + d.nPatches);
+    h.patchType.assign(d.patchType, d.patchType + d.nPatches);
+    {
+        int f = F;
+        for (int p = 0; p < d.nPatches; ++p) {
+            if (h.patchStart[p] != f || h.patchSize[p] < 0) throw InvalidError("patches must tile the boundary faces in order");
+            if (h.patchType[p] != S4F_PATCH_FIXED_DISPLACEMENT && h.patchType[p] != S4F_PATCH_TRACTION)
+                throw InvalidError("unsupported patch type");
+            f += h.patchSize[p];
+        }
+        if (f != NF) throw InvalidError("patches do not cover the boundary faces");
+    }
+    for (int f = 0; f < NF; ++f)
+        if (h.owner[f] < 0 || h.owner[f] >= C) throw InvalidError("owner out of range");
+    for (int f = 0; f < F; ++f)
+        if (h.neighbour[f] <= h.owner[f] || h.neighbour[f] >= C) throw InvalidError("faces must be upper-triangular (owner < neighbour < nCells)");
+
+    const double* Cc = d.C;
+    const double* Sf = d.Sf;
+    const double* Cf = d.Cf;
+    h.w.resize(F); h.nonOrthDelta.resize(F); h.magSf.resize(F);
+    for (int f = 0; f < F; ++f) {
+        const int o = h.owner[f], n = h.neighbour[f];
+        double a[3], b[3], dd[3], uA[3];
+        const double ms = std::sqrt(dot3(Sf + 3 * (size_t)f, Sf + 3 * (size_t)f));
+        h.magSf[f] = ms;
+        for (int q = 0; q < 3; ++q) {
+            a[q] = Cf[3 * (size_t)f + q] - Cc[3 * (size_t)o + q];
+            b[q] = Cc[3 * (size_t)n + q] - Cf[3 * (size_t)f + q];
+            dd[q] = Cc[3 * (size_t)n + q] - Cc[3 * (size_t)o + q];
+            uA[q] = Sf[3 * (size_t)f + q] / ms;
+        }
+        const double SfdOwn = std::fabs(dot3(Sf + 3 * (size_t)f, a)), SfdNei = std::fabs(dot3(Sf + 3 * (size_t)f, b));
+        h.w[f] = SfdNei / (SfdOwn + SfdNei);
+        h.nonOrthDelta[f] = 1.0 / std::max(dot3(uA, dd), 0.05 * std::sqrt(dot3(dd, dd)));
+    }
+    h.bCell.resize(B); h.bType.resize(B); h.bN.resize(3 * (size_t)B); h.bKv.resize(3 * (size_t)B);
+    h.bLs.resize(3 * (size_t)B); h.bDelta.resize(B); h.bMagSf.resize(B);
+    for (int p = 0; p < d.nPatches; ++p)
+        for (int q = 0; q < h.patchSize[p]; ++q) h.bType[h.patchStart[p] - F + q] = h.patchType[p];
+    // least-squares matrices
+    std::vector<double> dd6((size_t)C * 6, 0.0);
+    auto addDD = [&](int c, const double* v) {
+        const double md2 = dot3(v, v);
+        double* t = &dd6[6 * (size_t)c];
+        t[0] += v[0] * v[0] / md2; t[1] += v[0] * v[1] / md2; t[2] += v[0] * v[2] / md2;
+        t[3] += v[1] * v[1] / md2; t[4] += v[1] * v[2] / md2; t[5] += v[2] * v[2] / md2;
+    };
+    for (int f = 0; f < F; ++f) {
+        double v[3];
+        for (int q = 0; q < 3; ++q) v[q] = Cc[3 * (size_t)h.neighbour[f] + q] - Cc[3 * (size_t)h.owner[f] + q];
+        addDD(h.owner[f], v);
+        addDD(h.neighbour[f], v);
+    }
+    std::vector<double> bdl((size_t)B * 3);
+    for (int b = 0; b < B; ++b) {
+        const int f = F + b, P = h.owner[f];
+        h.bCell[b] = P;
+        const double ms = std::sqrt(dot3(Sf + 3 * (size_t)f, Sf + 3 * (size_t)f));
+        h.bMagSf[b] = ms;
+        double n[3], dl[3];
+        for (int q = 0; q < 3; ++q) {
+            n[q] = Sf[3 * (size_t)f + q] / ms;
+            dl[q] = Cf[3 * (size_t)f + q] - Cc[3 * (size_t)P + q];
+        }
+        const double nd = dot3(n, dl);
+        double dn[3];
+        for (int q = 0; q < 3; ++q) {
+            h.bN[(size_t)q * B + b] = n[q];
+            h.bKv[(size_t)q * B + b] = dl[q] - n[q] * nd;  // (I - nn).(Cf - C)
+            dn[q] = n[q] * nd;
+        }
+        h.bDelta[b] = 1.0 / std::sqrt(dot3(dn, dn));
+        const double* ld = d.lsFullDelta ? dl : dn;
+        for (int q = 0; q < 3; ++q) bdl[3 * (size_t)b + q] = ld[q];
+        addDD(P, ld);
+        const double r = 1.0 / dot3(ld, ld);
+        for (int q = 0; q < 3; ++q) h.bLs[(size_t)q * B + b] = ld[q] * r;
+    }
+    h.minv.resize(6 * (size_t)C);
+    h.cC.resize(3 * (size_t)C);
+    for (int c = 0; c < C; ++c) {
+        const double* s = &dd6[6 * (size_t)c];
+        const double xx = s[0], xy = s[1], xz = s[2], yy = s[3], yz = s[4], zz = s[5];
+        const double det = xx * yy * zz + xy * yz * xz + xz * xy * yz - xx * yz * yz - xy * xy * zz - xz * yy * xz;
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) throw InvalidError("singular least-squares matrix in cell " + std::to_string(c));
+        const double inv[6] = {(yy * zz - yz * yz) / det, (xz * yz - xy * zz) / det, (xy * yz - xz * yy) / det,
+                               (xx * zz - xz * xz) / det, (xy * xz - xx * yz) / det, (xx * yy - xy * xy) / det};
+        for (int q = 0; q < 6; ++q) h.minv[(size_t)q * C + c] = inv[q];
+        for (int q = 0; q < 3; ++q) h.cC[(size_t)q * C + c] = Cc[3 * (size_t)c + q];
+    }
+    // sliced ELL: row lists in face order, internal first then boundary
+    std::vector<int> cnt(C, 0);
+    for (int f = 0; f < NF; ++f) cnt[h.owner[f]]++;
+    for (int f = 0; f < F; ++f) cnt[h.neighbour[f]]++;
+    h.nSlices = (C + kSlice - 1) / kSlice;
+    h.sliceStart.resize((size_t)h.nSlices + 1);
+    size_t E = 0;
+    for (int s = 0; s < h.nSlices; ++s) {
+        h.sliceStart[s] = (int)E;
+        int wmax = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            const int c = s * kSlice + l;
+            if (c < C) wmax = std::max(wmax, cnt[c]);
+        }
+        E += (size_t)wmax * kSlice;
+        if (E > (size_t)INT32_MAX) throw InvalidError("mesh too large for int32 ELL offsets");
+    }
+    h.sliceStart[h.nSlices] = (int)E;
+    h.col.assign(E, kPad);
+    h.eS.assign(3 * E, 0.0);
+    h.eW.assign(E, 0.0);
+    h.entOwn.resize(F); h.entNei.resize(F); h.entB.resize(B);
+    std::vector<int> fill(C, 0);
+    auto slot = [&](int c) {
+        const int s = c / kSlice, l = c % kSlice;
+        return h.sliceStart[s] + fill[c]++ * kSlice + l;
+    };
+    // internal faces in face order: each cell receives its entries sorted by face index
+    for (int f = 0; f < F; ++f) {
+        const int o = h.owner[f], n = h.neighbour[f];
+        // neighbour side first only if its face index order requires: both lists are
+        // filled in face order, which is the accumulation order of each cell
+        const int eo = slot(o);
+        h.col[eo] = n;
+        h.entOwn[f] = eo;
+        h.eW[eo] = h.w[f];
+        for (int q = 0; q < 3; ++q) h.eS[(size_t)q * E + eo] = Sf[3 * (size_t)f + q];
+        const int en = slot(n);
+        h.col[en] = o;
+        h.entNei[f] = en;
+        h.eW[en] = 1.0 - h.w[f];
+        for (int q = 0; q < 3; ++q) h.eS[(size_t)q * E + en] = -Sf[3 * (size_t)f + q];
+    }
+    for (int b = 0; b < B; ++b) {
+        const int f = F + b, P = h.owner[f];
+        const int e = slot(P);
+        h.col[e] = -1 - b;
+        h.entB[b] = e;
+        for (int q = 0; q < 3; ++q) h.eS[(size_t)q * E + e] = Sf[3 * (size_t)f + q];
+    }
+    h.bVal.assign(3 * (size_t)B, 0.0);
+    h.bPres.assign(B, 0.0);
+    h.instCell0 = {0};
+    h.instNCell = {C};
+    h.instB0 = {0};
+    h.instNB = {B};
+}
+
+void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCell) {
+    const int C = h.C, F = h.F, B = h.B;
+    const int n = perCell ? C : 1;
+    h.uniformMaterial = !perCell;
+    h.mu.resize(n); h.K.resize(n); h.impK.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const double e = E[i], v = nu[i];
+        if (!(e > 0.0) || !std::isfinite(e) || !(v > -1.0 && v <= 0.5) || !std::isfinite(v))
+            throw InvalidError("linearElastic needs E > 0 and -1 < nu <= 0.5");
+        const double mu = e / (2.0 * (1.0 + v));
+        const double lam = v * e / ((1.0 + v) * (1.0 - 2.0 * v));
+        h.mu[i] = mu;
+        h.K[i] = e / (3.0 * (1.0 - 2.0 * v));
+        h.impK[i] = (v == 0.5) ? 2.0 * mu : 2.0 * mu + lam;
+    }
+    if (h.impK.size() == 1 && !std::isfinite(h.impK[0])) throw InvalidError("nu = 0.5 needs the pressure equation");
+    auto ik = [&](int c) { return perCell ? h.impK[c] : h.impK[0]; };
+    const size_t E_ = h.col.size();
+    h.eA.assign(E_, 0.0);
+    h.eC.assign(E_, 0.0);
+    h.diag.assign(C, 0.0);
+    const double rcScale = 0.1, ifScale = 0.01;  // defaults; scaled at solve time if different
+    for (int f = 0; f < F; ++f) {
+        const int o = h.owner[f], nn = h.neighbour[f];
+        const double wf = h.w[f];
+        const double impKf = wf * (ik(o) - ik(nn)) + ik(nn);
+        const double a = h.nonOrthDelta[f] * (impKf * h.magSf[f]);
+        const double go = rcScale * ik(o), gn = rcScale * ik(nn);
+        double gam = wf * (go - gn) + gn;
+        if (std::fabs(ik(o) - ik(nn)) > kSmall) gam = ifScale * 0.5 * (ik(o) + ik(nn));
+        const double c = gam * h.magSf[f] * h.nonOrthDelta[f];
+        h.eA[h.entOwn[f]] = a;
+        h.eA[h.entNei[f]] = a;
+        h.eC[h.entOwn[f]] = c;
+        h.eC[h.entNei[f]] = c;
+        h.diag[o] += a;
+        h.diag[nn] += a;
+    }
+    for (int b = 0; b < B; ++b)
+        if (h.bType[b] == S4F_PATCH_FIXED_DISPLACEMENT) {
+            const int P = h.bCell[b];
+            h.diag[P] += ik(P) * h.bMagSf[b] * h.bDelta[b];
+        }
+}
+
+void uploadMaterial(s4f_mesh* M) {
+    HostMesh& h = M->h;
+    hipStream_t s = M->ctx->stream;
+    M->eA.upload(h.eA, s);
+    M->eC.upload(h.eC, s);
+    M->diag.upload(h.diag, s);
+    std::vector<double> inv(h.C);
+    std::vector<float> a32(h.eA.size()), d32(h.C), i32(h.C);
+    for (int c = 0; c < h.C; ++c) {
+        inv[c] = 1.0 / h.diag[c];
+        d32[c] = (float)h.diag[c];
+        i32[c] = 1.0f / d32[c];
+    }
+    for (size_t e = 0; e < h.eA.size(); ++e) a32[e] = (float)h.eA[e];
+    M->invDiag.upload(inv, s);
+    M->eA32.upload(a32, s);
+    M->diag32.upload(d32, s);
+    M->invDiag32.upload(i32, s);
+    if (!h.uniformMaterial) {
+        M->mu.upload(h.mu, s);
+        M->K.upload(h.K, s);
+        M->impK.upload(h.impK, s);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// rhieChowScale / interfaceScale other than the defaults rescale eC
+void applyStabilisationScale(s4f_mesh* M, double rcScale, double ifScale) {
+    if (rcScale == M->rcScale && ifScale == M->ifScale) return;
+    HostMesh& h = M->h;
+    if (h.instCell0.size() != 1) throw InvalidError("batched solves use the stabilisation scales the meshes were set up with");
+    std::vector<double> eC(h.eC.size(), 0.0);
+    auto ik = [&](int c) { return h.uniformMaterial ? h.impK[0] : h.impK[c]; };
+    for (int f = 0; f < h.F; ++f) {
+        const int o = h.owner[f], nn = h.neighbour[f];
+        const double wf = h.w[f];
+        const double go = rcScale * ik(o), gn = rcScale * ik(nn);
+        double gam = wf * (go - gn) + gn;
+        if (std::fabs(ik(o) - ik(nn)) > kSmall) gam = ifScale * 0.5 * (ik(o) + ik(nn));
+        const double c = gam * h.magSf[f] * h.nonOrthDelta[f];
+        eC[h.entOwn[f]] = c;
+        eC[h.entNei[f]] = c;
+    }
+    M->eC.upload(eC, M->ctx->stream);
+    M->rcScale = rcScale;
+    M->ifScale = ifScale;
+}
+
+void uploadGeometry(s4f_mesh* M) {
+    HostMesh& h = M->h;
+    hipStream_t s = M->ctx->stream;
+    M->sliceStart.upload(h.sliceStart, s);
+    M->col.upload(h.col, s);
+    M->eS.upload(h.eS, s);
+    M->eW.upload(h.eW, s);
+    M->cC.upload(h.cC, s);
+    M->minv.upload(h.minv, s);
+    M->bCell.upload(h.bCell, s);
+    M->bType.upload(h.bType, s);
+    M->bN.upload(h.bN, s);
+    M->bKv.upload(h.bKv, s);
+    M->bDelta.upload(h.bDelta, s);
+    M->bLs.upload(h.bLs, s);
+    M->bVal.upload(h.bVal, s);
+    M->bPres.upload(h.bPres, s);
+    // workgroups: a common slice count per workgroup, every workgroup inside one instance
+    int spb = 0, nbAll = 0;
+    buildBlockMap(h.nSlices, gMaxBlocks, spb, nbAll);
+    const int nInst = (int)h.instCell0.size();
+    std::vector<int> inst, s0, s1, blk0(nInst), nblk(nInst);
+    for (int i = 0; i < nInst; ++i) {
+        const int a = h.instCell0[i] / kSlice;
+        const int e = a + (h.instNCell[i] + kSlice - 1) / kSlice;
+        blk0[i] = (int)inst.size();
+        for (int x = a; x < e; x += spb) {
+            inst.push_back(i);
+            s0.push_back(x);
+            s1.push_back(std::min(x + spb, e));
+        }
+        if ((int)inst.size() == blk0[i]) {  // empty instance: one idle workgroup keeps the reductions defined
+            inst.push_back(i);
+            s0.push_back(a);
+            s1.push_back(a);
+        }
+        nblk[i] = (int)inst.size() - blk0[i];
+    }
+    // pad to a multiple of 8 with idle workgroups of the last instance
+    while (inst.size() >= 8 && inst.size() % 8) {
+        inst.push_back(nInst - 1);
+        s0.push_back(0);
+        s1.push_back(0);
+        nblk[nInst - 1]++;
+    }
+    M->slicesPerBlock = spb;
+    M->nBlocks = (int)inst.size();
+    M->bmInst.upload(inst, s);
+    M->bmS0.upload(s0, s);
+    M->bmS1.upload(s1, s);
+    M->iBlk0.upload(blk0, s);
+    M->iNBlk.upload(nblk, s);
+    M->iCell0.upload(h.instCell0, s);
+    M->iNCell.upload(h.instNCell, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void aosToSoa(const double* a, int n, int k, std::vector<double>& out) {
+    out.resize((size_t)n * k);
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < k; ++q) out[(size_t)q * n + i] = a[(size_t)i * k + q];
+}
+void soaToAos(const std::vector<double>& s, int n, int k, double* out) {
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < k; ++q) out[(size_t)i * k + q] = s[(size_t)q * n + i];
+}
+std::vector<double> download(const DArr<double>& d, size_t n, hipStream_t s) {
+    std::vector<double> h(n);
+    if (n) HIP_CHECK(hipMemcpyAsync(h.data(), d.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return h;
+}
+
+Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
+    Ctl c;
+    std::memset(&c, 0, sizeof(c));
+    c.pcgTol = o.pcgTolerance;
+    c.pcgRelTol = o.pcgRelTol;
+    c.solTol = o.solutionTolerance;
+    c.altTol = o.alternativeTolerance;
+    c.pcgMaxIter = o.pcgMaxIter;
+    c.nCorr = o.nCorrectors;
+    c.histCap = histCap;
+    return c;
+}
+
+void checkOpts(const s4f_solve_opts& o) {
+    if (o.nCorrectors < 1) throw InvalidError("nCorrectors must be >= 1");
+    if (o.pcgMaxIter < 0) throw InvalidError("maxIter must be >= 0");
+    if (o.precision != S4F_PRECISION_FP64 && o.precision != S4F_PRECISION_MIXED) throw InvalidError("unknown precision");
+    if (o.preconditioner != S4F_PRECOND_DIAGONAL) throw InvalidError("preconditioner not available in this build");
+    if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
+        throw InvalidError("tolerances must be >= 0");
+}
+
+constexpr int kHistCap = 100000;
+
+template <typename T>
+struct PcgVecs {
+    T *r, *p, *q, *x;
+};
+
+template <typename T>
+PcgVecs<T> pcgVecs(SolveState& st, int C);
+template <>
+PcgVecs<double> pcgVecs<double>(SolveState& st, int C) {
+    st.r64.alloc(3 * (size_t)C); st.p64.alloc(3 * (size_t)C); st.q64.alloc(3 * (size_t)C); st.x64.alloc(3 * (size_t)C);
+    return {st.r64.p, st.p64.p, st.q64.p, st.x64.p};
+}
+template <>
+PcgVecs<float> pcgVecs<float>(SolveState& st, int C) {
+    st.r32.alloc(3 * (size_t)C); st.p32.alloc(3 * (size_t)C); st.q32.alloc(3 * (size_t)C); st.x32.alloc(3 * (size_t)C);
+    return {st.r32.p, st.p32.p, st.q32.p, st.x32.p};
+}
+
+void allocState(s4f_mesh* M) {
+    SolveState& st = M->st;
+    const int C = M->h.C, B = std::max(M->h.B, 1);
+    const int nInst = (int)M->h.instCell0.size();
+    st.D0.alloc(3 * (size_t)C); st.D1.alloc(3 * (size_t)C); st.Dold.alloc(3 * (size_t)C);
+    st.G.alloc(9 * (size_t)C);
+    st.Db.alloc(3 * (size_t)B); st.g.alloc(3 * (size_t)B); st.snb.alloc(3 * (size_t)B);
+    st.Gb.alloc(9 * (size_t)B); st.sigb.alloc(6 * (size_t)B);
+    st.ctl.alloc(nInst);
+    st.part.alloc((size_t)M->nBlocks * kMaxPartials);
+    st.histCap = nInst == 1 ? kHistCap : 4096;
+    st.hist.alloc(3 * (size_t)st.histCap * nInst);
+}
+
+// One launch of the inner PCG chunk: `n` iterations of spmv / update / pdir.
+template <typename T>
+void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, PcgVecs<T> v, Ctl* ctl, double* part, int n, hipStream_t s,
+                    Profiler* pf = nullptr, int inner0 = 0, int outer = 0) {
+    for (int i = 0; i < n; ++i) {
+        int x = pf ? pf->begin(K_SPMV, s, inner0 + i, outer) : -1;
+        hipLaunchKernelGGL((k_spmv<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, ctl, part, 1);
+        if (pf) pf->end(x, s);
+        x = pf ? pf->begin(K_UPDATE, s, inner0 + i, outer) : -1;
+        hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, v.p, v.r, v.q, ctl, part);
+        if (pf) pf->end(x, s);
+        // the direction update after the last working update is a no-op
+        x = pf ? pf->begin(K_PDIR, s, inner0 + i + 1, outer) : -1;
+        hipLaunchKernelGGL((k_pdir<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.r, v.p, ctl);
+        if (pf) pf->end(x, s);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+struct InstanceIO {
+    double* D_inout = nullptr;  // AoS nCells*3
+    double* sigma_out = nullptr;
+    double* gradD_out = nullptr;
+    double* Db_out = nullptr;
+    s4f_solve_report* rep = nullptr;
+    s4f_status status = S4F_OK;
+};
+
+// The outer momentum loop of linGeomTotalDispSolid::evolve on every instance
+// packed in M, with shared launches; instances finish independently.
+template <typename T>
+void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
+    s4f_ctx* ctx = M->ctx;
+    hipStream_t s = ctx->stream;
+    HostMesh& h = M->h;
+    const int C = h.C;
+    const int nInst = (int)h.instCell0.size();
+    allocState(M);
+    SolveState& st = M->st;
+    PcgVecs<T> v = pcgVecs<T>(st, C);
+    applyStabilisationScale(M, o.rhieChowScale, o.interfaceScale);
+    const DevMesh dm = M->dm();
+    const BlockMap bm = M->bmap();
+    // initial and old-time fields
+    std::vector<double> D0(3 * (size_t)C, 0.0);
+    bool warm = false;
+    for (int i = 0; i < nInst; ++i) {
+        if (!(o.warmStart && io[i].D_inout)) continue;
+        warm = true;
+        const int c0 = h.instCell0[i], n = h.instNCell[i];
+        for (int c = 0; c < n; ++c)
+            for (int q = 0; q < 3; ++q) D0[(size_t)q * C + c0 + c] = io[i].D_inout[3 * (size_t)c + q];
+    }
+    HIP_CHECK(hipMemcpyAsync(st.D0.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (warm) HIP_CHECK(hipMemcpyAsync(st.Dold.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
+    HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
+    std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
+    HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
+    Ctl* hctl = nullptr;
+    HIP_CHECK(hipHostMalloc((void**)&hctl, nInst * sizeof(Ctl), hipHostMallocDefault));
+    std::unique_ptr<Ctl, void (*)(Ctl*)> hctlGuard(hctl, [](Ctl* p) { (void)hipHostFree(p); });
+    const double* Dold = warm ? st.Dold.p : nullptr;
+    HIP_CHECK(hipEventRecord(ctx->ev0, s));
+    // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
+    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)nullptr,
+                       (double*)nullptr, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p,
+                       st.part.p, st.hist.p, 1);
+    HIP_CHECK(hipGetLastError());
+    double* Dcur = st.D0.p;
+    double* Dnext = st.D1.p;
+    int chunk = 4;
+    Profiler* pf = (ctx->prof.every > 0 && nInst == 1) ? &ctx->prof : nullptr;
+    for (int it = 0;; ++it) {
+        int xr = pf ? pf->begin(K_RESIDUAL, s, 0, it) : -1;
+        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, Dcur, st.G.p, st.sigb.p, v.r,
+                           v.p, st.ctl.p, st.part.p, 1);
+        if (pf) pf->end(xr, s);
+        bool allDone = false;
+        int total = 0;
+        for (;;) {
+            launchPcgChunk<T>(dm, bm, v, st.ctl.p, st.part.p, chunk, s, pf, total, it);
+            total += chunk;
+            HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            allDone = true;
+            bool anyInner = false;
+            for (int i = 0; i < nInst; ++i) {
+                if (!hctl[i].outerDone) allDone = false;
+                if (!hctl[i].outerDone && !hctl[i].innerDone) anyInner = true;
+            }
+            if (allDone || !anyInner) break;
+            if (total > o.pcgMaxIter + 2) break;  // maxIter deactivates every component before this
+        }
+        if (pf) {
+            // launches beyond the iterations the PCG ran (or of a finished solve) did no work
+            const int ran = allDone ? -1 : std::max({hctl[0].nIter[0], hctl[0].nIter[1], hctl[0].nIter[2]});
+            pf->resolve([&](const Profiler::Sample& x) {
+                if (x.outer != it) return x.outer < it;  // k_grad of the previous iteration
+                if (ran < 0) return false;
+                if (x.kernel == K_RESIDUAL) return true;
+                return x.inner < ran;
+            });
+        }
+        if (allDone) break;
+        int used = 1;
+        for (int i = 0; i < nInst; ++i)
+            if (!hctl[i].outerDone) used = std::max({used, hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});
+        chunk = used;
+        int xg = pf ? pf->begin(K_GRAD, s, 0, it) : -1;
+        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, Dcur, (const T*)v.x, Dnext,
+                           st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
+                           0);
+        if (pf) pf->end(xg, s);
+        HIP_CHECK(hipGetLastError());
+        std::swap(Dcur, Dnext);
+        if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
+    }
+    HIP_CHECK(hipEventRecord(ctx->ev1, s));
+    HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const bool anySigma = std::any_of(io.begin(), io.end(), [](const InstanceIO& x) { return x.sigma_out != nullptr; });
+    std::vector<double> hD0 = download(st.D0, 3 * (size_t)C, s), hD1 = download(st.D1, 3 * (size_t)C, s);
+    std::vector<double> hG, hSig, hDb;
+    if (std::any_of(io.begin(), io.end(), [](const InstanceIO& x) { return x.gradD_out != nullptr; }))
+        hG = download(st.G, 9 * (size_t)C, s);
+    if (anySigma) {
+        st.sig.alloc(6 * (size_t)C);
+        hipLaunchKernelGGL(k_sigma, dim3(std::min(4096, (C + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, dm, st.G.p,
+                           st.sig.p);
+        HIP_CHECK(hipGetLastError());
+        hSig = download(st.sig, 6 * (size_t)C, s);
+    }
+    if (std::any_of(io.begin(), io.end(), [](const InstanceIO& x) { return x.Db_out != nullptr; }) && h.B)
+        hDb = download(st.Db, 3 * (size_t)h.B, s);
+    st.hHist = download(st.hist, st.hist.n, s);
+    st.lastNOuter = hctl[0].nOuter;
+    for (int i = 0; i < nInst; ++i) {
+        const Ctl& cf = hctl[i];
+        const int c0i = h.instCell0[i], n = h.instNCell[i], b0 = h.instB0[i], nb = h.instNB[i];
+        // the final field of instance i is in the buffer written by its iteration nOuter-1
+        const std::vector<double>& Df = (cf.nOuter % 2 == 1) ? hD1 : hD0;
+        auto copyOut = [&](const std::vector<double>& src, int k, int off, int cnt, size_t stride, double* out) {
+            for (int c = 0; c < cnt; ++c)
+                for (int q = 0; q < k; ++q) out[(size_t)c * k + q] = src[(size_t)q * stride + off + c];
+        };
+        if (io[i].D_inout) copyOut(Df, 3, c0i, n, C, io[i].D_inout);
+        if (io[i].gradD_out) copyOut(hG, 9, c0i, n, C, io[i].gradD_out);
+        if (io[i].sigma_out) copyOut(hSig, 6, c0i, n, C, io[i].sigma_out);
+        if (io[i].Db_out && nb) copyOut(hDb, 3, b0, nb, h.B, io[i].Db_out);
+        if (io[i].rep) {
+            s4f_solve_report* rep = io[i].rep;
+            rep->nOuter = cf.nOuter;
+            rep->converged = cf.converged;
+            rep->innerTotal = cf.innerTotal;
+            rep->initialResidual = cf.initResMag;
+            rep->residualvf = cf.rvf;
+            rep->seconds = ms * 1e-3;
+            rep->mixedFallback = 0;
+        }
+        io[i].status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
+    }
+}
+
+void solveDispatch(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
+    if (o.precision == S4F_PRECISION_MIXED) solveT<float>(M, o, io);
+    else solveT<double>(M, o, io);
+}
+
+// Concatenates meshes (their material and boundary values included) into one
+// packed structure: each instance's cells start on a slice boundary; column,
+// boundary-face and cell indices are offset; material data is per cell.
+void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
+    P = HostMesh();
+    const int n = (int)ms.size();
+    long long C = 0, B = 0, E = 0, S = 0;
+    for (const HostMesh* m : ms) {
+        P.instCell0.push_back((int)C);
+        P.instNCell.push_back(m->C);
+        P.instB0.push_back((int)B);
+        P.instNB.push_back(m->B);
+        C += (long long)m->nSlices * kSlice;
+        B += m->B;
+        E += (long long)m->col.size();
+        S += m->nSlices;
+    }
+    if (C > INT32_MAX / 9 || E > INT32_MAX || B > INT32_MAX / 9) throw InvalidError("batch too large for int32 indexing");
+    P.C = (int)C; P.B = (int)B; P.nSlices = (int)S; P.F = 0;
+    P.uniformMaterial = false;
+    P.sliceStart.resize(S + 1);
+    P.col.resize(E);
+    P.eS.resize(3 * E); P.eW.resize(E); P.eA.resize(E); P.eC.resize(E);
+    P.cC.assign(3 * C, 0.0);
+    P.minv.assign(6 * C, 0.0);
+    P.diag.assign(C, 1.0);
+    P.mu.assign(C, 1.0); P.K.assign(C, 1.0); P.impK.assign(C, 1.0);
+    P.bCell.resize(B); P.bType.resize(B);
+    P.bN.resize(3 * B); P.bKv.resize(3 * B); P.bLs.resize(3 * B); P.bDelta.resize(B);
+    P.bVal.resize(3 * B); P.bPres.resize(B);
+    long long e0 = 0, s0 = 0;
+    for (int i = 0; i < n; ++i) {
+        const HostMesh& m = *ms[i];
+        const int c0 = P.instCell0[i], b0 = P.instB0[i];
+        const size_t Em = m.col.size();
+        for (int s = 0; s < m.nSlices; ++s) P.sliceStart[s0 + s] = (int)(e0 + m.sliceStart[s]);
+        for (size_t e = 0; e < Em; ++e) {
+            const int c = m.col[e];
+            P.col[e0 + e] = c >= 0 ? c + c0 : (c == kPad ? kPad : -1 - ((-1 - c) + b0));
+            for (int q = 0; q < 3; ++q) P.eS[(size_t)q * E + e0 + e] = m.eS[(size_t)q * Em + e];
+            P.eW[e0 + e] = m.eW[e];
+            P.eA[e0 + e] = m.eA[e];
+            P.eC[e0 + e] = m.eC[e];
+        }
+        for (int c = 0; c < m.C; ++c) {
+            for (int q = 0; q < 3; ++q) P.cC[(size_t)q * C + c0 + c] = m.cC[(size_t)q * m.C + c];
+            for (int q = 0; q < 6; ++q) P.minv[(size_t)q * C + c0 + c] = m.minv[(size_t)q * m.C + c];
+            P.diag[c0 + c] = m.diag[c];
+            const int mc = m.uniformMaterial ? 0 : c;
+            P.mu[c0 + c] = m.mu[mc];
+            P.K[c0 + c] = m.K[mc];
+            P.impK[c0 + c] = m.impK[mc];
+        }
+        for (int b = 0; b < m.B; ++b) {
+            P.bCell[b0 + b] = m.bCell[b] + c0;
+            P.bType[b0 + b] = m.bType[b];
+            P.bDelta[b0 + b] = m.bDelta[b];
+            P.bPres[b0 + b] = m.bPres[b];
+            for (int q = 0; q < 3; ++q) {
+                P.bN[(size_t)q * B + b0 + b] = m.bN[(size_t)q * m.B + b];
+                P.bKv[(size_t)q * B + b0 + b] = m.bKv[(size_t)q * m.B + b];
+                P.bLs[(size_t)q * B + b0 + b] = m.bLs[(size_t)q * m.B + b];
+                P.bVal[(size_t)q * B + b0 + b] = m.bVal[(size_t)q * m.B + b];
+            }
+        }
+        e0 += (long long)Em;
+        s0 += m.nSlices;
+    }
+    P.sliceStart[S] = (int)E;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* s4f_last_error(void) { return gLastError.c_str(); }
+const char* s4f_version(void) { return "s4f-mi355x 0.1 (gfx950)"; }
+
+s4f_status s4f_ctx_create(int device, s4f_ctx** out) {
+    return guarded([&]() -> s4f_status {
+        if (!out) throw InvalidError("null output");
+        int n = 0;
+        HIP_CHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw InvalidError("no HIP device " + std::to_string(device));
+        HIP_CHECK(hipSetDevice(device));
+        auto c = std::make_unique<s4f_ctx>();
+        c->device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&c->ev0));
+        HIP_CHECK(hipEventCreate(&c->ev1));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c->nCU = prop.multiProcessorCount;
+        *out = c.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_ctx_destroy(s4f_ctx* ctx) {
+    if (!ctx) return S4F_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return S4F_OK;
+}
+
+s4f_status s4f_polymesh_box(int nx, int ny, int nz, const double L[3], double jitter, uint64_t seed, int tets,
+                            s4f_polymesh** out) {
+    return guarded([&]() -> s4f_status {
+        if (!out || !L) throw InvalidError("null argument");
+        auto p = std::make_unique<s4f_polymesh>();
+        p->pm.reset(makeBoxMesh(nx, ny, nz, L, jitter, seed, tets != 0));
+        *out = p.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_polymesh_from_arrays(int32_t nPoints, const double* points, int32_t nCells, int32_t nFaces,
+                                    int32_t nInternalFaces, const int32_t* faceOffsets, const int32_t* faceVerts,
+                                    const int32_t* owner, const int32_t* neighbour, int32_t nPatches,
+                                    const int32_t* patchStart, const int32_t* patchSize, s4f_polymesh** out) {
+    return guarded([&]() -> s4f_status {
+        if (!out || !points || !faceOffsets || !faceVerts || !owner || (nInternalFaces && !neighbour))
+            throw InvalidError("null argument");
+        if (nCells <= 0 || nFaces <= 0 || nInternalFaces < 0 || nInternalFaces > nFaces || nPatches < 0)
+            throw InvalidError("invalid sizes");
+        auto p = std::make_unique<s4f_polymesh>();
+        p->pm = std::make_unique<PolyMesh>();
+        PolyMesh& m = *p->pm;
+        m.nPoints = nPoints; m.nCells = nCells; m.nFaces = nFaces; m.nInternalFaces = nInternalFaces; m.nPatches = nPatches;
+        m.points.assign(points, points + 3 * (size_t)nPoints);
+        m.fOff.assign(faceOffsets, faceOffsets + nFaces + 1);
+        for (int f = 0; f < nFaces; ++f)
+            if (m.fOff[f + 1] < m.fOff[f] + 3) throw InvalidError("faces need >= 3 points");
+        m.fVerts.assign(faceVerts, faceVerts + m.fOff[nFaces]);
+        for (int v : m.fVerts)
+            if (v < 0 || v >= nPoints) throw InvalidError("face point out of range");
+        m.owner.assign(owner, owner + nFaces);
+        m.neighbour.assign(neighbour, neighbour + nInternalFaces);
+        for (int f = 0; f < nFaces; ++f)
+            if (m.owner[f] < 0 || m.owner[f] >= nCells) throw InvalidError("owner out of range");
+        for (int f = 0; f < nInternalFaces; ++f)
+            if (m.neighbour[f] < 0 || m.neighbour[f] >= nCells) throw InvalidError("neighbour out of range");
+        m.patchStart.assign(patchStart, patchStart + nPatches);
+        m.patchSize.assign(patchSize, patchSize + nPatches);
+        m.computeGeometry();
+        *out = p.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_polymesh_free(s4f_polymesh* pm) {
+    delete pm;
+    return S4F_OK;
+}
+
+s4f_status s4f_polymesh_sizes(const s4f_polymesh* p, int64_t out[6]) {
+    if (!p || !out) return fail(S4F_ERR_INVALID, "null argument");
+    const PolyMesh& m = *p->pm;
+    out[0] = m.nPoints; out[1] = m.nCells; out[2] = m.nFaces; out[3] = m.nInternalFaces; out[4] = m.nPatches;
+    out[5] = m.fOff.empty() ? 0 : m.fOff[m.nFaces];
+    return S4F_OK;
+}
+
+s4f_status s4f_polymesh_addressing(const s4f_polymesh* p, double* points, int32_t* faceOffsets, int32_t* faceVerts,
+                                   int32_t* owner, int32_t* neighbour, int32_t* patchStart, int32_t* patchSize) {
+    if (!p) return fail(S4F_ERR_INVALID, "null argument");
+    const PolyMesh& m = *p->pm;
+    if (points) std::memcpy(points, m.points.data(), m.points.size() * sizeof(double));
+    if (faceOffsets) std::memcpy(faceOffsets, m.fOff.data(), m.fOff.size() * sizeof(int));
+    if (faceVerts) std::memcpy(faceVerts, m.fVerts.data(), m.fVerts.size() * sizeof(int));
+    if (owner) std::memcpy(owner, m.owner.data(), m.owner.size() * sizeof(int));
+    if (neighbour) std::memcpy(neighbour, m.neighbour.data(), m.neighbour.size() * sizeof(int));
+    if (patchStart) std::memcpy(patchStart, m.patchStart.data(), m.patchStart.size() * sizeof(int));
+    if (patchSize) std::memcpy(patchSize, m.patchSize.data(), m.patchSize.size() * sizeof(int));
+    return S4F_OK;
+}
+
+s4f_status s4f_polymesh_geometry(const s4f_polymesh* p, double* Cf, double* Sf, double* C, double* V) {
+    if (!p) return fail(S4F_ERR_INVALID, "null argument");
+    const PolyMesh& m = *p->pm;
+    if (Cf) std::memcpy(Cf, m.Cf.data(), m.Cf.size() * sizeof(double));
+    if (Sf) std::memcpy(Sf, m.Sf.data(), m.Sf.size() * sizeof(double));
+    if (C) std::memcpy(C, m.C.data(), m.C.size() * sizeof(double));
+    if (V) std::memcpy(V, m.V.data(), m.V.size() * sizeof(double));
+    return S4F_OK;
+}
+
+s4f_status s4f_mesh_create(s4f_ctx* ctx, const s4f_mesh_desc* desc, s4f_mesh** out) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx || !desc || !out) throw InvalidError("null argument");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto M = std::make_unique<s4f_mesh>();
+        M->ctx = ctx;
+        setupHostMesh(M->h, *desc);
+        M->bcSet.assign(M->h.nPatches, 0);
+        uploadGeometry(M.get());
+        *out = M.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_mesh_create_from_polymesh(s4f_ctx* ctx, const s4f_polymesh* p, const int32_t* patchType,
+                                         int32_t lsFullDelta, s4f_mesh** out) {
+    if (!p || !patchType) return fail(S4F_ERR_INVALID, "null argument");
+    const PolyMesh& m = *p->pm;
+    s4f_mesh_desc d{};
+    d.nCells = m.nCells;
+    d.nFaces = m.nFaces;
+    d.nInternalFaces = m.nInternalFaces;
+    d.owner = m.owner.data();
+    d.neighbour = m.neighbour.data();
+    d.C = m.C.data();
+    d.V = m.V.data();
+    d.Sf = m.Sf.data();
+    d.Cf = m.Cf.data();
+    d.nPatches = m.nPatches;
+    d.patchStart = m.patchStart.data();
+    d.patchSize = m.patchSize.data();
+    d.patchType = patchType;
+    d.lsFullDelta = lsFullDelta;
+    return s4f_mesh_create(ctx, &d, out);
+}
+
+s4f_status s4f_mesh_destroy(s4f_mesh* mesh) {
+    if (!mesh) return S4F_OK;
+    (void)hipSetDevice(mesh->ctx->device);
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+    delete mesh;
+    return S4F_OK;
+}
+
+s4f_status s4f_mesh_info(const s4f_mesh* m, int64_t out[6]) {
+    if (!m || !out) return fail(S4F_ERR_INVALID, "null argument");
+    out[0] = m->h.C; out[1] = m->h.F; out[2] = m->h.B; out[3] = (int64_t)m->h.col.size(); out[4] = m->h.nSlices;
+    size_t bytes = m->sliceStart.bytes() + m->col.bytes() + m->eS.bytes() + m->eW.bytes() + m->eA.bytes() + m->eC.bytes() +
+                   m->eA32.bytes() + m->cC.bytes() + m->minv.bytes() + m->diag.bytes() + m->invDiag.bytes() +
+                   m->diag32.bytes() + m->invDiag32.bytes() + m->bN.bytes() + m->bKv.bytes() + m->bLs.bytes() +
+                   m->bDelta.bytes() + m->bVal.bytes() + m->bPres.bytes() + m->bCell.bytes() + m->bType.bytes();
+    out[5] = (int64_t)bytes;
+    return S4F_OK;
+}
+
+s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int32_t perCell) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !E || !nu) throw InvalidError("null argument");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        computeMaterial(M->h, E, nu, perCell != 0);
+        uploadMaterial(M);
+        M->rcScale = 0.1;
+        M->ifScale = 0.01;
+        M->materialSet = true;
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_bc_set(s4f_mesh* M, int32_t patch, const double* value, const double* pressure) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !value) throw InvalidError("null argument");
+        HostMesh& h = M->h;
+        if (patch < 0 || patch >= h.nPatches) throw InvalidError("patch index out of range");
+        const int b0 = h.patchStart[patch] - h.F, n = h.patchSize[patch];
+        for (int i = 0; i < n; ++i) {
+            for (int q = 0; q < 3; ++q) {
+                const double v = value[3 * (size_t)i + q];
+                if (!std::isfinite(v)) throw InvalidError("non-finite boundary value");
+                h.bVal[(size_t)q * h.B + b0 + i] = v;
+            }
+            h.bPres[b0 + i] = pressure ? pressure[i] : 0.0;
+        }
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        M->bVal.upload(h.bVal, M->ctx->stream);
+        M->bPres.upload(h.bPres, M->ctx->stream);
+        HIP_CHECK(hipStreamSynchronize(M->ctx->stream));
+        M->bcSet[patch] = 1;
+        return S4F_OK;
+    });
+}
+
+void s4f_solve_opts_default(s4f_solve_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->nCorrectors = 10000;
+    o->solutionTolerance = 1e-6;
+    o->alternativeTolerance = 1e-7;
+    o->rhieChowScale = 0.1;
+    o->interfaceScale = 0.01;
+    o->pcgTolerance = 1e-9;
+    o->pcgRelTol = 0.1;
+    o->pcgMaxIter = 1000;
+    o->preconditioner = S4F_PRECOND_DIAGONAL;
+    o->chebyshevDegree = 4;
+    o->precision = S4F_PRECISION_FP64;
+    o->warmStart = 0;
+}
+
+s4f_status s4f_solve(s4f_mesh* M, const s4f_solve_opts* opts, double* D_inout, double* sigma_out, double* gradD_out,
+                     double* Db_out, s4f_solve_report* report) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !opts) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        checkOpts(*opts);
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        std::vector<InstanceIO> io(1);
+        io[0].D_inout = D_inout;
+        io[0].sigma_out = sigma_out;
+        io[0].gradD_out = gradD_out;
+        io[0].Db_out = Db_out;
+        io[0].rep = report;
+        solveDispatch(M, *opts, io);
+        if (io[0].status == S4F_ERR_DIVERGED) return fail(S4F_ERR_DIVERGED, "outer residual is not finite (diverged)");
+        if (io[0].status == S4F_WARN_MAXITER) return fail(S4F_WARN_MAXITER, "Max iterations reached within momentum loop");
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_solve_history(s4f_mesh* M, int32_t cap, double* initRes, double* rvf, int32_t* inner, int32_t* count) {
+    if (!M) return fail(S4F_ERR_INVALID, "null argument");
+    const SolveState& st = M->st;
+    const int n = std::min({cap, st.lastNOuter, st.histCap});
+    for (int i = 0; i < n; ++i) {
+        if (initRes) initRes[i] = st.hHist[i];
+        if (rvf) rvf[i] = st.hHist[(size_t)st.histCap + i];
+        if (inner) inner[i] = (int32_t)st.hHist[2 * (size_t)st.histCap + i];
+    }
+    if (count) *count = n;
+    return S4F_OK;
+}
+
+s4f_status s4f_grad(s4f_mesh* M, const double* D, const double* Db, const double* snb, double* gradD, double* gradDb) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !D || !Db || !snb || !gradD) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C, B = M->h.B;
+        allocState(M);
+        SolveState& st = M->st;
+        std::vector<double> t;
+        aosToSoa(D, C, 3, t);
+        HIP_CHECK(hipMemcpyAsync(st.D0.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        if (B) {
+            aosToSoa(Db, B, 3, t);
+            HIP_CHECK(hipMemcpyAsync(st.Db.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+            aosToSoa(snb, B, 3, t);
+            HIP_CHECK(hipMemcpyAsync(st.snb.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        }
+        Ctl c0;
+        std::memset(&c0, 0, sizeof(c0));
+        HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        hipLaunchKernelGGL((k_grad<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p,
+                           (const double*)nullptr, (double*)nullptr, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p,
+                           st.sigb.p, (const double*)nullptr, st.ctl.p, st.part.p, (double*)nullptr, 2);
+        HIP_CHECK(hipGetLastError());
+        soaToAos(download(st.G, 9 * (size_t)C, s), C, 9, gradD);
+        if (gradDb && B) soaToAos(download(st.Gb, 9 * (size_t)B, s), B, 9, gradDb);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_sigma(s4f_mesh* M, const double* gradD, double* sigma) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !gradD || !sigma) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C;
+        allocState(M);
+        SolveState& st = M->st;
+        st.sig.alloc(6 * (size_t)C);
+        std::vector<double> t;
+        aosToSoa(gradD, C, 9, t);
+        HIP_CHECK(hipMemcpyAsync(st.G.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_sigma, dim3(std::min(4096, (C + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, M->dm(), st.G.p,
+                           st.sig.p);
+        HIP_CHECK(hipGetLastError());
+        soaToAos(download(st.sig, 6 * (size_t)C, s), C, 6, sigma);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_residual(s4f_mesh* M, const double* D, const double* gradD, const double* sigmab, double* R) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !D || !gradD || !sigmab || !R) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C, B = M->h.B;
+        allocState(M);
+        SolveState& st = M->st;
+        PcgVecs<double> v = pcgVecs<double>(st, C);
+        std::vector<double> t;
+        aosToSoa(D, C, 3, t);
+        HIP_CHECK(hipMemcpyAsync(st.D0.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        aosToSoa(gradD, C, 9, t);
+        HIP_CHECK(hipMemcpyAsync(st.G.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        if (B) {
+            aosToSoa(sigmab, B, 6, t);
+            HIP_CHECK(hipMemcpyAsync(st.sigb.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+        }
+        Ctl c0;
+        std::memset(&c0, 0, sizeof(c0));
+        HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        hipLaunchKernelGGL((k_residual<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, st.G.p,
+                           st.sigb.p, v.r, v.p, st.ctl.p, st.part.p, 0);
+        HIP_CHECK(hipGetLastError());
+        soaToAos(download(st.r64, 3 * (size_t)C, s), C, 3, R);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !x || !y) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C;
+        allocState(M);
+        SolveState& st = M->st;
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        std::vector<double> t;
+        aosToSoa(x, C, 3, t);
+        if (precision == S4F_PRECISION_MIXED) {
+            PcgVecs<float> v = pcgVecs<float>(st, C);
+            std::vector<float> tf(t.begin(), t.end());
+            HIP_CHECK(hipMemcpyAsync(v.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_spmv<float>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, st.ctl.p,
+                               st.part.p, 0);
+            HIP_CHECK(hipGetLastError());
+            std::vector<float> yf(3 * (size_t)C);
+            HIP_CHECK(hipMemcpyAsync(yf.data(), v.q, yf.size() * 4, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            std::vector<double> yd(yf.begin(), yf.end());
+            soaToAos(yd, C, 3, y);
+        } else {
+            PcgVecs<double> v = pcgVecs<double>(st, C);
+            HIP_CHECK(hipMemcpyAsync(v.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_spmv<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, st.ctl.p,
+                               st.part.p, 0);
+            HIP_CHECK(hipGetLastError());
+            soaToAos(download(st.q64, 3 * (size_t)C, s), C, 3, y);
+        }
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_pcg3(s4f_mesh* M, const s4f_solve_opts* o, const double* b, const double normFactor[3], double* x,
+                    int32_t iters[3]) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !o || !b || !normFactor || !x) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        checkOpts(*o);
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C;
+        allocState(M);
+        SolveState& st = M->st;
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        std::vector<double> t;
+        aosToSoa(b, C, 3, t);
+        DArr<double> bd;
+        bd.upload(t, s);
+        Ctl c0 = makeCtl(*o, 0);
+        for (int j = 0; j < 3; ++j) c0.normFactor[j] = normFactor[j];
+        HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        Ctl hc;
+        auto run = [&](auto v, auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL((k_pcg_start<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, bd.p, v.r, v.p, st.ctl.p,
+                               st.part.p);
+            HIP_CHECK(hipGetLastError());
+            // x = 0 when no iteration runs
+            HIP_CHECK(hipMemsetAsync(v.x, 0, 3 * (size_t)C * sizeof(T), s));
+            int total = 0;
+            for (;;) {
+                launchPcgChunk<T>(dm, bm, v, st.ctl.p, st.part.p, 8, s);
+                total += 8;
+                HIP_CHECK(hipMemcpyAsync(&hc, st.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                if (hc.innerDone || total > o->pcgMaxIter + 8) break;
+            }
+            std::vector<T> xs(3 * (size_t)C);
+            HIP_CHECK(hipMemcpyAsync(xs.data(), v.x, xs.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            std::vector<double> xd(xs.begin(), xs.end());
+            soaToAos(xd, C, 3, x);
+        };
+        if (o->precision == S4F_PRECISION_MIXED) run(pcgVecs<float>(st, C), 0.0f);
+        else run(pcgVecs<double>(st, C), 0.0);
+        if (iters)
+            for (int j = 0; j < 3; ++j) iters[j] = hc.nIter[j];
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_matrix(s4f_mesh* M, double* diag, double* faceCoeff) {
+    if (!M) return fail(S4F_ERR_INVALID, "null argument");
+    if (!M->materialSet) return fail(S4F_ERR_INVALID, "material not set");
+    if (diag) std::memcpy(diag, M->h.diag.data(), M->h.diag.size() * sizeof(double));
+    if (faceCoeff)
+        for (int f = 0; f < M->h.F; ++f) faceCoeff[f] = M->h.eA[M->h.entOwn[f]];
+    return S4F_OK;
+}
+
+s4f_status s4f_profile(s4f_ctx* ctx, int32_t sampleEvery) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx) throw InvalidError("null argument");
+        if (sampleEvery < 0) throw InvalidError("sampleEvery must be >= 0");
+        ctx->prof.reset(sampleEvery);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_profile_read(s4f_ctx* ctx, int32_t kernel, double* avgMs, int64_t* samples, int64_t* launches) {
+    if (!ctx || kernel < 0 || kernel >= K_COUNT) return fail(S4F_ERR_INVALID, "bad kernel id");
+    const Profiler& p = ctx->prof;
+    if (avgMs) *avgMs = p.samples[kernel] ? p.ms[kernel] / p.samples[kernel] : 0.0;
+    if (samples) *samples = p.samples[kernel];
+    if (launches) *launches = p.launches[kernel];
+    return S4F_OK;
+}
+
+struct s4f_batch {
+    s4f_ctx* ctx = nullptr;
+    std::vector<s4f_mesh*> members;   // borrowed; must outlive the solve
+    std::unique_ptr<s4f_mesh> packed; // rebuilt when membership changes
+    bool dirty = true;
+};
+
+s4f_status s4f_batch_create(s4f_ctx* ctx, s4f_batch** out) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx || !out) throw InvalidError("null argument");
+        auto b = std::make_unique<s4f_batch>();
+        b->ctx = ctx;
+        *out = b.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_batch_add(s4f_batch* b, s4f_mesh* m, int32_t* index) {
+    return guarded([&]() -> s4f_status {
+        if (!b || !m) throw InvalidError("null argument");
+        if (!m->materialSet) throw InvalidError("material not set");
+        if (m->ctx != b->ctx) throw InvalidError("mesh belongs to another context");
+        if (m->h.instCell0.size() != 1) throw InvalidError("cannot nest batches");
+        b->members.push_back(m);
+        b->dirty = true;
+        if (index) *index = (int32_t)b->members.size() - 1;
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_inout, double** sigma_out,
+                           s4f_solve_report* reports) {
+    return guarded([&]() -> s4f_status {
+        if (!b || !opts) throw InvalidError("null argument");
+        if (b->members.empty()) throw InvalidError("empty batch");
+        checkOpts(*opts);
+        HIP_CHECK(hipSetDevice(b->ctx->device));
+        if (b->dirty || !b->packed) {
+            std::vector<const HostMesh*> hs;
+            for (s4f_mesh* m : b->members) hs.push_back(&m->h);
+            auto P = std::make_unique<s4f_mesh>();
+            P->ctx = b->ctx;
+            packMeshes(hs, P->h);
+            uploadGeometry(P.get());
+            uploadMaterial(P.get());
+            P->materialSet = true;
+            b->packed = std::move(P);
+            b->dirty = false;
+        } else {
+            // refresh boundary values (loads may change between solves)
+            HostMesh& P = b->packed->h;
+            for (size_t i = 0; i < b->members.size(); ++i) {
+                const HostMesh& m = b->members[i]->h;
+                for (int x = 0; x < m.B; ++x) {
+                    P.bPres[P.instB0[i] + x] = m.bPres[x];
+                    for (int q = 0; q < 3; ++q) P.bVal[(size_t)q * P.B + P.instB0[i] + x] = m.bVal[(size_t)q * m.B + x];
+                }
+            }
+            b->packed->bVal.upload(P.bVal, b->ctx->stream);
+            b->packed->bPres.upload(P.bPres, b->ctx->stream);
+        }
+        const int n = (int)b->members.size();
+        std::vector<InstanceIO> io(n);
+        for (int i = 0; i < n; ++i) {
+            io[i].D_inout = D_inout ? D_inout[i] : nullptr;
+            io[i].sigma_out = sigma_out ? sigma_out[i] : nullptr;
+            io[i].rep = reports ? reports + i : nullptr;
+        }
+        solveDispatch(b->packed.get(), *opts, io);
+        s4f_status worst = S4F_OK;
+        for (const InstanceIO& x : io) {
+            if (x.status == S4F_ERR_DIVERGED) worst = S4F_ERR_DIVERGED;
+            else if (x.status == S4F_WARN_MAXITER && worst == S4F_OK) worst = S4F_WARN_MAXITER;
+        }
+        if (worst != S4F_OK) return fail(worst, worst == S4F_ERR_DIVERGED ? "an instance diverged"
+                                                                          : "an instance reached nCorrectors");
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_batch_destroy(s4f_batch* b) {
+    if (!b) return S4F_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return S4F_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,716 @@
+// s4f_kernels.h -- HIP kernels of the segregated small-strain solid solver.
+//
+// All kernels walk the sliced-ELL cell->face lists: wave w of a workgroup
+// owns one 64-row slice at a time, lane l owns one cell, and each slot of the
+// slice is one coalesced load per array.  Face contributions are gathered
+// per cell in OpenFOAM's face order, so no atomics touch field data and the
+// summation order is fixed (run-to-run bitwise deterministic).  Reductions
+// are per workgroup (wave shuffles + LDS) into a partial array; the last
+// workgroup of each instance (counter in the control block) reduces the
+// partials in fixed order and updates the control block.
+//
+// Kernel <-> reference map (paths relative to the solids4foam tree):
+//   k_residual   linGeomTotalDispSolid.C:141-149: fvc::div(sigma) [Gauss linear]
+//                + momentumStabilisation.C:210-217 (RhieChow) + the normalisation
+//                of OpenFOAM lduMatrix::solver::normFactor for the PCG initial residual
+//   k_spmv       OpenFOAM lduMatrix::Amul (3 right-hand sides sharing M)
+//   k_update     OpenFOAM PCG solution/residual update + diagonal preconditioner
+//   k_pdir       OpenFOAM PCG search-direction update
+//   k_grad       solidTractionFvPatchVectorField.C:398-463 / fixedDisplacement
+//                boundary values, leastSquares grad(D) (mechanicalModel.C:571-582),
+//                gaussGrad boundary correction, linearElastic::correct on
+//                boundary faces (linearElastic.C:318-339), solidTraction
+//                updateCoeffs (:321-395 + linGeomTotalDispSolid.C:235-271) and
+//                solidModel::converged (solidModelTemplates.C:26-188)
+#pragma once
+
+#include "s4f_device.h"
+
+namespace s4f {
+
+struct BlockMap {
+    const int* inst;   // instance of each logical block
+    const int* s0;     // first slice
+    const int* s1;     // one past last slice
+    const int* iBlk0;  // per instance: first logical block
+    const int* iNBlk;  // per instance: number of blocks
+    const int* iCell0; // per instance: first cell
+    const int* iNCell; // per instance: number of cells
+    int nBlocks;
+};
+
+__device__ __forceinline__ int logicalBlock(int nb) {
+    const int b = blockIdx.x;
+    // consecutive logical blocks (neighbouring cells) land on the same XCD
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
+// NaN-propagating maximum (fmax would drop a NaN and hide a diverged solve)
+__device__ __forceinline__ double nanMax(double a, double b) { return (b > a || b != b) ? b : a; }
+
+template <bool MAX>
+__device__ __forceinline__ double waveReduce(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = MAX ? nanMax(v, o) : v + o;
+    }
+    return v;
+}
+
+template <bool MAX>
+__device__ __forceinline__ double blockReduce(double v, double* sh) {
+    v = waveReduce<MAX>(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; ++i) r = MAX ? nanMax(r, sh[i]) : r + sh[i];
+    return r;
+}
+
+__device__ __forceinline__ void storePartial(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double loadPartial(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// true in exactly one (the last arriving) workgroup of the instance
+__device__ __forceinline__ bool lastOfInstance(Ctl* c, int slot, int nblk) {
+    __shared__ int isLast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicInc(&c->counter[slot], (unsigned)(nblk - 1));
+        isLast = (t == (unsigned)(nblk - 1));
+    }
+    __syncthreads();
+    if (isLast) __threadfence();
+    return isLast;
+}
+
+// fixed-order reduction of partial value `i` over the instance's blocks
+template <bool MAX>
+__device__ double reducePartials(const double* part, int blk0, int nblk, int i, double* sh) {
+    double v = MAX ? 0.0 : 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
+        const double x = loadPartial(part + (size_t)(blk0 + b) * kMaxPartials + i);
+        v = MAX ? nanMax(v, x) : v + x;
+    }
+    return blockReduce<MAX>(v, sh);
+}
+
+// linearElastic: sigma = 2 mu dev(symm G) + K tr(symm G) I   (xx xy xz yy yz zz)
+__device__ __forceinline__ void hooke(const double* G, double mu, double K, double* s) {
+    const double e0 = G[0], e1 = 0.5 * (G[1] + G[3]), e2 = 0.5 * (G[2] + G[6]);
+    const double e3 = G[4], e4 = 0.5 * (G[5] + G[7]), e5 = G[8];
+    const double tr = e0 + e3 + e5;
+    const double sh = K * tr;
+    const double t3 = tr / 3.0;
+    s[0] = 2.0 * mu * (e0 - t3) + sh;
+    s[1] = 2.0 * mu * e1;
+    s[2] = 2.0 * mu * e2;
+    s[3] = 2.0 * mu * (e3 - t3) + sh;
+    s[4] = 2.0 * mu * e4;
+    s[5] = 2.0 * mu * (e5 - t3) + sh;
+}
+
+__device__ __forceinline__ double cellMu(const DevMesh& m, int c) { return m.mu ? m.mu[c] : m.mu0; }
+__device__ __forceinline__ double cellK(const DevMesh& m, int c) { return m.K ? m.K[c] : m.K0; }
+__device__ __forceinline__ double cellImpK(const DevMesh& m, int c) { return m.impK ? m.impK[c] : m.impK0; }
+
+template <typename T> struct MatSel;
+template <> struct MatSel<double> {
+    __device__ static const double* a(const DevMesh& m) { return m.eA; }
+    __device__ static const double* diag(const DevMesh& m) { return m.diag; }
+    __device__ static const double* invDiag(const DevMesh& m) { return m.invDiag; }
+};
+template <> struct MatSel<float> {
+    __device__ static const float* a(const DevMesh& m) { return m.eA32; }
+    __device__ static const float* diag(const DevMesh& m) { return m.diag32; }
+    __device__ static const float* invDiag(const DevMesh& m) { return m.invDiag32; }
+};
+
+// ---------------------------------------------------------------------------
+// k_residual: R = div(sigma) + RhieChow (integrated), r = R, p = z = R/diag,
+// and the PCG start: initial residual normalised by normFactor per component.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const BlockMap bm, const double* __restrict__ D,
+                                                     const double* __restrict__ G, const double* __restrict__ sigb,
+                                                     T* __restrict__ r, T* __restrict__ p, Ctl* __restrict__ ctlAll,
+                                                     double* __restrict__ part, int pcgStart) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone) return;
+    const int C = m.nCells;
+    const double xb0 = ctl->xbar[0], xb1 = ctl->xbar[1], xb2 = ctl->xbar[2];
+    double acc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) acc[i] = 0.0;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        const int e0 = m.sliceStart[s];
+        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        if (P >= cEnd) continue;
+        double DP[3], GP[9], CP[3], sP[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { DP[k] = D[(size_t)k * C + P]; CP[k] = m.cC[(size_t)k * C + P]; }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) GP[k] = G[(size_t)k * C + P];
+        const double muP = cellMu(m, P), KP = cellK(m, P);
+        hooke(GP, muP, KP, sP);
+        double div[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, off[3] = {0, 0, 0}, asum = 0.0;
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int n = m.col[e];
+            if (n >= 0) {
+                const double S0 = m.eS[e], S1 = m.eS[m.nEntries + e],
+                             S2 = m.eS[2 * m.nEntries + e];
+                const double w = m.eW[e];
+                double GN[9], sN[6], DN[3], d[3];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) GN[q] = G[(size_t)q * C + n];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) { DN[q] = D[(size_t)q * C + n]; d[q] = m.cC[(size_t)q * C + n] - CP[q]; }
+                hooke(GN, cellMu(m, n), cellK(m, n), sN);
+                double sf[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) sf[q] = w * (sP[q] - sN[q]) + sN[q];
+                div[0] += S0 * sf[0] + S1 * sf[1] + S2 * sf[2];
+                div[1] += S0 * sf[1] + S1 * sf[3] + S2 * sf[4];
+                div[2] += S0 * sf[2] + S1 * sf[4] + S2 * sf[5];
+                double Gf[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Gf[q] = w * (GP[q] - GN[q]) + GN[q];
+                const double cf = m.eC[e];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double dG = d[0] * Gf[j] + d[1] * Gf[3 + j] + d[2] * Gf[6 + j];
+                    rc[j] += cf * ((DN[j] - DP[j]) - dG);
+                }
+                const double a = m.eA[e];
+                asum += a;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) off[j] += a * DN[j];
+            } else if (n != kPad) {
+                const int b = -1 - n;
+                const size_t E = m.nEntries;
+                const double S0 = m.eS[e], S1 = m.eS[E + e], S2 = m.eS[2 * E + e];
+                double sb[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) sb[q] = sigb[(size_t)q * m.nBoundary + b];
+                div[0] += S0 * sb[0] + S1 * sb[1] + S2 * sb[2];
+                div[1] += S0 * sb[1] + S1 * sb[3] + S2 * sb[4];
+                div[2] += S0 * sb[2] + S1 * sb[4] + S2 * sb[5];
+            }
+        }
+        const double dg = m.diag[P];
+        const double rowsum = dg - asum;
+        const double xb[3] = {xb0, xb1, xb2};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double R = div[j] + rc[j];
+            const double MD = dg * DP[j] - off[j];
+            const double t = rowsum * xb[j];
+            acc[j] += fabs(R);
+            acc[3 + j] += fabs(MD - t);
+            acc[6 + j] += fabs((MD + R) - t);
+            const T rT = (T)R;
+            r[(size_t)j * C + P] = rT;
+            if (pcgStart) {
+                const T z = invD[P] * rT;
+                p[(size_t)j * C + P] = z;
+                acc[9 + j] += (double)z * (double)rT;
+            }
+        }
+    }
+    __shared__ double sh[kBlock / 64];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const double v = blockReduce<false>(acc[i], sh);
+        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    }
+    if (!pcgStart) return;
+    if (!lastOfInstance(ctl, 0, bm.iNBlk[inst])) return;
+    double tot[12];
+    for (int i = 0; i < 12; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+    if (threadIdx.x == 0) {
+        double mag2 = 0.0;
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            const double nf = tot[3 + j] + tot[6 + j] + 1e-20;
+            ctl->normFactor[j] = nf;
+            ctl->initRes[j] = tot[j] / nf;
+            ctl->finalRes[j] = ctl->initRes[j];
+            mag2 += ctl->initRes[j] * ctl->initRes[j];
+            ctl->rho[j] = tot[9 + j];
+            ctl->nIter[j] = 0;
+            ctl->alpha[j] = 0.0;
+            ctl->beta[j] = 0.0;
+            // PCG::solve: iterate unless already converged (finalRes < tolerance)
+            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
+            ctl->active[j] = act;
+            any |= act;
+        }
+        ctl->initResMag = sqrt(mag2);
+        ctl->innerDone = !any;
+        ctl->firstInner = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_spmv: q = M p (3 RHS), partial p.q; last workgroup: alpha = rho / p.q
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(const DevMesh m, const BlockMap bm, const T* __restrict__ p,
+                                                 T* __restrict__ q, Ctl* __restrict__ ctlAll, double* __restrict__ part,
+                                                 int reduce) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    Ctl* ctl = ctlAll + inst;
+    if (reduce && (ctl->outerDone || ctl->innerDone)) return;
+    const int C = m.nCells;
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ dg = MatSel<T>::diag(m);
+    double acc[3] = {0, 0, 0};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        const int e0 = m.sliceStart[s];
+        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        if (P >= cEnd) continue;
+        const T p0 = p[P], p1 = p[(size_t)C + P], p2 = p[2 * (size_t)C + P];
+        const T d = dg[P];
+        T y0 = d * p0, y1 = d * p1, y2 = d * p2;
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int n = m.col[e];
+            if (n >= 0) {
+                const T a = A[e];
+                y0 -= a * p[n];
+                y1 -= a * p[(size_t)C + n];
+                y2 -= a * p[2 * (size_t)C + n];
+            }
+        }
+        q[P] = y0;
+        q[(size_t)C + P] = y1;
+        q[2 * (size_t)C + P] = y2;
+        acc[0] += (double)p0 * (double)y0;
+        acc[1] += (double)p1 * (double)y1;
+        acc[2] += (double)p2 * (double)y2;
+    }
+    if (!reduce) return;
+    __shared__ double sh[kBlock / 64];
+    for (int i = 0; i < 3; ++i) {
+        const double v = blockReduce<false>(acc[i], sh);
+        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    }
+    if (!lastOfInstance(ctl, 1, bm.iNBlk[inst])) return;
+    double tot[3];
+    for (int i = 0; i < 3; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            if (!ctl->active[j]) { ctl->alpha[j] = 0.0; continue; }
+            // PCG singularity test: |p.Ap| / normFactor < VSMALL
+            if (fabs(tot[j]) / ctl->normFactor[j] < kVSmall) {
+                ctl->active[j] = 0;
+                ctl->alpha[j] = 0.0;
+                continue;
+            }
+            ctl->alpha[j] = ctl->rho[j] / tot[j];
+            any = 1;
+        }
+        if (!any) ctl->innerDone = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_update: x += alpha p, r -= alpha q, z = r/diag, rho' = z.r, sum|r|;
+// last workgroup: convergence per component (tolerance, relTol, maxIter), beta
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockMap bm, T* __restrict__ x,
+                                                   const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q,
+                                                   Ctl* __restrict__ ctlAll, double* __restrict__ part) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone || ctl->innerDone) return;
+    const int C = m.nCells;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const T al[3] = {(T)ctl->alpha[0], (T)ctl->alpha[1], (T)ctl->alpha[2]};
+    const int first = ctl->firstInner;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        const T iD = invD[P];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const size_t i = (size_t)j * C + P;
+            const T pv = p[i];
+            const T xv = first ? (T)0 : x[i];
+            x[i] = xv + al[j] * pv;
+            const T rv = r[i] - al[j] * q[i];
+            r[i] = rv;
+            const T z = iD * rv;
+            acc[j] += (double)z * (double)rv;
+            acc[3 + j] += fabs((double)rv);
+        }
+    }
+    __shared__ double sh[kBlock / 64];
+    for (int i = 0; i < 6; ++i) {
+        const double v = blockReduce<false>(acc[i], sh);
+        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    }
+    if (!lastOfInstance(ctl, 2, bm.iNBlk[inst])) return;
+    double tot[6];
+    for (int i = 0; i < 6; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            if (!ctl->active[j]) continue;
+            ctl->nIter[j] += 1;
+            ctl->innerTotal += 1;
+            const double fr = tot[3 + j] / ctl->normFactor[j];
+            ctl->finalRes[j] = fr;
+            const bool conv = fr < ctl->pcgTol || (ctl->pcgRelTol > 1e-20 && fr < ctl->pcgRelTol * ctl->initRes[j]);
+            const int act = !conv && ctl->nIter[j] < ctl->pcgMaxIter;
+            ctl->beta[j] = tot[j] / ctl->rho[j];
+            ctl->rho[j] = tot[j];
+            ctl->active[j] = act;
+            any |= act;
+        }
+        ctl->innerDone = !any;
+        ctl->firstInner = 0;
+    }
+}
+
+// p = z + beta p for the active components (z = r/diag recomputed)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pdir(const DevMesh m, const BlockMap bm, const T* __restrict__ r,
+                                                 T* __restrict__ p, const Ctl* __restrict__ ctlAll) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone || ctl->innerDone) return;
+    const int C = m.nCells;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const int act[3] = {ctl->active[0], ctl->active[1], ctl->active[2]};
+    const T be[3] = {(T)ctl->beta[0], (T)ctl->beta[1], (T)ctl->beta[2]};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        const T iD = invD[P];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * C + P;
+            p[i] = iD * r[i] + be[j] * p[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_grad: D' = D + x; boundary values D_b (solidTraction evaluate /
+// fixedDisplacement); least-squares grad(D') with the boundary correction;
+// boundary stress and the next traction snGrad; outer convergence test.
+// mode 0: outer iteration; mode 1: initialisation (x = 0, G_old = 0, g = 0,
+// no convergence test); mode 2: gradient from given D_b/snb (no boundary update).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap bm, const double* __restrict__ D,
+                                                 const T* __restrict__ x, double* __restrict__ Dn,
+                                                 double* __restrict__ G, double* __restrict__ Db,
+                                                 double* __restrict__ g, double* __restrict__ snb,
+                                                 double* __restrict__ Gb, double* __restrict__ sigb,
+                                                 const double* __restrict__ Dold, Ctl* __restrict__ ctlAll,
+                                                 double* __restrict__ part, double* __restrict__ hist, int mode) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    Ctl* ctl = ctlAll + inst;
+    if (mode == 0 && ctl->outerDone) return;
+    if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
+    const int C = m.nCells, B = m.nBoundary;
+    double acc[6] = {0, 0, 0, 0, 0, 0};  // max|D'-D|, max|D'-Dold|, max|D'|, sum D'
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        const int e0 = m.sliceStart[s];
+        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        if (P >= cEnd) continue;
+        double DP[3], DnP[3], CP[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            DP[k] = D[(size_t)k * C + P];
+            DnP[k] = x ? DP[k] + (double)x[(size_t)k * C + P] : DP[k];
+            CP[k] = m.cC[(size_t)k * C + P];
+        }
+        double A[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) A[q] = 0.0;
+        double Gold[9];
+        bool haveG = false;
+        int kb = width;  // first boundary slot
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int n = m.col[e];
+            if (n >= 0) {
+                double d[3], dv[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    d[q] = m.cC[(size_t)q * C + n] - CP[q];
+                    const double DN = x ? D[(size_t)q * C + n] + (double)x[(size_t)q * C + n] : D[(size_t)q * C + n];
+                    dv[q] = DN - DnP[q];
+                }
+                const double rr = 1.0 / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double di = d[i] * rr;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) A[3 * i + j] += di * dv[j];
+                }
+            } else if (n != kPad) {
+                if (kb == width) kb = k;
+                const int b = -1 - n;
+                double Dbv[3];
+                if (mode == 2) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) Dbv[q] = Db[(size_t)q * B + b];
+                } else {
+                    if (!haveG) {
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) Gold[q] = (mode == 1) ? 0.0 : G[(size_t)q * C + P];
+                        haveG = true;
+                    }
+                    const double k0 = m.bKv[b], k1 = m.bKv[(size_t)B + b], k2 = m.bKv[2 * (size_t)B + b];
+                    const double dl = m.bDelta[b];
+                    const bool trac = m.bType[b] == 1;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const double kG = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
+                        double sn;
+                        if (trac) {
+                            const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
+                            Dbv[j] = DnP[j] + kG + gv / dl;
+                            sn = gv;
+                        } else {
+                            Dbv[j] = m.bVal[(size_t)j * B + b];
+                            sn = (Dbv[j] - (DnP[j] + kG)) * dl;
+                        }
+                        Db[(size_t)j * B + b] = Dbv[j];
+                        snb[(size_t)j * B + b] = sn;
+                    }
+                }
+                const double l0 = m.bLs[b], l1 = m.bLs[(size_t)B + b], l2 = m.bLs[2 * (size_t)B + b];
+                const double lv[3] = {l0, l1, l2};
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) A[3 * i + j] += lv[i] * (Dbv[j] - DnP[j]);
+            }
+        }
+        const double M0 = m.minv[P], M1 = m.minv[(size_t)C + P], M2 = m.minv[2 * (size_t)C + P];
+        const double M3 = m.minv[3 * (size_t)C + P], M4 = m.minv[4 * (size_t)C + P], M5 = m.minv[5 * (size_t)C + P];
+        double Gn[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Gn[j] = M0 * A[j] + M1 * A[3 + j] + M2 * A[6 + j];
+            Gn[3 + j] = M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j];
+            Gn[6 + j] = M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) G[(size_t)q * C + P] = Gn[q];
+        if (Dn && mode == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Dn[(size_t)q * C + P] = DnP[q];
+        }
+        // boundary gradient, stress and the next traction snGrad
+        if (kb < width) {
+            const double muP = cellMu(m, P), KP = cellK(m, P), ik = cellImpK(m, P);
+            for (int k = kb; k < width; ++k) {
+                const int e = e0 + k * kSlice + lane;
+                const int n = m.col[e];
+                if (n >= 0 || n == kPad) continue;
+                const int b = -1 - n;
+                const double nv[3] = {m.bN[b], m.bN[(size_t)B + b], m.bN[2 * (size_t)B + b]};
+                double Gbv[9], sb[6];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double nG = nv[0] * Gn[j] + nv[1] * Gn[3 + j] + nv[2] * Gn[6 + j];
+                    const double sn = snb[(size_t)j * B + b];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Gbv[3 * i + j] = Gn[3 * i + j] + nv[i] * (sn - nG);
+                }
+                hooke(Gbv, muP, KP, sb);
+                if (Gb) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) Gb[(size_t)q * B + b] = Gbv[q];
+                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) sigb[(size_t)q * B + b] = sb[q];
+                if (mode != 2 && m.bType[b] == 1) {
+                    const double Tt[9] = {sb[0] - ik * Gbv[0], sb[1] - ik * Gbv[1], sb[2] - ik * Gbv[2],
+                                          sb[1] - ik * Gbv[3], sb[3] - ik * Gbv[4], sb[4] - ik * Gbv[5],
+                                          sb[2] - ik * Gbv[6], sb[4] - ik * Gbv[7], sb[5] - ik * Gbv[8]};
+                    const double rik = 1.0 / ik;
+                    const double pr = m.bPres[b];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const double nT = nv[0] * Tt[j] + nv[1] * Tt[3 + j] + nv[2] * Tt[6 + j];
+                        g[(size_t)j * B + b] = ((m.bVal[(size_t)j * B + b] - nv[j] * pr) - nT) * rik;
+                    }
+                }
+            }
+        }
+        if (mode != 2) {
+            double dx2 = 0.0, dold2 = 0.0, dn2 = 0.0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double a = DnP[q] - DP[q];
+                const double o = Dold ? DnP[q] - Dold[(size_t)q * C + P] : DnP[q];
+                dx2 += a * a;
+                dold2 += o * o;
+                dn2 += DnP[q] * DnP[q];
+                acc[3 + q] += DnP[q];
+            }
+            acc[0] = nanMax(acc[0], sqrt(dx2));
+            acc[1] = nanMax(acc[1], sqrt(dold2));
+            acc[2] = nanMax(acc[2], sqrt(dn2));
+        }
+    }
+    if (mode == 2) return;
+    __shared__ double sh[kBlock / 64];
+    for (int i = 0; i < 6; ++i) {
+        const double v = (i < 3) ? blockReduce<true>(acc[i], sh) : blockReduce<false>(acc[i], sh);
+        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    }
+    if (!lastOfInstance(ctl, 3, bm.iNBlk[inst])) return;
+    double tot[6];
+    for (int i = 0; i < 6; ++i)
+        tot[i] = (i < 3) ? reducePartials<true>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh)
+                         : reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+    if (threadIdx.x == 0) {
+        const double nc = (double)bm.iNCell[inst];
+        for (int j = 0; j < 3; ++j) ctl->xbar[j] = tot[3 + j] / nc;
+        if (mode == 0) {
+            // solidModel::converged, total-displacement form
+            double denom = tot[1];
+            if (denom < kSmall) denom = tot[2] < kSmall ? kSmall : tot[2];
+            const double rvf = tot[0] / denom;
+            const double ir = ctl->initResMag;
+            const int i = ctl->iCorr;
+            ctl->rvf = rvf;
+            const bool conv = i > 1 && ((ir < ctl->solTol && rvf < ctl->solTol) || rvf < ctl->altTol || ir < ctl->altTol);
+            if (hist && i < ctl->histCap) {
+                double* h = hist + (size_t)inst * 3 * ctl->histCap;
+                h[i] = ir;
+                h[ctl->histCap + i] = rvf;
+                h[2 * ctl->histCap + i] = (double)(ctl->nIter[0] + ctl->nIter[1] + ctl->nIter[2]);
+            }
+            if (!isfinite(ir) || !isfinite(rvf)) {
+                ctl->diverged = 1;
+                ctl->outerDone = 1;
+                ctl->nOuter = i + 1;
+            } else if (conv) {
+                ctl->converged = 1;
+                ctl->outerDone = 1;
+                ctl->nOuter = i + 1;
+            } else if (i + 1 >= ctl->nCorr) {
+                ctl->outerDone = 1;
+                ctl->nOuter = i + 1;
+            } else {
+                ctl->iCorr = i + 1;
+            }
+        }
+    }
+}
+
+// sigma = Hooke(G) for output (xx xy xz yy yz zz, SoA)
+__global__ __launch_bounds__(kBlock) void k_sigma(const DevMesh m, const double* __restrict__ G,
+                                                  double* __restrict__ sig) {
+    const int C = m.nCells;
+    for (int P = blockIdx.x * kBlock + threadIdx.x; P < C; P += gridDim.x * kBlock) {
+        double GP[9], s[6];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) GP[q] = G[(size_t)q * C + P];
+        hooke(GP, cellMu(m, P), cellK(m, P), s);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sig[(size_t)q * C + P] = s[q];
+    }
+}
+
+// PCG start from a given right-hand side b (for the standalone PCG entry):
+// r = b, p = z = b/diag; normalised initial residual with the supplied normFactor
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const BlockMap bm, const double* __restrict__ b,
+                                                      T* __restrict__ r, T* __restrict__ p, Ctl* __restrict__ ctlAll,
+                                                      double* __restrict__ part) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    Ctl* ctl = ctlAll + inst;
+    const int C = m.nCells;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const size_t i = (size_t)j * C + P;
+            const T rT = (T)b[i];
+            r[i] = rT;
+            const T z = invD[P] * rT;
+            p[i] = z;
+            acc[j] += fabs(b[i]);
+            acc[3 + j] += (double)z * (double)rT;
+        }
+    }
+    __shared__ double sh[kBlock / 64];
+    for (int i = 0; i < 6; ++i) {
+        const double v = blockReduce<false>(acc[i], sh);
+        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    }
+    if (!lastOfInstance(ctl, 0, bm.iNBlk[inst])) return;
+    double tot[6];
+    for (int i = 0; i < 6; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            ctl->initRes[j] = tot[j] / ctl->normFactor[j];
+            ctl->finalRes[j] = ctl->initRes[j];
+            ctl->rho[j] = tot[3 + j];
+            ctl->nIter[j] = 0;
+            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
+            ctl->active[j] = act;
+            any |= act;
+        }
+        ctl->innerDone = !any;
+        ctl->firstInner = 1;
+    }
+}
+
+}  // namespace s4f

@@ -1,0 +1,273 @@
+"""Host-side mirror of solids4foam's solidModel interface for the
+``linearGeometryTotalDisplacement`` path, over the libs4f C ABI.
+
+Reference interface (paths relative to the solids4foam tree):
+  solidModel::New / run-time table       src/solids4FoamModels/solidModels/solidModel/solidModel.C:1673-1749
+  linGeomTotalDispSolid (ctor, evolve)   src/solids4FoamModels/solidModels/linGeomTotalDispSolid/linGeomTotalDispSolid.C:64-232
+  solidModel::writeFields (sigmaEq)      solidModel.C:1936-1996
+Unknown model types raise ValueError listing the valid types, as
+solidModel::New aborts with FatalError (solidModel.C:1729-1743).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, f64, i32, lib, ptr
+
+
+class Context:
+    """One HIP device + stream (s4f_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().s4f_ctx_create(int(device), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s4f_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class PolyMesh:
+    """polyMesh addressing + OpenFOAM geometry (host)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        sz = np.zeros(6, np.int64)
+        check(lib().s4f_polymesh_sizes(self.h, ptr(sz)))
+        self.nPoints, self.nCells, self.nFaces, self.nInternalFaces, self.nPatches, self._nfv = map(int, sz)
+        self.nBoundaryFaces = self.nFaces - self.nInternalFaces
+        self._addr = None
+        self._geom = None
+
+    @classmethod
+    def box(cls, nx, ny, nz, L=(1.0, 1.0, 1.0), jitter=0.0, seed=0, tets=False):
+        h = ctypes.c_void_p()
+        Lv = f64(L)
+        check(lib().s4f_polymesh_box(int(nx), int(ny), int(nz), ptr(Lv), float(jitter), int(seed), int(bool(tets)),
+                                     ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, points, face_offsets, face_verts, owner, neighbour, n_cells, patch_start, patch_size):
+        pts, fo, fv, ow, nb = f64(points), i32(face_offsets), i32(face_verts), i32(owner), i32(neighbour)
+        ps, pz = i32(patch_start), i32(patch_size)
+        h = ctypes.c_void_p()
+        check(lib().s4f_polymesh_from_arrays(len(pts), ptr(pts), int(n_cells), len(ow), len(nb), ptr(fo), ptr(fv),
+                                             ptr(ow), ptr(nb), len(ps), ptr(ps), ptr(pz), ctypes.byref(h)))
+        return cls(h)
+
+    def addressing(self):
+        if self._addr is None:
+            a = dict(points=np.zeros((self.nPoints, 3)), fOff=np.zeros(self.nFaces + 1, np.int32),
+                     fVerts=np.zeros(self._nfv, np.int32), owner=np.zeros(self.nFaces, np.int32),
+                     neighbour=np.zeros(self.nInternalFaces, np.int32), patchStart=np.zeros(self.nPatches, np.int32),
+                     patchSize=np.zeros(self.nPatches, np.int32))
+            check(lib().s4f_polymesh_addressing(self.h, ptr(a["points"]), ptr(a["fOff"]), ptr(a["fVerts"]),
+                                                ptr(a["owner"]), ptr(a["neighbour"]), ptr(a["patchStart"]),
+                                                ptr(a["patchSize"])))
+            self._addr = a
+        return self._addr
+
+    def geometry(self):
+        if self._geom is None:
+            g = dict(Cf=np.zeros((self.nFaces, 3)), Sf=np.zeros((self.nFaces, 3)), C=np.zeros((self.nCells, 3)),
+                     V=np.zeros(self.nCells))
+            check(lib().s4f_polymesh_geometry(self.h, ptr(g["Cf"]), ptr(g["Sf"]), ptr(g["C"]), ptr(g["V"])))
+            self._geom = g
+        return self._geom
+
+    def patch_faces(self, p):
+        a = self.addressing()
+        s = int(a["patchStart"][p]) - self.nInternalFaces
+        return slice(s, s + int(a["patchSize"][p]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s4f_polymesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class SolverMesh:
+    """Device-resident mesh (s4f_mesh): addressing, geometry, LS matrices, ELL lists."""
+
+    def __init__(self, ctx, polymesh, patch_types, ls_full_delta=False):
+        self.ctx = ctx
+        self.poly = polymesh
+        self.patch_types = i32(patch_types)
+        if len(self.patch_types) != polymesh.nPatches:
+            raise ValueError("one patch type per patch")
+        h = ctypes.c_void_p()
+        check(lib().s4f_mesh_create_from_polymesh(ctx.h, polymesh.h, ptr(self.patch_types), int(bool(ls_full_delta)),
+                                                  ctypes.byref(h)))
+        self.h = h
+        self.nCells = polymesh.nCells
+        self.nBoundaryFaces = polymesh.nBoundaryFaces
+
+    def info(self):
+        out = np.zeros(6, np.int64)
+        check(lib().s4f_mesh_info(self.h, ptr(out)))
+        return dict(nCells=int(out[0]), nInternalFaces=int(out[1]), nBoundaryFaces=int(out[2]),
+                    ellEntries=int(out[3]), ellSlices=int(out[4]), deviceBytes=int(out[5]))
+
+    def set_material(self, E, nu):
+        E = np.atleast_1d(f64(E))
+        nu = np.atleast_1d(f64(nu))
+        per_cell = E.size > 1 or nu.size > 1
+        if per_cell:
+            E = f64(np.broadcast_to(E, (self.nCells,)))
+            nu = f64(np.broadcast_to(nu, (self.nCells,)))
+        check(lib().s4f_material_set(self.h, ptr(E), ptr(nu), int(per_cell)))
+
+    def set_bc(self, patch, value, pressure=None):
+        sl = self.poly.patch_faces(patch)
+        n = sl.stop - sl.start
+        v = f64(np.broadcast_to(np.asarray(value, np.float64), (n, 3)))
+        p = None if pressure is None else f64(np.broadcast_to(np.asarray(pressure, np.float64), (n,)))
+        check(lib().s4f_bc_set(self.h, int(patch), ptr(v), ptr(p)))
+
+    def solve(self, D0=None, want_sigma=True, want_grad=False, want_Db=False, **opts):
+        o = _abi.default_opts(**opts)
+        D = np.zeros((self.nCells, 3)) if D0 is None else f64(D0, (self.nCells, 3)).copy()
+        if D0 is not None:
+            o.warmStart = 1
+        sig = np.zeros((self.nCells, 6)) if want_sigma else None
+        G = np.zeros((self.nCells, 9)) if want_grad else None
+        Db = np.zeros((self.nBoundaryFaces, 3)) if want_Db else None
+        rep = _abi.SolveReport()
+        st = lib().s4f_solve(self.h, ctypes.byref(o), ptr(D), ptr(sig), ptr(G), ptr(Db), ctypes.byref(rep))
+        check(st, allow=(_abi.S4F_OK, _abi.S4F_WARN_MAXITER))
+        out = dict(D=D, sigma=sig, gradD=G, Db=Db, nOuter=rep.nOuter, converged=bool(rep.converged),
+                   innerTotal=int(rep.innerTotal), initialResidual=rep.initialResidual, residualvf=rep.residualvf,
+                   seconds=rep.seconds, status=st)
+        return out
+
+    def history(self, cap=100000):
+        ir, rv = np.zeros(cap), np.zeros(cap)
+        inn = np.zeros(cap, np.int32)
+        cnt = ctypes.c_int32()
+        check(lib().s4f_solve_history(self.h, cap, ptr(ir), ptr(rv), ptr(inn), ctypes.byref(cnt)))
+        n = cnt.value
+        return dict(initRes=ir[:n], rvf=rv[:n], inner=inn[:n])
+
+    # fine-grained operators ------------------------------------------------
+    def grad(self, D, Db, snb):
+        G = np.zeros((self.nCells, 9))
+        Gb = np.zeros((self.nBoundaryFaces, 9))
+        check(lib().s4f_grad(self.h, ptr(f64(D)), ptr(f64(Db)), ptr(f64(snb)), ptr(G), ptr(Gb)))
+        return G, Gb
+
+    def sigma(self, G):
+        s = np.zeros((self.nCells, 6))
+        check(lib().s4f_sigma(self.h, ptr(f64(G)), ptr(s)))
+        return s
+
+    def residual(self, D, G, sigmab):
+        R = np.zeros((self.nCells, 3))
+        check(lib().s4f_residual(self.h, ptr(f64(D)), ptr(f64(G)), ptr(f64(sigmab)), ptr(R)))
+        return R
+
+    def spmv3(self, x, mixed=False):
+        y = np.zeros((self.nCells, 3))
+        check(lib().s4f_spmv3(self.h, ptr(f64(x)), ptr(y), int(bool(mixed))))
+        return y
+
+    def pcg3(self, b, norm_factor, **opts):
+        o = _abi.default_opts(**opts)
+        x = np.zeros((self.nCells, 3))
+        it = np.zeros(3, np.int32)
+        nf = f64(norm_factor)
+        check(lib().s4f_pcg3(self.h, ctypes.byref(o), ptr(f64(b)), ptr(nf), ptr(x), ptr(it)))
+        return x, it
+
+    def matrix(self):
+        d = np.zeros(self.nCells)
+        a = np.zeros(self.poly.nInternalFaces)
+        check(lib().s4f_matrix(self.h, ptr(d), ptr(a)))
+        return d, a
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s4f_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def sigma_eq(sigma):
+    """von Mises sqrt(3/2 |dev sigma|^2) (solidModel.C:1957-1961); sigma (n,6) xx xy xz yy yz zz"""
+    s = np.asarray(sigma)
+    tr3 = (s[:, 0] + s[:, 3] + s[:, 5]) / 3.0
+    d0, d3, d5 = s[:, 0] - tr3, s[:, 3] - tr3, s[:, 5] - tr3
+    ms = d0 * d0 + d3 * d3 + d5 * d5 + 2.0 * (s[:, 1] ** 2 + s[:, 2] ** 2 + s[:, 4] ** 2)
+    return np.sqrt(1.5 * ms)
+
+
+class LinGeomTotalDispSolid:
+    """solidModel ``linearGeometryTotalDisplacement`` on the MI355X path.
+
+    Built from the same inputs a case gives the reference: the mesh, the
+    linearElastic law (E, nu), per-patch boundary conditions (fixedDisplacement
+    value / solidTraction traction + pressure) and the solidProperties
+    coefficients; ``evolve()`` runs the momentum loop
+    (linGeomTotalDispSolid.C:135-209) and leaves D, gradD and sigma populated.
+    """
+
+    typeName = "linearGeometryTotalDisplacement"
+
+    def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False):
+        # patches: list of dicts {type: "fixedDisplacement"|"solidTraction", value|traction, pressure}
+        types = []
+        for p in patches:
+            t = p["type"]
+            if t == "fixedDisplacement":
+                types.append(_abi.PATCH_FIXED_DISPLACEMENT)
+            elif t == "solidTraction":
+                types.append(_abi.PATCH_TRACTION)
+            else:
+                raise ValueError(f"Unsupported boundary condition type {t!r} "
+                                 "(valid: fixedDisplacement, solidTraction)")
+        self.mesh = SolverMesh(ctx, polymesh, types, ls_full_delta)
+        self.mesh.set_material(E, nu)
+        for i, p in enumerate(patches):
+            if p["type"] == "fixedDisplacement":
+                self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))
+            else:
+                self.mesh.set_bc(i, p.get("traction", (0.0, 0.0, 0.0)), p.get("pressure"))
+        self.opts = dict(coeffs or {})
+        self.opts.update(solver or {})
+        self.D = np.zeros((polymesh.nCells, 3))
+        self.sigma = None
+        self.gradD = None
+        self.report = None
+
+    def evolve(self, warm_start=False):
+        out = self.mesh.solve(D0=self.D if warm_start else None, want_sigma=True, want_grad=True, **self.opts)
+        self.D, self.sigma, self.gradD = out["D"], out["sigma"], out["gradD"]
+        self.report = {k: v for k, v in out.items() if k not in ("D", "sigma", "gradD", "Db")}
+        return True
+
+    def sigma_eq(self):
+        return sigma_eq(self.sigma)
+
+
+SOLID_MODELS = {LinGeomTotalDispSolid.typeName: LinGeomTotalDispSolid}
+
+
+def solid_model_new(solid_model_type, *args, **kw):
+    """Run-time selection by the solidProperties ``solidModel`` entry."""
+    try:
+        cls = SOLID_MODELS[solid_model_type]
+    except KeyError:
+        raise ValueError(f"Unknown solidModel type {solid_model_type!r}; valid solidModel types are: "
+                         f"{sorted(SOLID_MODELS)}") from None
+    return cls(*args, **kw)

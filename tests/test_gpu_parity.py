@@ -1,0 +1,183 @@
+"""Parity of the HIP path (called through the C ABI) with the FP64 CPU oracle.
+
+Operator level: each kernel against the oracle on the same state (FP64 to
+1e-12 relative, FP32 SpMV to 1e-6).  Solve level: converged displacement and
+stress against the committed oracle fixtures (tests/golden), plain FP64 to
+1e-8 relative L2, mixed precision to 1e-6.  Plus determinism, batching (alone
+== in a batch), and the divergence / max-iteration error paths.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests._cases import E_STEEL, NU_STEEL, FIXED, TRACTION, cantilever_bc, cube_tension_bc
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TIGHT = dict(nCorrectors=200000, solutionTolerance=1e-12, alternativeTolerance=1e-12, pcgTolerance=1e-13)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from solids4foam_amd import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _product_mesh(ctx, n, L=(1, 1, 1), jitter=0.0, seed=0, tets=False, bc="cube", ls_full_delta=False):
+    from solids4foam_amd import PolyMesh, SolverMesh
+    pm = PolyMesh.box(*n, L=L, jitter=jitter, seed=seed, tets=tets)
+    sl = [pm.patch_faces(p) for p in range(6)]
+    pt, bv = (cube_tension_bc if bc == "cube" else cantilever_bc)(pm.nBoundaryFaces, sl)
+    m = SolverMesh(ctx, pm, pt, ls_full_delta=ls_full_delta)
+    m.set_material(E_STEEL, NU_STEEL)
+    for p in range(6):
+        m.set_bc(p, bv[sl[p]])
+    return pm, m, pt, bv
+
+
+def _oracle_case(oracle_mod, n, L=(1, 1, 1), jitter=0.0, seed=0, tets=False, bc="cube", ls_full_delta=False):
+    om = oracle_mod.Mesh.box(*n, L=L, jitter=jitter, seed=seed, tets=tets)
+    sl = [om.patch_faces(p) for p in range(6)]
+    pt, bv = (cube_tension_bc if bc == "cube" else cantilever_bc)(om.nBoundaryFaces, sl)
+    return om, oracle_mod.Case(om, E_STEEL, NU_STEEL, pt, bv, ls_full_delta=ls_full_delta)
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_operators_match_oracle(ctx, oracle_mod, tets):
+    n = (5, 4, 4) if tets else (9, 7, 6)
+    print(f"[operators] tets={tets} n={n}", flush=True)
+    om, oc = _oracle_case(oracle_mod, n, jitter=0.15, seed=3, tets=tets, ls_full_delta=tets)
+    pm, m, _, _ = _product_mesh(ctx, n, jitter=0.15, seed=3, tets=tets, ls_full_delta=tets)
+    for _ in range(4):
+        oc.iterate(precond=oracle_mod.PRECOND_DIAGONAL)
+    C, B = om.nCells, om.nBoundaryFaces
+    D, Db, snb = oc.get("D").reshape(C, 3), oc.get("Db").reshape(B, 3), oc.get("snb").reshape(B, 3)
+    # gradient with the boundary correction
+    G, Gb = m.grad(D, Db, snb)
+    oc.op("grad")
+    Go, Gbo = oc.get("G").reshape(C, 9), oc.get("Gb").reshape(B, 9)
+    assert rel_l2(G, Go) < 1e-12
+    assert rel_l2(Gb, Gbo) < 1e-12
+    # Hooke
+    oc.op("sigma")
+    assert rel_l2(m.sigma(Go), oc.get("sigma").reshape(C, 6)) < 1e-13
+    # residual: div(sigma) + Rhie-Chow
+    R = m.residual(D, Go, oc.get("sigmab").reshape(B, 6))
+    oc.op("residual")
+    Ro = oc.get("R").reshape(C, 3)
+    assert np.abs(R - Ro).max() < 1e-10 * np.abs(Ro).max()
+    # matrix and 3-RHS SpMV
+    diag, a = m.matrix()
+    assert rel_l2(diag, oc.get("diag")) < 1e-14
+    assert rel_l2(-a, oc.get("upper")) < 1e-14
+    x = np.random.default_rng(0).normal(size=(C, 3))
+    y = m.spmv3(x)
+    yo = np.stack([oc.amul(np.ascontiguousarray(x[:, q])) for q in range(3)], axis=1)
+    assert rel_l2(y, yo) < 1e-13
+    y32 = m.spmv3(x, mixed=True)
+    assert rel_l2(y32, yo) < 1e-5
+    # PCG to a tight tolerance == dense solve
+    M = np.diag(oc.get("diag"))
+    o, nb = om.owner[:om.nInternalFaces], om.neighbour
+    M[o, nb] = oc.get("upper")
+    M[nb, o] = oc.get("upper")
+    xs, its = m.pcg3(Ro, np.abs(Ro).sum(axis=0), pcgTolerance=1e-14, pcgRelTol=0.0)
+    exact = np.linalg.solve(M, Ro)
+    assert rel_l2(xs, exact) < 1e-9
+    assert np.all(its > 0)
+
+
+@pytest.mark.parametrize("name,n,kw", [
+    ("cube16_jitter", (16, 16, 16), dict(jitter=0.15, seed=2)),
+    ("tet4_jitter", (4, 4, 4), dict(jitter=0.15, seed=5, tets=True, ls_full_delta=True)),
+    ("cfg1_cantilever", (50, 10, 10), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
+])
+def test_fp64_solve_matches_oracle_fixture(ctx, name, n, kw):
+    print(f"[fp64 solve] {name}", flush=True)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    pm, m, _, _ = _product_mesh(ctx, n, **kw)
+    out = m.solve(**TIGHT)
+    print(f"  nOuter={out['nOuter']} (oracle {int(g['nOuter'])}) converged={out['converged']} "
+          f"{out['seconds']:.2f}s err_D={rel_l2(out['D'], g['D']):.2e} "
+          f"err_sigma={rel_l2(out['sigma'], g['sigma']):.2e}", flush=True)
+    assert out["converged"]
+    assert rel_l2(out["D"], g["D"]) < 1e-8
+    assert rel_l2(out["sigma"], g["sigma"]) < 1e-8
+
+
+@pytest.mark.parametrize("name,n,kw", [
+    ("cube16_jitter", (16, 16, 16), dict(jitter=0.15, seed=2)),
+    ("cfg1_cantilever", (50, 10, 10), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
+])
+def test_mixed_precision_solve_error(ctx, name, n, kw):
+    from solids4foam_amd import sigma_eq
+    print(f"[mixed solve] {name}", flush=True)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    pm, m, _, _ = _product_mesh(ctx, n, **kw)
+    out = m.solve(precision=1, **TIGHT)
+    eD, eS = rel_l2(out["D"], g["D"]), rel_l2(sigma_eq(out["sigma"]), sigma_eq(g["sigma"]))
+    print(f"  nOuter={out['nOuter']} err_D={eD:.2e} err_vonMises={eS:.2e}", flush=True)
+    assert out["converged"]
+    assert eD < 1e-6 and eS < 1e-6
+
+
+def test_first_iterations_follow_oracle(ctx, oracle_mod):
+    # same preconditioner (diagonal) on both sides: the first outer iterations
+    # reproduce the oracle's residual history
+    n = (8, 8, 8)
+    om, oc = _oracle_case(oracle_mod, n, jitter=0.15, seed=4)
+    pm, m, _, _ = _product_mesh(ctx, n, jitter=0.15, seed=4)
+    ro = oc.solve(history=10, nCorr=4, precond=oracle_mod.PRECOND_DIAGONAL)
+    out = m.solve(nCorrectors=4)
+    h = m.history()
+    assert out["nOuter"] == 4
+    assert np.allclose(h["initRes"][:4], ro["history"]["initRes"][:4], rtol=1e-8, atol=1e-14)
+    assert np.array_equal(h["inner"][:4], ro["history"]["inner"][:4])
+    assert np.allclose(h["rvf"][:4], ro["history"]["rvf"][:4], rtol=1e-8)
+    assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-9
+
+
+def test_solve_is_bitwise_deterministic(ctx):
+    pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
+    a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)
+    b = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)
+    assert np.array_equal(a["D"], b["D"]) and np.array_equal(a["sigma"], b["sigma"])
+    assert a["nOuter"] == b["nOuter"]
+
+
+def test_batch_equals_alone(ctx):
+    from solids4foam_amd.batch import Batch
+    print("[batch]", flush=True)
+    specs = [((9, 7, 6), dict(jitter=0.1, seed=1)), ((16, 4, 4), dict(L=(1.0, 0.25, 0.25), bc="cantilever")),
+             ((5, 5, 5), dict(jitter=0.1, seed=3)), ((20, 11, 3), dict(jitter=0.1, seed=4))]
+    opts = dict(solutionTolerance=1e-10, alternativeTolerance=1e-10, nCorrectors=20000)
+    meshes = [_product_mesh(ctx, n, **kw) for n, kw in specs]
+    alone = [m.solve(**opts) for _, m, _, _ in meshes]
+    b = Batch(ctx)
+    for _, m, _, _ in meshes:
+        b.add(m)
+    res = b.solve(**opts)
+    for a, r in zip(alone, res):
+        assert r["converged"]
+        assert rel_l2(r["D"], a["D"]) < 1e-9
+        assert abs(r["nOuter"] - a["nOuter"]) <= 2
+
+
+def test_divergence_and_maxiter_status(ctx):
+    from solids4foam_amd import _abi
+    # tets with the normal-delta leastSquares gradient: the segregated loop diverges
+    pm, m, _, _ = _product_mesh(ctx, (3, 3, 3), jitter=0.0, tets=True, ls_full_delta=False)
+    with pytest.raises(_abi.S4FError) as e:
+        m.solve(nCorrectors=5000)
+    assert e.value.status == _abi.S4F_ERR_DIVERGED
+    pm, m, _, _ = _product_mesh(ctx, (6, 6, 6), jitter=0.1, seed=1)
+    out = m.solve(nCorrectors=3)
+    assert out["status"] == _abi.S4F_WARN_MAXITER and out["nOuter"] == 3 and not out["converged"]
