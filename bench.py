@@ -29,12 +29,12 @@ def algorithmic_bytes(kernel, C, F, B, T):
     """Compulsory HBM bytes per launch: each array a kernel touches counted
     once; neighbour gathers assumed cache-resident.  C cells, F internal faces,
     B boundary faces, T = bytes of the PCG scalar (8 FP64, 4 FP32)."""
-    if kernel == "spmv":            # diag, p, q per cell; a + col per ELL entry (2 per face)
-        return C * 7 * T + F * 2 * (T + 4)
+    if kernel == "spmv":            # r, 1/diag, p old/new, q, diag per cell; a + col per ELL entry (2 per face)
+        return C * 14 * T + F * 2 * (T + 4)
     if kernel == "pcg_update":      # x r/w, p, r r/w, q, 1/diag
         return C * 19 * T
-    if kernel == "pcg_direction":   # r, 1/diag, p r/w
-        return C * 10 * T
+    if kernel == "finalize":        # one workgroup: partial rows of 3 values
+        return 0
     if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
         return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
     if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w
@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--jitter", type=float, default=0.15)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--reltol", type=float, default=0.1)
     ap.add_argument("--profile-every", type=int, default=16)
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -99,7 +100,7 @@ def main():
     F = info["nInternalFaces"]
     print(f"[bench] mesh {args.n}^3: C={C} F={F} B={B} ELL entries={info['ellEntries']} "
           f"device={info['deviceBytes'] / 2**30:.2f} GiB setup={setup_s:.1f}s", file=sys.stderr, flush=True)
-    opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=0.1,
+    opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
                 nCorrectors=100000)
     L = _abi.lib()
 
@@ -140,11 +141,15 @@ def main():
     for k, p in prof.items():
         if p["samples"] == 0:
             continue
+        if k == "finalize":
+            kern[k] = dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"], samples=p["samples"],
+                           total_ms_est=round(p["avg_ms"] * p["launches"], 1))
+            continue
         b = algorithmic_bytes(k, C, F, B, T)
         kern[k] = dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"], samples=p["samples"],
                        alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1),
                        total_ms_est=round(p["avg_ms"] * p["launches"], 1))
-    dom = max(kern, key=lambda k: kern[k]["total_ms_est"]) if kern else None
+    dom = max((k for k in kern if k != "finalize"), key=lambda k: kern[k]["total_ms_est"]) if kern else None
     roofline = None
     if dom:
         a = kern[dom]["gbps"]
@@ -178,7 +183,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cube {args.n}^3 hex, jitter {args.jitter} seed {args.seed}, z-min fixed, "
                                f"z-max traction 1e7 Pa, linearElastic E=200GPa nu=0.3",
-                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": "diagonal, relTol 0.1"},
+                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"diagonal, relTol {args.reltol}"},
         "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
         "nOuter": nOuter,
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),

@@ -223,8 +223,11 @@ s4f_status s4f_matrix(s4f_mesh *mesh, double *diag, double *faceCoeff);
 /* per-time-step ExecutionTime/ClockTime print, solids4Foam.C:68-72)    */
 /* ------------------------------------------------------------------ */
 /* kernel ids for s4f_profile_read */
-enum { S4F_KERNEL_RESIDUAL = 0, S4F_KERNEL_SPMV = 1, S4F_KERNEL_PCG_UPDATE = 2, S4F_KERNEL_PCG_DIRECTION = 3,
-       S4F_KERNEL_GRAD = 4 };
+enum { S4F_KERNEL_RESIDUAL = 0,   /* div(sigma) + Rhie-Chow face loop */
+       S4F_KERNEL_SPMV = 1,       /* PCG direction update fused with the 3-RHS SpMV */
+       S4F_KERNEL_PCG_UPDATE = 2, /* PCG solution / residual update */
+       S4F_KERNEL_FINALIZE = 3,   /* one-workgroup reduction finalize (alpha) */
+       S4F_KERNEL_GRAD = 4 };     /* boundary update + least-squares gradient */
 /* Enables (sampleEvery > 0) or disables (0) sampled hipEvent timing of the
    solver kernels on the context's stream and resets the counters: every
    sampleEvery-th launch of each kernel is bracketed by an event pair; launches

@@ -73,7 +73,7 @@ EXPORTED = [
     "s4f_profile", "s4f_profile_read",
 ]
 
-KERNELS = {"residual": 0, "spmv": 1, "pcg_update": 2, "pcg_direction": 3, "grad": 4}
+KERNELS = {"residual": 0, "spmv": 1, "pcg_update": 2, "finalize": 3, "grad": 4}
 
 
 def lib():

@@ -80,7 +80,7 @@ using namespace s4f;
 // host synchronises; launches that found their solve already finished
 // (early-exit no-ops) are discarded so the average is that of working launches.
 namespace s4f {
-enum KernelId { K_RESIDUAL = 0, K_SPMV = 1, K_UPDATE = 2, K_PDIR = 3, K_GRAD = 4, K_COUNT = 5 };
+enum KernelId { K_RESIDUAL = 0, K_SPMV = 1, K_UPDATE = 2, K_FINALIZE = 3, K_GRAD = 4, K_COUNT = 5 };
 struct Profiler {
     int every = 0;
     struct Sample { int kernel; hipEvent_t a, b; int inner; int outer; };
@@ -133,13 +133,31 @@ struct Profiler {
 };
 }  // namespace s4f
 
+// A context lives until it has been destroyed AND its last mesh/batch is
+// gone: destroying it earlier only marks it, so handles never outlive it.
 struct s4f_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nCU = 0;
+    int children = 0;
+    bool destroyPending = false;
     s4f::Profiler prof;
 };
+
+namespace {
+void ctxRelease(s4f_ctx* ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+void ctxChildGone(s4f_ctx* ctx) {
+    if (--ctx->children == 0 && ctx->destroyPending) ctxRelease(ctx);
+}
+}  // namespace
 
 struct s4f_polymesh {
     std::unique_ptr<PolyMesh> pm;
@@ -152,7 +170,7 @@ struct HostMesh {
     std::vector<double> w, nonOrthDelta, magSf, bMagSf, bDelta;  // F, F, F, B, B
     // ELL
     int nSlices = 0;
-    std::vector<int> sliceStart, col;
+    std::vector<int> sliceStart, sliceWidth, col;
     std::vector<int> entOwn, entNei, entB;  // ELL entry of each face side
     std::vector<double> eS, eW;             // 3E, E
     std::vector<double> cC, minv;           // SoA 3C, 6C
@@ -171,8 +189,8 @@ struct HostMesh {
 
 struct SolveState {
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
-    DArr<double> r64, p64, q64, x64;
-    DArr<float> r32, p32, q32, x32;
+    DArr<double> r64, p64, q64, x64, pb64;
+    DArr<float> r32, p32, q32, x32, pb32;
     DArr<Ctl> ctl;
     DArr<double> part, hist;
     int histCap = 0;
@@ -184,7 +202,7 @@ struct s4f_mesh {
     s4f_ctx* ctx = nullptr;
     HostMesh h;
     // device geometry / addressing
-    DArr<int> sliceStart, col, bCell, bType;
+    DArr<int> sliceStart, sliceWidth, col, bCell, bType;
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
     DArr<float> eA32, diag32, invDiag32;
     // block map (single instance)
@@ -207,6 +225,7 @@ DevMesh s4f_mesh::dm() const {
     m.nBlocks = nBlocks;
     m.nEntries = h.col.size();
     m.sliceStart = sliceStart.p;
+    m.sliceWidth = sliceWidth.p;
     m.col = col.p;
     m.eS = eS.p;
     m.eW = eW.p;
@@ -281,17 +300,10 @@ s4f_status guarded(F&& f) {
 
 inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-// Number of workgroups for the reducing kernels: enough to fill the chip
-// (8 per CU), a multiple of 8 so that consecutive slices stay on one XCD.
+// Upper bound on the workgroup tiles of one instance (the reduction partial
+// rows its finalize kernel reads): 8 per CU on a 256-CU MI355X.
 int gMaxBlocks = 2048;
 
-void buildBlockMap(int nSlices, int maxBlocks, int& spb, int& nb) {
-    const int want = std::max(1, std::min(maxBlocks, (nSlices + 3) / 4));
-    spb = (nSlices + want - 1) / want;
-    if (spb < 1) spb = 1;
-    nb = (nSlices + spb - 1) / spb;
-    if (nb >= 8 && nb % 8) nb += 8 - nb % 8;
-}
 
 // Host-side precomputation shared by every entry point: OpenFOAM
 // surfaceInterpolation weights and deltas, boundary normals/correction
@@ -415,6 +427,8 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
         if (E > (size_t)INT32_MAX) throw InvalidError("mesh too large for int32 ELL offsets");
     }
     h.sliceStart[h.nSlices] = (int)E;
+    h.sliceWidth.resize(h.nSlices);
+    for (int s = 0; s < h.nSlices; ++s) h.sliceWidth[s] = (h.sliceStart[s + 1] - h.sliceStart[s]) / kSlice;
     h.col.assign(E, kPad);
     h.eS.assign(3 * E, 0.0);
     h.eW.assign(E, 0.0);
@@ -552,6 +566,7 @@ void uploadGeometry(s4f_mesh* M) {
     HostMesh& h = M->h;
     hipStream_t s = M->ctx->stream;
     M->sliceStart.upload(h.sliceStart, s);
+    M->sliceWidth.upload(h.sliceWidth, s);
     M->col.upload(h.col, s);
     M->eS.upload(h.eS, s);
     M->eW.upload(h.eW, s);
@@ -565,19 +580,24 @@ void uploadGeometry(s4f_mesh* M) {
     M->bLs.upload(h.bLs, s);
     M->bVal.upload(h.bVal, s);
     M->bPres.upload(h.bPres, s);
-    // workgroups: a common slice count per workgroup, every workgroup inside one instance
-    int spb = 0, nbAll = 0;
-    buildBlockMap(h.nSlices, gMaxBlocks, spb, nbAll);
+    // workgroups: each instance is cut into workgroup tiles by a rule that
+    // depends on that instance alone (>= 4 slices = one per wave, at most
+    // gMaxBlocks tiles), so its reductions -- and therefore its result -- are
+    // bitwise the same whether it is solved alone or packed with others
     const int nInst = (int)h.instCell0.size();
     std::vector<int> inst, s0, s1, blk0(nInst), nblk(nInst);
+    int spb = 0;
     for (int i = 0; i < nInst; ++i) {
         const int a = h.instCell0[i] / kSlice;
-        const int e = a + (h.instNCell[i] + kSlice - 1) / kSlice;
+        const int nS = (h.instNCell[i] + kSlice - 1) / kSlice;
+        const int e = a + nS;
+        const int spbI = std::max(4, (nS + gMaxBlocks - 1) / gMaxBlocks);
+        spb = std::max(spb, spbI);
         blk0[i] = (int)inst.size();
-        for (int x = a; x < e; x += spb) {
+        for (int x = a; x < e; x += spbI) {
             inst.push_back(i);
             s0.push_back(x);
-            s1.push_back(std::min(x + spb, e));
+            s1.push_back(std::min(x + spbI, e));
         }
         if ((int)inst.size() == blk0[i]) {  // empty instance: one idle workgroup keeps the reductions defined
             inst.push_back(i);
@@ -585,13 +605,6 @@ void uploadGeometry(s4f_mesh* M) {
             s1.push_back(a);
         }
         nblk[i] = (int)inst.size() - blk0[i];
-    }
-    // pad to a multiple of 8 with idle workgroups of the last instance
-    while (inst.size() >= 8 && inst.size() % 8) {
-        inst.push_back(nInst - 1);
-        s0.push_back(0);
-        s1.push_back(0);
-        nblk[nInst - 1]++;
     }
     M->slicesPerBlock = spb;
     M->nBlocks = (int)inst.size();
@@ -647,7 +660,7 @@ constexpr int kHistCap = 100000;
 
 template <typename T>
 struct PcgVecs {
-    T *r, *p, *q, *x;
+    T *r, *p, *q, *x, *p2;  // p and p2 alternate as old/new search direction
 };
 
 template <typename T>
@@ -655,12 +668,14 @@ PcgVecs<T> pcgVecs(SolveState& st, int C);
 template <>
 PcgVecs<double> pcgVecs<double>(SolveState& st, int C) {
     st.r64.alloc(3 * (size_t)C); st.p64.alloc(3 * (size_t)C); st.q64.alloc(3 * (size_t)C); st.x64.alloc(3 * (size_t)C);
-    return {st.r64.p, st.p64.p, st.q64.p, st.x64.p};
+    st.pb64.alloc(3 * (size_t)C);
+    return {st.r64.p, st.p64.p, st.q64.p, st.x64.p, st.pb64.p};
 }
 template <>
 PcgVecs<float> pcgVecs<float>(SolveState& st, int C) {
     st.r32.alloc(3 * (size_t)C); st.p32.alloc(3 * (size_t)C); st.q32.alloc(3 * (size_t)C); st.x32.alloc(3 * (size_t)C);
-    return {st.r32.p, st.p32.p, st.q32.p, st.x32.p};
+    st.pb32.alloc(3 * (size_t)C);
+    return {st.r32.p, st.p32.p, st.q32.p, st.x32.p, st.pb32.p};
 }
 
 void allocState(s4f_mesh* M) {
@@ -677,21 +692,25 @@ void allocState(s4f_mesh* M) {
     st.hist.alloc(3 * (size_t)st.histCap * nInst);
 }
 
-// One launch of the inner PCG chunk: `n` iterations of spmv / update / pdir.
+// `n` inner PCG iterations: fused direction update + SpMV, alpha, solution /
+// residual update, convergence + beta.  v.p / v.p2 swap roles every iteration.
 template <typename T>
-void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, PcgVecs<T> v, Ctl* ctl, double* part, int n, hipStream_t s,
-                    Profiler* pf = nullptr, int inner0 = 0, int outer = 0) {
+void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, int nInst, PcgVecs<T>& v, Ctl* ctl, double* part, int n,
+                    hipStream_t s, Profiler* pf = nullptr, int inner0 = 0, int outer = 0) {
     for (int i = 0; i < n; ++i) {
         int x = pf ? pf->begin(K_SPMV, s, inner0 + i, outer) : -1;
-        hipLaunchKernelGGL((k_spmv<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, ctl, part, 1);
+        hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
+                           v.p2, v.q, ctl, part, 1);
+        if (pf) pf->end(x, s);
+        x = pf ? pf->begin(K_FINALIZE, s, inner0 + i, outer) : -1;
+        hipLaunchKernelGGL(k_fin_spmv, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         if (pf) pf->end(x, s);
         x = pf ? pf->begin(K_UPDATE, s, inner0 + i, outer) : -1;
-        hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, v.p, v.r, v.q, ctl, part);
+        hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
+                           (const T*)v.q, (const Ctl*)ctl, part);
         if (pf) pf->end(x, s);
-        // the direction update after the last working update is a no-op
-        x = pf ? pf->begin(K_PDIR, s, inner0 + i + 1, outer) : -1;
-        hipLaunchKernelGGL((k_pdir<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.r, v.p, ctl);
-        if (pf) pf->end(x, s);
+        hipLaunchKernelGGL(k_fin_update, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        std::swap(v.p, v.p2);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -744,23 +763,25 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     HIP_CHECK(hipEventRecord(ctx->ev0, s));
     // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
     hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)nullptr,
-                       (double*)nullptr, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p,
-                       st.part.p, st.hist.p, 1);
+                       st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p,
+                       st.hist.p, 1);
+    hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
+                       st.hist.p, 1);
     HIP_CHECK(hipGetLastError());
-    double* Dcur = st.D0.p;
-    double* Dnext = st.D1.p;
     int chunk = 4;
     Profiler* pf = (ctx->prof.every > 0 && nInst == 1) ? &ctx->prof : nullptr;
     for (int it = 0;; ++it) {
         int xr = pf ? pf->begin(K_RESIDUAL, s, 0, it) : -1;
-        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, Dcur, st.G.p, st.sigb.p, v.r,
-                           v.p, st.ctl.p, st.part.p, 1);
+        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
+                           (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
         if (pf) pf->end(xr, s);
+        hipLaunchKernelGGL(k_fin_residual, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
         bool allDone = false;
         int total = 0;
         for (;;) {
-            launchPcgChunk<T>(dm, bm, v, st.ctl.p, st.part.p, chunk, s, pf, total, it);
+            launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s, pf, total, it);
             total += chunk;
+            chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
             HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
             allDone = true;
@@ -779,21 +800,23 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
                 if (x.outer != it) return x.outer < it;  // k_grad of the previous iteration
                 if (ran < 0) return false;
                 if (x.kernel == K_RESIDUAL) return true;
-                return x.inner < ran;
+                return x.inner < ran;  // spmv, alpha finalize and update of working iterations
             });
         }
         if (allDone) break;
         int used = 1;
         for (int i = 0; i < nInst; ++i)
             if (!hctl[i].outerDone) used = std::max({used, hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});
-        chunk = used;
+        // next outer iteration: launch just under the last count, then top up
+        chunk = std::max(1, used - 1);
         int xg = pf ? pf->begin(K_GRAD, s, 0, it) : -1;
-        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, Dcur, (const T*)v.x, Dnext,
+        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
                            st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
                            0);
         if (pf) pf->end(xg, s);
+        hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
+                           st.hist.p, 0);
         HIP_CHECK(hipGetLastError());
-        std::swap(Dcur, Dnext);
         if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
     }
     HIP_CHECK(hipEventRecord(ctx->ev1, s));
@@ -820,8 +843,8 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     for (int i = 0; i < nInst; ++i) {
         const Ctl& cf = hctl[i];
         const int c0i = h.instCell0[i], n = h.instNCell[i], b0 = h.instB0[i], nb = h.instNB[i];
-        // the final field of instance i is in the buffer written by its iteration nOuter-1
-        const std::vector<double>& Df = (cf.nOuter % 2 == 1) ? hD1 : hD0;
+        // the final field of instance i is in the buffer its parity flag names
+        const std::vector<double>& Df = cf.dpar ? hD1 : hD0;
         auto copyOut = [&](const std::vector<double>& src, int k, int off, int cnt, size_t stride, double* out) {
             for (int c = 0; c < cnt; ++c)
                 for (int q = 0; q < k; ++q) out[(size_t)c * k + q] = src[(size_t)q * stride + off + c];
@@ -870,6 +893,7 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
     P.C = (int)C; P.B = (int)B; P.nSlices = (int)S; P.F = 0;
     P.uniformMaterial = false;
     P.sliceStart.resize(S + 1);
+    P.sliceWidth.resize(S);
     P.col.resize(E);
     P.eS.resize(3 * E); P.eW.resize(E); P.eA.resize(E); P.eC.resize(E);
     P.cC.assign(3 * C, 0.0);
@@ -884,7 +908,10 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
         const HostMesh& m = *ms[i];
         const int c0 = P.instCell0[i], b0 = P.instB0[i];
         const size_t Em = m.col.size();
-        for (int s = 0; s < m.nSlices; ++s) P.sliceStart[s0 + s] = (int)(e0 + m.sliceStart[s]);
+        for (int s = 0; s < m.nSlices; ++s) {
+            P.sliceStart[s0 + s] = (int)(e0 + m.sliceStart[s]);
+            P.sliceWidth[s0 + s] = m.sliceWidth[s];
+        }
         for (size_t e = 0; e < Em; ++e) {
             const int c = m.col[e];
             P.col[e0 + e] = c >= 0 ? c + c0 : (c == kPad ? kPad : -1 - ((-1 - c) + b0));
@@ -952,12 +979,9 @@ s4f_status s4f_ctx_create(int device, s4f_ctx** out) {
 
 s4f_status s4f_ctx_destroy(s4f_ctx* ctx) {
     if (!ctx) return S4F_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipEventDestroy(ctx->ev0);
-    (void)hipEventDestroy(ctx->ev1);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (ctx->destroyPending) return fail(S4F_ERR_INVALID, "context already destroyed");
+    ctx->destroyPending = true;
+    if (ctx->children == 0) ctxRelease(ctx);
     return S4F_OK;
 }
 
@@ -1046,12 +1070,14 @@ s4f_status s4f_polymesh_geometry(const s4f_polymesh* p, double* Cf, double* Sf, 
 s4f_status s4f_mesh_create(s4f_ctx* ctx, const s4f_mesh_desc* desc, s4f_mesh** out) {
     return guarded([&]() -> s4f_status {
         if (!ctx || !desc || !out) throw InvalidError("null argument");
+        if (ctx->destroyPending) throw InvalidError("context was destroyed");
         HIP_CHECK(hipSetDevice(ctx->device));
         auto M = std::make_unique<s4f_mesh>();
         M->ctx = ctx;
         setupHostMesh(M->h, *desc);
         M->bcSet.assign(M->h.nPatches, 0);
         uploadGeometry(M.get());
+        ctx->children++;
         *out = M.release();
         return S4F_OK;
     });
@@ -1081,9 +1107,11 @@ s4f_status s4f_mesh_create_from_polymesh(s4f_ctx* ctx, const s4f_polymesh* p, co
 
 s4f_status s4f_mesh_destroy(s4f_mesh* mesh) {
     if (!mesh) return S4F_OK;
-    (void)hipSetDevice(mesh->ctx->device);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
+    s4f_ctx* ctx = mesh->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     delete mesh;
+    ctxChildGone(ctx);
     return S4F_OK;
 }
 
@@ -1208,7 +1236,7 @@ s4f_status s4f_grad(s4f_mesh* M, const double* D, const double* Db, const double
         const DevMesh dm = M->dm();
         const BlockMap bm = M->bmap();
         hipLaunchKernelGGL((k_grad<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p,
-                           (const double*)nullptr, (double*)nullptr, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p,
+                           (const double*)nullptr, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p,
                            st.sigb.p, (const double*)nullptr, st.ctl.p, st.part.p, (double*)nullptr, 2);
         HIP_CHECK(hipGetLastError());
         soaToAos(download(st.G, 9 * (size_t)C, s), C, 9, gradD);
@@ -1262,8 +1290,8 @@ s4f_status s4f_residual(s4f_mesh* M, const double* D, const double* gradD, const
         HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
         const DevMesh dm = M->dm();
         const BlockMap bm = M->bmap();
-        hipLaunchKernelGGL((k_residual<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, st.G.p,
-                           st.sigb.p, v.r, v.p, st.ctl.p, st.part.p, 0);
+        hipLaunchKernelGGL((k_residual<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
+                           (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
         HIP_CHECK(hipGetLastError());
         soaToAos(download(st.r64, 3 * (size_t)C, s), C, 3, R);
         return S4F_OK;
@@ -1287,8 +1315,8 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
             PcgVecs<float> v = pcgVecs<float>(st, C);
             std::vector<float> tf(t.begin(), t.end());
             HIP_CHECK(hipMemcpyAsync(v.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_spmv<float>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, st.ctl.p,
-                               st.part.p, 0);
+            hipLaunchKernelGGL((k_spmvp<float>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const float*)nullptr,
+                               (const float*)v.p, (float*)nullptr, v.q, st.ctl.p, st.part.p, 0);
             HIP_CHECK(hipGetLastError());
             std::vector<float> yf(3 * (size_t)C);
             HIP_CHECK(hipMemcpyAsync(yf.data(), v.q, yf.size() * 4, hipMemcpyDeviceToHost, s));
@@ -1298,8 +1326,8 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
         } else {
             PcgVecs<double> v = pcgVecs<double>(st, C);
             HIP_CHECK(hipMemcpyAsync(v.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_spmv<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.p, v.q, st.ctl.p,
-                               st.part.p, 0);
+            hipLaunchKernelGGL((k_spmvp<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)nullptr,
+                               (const double*)v.p, (double*)nullptr, v.q, st.ctl.p, st.part.p, 0);
             HIP_CHECK(hipGetLastError());
             soaToAos(download(st.q64, 3 * (size_t)C, s), C, 3, y);
         }
@@ -1332,12 +1360,13 @@ s4f_status s4f_pcg3(s4f_mesh* M, const s4f_solve_opts* o, const double* b, const
             using T = decltype(tag);
             hipLaunchKernelGGL((k_pcg_start<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, bd.p, v.r, v.p, st.ctl.p,
                                st.part.p);
+            hipLaunchKernelGGL(k_fin_pcg_start, dim3(1), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
             HIP_CHECK(hipGetLastError());
             // x = 0 when no iteration runs
             HIP_CHECK(hipMemsetAsync(v.x, 0, 3 * (size_t)C * sizeof(T), s));
             int total = 0;
             for (;;) {
-                launchPcgChunk<T>(dm, bm, v, st.ctl.p, st.part.p, 8, s);
+                launchPcgChunk<T>(dm, bm, 1, v, st.ctl.p, st.part.p, 8, s);
                 total += 8;
                 HIP_CHECK(hipMemcpyAsync(&hc, st.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
                 HIP_CHECK(hipStreamSynchronize(s));
@@ -1394,8 +1423,10 @@ struct s4f_batch {
 s4f_status s4f_batch_create(s4f_ctx* ctx, s4f_batch** out) {
     return guarded([&]() -> s4f_status {
         if (!ctx || !out) throw InvalidError("null argument");
+        if (ctx->destroyPending) throw InvalidError("context was destroyed");
         auto b = std::make_unique<s4f_batch>();
         b->ctx = ctx;
+        ctx->children++;
         *out = b.release();
         return S4F_OK;
     });
@@ -1466,9 +1497,11 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
 
 s4f_status s4f_batch_destroy(s4f_batch* b) {
     if (!b) return S4F_OK;
-    (void)hipSetDevice(b->ctx->device);
-    (void)hipStreamSynchronize(b->ctx->stream);
+    s4f_ctx* ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     delete b;
+    ctxChildGone(ctx);
     return S4F_OK;
 }
 

@@ -24,7 +24,8 @@ constexpr double kVSmall = 1e-300;   // OpenFOAM VSMALL (double)
 struct DevMesh {
     int nCells, nBoundary, nSlices, slicesPerBlock, nBlocks;
     size_t nEntries;
-    const int* sliceStart;
+    const int* sliceStart;  // first ELL entry of each slice
+    const int* sliceWidth;  // slots of each slice
     const int* col;
     const double* eS;      // 3*nEntries: area vector pointing out of the row cell
     const double* eW;      // interpolation weight of the row cell
@@ -61,6 +62,7 @@ struct DevMesh {
 struct Ctl {
     int outerDone, innerDone, converged, diverged;
     int iCorr, nOuter, firstInner, warm;
+    int dpar, pad0;  // which displacement buffer holds the current field
     int active[4];
     int nIter[4];
     double initRes[4], normFactor[4], finalRes[4];

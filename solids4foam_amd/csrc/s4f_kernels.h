@@ -5,17 +5,19 @@
 // slice is one coalesced load per array.  Face contributions are gathered
 // per cell in OpenFOAM's face order, so no atomics touch field data and the
 // summation order is fixed (run-to-run bitwise deterministic).  Reductions
-// are per workgroup (wave shuffles + LDS) into a partial array; the last
-// workgroup of each instance (counter in the control block) reduces the
-// partials in fixed order and updates the control block.
+// are per workgroup (wave shuffles + LDS) into a partial array; a one-block-
+// per-instance finalize kernel then reduces the instance's partials in fixed
+// order and updates the control block (kernel boundaries order the writes of
+// every XCD before the finalize reads them).
 //
 // Kernel <-> reference map (paths relative to the solids4foam tree):
 //   k_residual   linGeomTotalDispSolid.C:141-149: fvc::div(sigma) [Gauss linear]
 //                + momentumStabilisation.C:210-217 (RhieChow) + the normalisation
 //                of OpenFOAM lduMatrix::solver::normFactor for the PCG initial residual
-//   k_spmv       OpenFOAM lduMatrix::Amul (3 right-hand sides sharing M)
+//   k_spmvp      OpenFOAM PCG search-direction update p = z + beta p fused with
+//                lduMatrix::Amul (3 right-hand sides sharing M)
 //   k_update     OpenFOAM PCG solution/residual update + diagonal preconditioner
-//   k_pdir       OpenFOAM PCG search-direction update
+//   k_fin_*      the scalar parts of PCG::solve / normFactor / solidModel::converged
 //   k_grad       solidTractionFvPatchVectorField.C:398-463 / fixedDisplacement
 //                boundary values, leastSquares grad(D) (mechanicalModel.C:571-582),
 //                gaussGrad boundary correction, linearElastic::correct on
@@ -39,10 +41,13 @@ struct BlockMap {
     int nBlocks;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (hardware block b runs
+// on XCD b % 8).  Give XCD x a contiguous run of logical blocks so that
+// neighbouring cells (and their gathers) stay in one XCD's L2.
 __device__ __forceinline__ int logicalBlock(int nb) {
     const int b = blockIdx.x;
-    // consecutive logical blocks (neighbouring cells) land on the same XCD
-    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // NaN-propagating maximum (fmax would drop a NaN and hide a diverged solve)
@@ -58,49 +63,55 @@ __device__ __forceinline__ double waveReduce(double v) {
     return v;
 }
 
-template <bool MAX>
-__device__ __forceinline__ double blockReduce(double v, double* sh) {
-    v = waveReduce<MAX>(v);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __syncthreads();
-    if (l == 0) sh[w] = v;
-    __syncthreads();
-    double r = sh[0];
+constexpr int kFinBlock = 256;
+
+// Fixed-order reduction of the instance's partial rows (one workgroup per
+// instance); bit i of maxMask selects a maximum instead of a sum.  Result in
+// out[0..NV) (shared), valid for every thread after the call.
+template <int NV>
+__device__ void finReduce(const double* __restrict__ part, int blk0, int nblk, unsigned maxMask, double* out) {
+    double v[NV];
 #pragma unroll
-    for (int i = 1; i < kBlock / 64; ++i) r = MAX ? nanMax(r, sh[i]) : r + sh[i];
-    return r;
-}
-
-__device__ __forceinline__ void storePartial(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double loadPartial(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// true in exactly one (the last arriving) workgroup of the instance
-__device__ __forceinline__ bool lastOfInstance(Ctl* c, int slot, int nblk) {
-    __shared__ int isLast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = atomicInc(&c->counter[slot], (unsigned)(nblk - 1));
-        isLast = (t == (unsigned)(nblk - 1));
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kFinBlock) {
+        const double* row = part + (size_t)(blk0 + b) * kMaxPartials;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = ((maxMask >> i) & 1u) ? nanMax(v[i], row[i]) : v[i] + row[i];
+    }
+    __shared__ double sh[NV][kFinBlock / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        v[i] = ((maxMask >> i) & 1u) ? waveReduce<true>(v[i]) : waveReduce<false>(v[i]);
+        if (l == 0) sh[i][w] = v[i];
     }
     __syncthreads();
-    if (isLast) __threadfence();
-    return isLast;
+    if (threadIdx.x < NV) {
+        const int i = threadIdx.x;
+        double r = sh[i][0];
+        for (int k = 1; k < kFinBlock / 64; ++k) r = ((maxMask >> i) & 1u) ? nanMax(r, sh[i][k]) : r + sh[i][k];
+        out[i] = r;
+    }
+    __syncthreads();
 }
 
-// fixed-order reduction of partial value `i` over the instance's blocks
-template <bool MAX>
-__device__ double reducePartials(const double* part, int blk0, int nblk, int i, double* sh) {
-    double v = MAX ? 0.0 : 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kBlock) {
-        const double x = loadPartial(part + (size_t)(blk0 + b) * kMaxPartials + i);
-        v = MAX ? nanMax(v, x) : v + x;
+// block-level reduction of NV per-thread values; thread 0 stores the row
+template <int NV>
+__device__ __forceinline__ void storeBlockPartials(const double* acc, unsigned maxMask, double* row) {
+    __shared__ double sh[NV][kBlock / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double v = ((maxMask >> i) & 1u) ? waveReduce<true>(acc[i]) : waveReduce<false>(acc[i]);
+        if (l == 0) sh[i][w] = v;
     }
-    return blockReduce<MAX>(v, sh);
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int i = threadIdx.x;
+        double r = sh[i][0];
+        for (int k = 1; k < kBlock / 64; ++k) r = ((maxMask >> i) & 1u) ? nanMax(r, sh[i][k]) : r + sh[i][k];
+        row[i] = r;
+    }
 }
 
 // linearElastic: sigma = 2 mu dev(symm G) + K tr(symm G) I   (xx xy xz yy yz zz)
@@ -135,18 +146,19 @@ template <> struct MatSel<float> {
 };
 
 // ---------------------------------------------------------------------------
-// k_residual: R = div(sigma) + RhieChow (integrated), r = R, p = z = R/diag,
-// and the PCG start: initial residual normalised by normFactor per component.
+// k_residual: R = div(sigma) + RhieChow (integrated), r = R, and per-workgroup
+// partials of sum|R|, the two normFactor sums and z.r (k_fin_residual).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const BlockMap bm, const double* __restrict__ D,
-                                                     const double* __restrict__ G, const double* __restrict__ sigb,
-                                                     T* __restrict__ r, T* __restrict__ p, Ctl* __restrict__ ctlAll,
-                                                     double* __restrict__ part, int pcgStart) {
+__global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const BlockMap bm, const double* __restrict__ D0,
+                                                     const double* __restrict__ D1, const double* __restrict__ G, const double* __restrict__ sigb,
+                                                     T* __restrict__ r, Ctl* __restrict__ ctlAll,
+                                                     double* __restrict__ part) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone) return;
+    const double* __restrict__ D = ctl->dpar ? D1 : D0;
     const int C = m.nCells;
     const double xb0 = ctl->xbar[0], xb1 = ctl->xbar[1], xb2 = ctl->xbar[2];
     double acc[12];
@@ -159,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
     for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
         const int P = s * kSlice + lane;
         const int e0 = m.sliceStart[s];
-        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
         double DP[3], GP[9], CP[3], sP[6];
 #pragma unroll
@@ -226,61 +238,42 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
             acc[6 + j] += fabs((MD + R) - t);
             const T rT = (T)R;
             r[(size_t)j * C + P] = rT;
-            if (pcgStart) {
-                const T z = invD[P] * rT;
-                p[(size_t)j * C + P] = z;
-                acc[9 + j] += (double)z * (double)rT;
-            }
+            const T z = invD[P] * rT;
+            acc[9 + j] += (double)z * (double)rT;
         }
     }
-    __shared__ double sh[kBlock / 64];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) {
-        const double v = blockReduce<false>(acc[i], sh);
-        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
-    }
-    if (!pcgStart) return;
-    if (!lastOfInstance(ctl, 0, bm.iNBlk[inst])) return;
-    double tot[12];
-    for (int i = 0; i < 12; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
-    if (threadIdx.x == 0) {
-        double mag2 = 0.0;
-        int any = 0;
-        for (int j = 0; j < 3; ++j) {
-            const double nf = tot[3 + j] + tot[6 + j] + 1e-20;
-            ctl->normFactor[j] = nf;
-            ctl->initRes[j] = tot[j] / nf;
-            ctl->finalRes[j] = ctl->initRes[j];
-            mag2 += ctl->initRes[j] * ctl->initRes[j];
-            ctl->rho[j] = tot[9 + j];
-            ctl->nIter[j] = 0;
-            ctl->alpha[j] = 0.0;
-            ctl->beta[j] = 0.0;
-            // PCG::solve: iterate unless already converged (finalRes < tolerance)
-            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
-            ctl->active[j] = act;
-            any |= act;
-        }
-        ctl->initResMag = sqrt(mag2);
-        ctl->innerDone = !any;
-        ctl->firstInner = 1;
-    }
+    storeBlockPartials<12>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
 // ---------------------------------------------------------------------------
-// k_spmv: q = M p (3 RHS), partial p.q; last workgroup: alpha = rho / p.q
+// k_spmvp: p' = z + beta p (z = r/diag; p' = z on the first PCG iteration),
+// recomputed for the row and for every neighbour it gathers, and q = M p'
+// (3 RHS sharing M); partial p'.q.  Components already converged are skipped.
+// With reduce == 0 it is the plain product q = M pOld (no direction update).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv(const DevMesh m, const BlockMap bm, const T* __restrict__ p,
-                                                 T* __restrict__ q, Ctl* __restrict__ ctlAll, double* __restrict__ part,
-                                                 int reduce) {
+__global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMap bm, const T* __restrict__ r,
+                                                  const T* __restrict__ pOld, T* __restrict__ pNew, T* __restrict__ q,
+                                                  Ctl* __restrict__ ctlAll, double* __restrict__ part, int reduce) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
-    Ctl* ctl = ctlAll + inst;
+    const Ctl* ctl = ctlAll + inst;
     if (reduce && (ctl->outerDone || ctl->innerDone)) return;
     const int C = m.nCells;
     const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ dg = MatSel<T>::diag(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const int first = reduce ? ctl->firstInner : 0;
+    const bool act[3] = {reduce ? ctl->active[0] != 0 : true, reduce ? ctl->active[1] != 0 : true,
+                         reduce ? ctl->active[2] != 0 : true};
+    const T be[3] = {reduce ? (T)ctl->beta[0] : (T)0, reduce ? (T)ctl->beta[1] : (T)0, reduce ? (T)ctl->beta[2] : (T)0};
+    // direction component of cell c
+    auto dir = [&](int j, int c) -> T {
+        const size_t i = (size_t)j * C + c;
+        if (!reduce) return pOld[i];
+        const T z = invD[c] * r[i];
+        return first ? z : z + be[j] * pOld[i];
+    };
     double acc[3] = {0, 0, 0};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s1 = bm.s1[lb];
@@ -288,69 +281,53 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const DevMesh m, const BlockMap
     for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
         const int P = s * kSlice + lane;
         const int e0 = m.sliceStart[s];
-        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
-        const T p0 = p[P], p1 = p[(size_t)C + P], p2 = p[2 * (size_t)C + P];
+        T pP[3], y[3];
         const T d = dg[P];
-        T y0 = d * p0, y1 = d * p1, y2 = d * p2;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            pP[j] = act[j] ? dir(j, P) : (T)0;
+            y[j] = d * pP[j];
+        }
         for (int k = 0; k < width; ++k) {
             const int e = e0 + k * kSlice + lane;
             const int n = m.col[e];
             if (n >= 0) {
                 const T a = A[e];
-                y0 -= a * p[n];
-                y1 -= a * p[(size_t)C + n];
-                y2 -= a * p[2 * (size_t)C + n];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (act[j]) y[j] -= a * dir(j, n);
             }
         }
-        q[P] = y0;
-        q[(size_t)C + P] = y1;
-        q[2 * (size_t)C + P] = y2;
-        acc[0] += (double)p0 * (double)y0;
-        acc[1] += (double)p1 * (double)y1;
-        acc[2] += (double)p2 * (double)y2;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * C + P;
+            if (reduce) pNew[i] = pP[j];
+            q[i] = y[j];
+            acc[j] += (double)pP[j] * (double)y[j];
+        }
     }
     if (!reduce) return;
-    __shared__ double sh[kBlock / 64];
-    for (int i = 0; i < 3; ++i) {
-        const double v = blockReduce<false>(acc[i], sh);
-        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
-    }
-    if (!lastOfInstance(ctl, 1, bm.iNBlk[inst])) return;
-    double tot[3];
-    for (int i = 0; i < 3; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
-    if (threadIdx.x == 0) {
-        int any = 0;
-        for (int j = 0; j < 3; ++j) {
-            if (!ctl->active[j]) { ctl->alpha[j] = 0.0; continue; }
-            // PCG singularity test: |p.Ap| / normFactor < VSMALL
-            if (fabs(tot[j]) / ctl->normFactor[j] < kVSmall) {
-                ctl->active[j] = 0;
-                ctl->alpha[j] = 0.0;
-                continue;
-            }
-            ctl->alpha[j] = ctl->rho[j] / tot[j];
-            any = 1;
-        }
-        if (!any) ctl->innerDone = 1;
-    }
+    storeBlockPartials<3>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
 // ---------------------------------------------------------------------------
-// k_update: x += alpha p, r -= alpha q, z = r/diag, rho' = z.r, sum|r|;
-// last workgroup: convergence per component (tolerance, relTol, maxIter), beta
+// k_update: x += alpha p, r -= alpha q, z = r/diag; partials z.r and sum|r|
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockMap bm, T* __restrict__ x,
                                                    const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q,
-                                                   Ctl* __restrict__ ctlAll, double* __restrict__ part) {
+                                                   const Ctl* __restrict__ ctlAll, double* __restrict__ part) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
-    Ctl* ctl = ctlAll + inst;
+    const Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     const int C = m.nCells;
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     const T al[3] = {(T)ctl->alpha[0], (T)ctl->alpha[1], (T)ctl->alpha[2]};
+    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     const int first = ctl->firstInner;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -363,9 +340,12 @@ __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockM
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const size_t i = (size_t)j * C + P;
-            const T pv = p[i];
+            if (!act[j]) {
+                if (first) x[i] = (T)0;
+                continue;
+            }
             const T xv = first ? (T)0 : x[i];
-            x[i] = xv + al[j] * pv;
+            x[i] = xv + al[j] * p[i];
             const T rv = r[i] - al[j] * q[i];
             r[i] = rv;
             const T z = iD * rv;
@@ -373,60 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockM
             acc[3 + j] += fabs((double)rv);
         }
     }
-    __shared__ double sh[kBlock / 64];
-    for (int i = 0; i < 6; ++i) {
-        const double v = blockReduce<false>(acc[i], sh);
-        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
-    }
-    if (!lastOfInstance(ctl, 2, bm.iNBlk[inst])) return;
-    double tot[6];
-    for (int i = 0; i < 6; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
-    if (threadIdx.x == 0) {
-        int any = 0;
-        for (int j = 0; j < 3; ++j) {
-            if (!ctl->active[j]) continue;
-            ctl->nIter[j] += 1;
-            ctl->innerTotal += 1;
-            const double fr = tot[3 + j] / ctl->normFactor[j];
-            ctl->finalRes[j] = fr;
-            const bool conv = fr < ctl->pcgTol || (ctl->pcgRelTol > 1e-20 && fr < ctl->pcgRelTol * ctl->initRes[j]);
-            const int act = !conv && ctl->nIter[j] < ctl->pcgMaxIter;
-            ctl->beta[j] = tot[j] / ctl->rho[j];
-            ctl->rho[j] = tot[j];
-            ctl->active[j] = act;
-            any |= act;
-        }
-        ctl->innerDone = !any;
-        ctl->firstInner = 0;
-    }
-}
-
-// p = z + beta p for the active components (z = r/diag recomputed)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pdir(const DevMesh m, const BlockMap bm, const T* __restrict__ r,
-                                                 T* __restrict__ p, const Ctl* __restrict__ ctlAll) {
-    const int lb = logicalBlock(bm.nBlocks);
-    const int inst = bm.inst[lb];
-    const Ctl* ctl = ctlAll + inst;
-    if (ctl->outerDone || ctl->innerDone) return;
-    const int C = m.nCells;
-    const T* __restrict__ invD = MatSel<T>::invDiag(m);
-    const int act[3] = {ctl->active[0], ctl->active[1], ctl->active[2]};
-    const T be[3] = {(T)ctl->beta[0], (T)ctl->beta[1], (T)ctl->beta[2]};
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s1 = bm.s1[lb];
-    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
-    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
-        const int P = s * kSlice + lane;
-        if (P >= cEnd) continue;
-        const T iD = invD[P];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            if (!act[j]) continue;
-            const size_t i = (size_t)j * C + P;
-            p[i] = iD * r[i] + be[j] * p[i];
-        }
-    }
+    storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,8 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_pdir(const DevMesh m, const BlockMap
 // no convergence test); mode 2: gradient from given D_b/snb (no boundary update).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap bm, const double* __restrict__ D,
-                                                 const T* __restrict__ x, double* __restrict__ Dn,
+__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap bm, double* __restrict__ D0,
+                                                 const T* __restrict__ x, double* __restrict__ D1,
                                                  double* __restrict__ G, double* __restrict__ Db,
                                                  double* __restrict__ g, double* __restrict__ snb,
                                                  double* __restrict__ Gb, double* __restrict__ sigb,
@@ -449,6 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     Ctl* ctl = ctlAll + inst;
     if (mode == 0 && ctl->outerDone) return;
     if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
+    // current field and the buffer receiving D' (per-instance ping-pong)
+    const double* __restrict__ D = ctl->dpar ? D1 : D0;
+    double* __restrict__ Dn = (mode == 0) ? (ctl->dpar ? D0 : D1) : nullptr;
     const int C = m.nCells, B = m.nBoundary;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // max|D'-D|, max|D'-Dold|, max|D'|, sum D'
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -457,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
         const int P = s * kSlice + lane;
         const int e0 = m.sliceStart[s];
-        const int width = (m.sliceStart[s + 1] - e0) / kSlice;
+        const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
         double DP[3], DnP[3], CP[3];
 #pragma unroll
@@ -541,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         }
 #pragma unroll
         for (int q = 0; q < 9; ++q) G[(size_t)q * C + P] = Gn[q];
-        if (Dn && mode == 0) {
+        if (Dn) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) Dn[(size_t)q * C + P] = DnP[q];
         }
@@ -600,50 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         }
     }
     if (mode == 2) return;
-    __shared__ double sh[kBlock / 64];
-    for (int i = 0; i < 6; ++i) {
-        const double v = (i < 3) ? blockReduce<true>(acc[i], sh) : blockReduce<false>(acc[i], sh);
-        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
-    }
-    if (!lastOfInstance(ctl, 3, bm.iNBlk[inst])) return;
-    double tot[6];
-    for (int i = 0; i < 6; ++i)
-        tot[i] = (i < 3) ? reducePartials<true>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh)
-                         : reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
-    if (threadIdx.x == 0) {
-        const double nc = (double)bm.iNCell[inst];
-        for (int j = 0; j < 3; ++j) ctl->xbar[j] = tot[3 + j] / nc;
-        if (mode == 0) {
-            // solidModel::converged, total-displacement form
-            double denom = tot[1];
-            if (denom < kSmall) denom = tot[2] < kSmall ? kSmall : tot[2];
-            const double rvf = tot[0] / denom;
-            const double ir = ctl->initResMag;
-            const int i = ctl->iCorr;
-            ctl->rvf = rvf;
-            const bool conv = i > 1 && ((ir < ctl->solTol && rvf < ctl->solTol) || rvf < ctl->altTol || ir < ctl->altTol);
-            if (hist && i < ctl->histCap) {
-                double* h = hist + (size_t)inst * 3 * ctl->histCap;
-                h[i] = ir;
-                h[ctl->histCap + i] = rvf;
-                h[2 * ctl->histCap + i] = (double)(ctl->nIter[0] + ctl->nIter[1] + ctl->nIter[2]);
-            }
-            if (!isfinite(ir) || !isfinite(rvf)) {
-                ctl->diverged = 1;
-                ctl->outerDone = 1;
-                ctl->nOuter = i + 1;
-            } else if (conv) {
-                ctl->converged = 1;
-                ctl->outerDone = 1;
-                ctl->nOuter = i + 1;
-            } else if (i + 1 >= ctl->nCorr) {
-                ctl->outerDone = 1;
-                ctl->nOuter = i + 1;
-            } else {
-                ctl->iCorr = i + 1;
-            }
-        }
-    }
+    storeBlockPartials<6>(acc, 7u, part + (size_t)lb * kMaxPartials);
 }
 
 // sigma = Hooke(G) for output (xx xy xz yy yz zz, SoA)
@@ -668,7 +555,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
                                                       double* __restrict__ part) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
-    Ctl* ctl = ctlAll + inst;
     const int C = m.nCells;
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -689,14 +575,53 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
             acc[3 + j] += (double)z * (double)rT;
         }
     }
-    __shared__ double sh[kBlock / 64];
-    for (int i = 0; i < 6; ++i) {
-        const double v = blockReduce<false>(acc[i], sh);
-        if (threadIdx.x == 0) storePartial(part + (size_t)lb * kMaxPartials + i, v);
+    storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
+}
+
+// ===========================================================================
+// finalize kernels: one workgroup per instance (grid = number of instances)
+// ===========================================================================
+
+// PCG::solve start: normFactor (lduMatrix::solver::normFactor), normalised
+// initial residual, convergence check before the first iteration, z.r
+__global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                            const double* __restrict__ part) {
+    const int inst = blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone) return;
+    __shared__ double tot[12];
+    finReduce<12>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    if (threadIdx.x == 0) {
+        double mag2 = 0.0;
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            const double nf = tot[3 + j] + tot[6 + j] + 1e-20;
+            ctl->normFactor[j] = nf;
+            ctl->initRes[j] = tot[j] / nf;
+            ctl->finalRes[j] = ctl->initRes[j];
+            mag2 += ctl->initRes[j] * ctl->initRes[j];
+            ctl->rho[j] = tot[9 + j];
+            ctl->nIter[j] = 0;
+            ctl->alpha[j] = 0.0;
+            ctl->beta[j] = 0.0;
+            // iterate unless already converged (finalRes = initRes < tolerance)
+            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
+            ctl->active[j] = act;
+            any |= act;
+        }
+        ctl->initResMag = sqrt(mag2);
+        ctl->innerDone = !any;
+        ctl->firstInner = 1;
     }
-    if (!lastOfInstance(ctl, 0, bm.iNBlk[inst])) return;
-    double tot[6];
-    for (int i = 0; i < 6; ++i) tot[i] = reducePartials<false>(part, bm.iBlk0[inst], bm.iNBlk[inst], i, sh);
+}
+
+// PCG start from a caller-supplied right-hand side (normFactor given)
+__global__ __launch_bounds__(kFinBlock) void k_fin_pcg_start(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                             const double* __restrict__ part) {
+    const int inst = blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    __shared__ double tot[6];
+    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         int any = 0;
         for (int j = 0; j < 3; ++j) {
@@ -710,6 +635,102 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
         }
         ctl->innerDone = !any;
         ctl->firstInner = 1;
+    }
+}
+
+// alpha = rho / p.q, with PCG's singularity test |p.q|/normFactor < VSMALL
+__global__ __launch_bounds__(kFinBlock) void k_fin_spmv(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                        const double* __restrict__ part) {
+    const int inst = blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone || ctl->innerDone) return;
+    __shared__ double tot[3];
+    finReduce<3>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            if (!ctl->active[j]) { ctl->alpha[j] = 0.0; continue; }
+            if (fabs(tot[j]) / ctl->normFactor[j] < kVSmall) {
+                ctl->active[j] = 0;
+                ctl->alpha[j] = 0.0;
+                continue;
+            }
+            ctl->alpha[j] = ctl->rho[j] / tot[j];
+            any = 1;
+        }
+        if (!any) ctl->innerDone = 1;
+    }
+}
+
+// PCG convergence per component (tolerance, relTol, maxIter) and beta
+__global__ __launch_bounds__(kFinBlock) void k_fin_update(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                          const double* __restrict__ part) {
+    const int inst = blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone || ctl->innerDone) return;
+    __shared__ double tot[6];
+    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < 3; ++j) {
+            if (!ctl->active[j]) continue;
+            ctl->nIter[j] += 1;
+            ctl->innerTotal += 1;
+            const double fr = tot[3 + j] / ctl->normFactor[j];
+            ctl->finalRes[j] = fr;
+            const bool conv = fr < ctl->pcgTol || (ctl->pcgRelTol > 1e-20 && fr < ctl->pcgRelTol * ctl->initRes[j]);
+            const int act = !conv && ctl->nIter[j] < ctl->pcgMaxIter;
+            ctl->beta[j] = tot[j] / ctl->rho[j];
+            ctl->rho[j] = tot[j];
+            ctl->active[j] = act;
+            any |= act;
+        }
+        ctl->innerDone = !any;
+        ctl->firstInner = 0;
+    }
+}
+
+// solidModel::converged (total-displacement form) and the mean field for the
+// next normFactor; mode 1 (initialisation) only sets the mean
+__global__ __launch_bounds__(kFinBlock) void k_fin_grad(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                        const double* __restrict__ part, double* __restrict__ hist,
+                                                        int mode) {
+    const int inst = blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    if (mode == 0 && ctl->outerDone) return;
+    __shared__ double tot[6];
+    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 7u, tot);
+    if (threadIdx.x != 0) return;
+    const double nc = (double)bm.iNCell[inst];
+    for (int j = 0; j < 3; ++j) ctl->xbar[j] = tot[3 + j] / nc;
+    if (mode != 0) return;
+    ctl->dpar ^= 1;  // k_grad wrote D' into the other buffer
+    double denom = tot[1];
+    if (denom < kSmall) denom = tot[2] < kSmall ? kSmall : tot[2];
+    const double rvf = tot[0] / denom;
+    const double ir = ctl->initResMag;
+    const int i = ctl->iCorr;
+    ctl->rvf = rvf;
+    const bool conv = i > 1 && ((ir < ctl->solTol && rvf < ctl->solTol) || rvf < ctl->altTol || ir < ctl->altTol);
+    if (hist && i < ctl->histCap) {
+        double* h = hist + (size_t)inst * 3 * ctl->histCap;
+        h[i] = ir;
+        h[ctl->histCap + i] = rvf;
+        h[2 * ctl->histCap + i] = (double)(ctl->nIter[0] + ctl->nIter[1] + ctl->nIter[2]);
+    }
+    if (!isfinite(ir) || !isfinite(rvf)) {
+        ctl->diverged = 1;
+        ctl->outerDone = 1;
+        ctl->nOuter = i + 1;
+    } else if (conv) {
+        ctl->converged = 1;
+        ctl->outerDone = 1;
+        ctl->nOuter = i + 1;
+    } else if (i + 1 >= ctl->nCorr) {
+        ctl->outerDone = 1;
+        ctl->nOuter = i + 1;
+    } else {
+        ctl->iCorr = i + 1;
     }
 }
 

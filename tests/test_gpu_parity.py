@@ -166,9 +166,10 @@ def test_batch_equals_alone(ctx):
         b.add(m)
     res = b.solve(**opts)
     for a, r in zip(alone, res):
+        # an instance's tiles and reductions do not depend on the rest of the batch
         assert r["converged"]
-        assert rel_l2(r["D"], a["D"]) < 1e-9
-        assert abs(r["nOuter"] - a["nOuter"]) <= 2
+        assert r["nOuter"] == a["nOuter"]
+        assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
 
 
 def test_divergence_and_maxiter_status(ctx):
