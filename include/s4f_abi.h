@@ -248,6 +248,40 @@ s4f_status s4f_batch_solve(s4f_batch *batch, const s4f_solve_opts *opts, double 
                            double **sigma_out, s4f_solve_report *reports);
 s4f_status s4f_batch_destroy(s4f_batch *batch);
 
+/* ------------------------------------------------------------------ */
+/* serving: continuous batching of solve requests (no counterpart in   */
+/* the reference, which runs one case per process)                     */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    s4f_mesh *mesh;            /* mesh with material set; must outlive the request */
+    const double *bcValue;     /* nBoundaryFaces*3: fixed displacement / traction per boundary face (copied) */
+    const double *bcPressure;  /* nBoundaryFaces or NULL (copied) */
+    int32_t warmStart;         /* 1: start from this mesh's last converged field if it is still resident */
+} s4f_request;
+
+/* Starts a server thread that owns an HBM arena of `budgetBytes` and up to
+   `maxSlots` concurrent solves, all with the same options.  Requests are
+   admitted in arrival order as soon as their cells, faces and slots fit;
+   in-flight solves of different sizes advance together, one outer iteration
+   per shared launch sequence, and retire on convergence.  Meshes stay
+   resident after their solve (evicted least-recently-used when space is
+   needed), so a repeated mesh skips the upload and can warm start. */
+s4f_status s4f_server_create(s4f_ctx *ctx, int64_t budgetBytes, int32_t maxSlots,
+                             const s4f_solve_opts *opts, s4f_server **out);
+/* Queues a request; S4F_ERR_ADMISSION if it can never fit the budget. */
+s4f_status s4f_server_submit(s4f_server *srv, const s4f_request *req, int64_t *id);
+/* Id of the next finished request (waits up to waitMs); S4F_ERR_NOT_READY if none. */
+s4f_status s4f_server_poll(s4f_server *srv, int32_t waitMs, int64_t *id);
+/* Copies a finished result out and releases it: D (nCells*3), sigma
+   (nCells*6), report, timing = {submit, start, finish} seconds on the host
+   monotonic clock.  Returns the solve's status (OK, WARN_MAXITER, ERR_DIVERGED). */
+s4f_status s4f_server_result(s4f_server *srv, int64_t id, double *D_out, double *sigma_out,
+                             s4f_solve_report *report, double timing[3]);
+/* queued, finished, arena bytes resident, budget, cell capacity, slots */
+s4f_status s4f_server_stats(s4f_server *srv, int64_t out[6]);
+/* Stops the server thread (in-flight requests are dropped) and frees the arena. */
+s4f_status s4f_server_destroy(s4f_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

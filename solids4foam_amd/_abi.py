@@ -71,7 +71,14 @@ EXPORTED = [
     "s4f_grad", "s4f_sigma", "s4f_residual", "s4f_spmv3", "s4f_pcg3", "s4f_matrix",
     "s4f_batch_create", "s4f_batch_add", "s4f_batch_solve", "s4f_batch_destroy",
     "s4f_profile", "s4f_profile_read",
+    "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
+    "s4f_server_destroy",
 ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_void_p), ("bcValue", ctypes.c_void_p), ("bcPressure", ctypes.c_void_p),
+                ("warmStart", ctypes.c_int32)]
 
 KERNELS = {"residual": 0, "spmv": 1, "pcg_update": 2, "finalize": 3, "grad": 4}
 
@@ -120,6 +127,13 @@ def lib():
         L.s4f_batch_destroy.argtypes = [vp]
         L.s4f_profile.argtypes = [vp, ctypes.c_int32]
         L.s4f_profile_read.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+        L.s4f_server_create.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(SolveOpts),
+                                        ctypes.POINTER(vp)]
+        L.s4f_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]
+        L.s4f_server_poll.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+        L.s4f_server_result.argtypes = [vp, ctypes.c_int64, dp, dp, ctypes.POINTER(SolveReport), dp]
+        L.s4f_server_stats.argtypes = [vp, vp]
+        L.s4f_server_destroy.argtypes = [vp]
         for name in EXPORTED:
             f = getattr(L, name)
             if name not in ("s4f_last_error", "s4f_version", "s4f_solve_opts_default"):

@@ -5,7 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -209,6 +216,7 @@ struct s4f_mesh {
     DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell;
     int nBlocks = 0, slicesPerBlock = 0;
     bool materialSet = false;
+    unsigned generation = 0;  // bumped when material data changes (serving cache key)
     double rcScale = 0.1, ifScale = 0.01;  // stabilisation scales baked into eC
     std::vector<char> bcSet;
     SolveState st;
@@ -270,6 +278,7 @@ BlockMap s4f_mesh::bmap() const {
     b.iNBlk = iNBlk.p;
     b.iCell0 = iCell0.p;
     b.iNCell = iNCell.p;
+    b.finInst = nullptr;
     b.nBlocks = nBlocks;
     return b;
 }
@@ -832,7 +841,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     if (anySigma) {
         st.sig.alloc(6 * (size_t)C);
         hipLaunchKernelGGL(k_sigma, dim3(std::min(4096, (C + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, dm, st.G.p,
-                           st.sig.p);
+                           st.sig.p, 0, C);
         HIP_CHECK(hipGetLastError());
         hSig = download(st.sig, 6 * (size_t)C, s);
     }
@@ -1132,6 +1141,7 @@ s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int3
         HIP_CHECK(hipSetDevice(M->ctx->device));
         computeMaterial(M->h, E, nu, perCell != 0);
         uploadMaterial(M);
+        M->generation++;
         M->rcScale = 0.1;
         M->ifScale = 0.01;
         M->materialSet = true;
@@ -1259,7 +1269,7 @@ s4f_status s4f_sigma(s4f_mesh* M, const double* gradD, double* sigma) {
         aosToSoa(gradD, C, 9, t);
         HIP_CHECK(hipMemcpyAsync(st.G.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_sigma, dim3(std::min(4096, (C + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, M->dm(), st.G.p,
-                           st.sig.p);
+                           st.sig.p, 0, C);
         HIP_CHECK(hipGetLastError());
         soaToAos(download(st.sig, 6 * (size_t)C, s), C, 6, sigma);
         return S4F_OK;
@@ -1501,6 +1511,686 @@ s4f_status s4f_batch_destroy(s4f_batch* b) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete b;
+    ctxChildGone(ctx);
+    return S4F_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving: continuous batching over an HBM arena
+// ===========================================================================
+namespace {
+
+double nowSeconds() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, cap) (units: cells, entries or faces).
+struct RangeAlloc {
+    std::map<long long, long long> freeMap;  // start -> length
+    long long cap = 0, used = 0;
+    void init(long long c) { cap = c; used = 0; freeMap.clear(); if (c > 0) freeMap[0] = c; }
+    long long alloc(long long n, long long align) {
+        if (n <= 0) return 0;
+        for (auto it = freeMap.begin(); it != freeMap.end(); ++it) {
+            const long long st = (it->first + align - 1) / align * align;
+            const long long end = it->first + it->second;
+            if (st + n > end) continue;
+            const long long fs = it->first, fl = it->second;
+            freeMap.erase(it);
+            if (st > fs) freeMap[fs] = st - fs;
+            if (st + n < fs + fl) freeMap[st + n] = fs + fl - (st + n);
+            used += n;
+            return st;
+        }
+        return -1;
+    }
+    void release(long long st, long long n) {
+        if (n <= 0) return;
+        used -= n;
+        auto it = freeMap.emplace(st, n).first;
+        auto nx = std::next(it);
+        if (nx != freeMap.end() && it->first + it->second == nx->first) { it->second += nx->second; freeMap.erase(nx); }
+        if (it != freeMap.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; freeMap.erase(it); }
+        }
+    }
+};
+
+struct Placement {
+    const s4f_mesh* mesh = nullptr;
+    unsigned gen = 0;
+    long long c0 = -1, nCellsPad = 0, e0 = -1, nE = 0, b0 = -1, nB = 0;
+    int nCells = 0, slice0 = 0, nSlices = 0;
+    int dpar = 0;          // buffer holding the resident field
+    bool hasField = false; // a converged field is resident (warm start possible)
+    double lastUse = 0;
+};
+
+struct Request {
+    long long id = 0;
+    s4f_mesh* mesh = nullptr;
+    std::vector<double> bVal, bPres;  // SoA 3B, B (the mesh's boundary-face order)
+    int warm = 0;
+    double tSubmit = 0, tStart = 0, tFinish = 0;
+    int slot = -1;
+    int place = -1;
+};
+
+struct Result {
+    std::vector<double> D, sigma;  // AoS
+    s4f_solve_report rep{};
+    double tSubmit = 0, tStart = 0, tFinish = 0;
+    s4f_status status = S4F_OK;
+};
+
+}  // namespace
+
+struct s4f_server {
+    s4f_ctx* ctx = nullptr;
+    s4f_solve_opts opts{};
+    int maxSlots = 0;
+    long long budget = 0;
+    long long Ccap = 0, Ecap = 0, Bcap = 0;
+    RangeAlloc cells, ents, bnds;
+    // arena: device geometry and state, laid out like one packed s4f_mesh
+    DArr<int> sliceStart, sliceWidth, col, bCell, bType;
+    DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
+    DArr<float> eA32, diag32, invDiag32;
+    DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
+    DArr<double> r64, p64, pb64, q64, x64;
+    DArr<float> r32, p32, pb32, q32, x32;
+    DArr<Ctl> ctl;
+    DArr<double> part;
+    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, finList;
+    DArr<int> nbmInst, nbmS0, nbmS1, nfinList;  // init map of newly admitted instances
+    std::vector<int> hiBlk0, hiNBlk, hiCell0, hiNCell;
+    int nTiles = 0, nActive = 0, chunk = 4;
+    Ctl* hctl = nullptr;
+    // bookkeeping (server thread only)
+    std::vector<Placement> places;
+    std::vector<int> slotReq;  // request index per slot (-1 free)
+    std::vector<Request> live;
+    // shared with clients
+    std::mutex mu_;
+    std::condition_variable cvIn, cvOut;
+    std::deque<Request> incoming;
+    std::map<long long, Result> results;
+    std::deque<long long> doneOrder;
+    long long nextId = 1;
+    bool stop = false;
+    std::string threadError;
+    std::thread th;
+    std::atomic<long long> nQueued{0}, nDone{0};
+    std::atomic<long long> bytesInUse{0};
+};
+
+namespace {
+
+constexpr double kCellBytes = 24 + 48 + 8 + 8 + 4 + 4 + 24 + 72 + 72 + 48 + 5 * 3 * 8;  // geometry + state + PCG (FP64)
+constexpr double kEntryBytes = 4 + 24 + 8 + 8 + 8 + 4;
+constexpr double kBndBytes = 4 + 4 + 24 + 24 + 8 + 24 + 24 + 8 + 24 + 24 + 24 + 72 + 48;
+
+long long footprintBytes(const HostMesh& h) {
+    const long long C = (long long)h.nSlices * kSlice;
+    return (long long)(C * kCellBytes + (double)h.col.size() * kEntryBytes + h.B * kBndBytes);
+}
+
+void serverAllocate(s4f_server* S) {
+    // split the budget over cells, ELL entries (7 per cell) and boundary faces (1 per cell)
+    const double perCell = kCellBytes + 7.0 * kEntryBytes + 1.0 * kBndBytes;
+    long long C = (long long)(S->budget / perCell);
+    C = std::max<long long>(C / kSlice * kSlice, kSlice);
+    if (C > INT32_MAX / 16) C = INT32_MAX / 16 / kSlice * kSlice;
+    S->Ccap = C;
+    S->Ecap = 7 * C;
+    S->Bcap = C;
+    S->cells.init(S->Ccap);
+    S->ents.init(S->Ecap);
+    S->bnds.init(S->Bcap);
+    const size_t Cc = (size_t)S->Ccap, Ec = (size_t)S->Ecap, Bc = (size_t)S->Bcap;
+    const size_t nS = Cc / kSlice;
+    S->sliceStart.alloc(nS + 1); S->sliceWidth.alloc(nS); S->col.alloc(Ec);
+    S->eS.alloc(3 * Ec); S->eW.alloc(Ec); S->eA.alloc(Ec); S->eC.alloc(Ec); S->eA32.alloc(Ec);
+    S->cC.alloc(3 * Cc); S->minv.alloc(6 * Cc); S->diag.alloc(Cc); S->invDiag.alloc(Cc);
+    S->diag32.alloc(Cc); S->invDiag32.alloc(Cc); S->mu.alloc(Cc); S->K.alloc(Cc); S->impK.alloc(Cc);
+    S->bCell.alloc(Bc); S->bType.alloc(Bc); S->bN.alloc(3 * Bc); S->bKv.alloc(3 * Bc); S->bDelta.alloc(Bc);
+    S->bLs.alloc(3 * Bc); S->bVal.alloc(3 * Bc); S->bPres.alloc(Bc);
+    S->D0.alloc(3 * Cc); S->D1.alloc(3 * Cc); S->Dold.alloc(3 * Cc); S->G.alloc(9 * Cc); S->sig.alloc(6 * Cc);
+    S->Db.alloc(3 * Bc); S->g.alloc(3 * Bc); S->snb.alloc(3 * Bc); S->Gb.alloc(9 * Bc); S->sigb.alloc(6 * Bc);
+    if (S->opts.precision == S4F_PRECISION_MIXED) {
+        S->r32.alloc(3 * Cc); S->p32.alloc(3 * Cc); S->pb32.alloc(3 * Cc); S->q32.alloc(3 * Cc); S->x32.alloc(3 * Cc);
+    } else {
+        S->r64.alloc(3 * Cc); S->p64.alloc(3 * Cc); S->pb64.alloc(3 * Cc); S->q64.alloc(3 * Cc); S->x64.alloc(3 * Cc);
+    }
+    S->ctl.alloc(S->maxSlots);
+    std::vector<Ctl> zero(S->maxSlots);
+    std::memset(zero.data(), 0, zero.size() * sizeof(Ctl));
+    for (Ctl& c : zero) c.outerDone = 1;
+    HIP_CHECK(hipMemcpy(S->ctl.p, zero.data(), zero.size() * sizeof(Ctl), hipMemcpyHostToDevice));
+    HIP_CHECK(hipHostMalloc((void**)&S->hctl, S->maxSlots * sizeof(Ctl), hipHostMallocDefault));
+    S->hiBlk0.assign(S->maxSlots, 0); S->hiNBlk.assign(S->maxSlots, 0);
+    S->hiCell0.assign(S->maxSlots, 0); S->hiNCell.assign(S->maxSlots, 0);
+    S->iBlk0.alloc(S->maxSlots); S->iNBlk.alloc(S->maxSlots); S->iCell0.alloc(S->maxSlots); S->iNCell.alloc(S->maxSlots);
+    S->slotReq.assign(S->maxSlots, -1);
+}
+
+DevMesh serverDm(const s4f_server* S) {
+    DevMesh m{};
+    m.nCells = (int)S->Ccap;
+    m.nBoundary = (int)S->Bcap;
+    m.nSlices = (int)(S->Ccap / kSlice);
+    m.nEntries = (size_t)S->Ecap;
+    m.sliceStart = S->sliceStart.p; m.sliceWidth = S->sliceWidth.p; m.col = S->col.p;
+    m.eS = S->eS.p; m.eW = S->eW.p; m.eA = S->eA.p; m.eC = S->eC.p; m.eA32 = S->eA32.p;
+    m.cC = S->cC.p; m.minv = S->minv.p; m.diag = S->diag.p; m.invDiag = S->invDiag.p;
+    m.diag32 = S->diag32.p; m.invDiag32 = S->invDiag32.p;
+    m.mu = S->mu.p; m.K = S->K.p; m.impK = S->impK.p;
+    m.bCell = S->bCell.p; m.bType = S->bType.p; m.bN = S->bN.p; m.bKv = S->bKv.p; m.bDelta = S->bDelta.p;
+    m.bLs = S->bLs.p; m.bVal = S->bVal.p; m.bPres = S->bPres.p;
+    m.nInst = S->maxSlots;
+    return m;
+}
+
+template <typename T>
+void hcopy(T* dst, const T* src, size_t n) {
+    if (n) HIP_CHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// writes one mesh's geometry/material into its arena ranges (offsets applied)
+void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
+    const size_t Cc = (size_t)S->Ccap, Ec = (size_t)S->Ecap, Bc = (size_t)S->Bcap;
+    const size_t E = h.col.size();
+    const int c0 = (int)pl.c0, e0 = (int)pl.e0, b0 = (int)pl.b0;
+    std::vector<int> ss(h.nSlices), col(E);
+    for (int s = 0; s < h.nSlices; ++s) ss[s] = e0 + h.sliceStart[s];
+    hcopy(S->sliceStart.p + pl.slice0, ss.data(), ss.size());
+    hcopy(S->sliceWidth.p + pl.slice0, h.sliceWidth.data(), h.sliceWidth.size());
+    for (size_t e = 0; e < E; ++e) {
+        const int c = h.col[e];
+        col[e] = c >= 0 ? c + c0 : (c == kPad ? kPad : -1 - ((-1 - c) + b0));
+    }
+    hcopy(S->col.p + e0, col.data(), E);
+    for (int q = 0; q < 3; ++q) hcopy(S->eS.p + (size_t)q * Ec + e0, h.eS.data() + (size_t)q * E, E);
+    hcopy(S->eW.p + e0, h.eW.data(), E);
+    hcopy(S->eA.p + e0, h.eA.data(), E);
+    hcopy(S->eC.p + e0, h.eC.data(), E);
+    std::vector<float> a32(E);
+    for (size_t e = 0; e < E; ++e) a32[e] = (float)h.eA[e];
+    hcopy(S->eA32.p + e0, a32.data(), E);
+    const int C = h.C, Cp = (int)pl.nCellsPad;
+    // cell arrays (padding rows: unit diagonal, zero geometry)
+    std::vector<double> t(Cp);
+    auto putCell = [&](double* base, const std::vector<double>& src, int k, int strideSrc, double pad) {
+        for (int q = 0; q < k; ++q) {
+            for (int c = 0; c < Cp; ++c) t[c] = c < C ? src[(size_t)q * strideSrc + c] : pad;
+            hcopy(base + (size_t)q * Cc + c0, t.data(), (size_t)Cp);
+        }
+    };
+    putCell(S->cC.p, h.cC, 3, C, 0.0);
+    putCell(S->minv.p, h.minv, 6, C, 0.0);
+    putCell(S->diag.p, h.diag, 1, C, 1.0);
+    std::vector<double> inv(C), muv(C), Kv(C), ikv(C);
+    std::vector<float> d32(Cp, 1.0f), i32(Cp, 1.0f);
+    for (int c = 0; c < C; ++c) {
+        inv[c] = 1.0 / h.diag[c];
+        d32[c] = (float)h.diag[c];
+        i32[c] = 1.0f / d32[c];
+        const int mc = h.uniformMaterial ? 0 : c;
+        muv[c] = h.mu[mc]; Kv[c] = h.K[mc]; ikv[c] = h.impK[mc];
+    }
+    putCell(S->invDiag.p, inv, 1, C, 1.0);
+    putCell(S->mu.p, muv, 1, C, 1.0);
+    putCell(S->K.p, Kv, 1, C, 1.0);
+    putCell(S->impK.p, ikv, 1, C, 1.0);
+    hcopy(S->diag32.p + c0, d32.data(), (size_t)Cp);
+    hcopy(S->invDiag32.p + c0, i32.data(), (size_t)Cp);
+    const int B = h.B;
+    std::vector<int> bc(B);
+    for (int b = 0; b < B; ++b) bc[b] = h.bCell[b] + c0;
+    hcopy(S->bCell.p + b0, bc.data(), (size_t)B);
+    hcopy(S->bType.p + b0, h.bType.data(), (size_t)B);
+    hcopy(S->bDelta.p + b0, h.bDelta.data(), (size_t)B);
+    for (int q = 0; q < 3; ++q) {
+        hcopy(S->bN.p + (size_t)q * Bc + b0, h.bN.data() + (size_t)q * B, (size_t)B);
+        hcopy(S->bKv.p + (size_t)q * Bc + b0, h.bKv.data() + (size_t)q * B, (size_t)B);
+        hcopy(S->bLs.p + (size_t)q * Bc + b0, h.bLs.data() + (size_t)q * B, (size_t)B);
+    }
+}
+
+void placeLoads(s4f_server* S, const Request& rq, const Placement& pl) {
+    const size_t Bc = (size_t)S->Bcap;
+    const int B = (int)pl.nB;
+    for (int q = 0; q < 3; ++q) hcopy(S->bVal.p + (size_t)q * Bc + pl.b0, rq.bVal.data() + (size_t)q * B, (size_t)B);
+    hcopy(S->bPres.p + pl.b0, rq.bPres.data(), (size_t)B);
+}
+
+void releasePlacement(s4f_server* S, int pi) {
+    Placement& pl = S->places[pi];
+    S->cells.release(pl.c0, pl.nCellsPad);
+    S->ents.release(pl.e0, pl.nE);
+    S->bnds.release(pl.b0, pl.nB);
+    S->bytesInUse -= footprintBytes(pl.mesh->h);
+    pl.mesh = nullptr;
+}
+
+bool placementBusy(const s4f_server* S, int pi) {
+    for (const Request& r : S->live)
+        if (r.place == pi) return true;
+    return false;
+}
+
+// finds or creates the placement of a request's mesh; -1 when it does not fit
+int acquirePlacement(s4f_server* S, const Request& rq) {
+    const HostMesh& h = rq.mesh->h;
+    for (int i = 0; i < (int)S->places.size(); ++i) {
+        Placement& p = S->places[i];
+        if (p.mesh == rq.mesh && p.gen == rq.mesh->generation && !placementBusy(S, i)) return i;
+    }
+    Placement pl;
+    pl.mesh = rq.mesh;
+    pl.gen = rq.mesh->generation;
+    pl.nCells = h.C;
+    pl.nSlices = h.nSlices;
+    pl.nCellsPad = (long long)h.nSlices * kSlice;
+    pl.nE = (long long)h.col.size();
+    pl.nB = h.B;
+    for (;;) {
+        const long long c0 = S->cells.alloc(pl.nCellsPad, kSlice);
+        const long long e0 = c0 >= 0 ? S->ents.alloc(pl.nE, kSlice) : -1;
+        const long long b0 = e0 >= 0 ? S->bnds.alloc(pl.nB, 1) : -1;
+        if (b0 >= 0) {
+            pl.c0 = c0; pl.e0 = e0; pl.b0 = b0;
+            pl.slice0 = (int)(c0 / kSlice);
+            break;
+        }
+        if (e0 >= 0) S->ents.release(e0, pl.nE);
+        if (c0 >= 0) S->cells.release(c0, pl.nCellsPad);
+        // evict the least recently used resident mesh that no solve is using
+        int victim = -1;
+        for (int i = 0; i < (int)S->places.size(); ++i) {
+            const Placement& p = S->places[i];
+            if (!p.mesh || placementBusy(S, i)) continue;
+            if (victim < 0 || p.lastUse < S->places[victim].lastUse) victim = i;
+        }
+        if (victim < 0) return -1;
+        releasePlacement(S, victim);
+    }
+    placeMesh(S, h, pl);
+    S->bytesInUse += footprintBytes(h);
+    for (int i = 0; i < (int)S->places.size(); ++i)
+        if (!S->places[i].mesh) { S->places[i] = pl; return i; }
+    S->places.push_back(pl);
+    return (int)S->places.size() - 1;
+}
+
+// tile list of the given slots (the same per-instance rule as a plain solve)
+void buildTiles(const s4f_server* S, const std::vector<int>& slots, std::vector<int>& inst, std::vector<int>& s0,
+                std::vector<int>& s1, std::vector<int>& blk0, std::vector<int>& nblk) {
+    inst.clear(); s0.clear(); s1.clear();
+    for (int slot : slots) {
+        const Request& rq = S->live[S->slotReq[slot]];
+        const Placement& pl = S->places[rq.place];
+        const int a = pl.slice0, nS = pl.nSlices, e = a + nS;
+        const int spbI = std::max(4, (nS + gMaxBlocks - 1) / gMaxBlocks);
+        blk0[slot] = (int)inst.size();
+        for (int x = a; x < e; x += spbI) {
+            inst.push_back(slot);
+            s0.push_back(x);
+            s1.push_back(std::min(x + spbI, e));
+        }
+        nblk[slot] = (int)inst.size() - blk0[slot];
+    }
+}
+
+template <typename T>
+PcgVecs<T> serverVecs(s4f_server* S);
+template <>
+PcgVecs<double> serverVecs<double>(s4f_server* S) { return {S->r64.p, S->p64.p, S->q64.p, S->x64.p, S->pb64.p}; }
+template <>
+PcgVecs<float> serverVecs<float>(s4f_server* S) { return {S->r32.p, S->p32.p, S->q32.p, S->x32.p, S->pb32.p}; }
+
+BlockMap makeMap(const s4f_server* S, const int* inst, const int* s0, const int* s1, const int* fin, int nBlocks) {
+    BlockMap b{};
+    b.inst = inst; b.s0 = s0; b.s1 = s1;
+    b.iBlk0 = S->iBlk0.p; b.iNBlk = S->iNBlk.p; b.iCell0 = S->iCell0.p; b.iNCell = S->iNCell.p;
+    b.finInst = fin;
+    b.nBlocks = nBlocks;
+    return b;
+}
+
+// admits queued requests in arrival order while they fit
+template <typename T>
+void serverAdmit(s4f_server* S) {
+    std::vector<int> newSlots;
+    for (;;) {
+        Request rq;
+        {
+            std::lock_guard<std::mutex> lk(S->mu_);
+            if (S->incoming.empty()) break;
+            rq = S->incoming.front();
+        }
+        int slot = -1;
+        for (int i = 0; i < S->maxSlots; ++i)
+            if (S->slotReq[i] < 0) { slot = i; break; }
+        if (slot < 0) break;
+        const int pi = acquirePlacement(S, rq);
+        if (pi < 0) break;  // head of the queue waits for memory (arrival order is kept)
+        {
+            std::lock_guard<std::mutex> lk(S->mu_);
+            S->incoming.pop_front();
+        }
+        S->nQueued--;
+        Placement& pl = S->places[pi];
+        rq.place = pi;
+        rq.slot = slot;
+        rq.tStart = nowSeconds();
+        const int warm = (rq.warm && pl.hasField) ? 1 : 0;
+        placeLoads(S, rq, pl);
+        Ctl c = makeCtl(S->opts, 0);
+        c.dpar = warm ? pl.dpar : 0;
+        c.warm = warm;
+        HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+        HIP_CHECK(hipMemcpy(S->ctl.p + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+        const int n = (int)pl.nCells;
+        hipLaunchKernelGGL(k_slot_init, dim3(std::max(1, std::min(2048, (n + kBlock - 1) / kBlock))), dim3(kBlock), 0,
+                           S->ctx->stream, S->D0.p, S->D1.p, S->Dold.p, S->G.p, S->g.p, (int)S->Ccap, (int)S->Bcap,
+                           (int)pl.c0, n, (int)pl.b0, (int)pl.nB, warm, c.dpar);
+        HIP_CHECK(hipGetLastError());
+        S->hiCell0[slot] = (int)pl.c0;
+        S->hiNCell[slot] = n;
+        pl.hasField = false;
+        S->live.push_back(rq);
+        S->slotReq[slot] = (int)S->live.size() - 1;
+        newSlots.push_back(slot);
+    }
+    if (newSlots.empty()) return;
+    // initialisation (constructor grad) of the new instances only
+    std::vector<int> inst, s0, s1;
+    buildTiles(S, newSlots, inst, s0, s1, S->hiBlk0, S->hiNBlk);
+    hipStream_t s = S->ctx->stream;
+    S->nbmInst.upload(inst, s); S->nbmS0.upload(s0, s); S->nbmS1.upload(s1, s); S->nfinList.upload(newSlots, s);
+    S->iBlk0.upload(S->hiBlk0, s); S->iNBlk.upload(S->hiNBlk, s);
+    S->iCell0.upload(S->hiCell0, s); S->iNCell.upload(S->hiNCell, s);
+    S->part.alloc(std::max<size_t>(S->part.n, inst.size() * kMaxPartials));
+    const BlockMap bm = makeMap(S, S->nbmInst.p, S->nbmS0.p, S->nbmS1.p, S->nfinList.p, (int)inst.size());
+    const DevMesh dm = serverDm(S);
+    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, S->D0.p, (const T*)nullptr, S->D1.p,
+                       S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p, (const double*)S->Dold.p, S->ctl.p,
+                       S->part.p, (double*)nullptr, 1);
+    hipLaunchKernelGGL(k_fin_grad, dim3((int)newSlots.size()), dim3(kFinBlock), 0, s, bm, S->ctl.p,
+                       (const double*)S->part.p, (double*)nullptr, 1);
+    HIP_CHECK(hipGetLastError());
+    S->nTiles = -1;  // active map must be rebuilt
+}
+
+void serverRebuildMap(s4f_server* S) {
+    std::vector<int> slots;
+    for (int i = 0; i < S->maxSlots; ++i)
+        if (S->slotReq[i] >= 0) slots.push_back(i);
+    std::vector<int> inst, s0, s1;
+    buildTiles(S, slots, inst, s0, s1, S->hiBlk0, S->hiNBlk);
+    hipStream_t s = S->ctx->stream;
+    S->bmInst.upload(inst, s); S->bmS0.upload(s0, s); S->bmS1.upload(s1, s); S->finList.upload(slots, s);
+    S->iBlk0.upload(S->hiBlk0, s); S->iNBlk.upload(S->hiNBlk, s);
+    S->part.alloc(std::max<size_t>(S->part.n, inst.size() * kMaxPartials));
+    S->nTiles = (int)inst.size();
+    S->nActive = (int)slots.size();
+}
+
+// one outer iteration of every in-flight solve, then retire the converged
+template <typename T>
+void serverStep(s4f_server* S) {
+    hipStream_t s = S->ctx->stream;
+    if (S->nTiles < 0) serverRebuildMap(S);
+    if (S->nActive == 0) return;
+    const DevMesh dm = serverDm(S);
+    const BlockMap bm = makeMap(S, S->bmInst.p, S->bmS0.p, S->bmS1.p, S->finList.p, S->nTiles);
+    PcgVecs<T> v = serverVecs<T>(S);
+    const int nA = S->nActive;
+    std::vector<int> slots;
+    for (int i = 0; i < S->maxSlots; ++i)
+        if (S->slotReq[i] >= 0) slots.push_back(i);
+    hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)S->D0.p,
+                       (const double*)S->D1.p, S->G.p, S->sigb.p, v.r, S->ctl.p, S->part.p);
+    hipLaunchKernelGGL(k_fin_residual, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p);
+    int total = 0, chunk = S->chunk;
+    for (;;) {
+        launchPcgChunk<T>(dm, bm, nA, v, S->ctl.p, S->part.p, chunk, s);
+        total += chunk;
+        chunk = std::max(2, chunk / 4);
+        HIP_CHECK(hipMemcpyAsync(S->hctl, S->ctl.p, S->maxSlots * sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        bool any = false;
+        for (int sl : slots)
+            if (!S->hctl[sl].outerDone && !S->hctl[sl].innerDone) any = true;
+        if (!any || total > S->opts.pcgMaxIter + 2) break;
+    }
+    int used = 1;
+    for (int sl : slots)
+        if (!S->hctl[sl].outerDone) used = std::max({used, S->hctl[sl].nIter[0], S->hctl[sl].nIter[1], S->hctl[sl].nIter[2]});
+    S->chunk = std::max(1, used - 1);
+    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, S->D0.p, (const T*)v.x, S->D1.p, S->G.p,
+                       S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p, (const double*)S->Dold.p, S->ctl.p, S->part.p,
+                       (double*)nullptr, 0);
+    hipLaunchKernelGGL(k_fin_grad, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p,
+                       (double*)nullptr, 0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(S->hctl, S->ctl.p, S->maxSlots * sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    // retire
+    bool changed = false;
+    for (int sl : slots) {
+        const Ctl& cf = S->hctl[sl];
+        if (!cf.outerDone) continue;
+        Request& rq = S->live[S->slotReq[sl]];
+        Placement& pl = S->places[rq.place];
+        const int n = pl.nCells;
+        hipLaunchKernelGGL(k_sigma, dim3(std::max(1, std::min(2048, (n + kBlock - 1) / kBlock))), dim3(kBlock), 0, s, dm,
+                           S->G.p, S->sig.p, (int)pl.c0, n);
+        HIP_CHECK(hipGetLastError());
+        Result res;
+        res.D.resize(3 * (size_t)n);
+        res.sigma.resize(6 * (size_t)n);
+        std::vector<double> tmp(n);
+        const double* Df = cf.dpar ? S->D1.p : S->D0.p;
+        for (int q = 0; q < 3; ++q) {
+            HIP_CHECK(hipMemcpyAsync(tmp.data(), Df + (size_t)q * S->Ccap + pl.c0, n * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int c = 0; c < n; ++c) res.D[3 * (size_t)c + q] = tmp[c];
+        }
+        for (int q = 0; q < 6; ++q) {
+            HIP_CHECK(hipMemcpyAsync(tmp.data(), S->sig.p + (size_t)q * S->Ccap + pl.c0, n * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int c = 0; c < n; ++c) res.sigma[6 * (size_t)c + q] = tmp[c];
+        }
+        res.rep.nOuter = cf.nOuter;
+        res.rep.converged = cf.converged;
+        res.rep.innerTotal = cf.innerTotal;
+        res.rep.initialResidual = cf.initResMag;
+        res.rep.residualvf = cf.rvf;
+        res.status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
+        res.tSubmit = rq.tSubmit;
+        res.tStart = rq.tStart;
+        res.tFinish = nowSeconds();
+        res.rep.seconds = res.tFinish - res.tStart;
+        pl.dpar = cf.dpar;
+        pl.hasField = cf.converged != 0;
+        pl.lastUse = res.tFinish;
+        const long long id = rq.id;
+        rq.place = -1;
+        S->slotReq[sl] = -1;
+        changed = true;
+        {
+            std::lock_guard<std::mutex> lk(S->mu_);
+            S->results[id] = std::move(res);
+            S->doneOrder.push_back(id);
+        }
+        S->nDone++;
+        S->cvOut.notify_all();
+    }
+    if (changed) {
+        // compact the live list (slot -> request indices)
+        std::vector<Request> keep;
+        for (int i = 0; i < S->maxSlots; ++i)
+            if (S->slotReq[i] >= 0) {
+                keep.push_back(S->live[S->slotReq[i]]);
+                S->slotReq[i] = (int)keep.size() - 1;
+            }
+        S->live.swap(keep);
+        S->nTiles = -1;
+    }
+}
+
+template <typename T>
+void serverLoop(s4f_server* S) {
+    try {
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(S->mu_);
+                if (S->stop) break;
+                if (S->live.empty() && S->incoming.empty())
+                    S->cvIn.wait_for(lk, std::chrono::milliseconds(20));
+                if (S->stop) break;
+            }
+            serverAdmit<T>(S);
+            if (S->live.empty()) continue;
+            serverStep<T>(S);
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(S->mu_);
+        S->threadError = e.what();
+        S->stop = true;
+        S->cvOut.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+s4f_status s4f_server_create(s4f_ctx* ctx, int64_t budgetBytes, int32_t maxSlots, const s4f_solve_opts* opts,
+                             s4f_server** out) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx || !opts || !out) throw InvalidError("null argument");
+        if (ctx->destroyPending) throw InvalidError("context was destroyed");
+        if (budgetBytes < (1 << 20)) throw InvalidError("budget must be at least 1 MiB");
+        if (maxSlots < 1 || maxSlots > 4096) throw InvalidError("maxSlots must be in 1..4096");
+        checkOpts(*opts);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto S = std::make_unique<s4f_server>();
+        S->ctx = ctx;
+        S->opts = *opts;
+        S->opts.warmStart = 0;
+        S->maxSlots = maxSlots;
+        S->budget = budgetBytes;
+        serverAllocate(S.get());
+        s4f_server* raw = S.get();
+        if (S->opts.precision == S4F_PRECISION_MIXED) S->th = std::thread(serverLoop<float>, raw);
+        else S->th = std::thread(serverLoop<double>, raw);
+        ctx->children++;
+        *out = S.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_server_submit(s4f_server* S, const s4f_request* rq, int64_t* id) {
+    return guarded([&]() -> s4f_status {
+        if (!S || !rq || !rq->mesh || !rq->bcValue) throw InvalidError("null argument");
+        s4f_mesh* m = rq->mesh;
+        if (!m->materialSet) throw InvalidError("material not set");
+        if (m->ctx != S->ctx) throw InvalidError("mesh belongs to another context");
+        if (m->h.instCell0.size() != 1) throw InvalidError("requests take plain meshes");
+        const HostMesh& h = m->h;
+        if ((long long)h.nSlices * kSlice > S->Ccap || (long long)h.col.size() > S->Ecap || h.B > S->Bcap)
+            return fail(S4F_ERR_ADMISSION, "request footprint exceeds the server's HBM budget");
+        Request r;
+        r.mesh = m;
+        r.warm = rq->warmStart;
+        r.bVal.resize(3 * (size_t)h.B);
+        r.bPres.assign(h.B, 0.0);
+        for (int b = 0; b < h.B; ++b) {
+            for (int q = 0; q < 3; ++q) {
+                const double v = rq->bcValue[3 * (size_t)b + q];
+                if (!std::isfinite(v)) throw InvalidError("non-finite boundary value");
+                r.bVal[(size_t)q * h.B + b] = v;
+            }
+            if (rq->bcPressure) r.bPres[b] = rq->bcPressure[b];
+        }
+        r.tSubmit = nowSeconds();
+        {
+            std::lock_guard<std::mutex> lk(S->mu_);
+            if (S->stop) return fail(S4F_ERR_HIP, S->threadError.empty() ? "server stopped" : S->threadError);
+            r.id = S->nextId++;
+            if (id) *id = r.id;
+            S->incoming.push_back(std::move(r));
+        }
+        S->nQueued++;
+        S->cvIn.notify_all();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_server_poll(s4f_server* S, int32_t waitMs, int64_t* id) {
+    if (!S || !id) return fail(S4F_ERR_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(S->mu_);
+    if (S->doneOrder.empty() && waitMs > 0)
+        S->cvOut.wait_for(lk, std::chrono::milliseconds(waitMs), [&] { return !S->doneOrder.empty() || S->stop; });
+    if (!S->threadError.empty()) return fail(S4F_ERR_HIP, S->threadError);
+    if (S->doneOrder.empty()) return fail(S4F_ERR_NOT_READY, "no finished request");
+    *id = S->doneOrder.front();
+    S->doneOrder.pop_front();
+    return S4F_OK;
+}
+
+s4f_status s4f_server_result(s4f_server* S, int64_t id, double* D_out, double* sigma_out, s4f_solve_report* rep,
+                             double timing[3]) {
+    if (!S) return fail(S4F_ERR_INVALID, "null argument");
+    Result r;
+    {
+        std::lock_guard<std::mutex> lk(S->mu_);
+        auto it = S->results.find(id);
+        if (it == S->results.end()) return fail(S4F_ERR_NOT_READY, "no such finished request");
+        r = std::move(it->second);
+        S->results.erase(it);
+    }
+    if (D_out) std::memcpy(D_out, r.D.data(), r.D.size() * sizeof(double));
+    if (sigma_out) std::memcpy(sigma_out, r.sigma.data(), r.sigma.size() * sizeof(double));
+    if (rep) *rep = r.rep;
+    if (timing) { timing[0] = r.tSubmit; timing[1] = r.tStart; timing[2] = r.tFinish; }
+    return r.status;
+}
+
+s4f_status s4f_server_stats(s4f_server* S, int64_t out[6]) {
+    if (!S || !out) return fail(S4F_ERR_INVALID, "null argument");
+    out[0] = S->nQueued.load();
+    out[1] = S->nDone.load();
+    out[2] = S->bytesInUse.load();
+    out[3] = S->budget;
+    out[4] = S->Ccap;
+    out[5] = S->maxSlots;
+    return S4F_OK;
+}
+
+s4f_status s4f_server_destroy(s4f_server* S) {
+    if (!S) return S4F_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu_);
+        S->stop = true;
+    }
+    S->cvIn.notify_all();
+    if (S->th.joinable()) S->th.join();
+    s4f_ctx* ctx = S->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (S->hctl) (void)hipHostFree(S->hctl);
+    delete S;
     ctxChildGone(ctx);
     return S4F_OK;
 }

@@ -38,6 +38,7 @@ struct BlockMap {
     const int* iNBlk;  // per instance: number of blocks
     const int* iCell0; // per instance: first cell
     const int* iNCell; // per instance: number of cells
+    const int* finInst;// instance finalized by finalize-kernel workgroup i (nullptr: i itself)
     int nBlocks;
 };
 
@@ -535,9 +536,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
 
 // sigma = Hooke(G) for output (xx xy xz yy yz zz, SoA)
 __global__ __launch_bounds__(kBlock) void k_sigma(const DevMesh m, const double* __restrict__ G,
-                                                  double* __restrict__ sig) {
+                                                  double* __restrict__ sig, int c0, int n) {
     const int C = m.nCells;
-    for (int P = blockIdx.x * kBlock + threadIdx.x; P < C; P += gridDim.x * kBlock) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const int P = c0 + i;
         double GP[9], s[6];
 #pragma unroll
         for (int q = 0; q < 9; ++q) GP[q] = G[(size_t)q * C + P];
@@ -545,6 +547,41 @@ __global__ __launch_bounds__(kBlock) void k_sigma(const DevMesh m, const double*
 #pragma unroll
         for (int q = 0; q < 6; ++q) sig[(size_t)q * C + P] = s[q];
     }
+}
+
+// von Mises equivalent stress sqrt(3/2 |dev sigma|^2) (solidModel.C:1957-1961)
+__global__ __launch_bounds__(kBlock) void k_sigma_eq(const double* __restrict__ sig, double* __restrict__ eq, int n) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double xx = sig[i], xy = sig[(size_t)n + i], xz = sig[2 * (size_t)n + i];
+        const double yy = sig[3 * (size_t)n + i], yz = sig[4 * (size_t)n + i], zz = sig[5 * (size_t)n + i];
+        const double t3 = (xx + yy + zz) / 3.0;
+        const double a = xx - t3, b = yy - t3, c = zz - t3;
+        eq[i] = sqrt(1.5 * (a * a + b * b + c * c + 2.0 * (xy * xy + xz * xz + yz * yz)));
+    }
+}
+
+// serving: (re)initialise one instance's state ranges; cold start zeroes the
+// displacement, warm start keeps the resident field of buffer `dpar` and makes
+// it the old-time field
+__global__ __launch_bounds__(kBlock) void k_slot_init(double* __restrict__ D0, double* __restrict__ D1,
+                                                      double* __restrict__ Dold, double* __restrict__ G,
+                                                      double* __restrict__ g, int C, int B, int c0, int n, int b0,
+                                                      int nb, int warm, int dpar) {
+    double* D = dpar ? D1 : D0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const int P = c0 + i;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const size_t k = (size_t)q * C + P;
+            if (!warm) D[k] = 0.0;
+            Dold[k] = D[k];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) G[(size_t)q * C + P] = 0.0;
+    }
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += gridDim.x * kBlock)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) g[(size_t)q * B + b0 + i] = 0.0;
 }
 
 // PCG start from a given right-hand side b (for the standalone PCG entry):
@@ -586,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
 // initial residual, convergence check before the first iteration, z.r
 __global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, Ctl* __restrict__ ctlAll,
                                                             const double* __restrict__ part) {
-    const int inst = blockIdx.x;
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone) return;
     __shared__ double tot[12];
@@ -618,7 +655,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, C
 // PCG start from a caller-supplied right-hand side (normFactor given)
 __global__ __launch_bounds__(kFinBlock) void k_fin_pcg_start(const BlockMap bm, Ctl* __restrict__ ctlAll,
                                                              const double* __restrict__ part) {
-    const int inst = blockIdx.x;
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     __shared__ double tot[6];
     finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
@@ -641,7 +678,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_pcg_start(const BlockMap bm, 
 // alpha = rho / p.q, with PCG's singularity test |p.q|/normFactor < VSMALL
 __global__ __launch_bounds__(kFinBlock) void k_fin_spmv(const BlockMap bm, Ctl* __restrict__ ctlAll,
                                                         const double* __restrict__ part) {
-    const int inst = blockIdx.x;
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     __shared__ double tot[3];
@@ -665,7 +702,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_spmv(const BlockMap bm, Ctl* 
 // PCG convergence per component (tolerance, relTol, maxIter) and beta
 __global__ __launch_bounds__(kFinBlock) void k_fin_update(const BlockMap bm, Ctl* __restrict__ ctlAll,
                                                           const double* __restrict__ part) {
-    const int inst = blockIdx.x;
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     __shared__ double tot[6];
@@ -695,7 +732,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_update(const BlockMap bm, Ctl
 __global__ __launch_bounds__(kFinBlock) void k_fin_grad(const BlockMap bm, Ctl* __restrict__ ctlAll,
                                                         const double* __restrict__ part, double* __restrict__ hist,
                                                         int mode) {
-    const int inst = blockIdx.x;
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     if (mode == 0 && ctl->outerDone) return;
     __shared__ double tot[6];
