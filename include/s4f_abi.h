@@ -249,6 +249,27 @@ s4f_status s4f_batch_solve(s4f_batch *batch, const s4f_solve_opts *opts, double 
 s4f_status s4f_batch_destroy(s4f_batch *batch);
 
 /* ------------------------------------------------------------------ */
+/* lduMatrix::solver seam: PCG on a caller's symmetric LDU matrix      */
+/* (replaces OpenFOAM's PCG selected by fvSolution "solver", called    */
+/* per component from fvMatrix::solveSegregated,                       */
+/* linGeomTotalDispSolid.C:164)                                        */
+/* ------------------------------------------------------------------ */
+typedef struct s4f_ldu s4f_ldu;
+/* Addressing of an lduMatrix: lowerAddr (owner) and upperAddr (neighbour) of
+   its nFaces off-diagonal pairs; builds the device sliced-ELL structure once. */
+s4f_status s4f_ldu_create(s4f_ctx *ctx, int32_t nCells, int32_t nFaces, const int32_t *lowerAddr,
+                          const int32_t *upperAddr, s4f_ldu **out);
+/* PCG with the diagonal preconditioner on (diag, upper) for one component,
+   from psi (updated in place), with OpenFOAM's normalised residual
+   (lduMatrix::solver::normFactor) and convergence test (tolerance, relTol,
+   maxIter).  Returns the initial and final normalised residuals and the
+   iteration count, as SolverPerformance reports them. */
+s4f_status s4f_ldu_solve(s4f_ldu *ldu, const double *diag, const double *upper, const double *source,
+                         double *psi, double tolerance, double relTol, int32_t maxIter,
+                         int32_t precision, double *initRes, double *finalRes, int32_t *nIter);
+s4f_status s4f_ldu_destroy(s4f_ldu *ldu);
+
+/* ------------------------------------------------------------------ */
 /* serving: continuous batching of solve requests (no counterpart in   */
 /* the reference, which runs one case per process)                     */
 /* ------------------------------------------------------------------ */

@@ -72,7 +72,7 @@ EXPORTED = [
     "s4f_batch_create", "s4f_batch_add", "s4f_batch_solve", "s4f_batch_destroy",
     "s4f_profile", "s4f_profile_read",
     "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
-    "s4f_server_destroy",
+    "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
 ]
 
 
@@ -134,6 +134,10 @@ def lib():
         L.s4f_server_result.argtypes = [vp, ctypes.c_int64, dp, dp, ctypes.POINTER(SolveReport), dp]
         L.s4f_server_stats.argtypes = [vp, vp]
         L.s4f_server_destroy.argtypes = [vp]
+        L.s4f_ldu_create.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ip, ip, ctypes.POINTER(vp)]
+        L.s4f_ldu_solve.argtypes = [vp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                    ctypes.c_int32, vp, vp, vp]
+        L.s4f_ldu_destroy.argtypes = [vp]
         for name in EXPORTED:
             f = getattr(L, name)
             if name not in ("s4f_last_error", "s4f_version", "s4f_solve_opts_default"):

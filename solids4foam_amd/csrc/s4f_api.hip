@@ -2196,3 +2196,160 @@ s4f_status s4f_server_destroy(s4f_server* S) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// lduMatrix::solver entry: PCG on a caller's LDU matrix (OpenFOAM addressing)
+// ===========================================================================
+struct s4f_ldu {
+    s4f_ctx* ctx = nullptr;
+    std::unique_ptr<s4f_mesh> M;   // ELL structure + PCG state (no geometry)
+    std::vector<int> faceEntOwn, faceEntNei;
+};
+
+extern "C" {
+
+s4f_status s4f_ldu_create(s4f_ctx* ctx, int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
+                          const int32_t* upperAddr, s4f_ldu** out) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx || !out || nCells <= 0 || nFaces < 0 || (nFaces && (!lowerAddr || !upperAddr)))
+            throw InvalidError("invalid LDU addressing");
+        if (ctx->destroyPending) throw InvalidError("context was destroyed");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto L = std::make_unique<s4f_ldu>();
+        L->ctx = ctx;
+        L->M = std::make_unique<s4f_mesh>();
+        s4f_mesh* M = L->M.get();
+        M->ctx = ctx;
+        HostMesh& h = M->h;
+        h.C = nCells; h.F = nFaces; h.B = 0; h.nPatches = 0;
+        std::vector<int> cnt(nCells, 0);
+        for (int f = 0; f < nFaces; ++f) {
+            const int l = lowerAddr[f], u = upperAddr[f];
+            if (l < 0 || u < 0 || l >= nCells || u >= nCells || l == u) throw InvalidError("LDU address out of range");
+            cnt[l]++; cnt[u]++;
+        }
+        h.nSlices = (nCells + kSlice - 1) / kSlice;
+        h.sliceStart.resize((size_t)h.nSlices + 1);
+        h.sliceWidth.resize(h.nSlices);
+        size_t E = 0;
+        for (int sI = 0; sI < h.nSlices; ++sI) {
+            h.sliceStart[sI] = (int)E;
+            int wmax = 0;
+            for (int l = 0; l < kSlice; ++l) {
+                const int c = sI * kSlice + l;
+                if (c < nCells) wmax = std::max(wmax, cnt[c]);
+            }
+            h.sliceWidth[sI] = wmax;
+            E += (size_t)wmax * kSlice;
+        }
+        h.sliceStart[h.nSlices] = (int)E;
+        h.col.assign(E, kPad);
+        h.eS.assign(3 * E, 0.0); h.eW.assign(E, 0.0);
+        std::vector<int> fill(nCells, 0);
+        auto slot = [&](int c) { return h.sliceStart[c / kSlice] + fill[c]++ * kSlice + c % kSlice; };
+        L->faceEntOwn.resize(nFaces); L->faceEntNei.resize(nFaces);
+        for (int f = 0; f < nFaces; ++f) {
+            const int eo = slot(lowerAddr[f]);
+            h.col[eo] = upperAddr[f];
+            L->faceEntOwn[f] = eo;
+            const int en = slot(upperAddr[f]);
+            h.col[en] = lowerAddr[f];
+            L->faceEntNei[f] = en;
+        }
+        h.cC.assign(3 * (size_t)nCells, 0.0);
+        h.minv.assign(6 * (size_t)nCells, 0.0);
+        h.bVal.clear(); h.bPres.clear();
+        h.instCell0 = {0}; h.instNCell = {nCells}; h.instB0 = {0}; h.instNB = {0};
+        h.mu = {1.0}; h.K = {1.0}; h.impK = {1.0};
+        uploadGeometry(M);
+        ctx->children++;
+        *out = L.release();
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_ldu_solve(s4f_ldu* L, const double* diag, const double* upper, const double* source, double* psi,
+                         double tolerance, double relTol, int32_t maxIter, int32_t precision, double* initRes,
+                         double* finalRes, int32_t* nIter) {
+    return guarded([&]() -> s4f_status {
+        if (!L || !diag || !upper || !source || !psi) throw InvalidError("null argument");
+        s4f_mesh* M = L->M.get();
+        HostMesh& h = M->h;
+        hipStream_t s = L->ctx->stream;
+        HIP_CHECK(hipSetDevice(L->ctx->device));
+        const int C = h.C;
+        // off-diagonal entries of a symmetric lduMatrix: -eA = upper
+        h.eA.assign(h.col.size(), 0.0);
+        for (int f = 0; f < h.F; ++f) {
+            h.eA[L->faceEntOwn[f]] = -upper[f];
+            h.eA[L->faceEntNei[f]] = -upper[f];
+        }
+        h.diag.assign(diag, diag + C);
+        h.eC.assign(h.col.size(), 0.0);
+        for (int c = 0; c < C; ++c)
+            if (!(h.diag[c] != 0.0)) throw InvalidError("zero diagonal");
+        uploadMaterial(M);
+        M->materialSet = true;
+        allocState(M);
+        SolveState& st = M->st;
+        double avg = 0.0;
+        for (int c = 0; c < C; ++c) avg += psi[c];
+        avg /= C;
+        DArr<double> dpsi, dsrc;
+        dpsi.upload(std::vector<double>(psi, psi + C), s);
+        dsrc.upload(std::vector<double>(source, source + C), s);
+        s4f_solve_opts o;
+        s4f_solve_opts_default(&o);
+        o.pcgTolerance = tolerance;
+        o.pcgRelTol = relTol;
+        o.pcgMaxIter = maxIter;
+        o.precision = precision;
+        checkOpts(o);
+        Ctl c0 = makeCtl(o, 0);
+        HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        Ctl hc;
+        std::vector<double> x(C, 0.0);
+        auto run = [&](auto v, auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL((k_ldu_start<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)dpsi.p,
+                               (const double*)dsrc.p, avg, v.r, st.part.p);
+            hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
+            HIP_CHECK(hipMemsetAsync(v.x, 0, 3 * (size_t)C * sizeof(T), s));
+            HIP_CHECK(hipGetLastError());
+            int total = 0, chunk = 8;
+            for (;;) {
+                HIP_CHECK(hipMemcpyAsync(&hc, st.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                if (hc.innerDone || total > maxIter + 8) break;
+                launchPcgChunk<T>(dm, bm, 1, v, st.ctl.p, st.part.p, chunk, s);
+                total += chunk;
+            }
+            std::vector<T> xs(C);
+            HIP_CHECK(hipMemcpyAsync(xs.data(), v.x, C * sizeof(T), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (!hc.firstInner)
+                for (int c = 0; c < C; ++c) x[c] = (double)xs[c];
+        };
+        if (precision == S4F_PRECISION_MIXED) run(pcgVecs<float>(st, C), 0.0f);
+        else run(pcgVecs<double>(st, C), 0.0);
+        for (int c = 0; c < C; ++c) psi[c] += x[c];
+        if (initRes) *initRes = hc.initRes[0];
+        if (finalRes) *finalRes = hc.finalRes[0];
+        if (nIter) *nIter = hc.nIter[0];
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_ldu_destroy(s4f_ldu* L) {
+    if (!L) return S4F_OK;
+    s4f_ctx* ctx = L->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete L;
+    ctxChildGone(ctx);
+    return S4F_OK;
+}
+
+}  // extern "C"

@@ -615,6 +615,54 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
     storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
+// lduMatrix::solver start for one component (component 0) of a caller's LDU
+// matrix: r = source - A psi, and the partials of sum|r| and of OpenFOAM's
+// normFactor, sum |A psi - rowSum avg(psi)| + |source - rowSum avg(psi)|,
+// laid out as k_fin_residual reads them (components 1 and 2 zero)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ldu_start(const DevMesh m, const BlockMap bm, const double* __restrict__ psi,
+                                                      const double* __restrict__ src, double avg, T* __restrict__ r,
+                                                      double* __restrict__ part) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const int C = m.nCells;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    double acc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) acc[i] = 0.0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        const int e0 = m.sliceStart[s];
+        const int width = m.sliceWidth[s];
+        if (P >= cEnd) continue;
+        const double dg = m.diag[P];
+        double Ap = dg * psi[P], asum = 0.0;
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int n = m.col[e];
+            if (n >= 0) {
+                const double a = m.eA[e];
+                Ap -= a * psi[n];
+                asum += a;
+            }
+        }
+        const double t = (dg - asum) * avg;
+        const double rv = src[P] - Ap;
+        const T rT = (T)rv;
+        r[P] = rT;
+        r[(size_t)C + P] = (T)0;
+        r[2 * (size_t)C + P] = (T)0;
+        acc[0] += fabs(rv);
+        acc[3] += fabs(Ap - t);
+        acc[6] += fabs(src[P] - t);
+        acc[9] += (double)(invD[P] * rT) * (double)rT;
+    }
+    storeBlockPartials<12>(acc, 0u, part + (size_t)lb * kMaxPartials);
+}
+
 // ===========================================================================
 // finalize kernels: one workgroup per instance (grid = number of instances)
 // ===========================================================================

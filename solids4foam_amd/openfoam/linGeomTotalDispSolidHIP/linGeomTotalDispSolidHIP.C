@@ -1,0 +1,294 @@
+/*---------------------------------------------------------------------------*\
+    linGeomTotalDispSolidHIP: see the header.  Compiled with wmake against an
+    OpenFOAM + solids4foam installation (Make/options), linking libs4f.
+\*---------------------------------------------------------------------------*/
+#include "linGeomTotalDispSolidHIP.H"
+#include "addToRunTimeSelectionTable.H"
+#include "fvc.H"
+#include "linearElastic.H"
+#include "solidTractionFvPatchVectorField.H"
+#include "fixedDisplacementFvPatchVectorField.H"
+
+#include <vector>
+
+namespace Foam
+{
+namespace solidModels
+{
+
+defineTypeNameAndDebug(linGeomTotalDispSolidHIP, 0);
+addToRunTimeSelectionTable(solidModel, linGeomTotalDispSolidHIP, dictionary);
+
+
+void linGeomTotalDispSolidHIP::check(s4f_status st, const char* what) const
+{
+    if (st != S4F_OK && st != S4F_WARN_MAXITER)
+    {
+        FatalErrorIn(type() + "::" + what)
+            << "libs4f: " << s4f_last_error() << abort(FatalError);
+    }
+}
+
+
+// flat owner/neighbour/geometry arrays in face order (internal faces, then
+// every patch in order) -- the layout OpenFOAM's polyMesh uses
+void linGeomTotalDispSolidHIP::createDeviceMesh()
+{
+    const fvMesh& m = mesh();
+    const label nCells = m.nCells();
+    const label nInt = m.nInternalFaces();
+    const label nFaces = m.nFaces();
+
+    patchType_.setSize(m.boundary().size());
+    forAll(D().boundaryField(), patchI)
+    {
+        const word t = D().boundaryField()[patchI].type();
+        if (t == fixedDisplacementFvPatchVectorField::typeName)
+        {
+            patchType_[patchI] = S4F_PATCH_FIXED_DISPLACEMENT;
+        }
+        else if (t == solidTractionFvPatchVectorField::typeName)
+        {
+            patchType_[patchI] = S4F_PATCH_TRACTION;
+        }
+        else
+        {
+            FatalErrorIn(type() + "::createDeviceMesh()")
+                << "patch " << m.boundary()[patchI].name() << " has type " << t
+                << "; the device path supports fixedDisplacement and solidTraction"
+                << abort(FatalError);
+        }
+    }
+
+    std::vector<int32_t> owner(nFaces), neighbour(nInt), pStart, pSize, pType;
+    std::vector<double> C(3*nCells), V(nCells), Sf(3*nFaces), Cf(3*nFaces);
+    forAll(m.owner(), faceI) { owner[faceI] = m.owner()[faceI]; }
+    forAll(m.neighbour(), faceI) { neighbour[faceI] = m.neighbour()[faceI]; }
+    forAll(m.C(), cellI)
+    {
+        for (direction d = 0; d < 3; ++d) { C[3*cellI + d] = m.C()[cellI][d]; }
+        V[cellI] = m.V()[cellI];
+    }
+    for (label faceI = 0; faceI < nInt; ++faceI)
+    {
+        for (direction d = 0; d < 3; ++d)
+        {
+            Sf[3*faceI + d] = m.Sf()[faceI][d];
+            Cf[3*faceI + d] = m.Cf()[faceI][d];
+        }
+    }
+    forAll(m.boundary(), patchI)
+    {
+        const fvPatch& p = m.boundary()[patchI];
+        pStart.push_back(p.start());
+        pSize.push_back(p.size());
+        pType.push_back(patchType_[patchI]);
+        forAll(p, i)
+        {
+            const label faceI = p.start() + i;
+            for (direction d = 0; d < 3; ++d)
+            {
+                Sf[3*faceI + d] = m.Sf().boundaryField()[patchI][i][d];
+                Cf[3*faceI + d] = m.Cf().boundaryField()[patchI][i][d];
+            }
+        }
+    }
+
+    s4f_mesh_desc desc;
+    desc.nCells = nCells;
+    desc.nFaces = nFaces;
+    desc.nInternalFaces = nInt;
+    desc.owner = owner.data();
+    desc.neighbour = neighbour.data();
+    desc.C = C.data();
+    desc.V = V.data();
+    desc.Sf = Sf.data();
+    desc.Cf = Cf.data();
+    desc.nPatches = pStart.size();
+    desc.patchStart = pStart.data();
+    desc.patchSize = pSize.data();
+    desc.patchType = pType.data();
+    // leastSquares -> normal patch delta; extendedLeastSquares 0 -> full delta
+    const word gradScheme(m.gradScheme("grad(D)"));
+    desc.lsFullDelta = gradScheme.find("extendedLeastSquares") != string::npos;
+    check(s4f_mesh_create(ctx_, &desc, &gpuMesh_), "createDeviceMesh()");
+
+    // one linearElastic law (linearElastic.C:62-133)
+    const PtrList<mechanicalLaw>& laws = mechanical();
+    if (laws.size() != 1 || !isA<linearElastic>(laws[0]))
+    {
+        FatalErrorIn(type() + "::createDeviceMesh()")
+            << "the device path takes a single linearElastic law"
+            << abort(FatalError);
+    }
+    const linearElastic& law = refCast<const linearElastic>(laws[0]);
+    const double E = law.E().value();
+    const double nu = law.nu().value();
+    check(s4f_material_set(gpuMesh_, &E, &nu, 0), "createDeviceMesh()");
+}
+
+
+void linGeomTotalDispSolidHIP::uploadBoundaryValues()
+{
+    const fvMesh& m = mesh();
+    forAll(m.boundary(), patchI)
+    {
+        const label n = m.boundary()[patchI].size();
+        std::vector<double> v(3*n), p(n, 0.0);
+        if (patchType_[patchI] == S4F_PATCH_TRACTION)
+        {
+            const solidTractionFvPatchVectorField& bc =
+                refCast<const solidTractionFvPatchVectorField>(D().boundaryField()[patchI]);
+            forAll(bc.traction(), i)
+            {
+                for (direction d = 0; d < 3; ++d) { v[3*i + d] = bc.traction()[i][d]; }
+                p[i] = bc.pressure()[i];
+            }
+        }
+        else
+        {
+            const fvPatchVectorField& bc = D().boundaryField()[patchI];
+            forAll(bc, i)
+            {
+                for (direction d = 0; d < 3; ++d) { v[3*i + d] = bc[i][d]; }
+            }
+        }
+        check(s4f_bc_set(gpuMesh_, patchI, v.data(), p.data()), "uploadBoundaryValues()");
+    }
+}
+
+
+s4f_solve_opts linGeomTotalDispSolidHIP::solveOptions() const
+{
+    s4f_solve_opts o;
+    s4f_solve_opts_default(&o);
+    o.nCorrectors = nCorr();
+    o.solutionTolerance = solutionTol();
+    o.alternativeTolerance =
+        solidModelDict().lookupOrDefault<scalar>("alternativeTolerance", 1e-7);
+    const dictionary& stab = solidModelDict().subDict("stabilisation");
+    if (word(stab.lookup("type")) != "RhieChow")
+    {
+        FatalErrorIn(type() + "::solveOptions()")
+            << "the device path implements the RhieChow stabilisation"
+            << abort(FatalError);
+    }
+    o.rhieChowScale = readScalar(stab.lookup("scaleFactor"));
+    o.interfaceScale = stab.lookupOrDefault<scalar>("interfaceScaleFactor", 0.01);
+    const dictionary& sol = mesh().solverDict(D().name());
+    o.pcgTolerance = sol.lookupOrDefault<scalar>("tolerance", 1e-6);
+    o.pcgRelTol = sol.lookupOrDefault<scalar>("relTol", 0);
+    o.pcgMaxIter = sol.lookupOrDefault<label>("maxIter", 1000);
+    o.precision =
+        word(solidModelDict().lookupOrDefault<word>("precision", "fp64")) == "mixed"
+      ? S4F_PRECISION_MIXED : S4F_PRECISION_FP64;
+    o.warmStart = 1;
+    return o;
+}
+
+
+linGeomTotalDispSolidHIP::linGeomTotalDispSolidHIP
+(
+    Time& runTime,
+    const word& region
+)
+:
+    solidModel(typeName, runTime, region),
+    impK_(mechanical().impK()),
+    rImpK_(1.0/impK_),
+    ctx_(nullptr),
+    gpuMesh_(nullptr),
+    patchType_()
+{
+    DisRequired();
+    if (solidModelDict().lookupOrDefault<Switch>("predictor", false))
+    {
+        FatalErrorIn(type() + "::" + type())
+            << "predictor is not available on the device path" << abort(FatalError);
+    }
+    check(s4f_ctx_create(0, &ctx_), "linGeomTotalDispSolidHIP()");
+    createDeviceMesh();
+    D().correctBoundaryConditions();
+    D().storePrevIter();
+    mechanical().grad(D(), gradD());
+}
+
+
+linGeomTotalDispSolidHIP::~linGeomTotalDispSolidHIP()
+{
+    s4f_mesh_destroy(gpuMesh_);
+    s4f_ctx_destroy(ctx_);
+}
+
+
+bool linGeomTotalDispSolidHIP::evolve()
+{
+    Info<< "Evolving solid solver (MI355X)" << endl;
+
+    uploadBoundaryValues();
+    const s4f_solve_opts o = solveOptions();
+
+    const label nCells = mesh().nCells();
+    std::vector<double> Dv(3*nCells), sig(6*nCells), G(9*nCells);
+    forAll(D(), cellI)
+    {
+        for (direction d = 0; d < 3; ++d) { Dv[3*cellI + d] = D()[cellI][d]; }
+    }
+    s4f_solve_report rep;
+    const s4f_status st =
+        s4f_solve(gpuMesh_, &o, Dv.data(), sig.data(), G.data(), nullptr, &rep);
+    check(st, "evolve()");
+    if (st == S4F_WARN_MAXITER)
+    {
+        Warning<< "Max iterations reached within momentum loop" << endl;
+    }
+    Info<< "    Corr, res, relRes, iters: " << rep.nOuter << ", "
+        << rep.initialResidual << ", " << rep.residualvf << ", "
+        << rep.innerTotal << " (" << rep.seconds << " s on the device)" << endl;
+
+    D().storePrevIter();
+    forAll(D(), cellI)
+    {
+        for (direction d = 0; d < 3; ++d) { D()[cellI][d] = Dv[3*cellI + d]; }
+    }
+    D().correctBoundaryConditions();
+
+    // boundary values of gradD and sigma are recomputed by the reference's own
+    // operators so every patch and function object sees consistent fields
+    mechanical().grad(D(), gradD());
+    mechanical().correct(sigma());
+
+    DD() = D() - D().oldTime();
+    mechanical().interpolate(D(), gradD(), pointD());
+    pointDD() = pointD() - pointD().oldTime();
+    U() = fvc::ddt(D());
+
+    return true;
+}
+
+
+tmp<vectorField> linGeomTotalDispSolidHIP::tractionBoundarySnGrad
+(
+    const vectorField& traction,
+    const scalarField& pressure,
+    const fvPatch& patch
+) const
+{
+    // same expression as linGeomTotalDispSolid.C:235-271
+    const label patchID = patch.index();
+    const scalarField& impK = impK_.boundaryField()[patchID];
+    const scalarField& rImpK = rImpK_.boundaryField()[patchID];
+    const tensorField& pGradD = gradD().boundaryField()[patchID];
+    const symmTensorField& pSigma = sigma().boundaryField()[patchID];
+    const vectorField n(patch.nf());
+    return tmp<vectorField>
+    (
+        new vectorField
+        (
+            ((traction - n*pressure) - (n & (pSigma - impK*pGradD)))*rImpK
+        )
+    );
+}
+
+} // End namespace solidModels
+} // End namespace Foam

@@ -271,3 +271,33 @@ def solid_model_new(solid_model_type, *args, **kw):
         raise ValueError(f"Unknown solidModel type {solid_model_type!r}; valid solidModel types are: "
                          f"{sorted(SOLID_MODELS)}") from None
     return cls(*args, **kw)
+
+
+class LduSolver:
+    """PCG on a caller's symmetric LDU matrix (OpenFOAM lduMatrix addressing):
+    the lduMatrix::solver seam (s4f_ldu_* in include/s4f_abi.h)."""
+
+    def __init__(self, ctx, n_cells, lower_addr, upper_addr):
+        self.ctx = ctx
+        self.nCells = int(n_cells)
+        lo, up = i32(lower_addr), i32(upper_addr)
+        h = ctypes.c_void_p()
+        check(lib().s4f_ldu_create(ctx.h, self.nCells, len(lo), ptr(lo), ptr(up), ctypes.byref(h)))
+        self.h = h
+
+    def solve(self, diag, upper, source, psi, tolerance=1e-9, rel_tol=0.0, max_iter=1000, mixed=False):
+        psi = f64(psi).copy()
+        ir, fr = ctypes.c_double(), ctypes.c_double()
+        it = ctypes.c_int32()
+        check(lib().s4f_ldu_solve(self.h, ptr(f64(diag)), ptr(f64(upper)), ptr(f64(source)), ptr(psi),
+                                  float(tolerance), float(rel_tol), int(max_iter), int(bool(mixed)),
+                                  ctypes.byref(ir), ctypes.byref(fr), ctypes.byref(it)))
+        return psi, dict(initialResidual=ir.value, finalResidual=fr.value, nIterations=it.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s4f_ldu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -182,3 +182,36 @@ def test_divergence_and_maxiter_status(ctx):
     pm, m, _, _ = _product_mesh(ctx, (6, 6, 6), jitter=0.1, seed=1)
     out = m.solve(nCorrectors=3)
     assert out["status"] == _abi.S4F_WARN_MAXITER and out["nOuter"] == 3 and not out["converged"]
+
+
+def test_ldu_solver_matches_oracle_pcg(ctx, oracle_mod):
+    # lduMatrix::solver seam: one component of the oracle's M D = M D0 + R,
+    # PCG with the diagonal preconditioner from the warm start D0, relTol 0.1:
+    # same normalised initial residual and (to rounding) the same iterate
+    from solids4foam_amd.solid import LduSolver
+    om, oc = _oracle_case(oracle_mod, (8, 7, 6), jitter=0.15, seed=6)
+    for _ in range(3):
+        oc.iterate(precond=oracle_mod.PRECOND_DIAGONAL)
+    D0 = oc.get("D").reshape(-1, 3).copy()
+    oc.op("update_traction")
+    oc.op("residual")
+    R = oc.get("R").reshape(-1, 3)
+    diag, upper = oc.get("diag"), oc.get("upper")
+    L = LduSolver(ctx, om.nCells, om.owner[:om.nInternalFaces], om.neighbour)
+    q = 1
+    src = oc.amul(np.ascontiguousarray(D0[:, q])) + R[:, q]
+    psi, perf = L.solve(diag, upper, src, D0[:, q], tolerance=1e-9, rel_tol=0.1)
+    oc.set("D", D0)
+    conv, ir, rvf, inner = oc.iterate(precond=oracle_mod.PRECOND_DIAGONAL)
+    D1 = oc.get("D").reshape(-1, 3)
+    assert perf["nIterations"] > 0
+    assert abs(perf["finalResidual"]) < 0.1 * perf["initialResidual"]
+    assert rel_l2(psi, D1[:, q]) < 1e-10
+    # tight solve == dense
+    M = np.diag(diag)
+    o, nb = om.owner[:om.nInternalFaces], om.neighbour
+    M[o, nb] = upper
+    M[nb, o] = upper
+    psi2, perf2 = L.solve(diag, upper, src, D0[:, q], tolerance=1e-15, rel_tol=0.0)
+    assert rel_l2(psi2, np.linalg.solve(M, src)) < 1e-10
+    L.close()
