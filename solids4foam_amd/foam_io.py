@@ -1,0 +1,443 @@
+"""OpenFOAM case I/O for the linearGeometryTotalDisplacement path.
+
+Reads what a solids4foam case gives the reference and writes what it writes:
+  * dictionaries (FoamFile subset: keywords, sub-dictionaries, lists,
+    ``uniform``/``nonuniform List<...>`` fields, dimensioned scalars, quoted
+    regex keys, // and /* */ comments; ``#include`` is not supported);
+  * ASCII polyMesh (points, faces, owner, neighbour, boundary), optionally
+    gzip-compressed (``*.gz``);
+  * the ``0/D`` boundary conditions the device path supports
+    (fixedDisplacement, solidTraction) and the material of
+    ``constant/mechanicalProperties`` (one linearElastic law);
+  * ASCII volVectorField / volSymmTensorField / volScalarField output.
+
+Reference: solidModel field I/O (src/solids4FoamModels/solidModels/solidModel/solidModel.C:1017-1124,
+1936-1996), solidTractionFvPatchVectorField.C:60-120 and
+fixedDisplacementFvPatchVectorField.C:150-190 (dictionary keys), linearElastic.C:62-104.
+"""
+import gzip
+import os
+import re
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# tokenizer / dictionary parser
+# ---------------------------------------------------------------------------
+_TOKEN = re.compile(r'"(?:[^"\\]|\\.)*"|[{}();\[\]]|[^\s{}();\[\]"]+')
+
+
+def _strip_comments(text):
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", text)
+
+
+def _open(path):
+    if os.path.exists(path):
+        return open(path, "rt")
+    if os.path.exists(path + ".gz"):
+        return gzip.open(path + ".gz", "rt")
+    raise FileNotFoundError(path)
+
+
+def read_text(path):
+    with _open(path) as f:
+        return f.read()
+
+
+def _num(tok):
+    try:
+        if re.fullmatch(r"[-+]?\d+", tok):
+            return int(tok)
+        return float(tok)
+    except ValueError:
+        return None
+
+
+class _Parser:
+    def __init__(self, text):
+        self.toks = _TOKEN.findall(_strip_comments(text))
+        self.i = 0
+
+    def peek(self):
+        return self.toks[self.i] if self.i < len(self.toks) else None
+
+    def next(self):
+        t = self.toks[self.i]
+        self.i += 1
+        return t
+
+    def expect(self, t):
+        got = self.next()
+        if got != t:
+            raise ValueError(f"expected {t!r}, got {got!r} (token {self.i})")
+
+    def parse_dict_body(self, end="}"):
+        d = {}
+        while self.peek() is not None and self.peek() != end:
+            key = self.next()
+            if key.startswith("#"):
+                raise ValueError(f"{key} directives are not supported")
+            if key.startswith('"'):
+                key = key[1:-1]
+            if self.peek() == "{":
+                self.next()
+                d[key] = self.parse_dict_body("}")
+                self.expect("}")
+            else:
+                vals = []
+                while self.peek() != ";":
+                    vals.append(self.parse_value())
+                self.expect(";")
+                d[key] = vals[0] if len(vals) == 1 else vals
+        return d
+
+    def parse_value(self):
+        t = self.peek()
+        if t == "(":
+            return self.parse_list()
+        if t == "[":  # dimensions
+            self.next()
+            dims = []
+            while self.peek() != "]":
+                dims.append(self.next())
+            self.next()
+            return ("dimensions", dims)
+        if t == "{":
+            self.next()
+            d = self.parse_dict_body("}")
+            self.expect("}")
+            return d
+        self.next()
+        n = _num(t)
+        if n is not None and self.peek() == "(":  # counted list N ( ... )
+            return self.parse_list()
+        if t.startswith("List<") and self.peek() is not None:
+            return ("List", t[5:-1])
+        return n if n is not None else t
+
+    def parse_list(self):
+        self.expect("(")
+        out = []
+        while self.peek() != ")":
+            if self.peek() == "(":
+                out.append(self.parse_list())
+            else:
+                t = self.next()
+                n = _num(t)
+                if n is not None and self.peek() == "(":
+                    out.append(self.parse_list())
+                elif self.peek() == "{" and n is None:
+                    self.next()
+                    d = self.parse_dict_body("}")
+                    self.expect("}")
+                    out.append((t, d))
+                else:
+                    out.append(n if n is not None else t)
+        self.expect(")")
+        return out
+
+
+def parse_dict(text):
+    """OpenFOAM dictionary text -> nested dict (FoamFile header kept)."""
+    return _Parser(text).parse_dict_body(end=None)
+
+
+def read_dict(path):
+    return parse_dict(read_text(path))
+
+
+def lookup(d, key, default=None):
+    """dictionary lookup that honours quoted regex keys ("D|DD")"""
+    if key in d:
+        return d[key]
+    for k, v in d.items():
+        try:
+            if re.fullmatch(k, key):
+                return v
+        except re.error:
+            continue
+    return default
+
+
+def dimensioned(v):
+    """value of `E [dims] 200e9`, `E E [dims] 200e9` or a plain number"""
+    if isinstance(v, list):
+        return float(v[-1])
+    return float(v)
+
+
+# ---------------------------------------------------------------------------
+# polyMesh
+# ---------------------------------------------------------------------------
+def _header_and_body(text):
+    text = _strip_comments(text)
+    m = re.search(r"FoamFile\s*\{.*?\}", text, flags=re.S)
+    return text[m.end():] if m else text
+
+
+def _read_list_block(text):
+    """body of an OpenFOAM list file: N ( entries )"""
+    body = _header_and_body(text)
+    m = re.search(r"(\d+)\s*\(", body)
+    if not m:
+        raise ValueError("no list found")
+    n = int(m.group(1))
+    start = m.end()
+    end = body.rfind(")")
+    return n, body[start:end]
+
+
+def read_points(path):
+    n, inner = _read_list_block(read_text(path))
+    vals = np.array(inner.replace("(", " ").replace(")", " ").split(), dtype=np.float64)
+    return vals.reshape(n, 3)
+
+
+def read_faces(path):
+    n, inner = _read_list_block(read_text(path))
+    toks = inner.replace("(", " ( ").replace(")", " ) ").split()
+    off = [0]
+    verts = []
+    i = 0
+    while i < len(toks):
+        k = int(toks[i])
+        assert toks[i + 1] == "("
+        verts.extend(int(x) for x in toks[i + 2:i + 2 + k])
+        i += k + 3
+        off.append(len(verts))
+    if len(off) - 1 != n:
+        raise ValueError(f"faces: expected {n}, read {len(off) - 1}")
+    return np.array(off, np.int32), np.array(verts, np.int32)
+
+
+def read_labels(path):
+    text = read_text(path)
+    n, inner = _read_list_block(text)
+    vals = np.array(inner.split(), dtype=np.int64).astype(np.int32)
+    if len(vals) != n:
+        raise ValueError(f"{path}: expected {n} labels, read {len(vals)}")
+    note = re.search(r'note\s+"([^"]*)"', text)
+    return vals, (note.group(1) if note else "")
+
+
+def read_boundary(path):
+    n, inner = _read_list_block(read_text(path))
+    entries = _Parser("(" + inner + ")").parse_list()
+    out = []
+    for name, d in entries:
+        out.append(dict(name=name, type=d.get("type"), startFace=int(d["startFace"]), nFaces=int(d["nFaces"])))
+    return out
+
+
+def read_polymesh(case_dir):
+    pm = os.path.join(case_dir, "constant", "polyMesh")
+    points = read_points(os.path.join(pm, "points"))
+    fOff, fVerts = read_faces(os.path.join(pm, "faces"))
+    owner, note = read_labels(os.path.join(pm, "owner"))
+    neighbour, _ = read_labels(os.path.join(pm, "neighbour"))
+    boundary = read_boundary(os.path.join(pm, "boundary"))
+    nCells = int(owner.max()) + 1
+    m = re.search(r"nCells:\s*(\d+)", note)
+    if m:
+        nCells = int(m.group(1))
+    return dict(points=points, fOff=fOff, fVerts=fVerts, owner=owner, neighbour=neighbour, boundary=boundary,
+                nCells=nCells)
+
+
+def _foam_header(cls, obj, location=None):
+    loc = f'    location    "{location}";\n' if location else ""
+    return ("FoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+            f"    class       {cls};\n{loc}    object      {obj};\n}}\n\n")
+
+
+def write_polymesh(case_dir, points, fOff, fVerts, owner, neighbour, patches, nCells):
+    """patches: list of dict(name, type, startFace, nFaces)"""
+    pm = os.path.join(case_dir, "constant", "polyMesh")
+    os.makedirs(pm, exist_ok=True)
+    with open(os.path.join(pm, "points"), "w") as f:
+        f.write(_foam_header("vectorField", "points", "constant/polyMesh"))
+        f.write(f"{len(points)}\n(\n")
+        f.writelines(f"({p[0]:.17g} {p[1]:.17g} {p[2]:.17g})\n" for p in points)
+        f.write(")\n")
+    with open(os.path.join(pm, "faces"), "w") as f:
+        f.write(_foam_header("faceList", "faces", "constant/polyMesh"))
+        f.write(f"{len(fOff) - 1}\n(\n")
+        for i in range(len(fOff) - 1):
+            v = fVerts[fOff[i]:fOff[i + 1]]
+            f.write(f"{len(v)}(" + " ".join(map(str, v)) + ")\n")
+        f.write(")\n")
+    note = f"nPoints:{len(points)}  nCells:{nCells}  nFaces:{len(owner)}  nInternalFaces:{len(neighbour)}"
+    for name, arr in (("owner", owner), ("neighbour", neighbour)):
+        with open(os.path.join(pm, name), "w") as f:
+            f.write(_foam_header("labelList", name, "constant/polyMesh").replace(
+                "    object", f'    note        "{note}";\n    object'))
+            f.write(f"{len(arr)}\n(\n")
+            f.write("\n".join(map(str, arr)))
+            f.write("\n)\n")
+    with open(os.path.join(pm, "boundary"), "w") as f:
+        f.write(_foam_header("polyBoundaryMesh", "boundary", "constant/polyMesh"))
+        f.write(f"{len(patches)}\n(\n")
+        for p in patches:
+            f.write(f"    {p['name']}\n    {{\n        type            {p.get('type', 'patch')};\n"
+                    f"        nFaces          {p['nFaces']};\n        startFace       {p['startFace']};\n    }}\n")
+        f.write(")\n")
+
+
+# ---------------------------------------------------------------------------
+# fields and boundary conditions
+# ---------------------------------------------------------------------------
+def _field_value(v, n, width):
+    """uniform x | uniform (x y z) | nonuniform List<T> N ( ... ) -> (n, width) array"""
+    if isinstance(v, list) and len(v) >= 2 and v[0] == "uniform":
+        val = v[1]
+        arr = np.array(val if isinstance(val, list) else [val], dtype=np.float64).reshape(1, -1)
+        return np.broadcast_to(arr, (n, width)).copy()
+    if isinstance(v, list) and len(v) >= 3 and v[0] == "nonuniform":
+        data = v[-1]
+        arr = np.array(data, dtype=np.float64).reshape(-1, width)
+        if len(arr) != n:
+            raise ValueError(f"nonuniform field has {len(arr)} values, patch has {n}")
+        return arr
+    if isinstance(v, (int, float)):
+        return np.full((n, width), float(v))
+    raise ValueError(f"unsupported field value {v!r}")
+
+
+SUPPORTED_D_BCS = ("fixedDisplacement", "solidTraction")
+
+
+def read_D_boundary(case_dir, boundary, time="0"):
+    """per patch: dict(type, value (n,3) | traction (n,3), pressure (n,))"""
+    d = read_dict(os.path.join(case_dir, time, "D"))
+    bf = d["boundaryField"]
+    out = []
+    for p in boundary:
+        e = lookup(bf, p["name"])
+        if e is None:
+            raise ValueError(f"0/D has no entry for patch {p['name']}")
+        t = e.get("type")
+        n = p["nFaces"]
+        if t == "fixedDisplacement":
+            disp = e.get("displacement", e.get("value", ["uniform", [0, 0, 0]]))
+            out.append(dict(type=t, value=_field_value(disp, n, 3)))
+        elif t == "solidTraction":
+            tr = e.get("traction", ["uniform", [0, 0, 0]])
+            pr = e.get("pressure", ["uniform", 0])
+            out.append(dict(type=t, traction=_field_value(tr, n, 3), pressure=_field_value(pr, n, 1)[:, 0]))
+        else:
+            raise ValueError(f"patch {p['name']}: boundary condition {t!r} is not supported on the device path "
+                             f"(supported: {', '.join(SUPPORTED_D_BCS)})")
+    return out
+
+
+def read_linear_elastic(case_dir):
+    """(E, nu) of the single linearElastic law (linearElastic.C:62-104)"""
+    d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
+    laws = d.get("mechanical")
+    if isinstance(laws, list):
+        if len(laws) != 1:
+            raise ValueError("the device path takes one mechanical law")
+        _, law = laws[0]
+    else:
+        law = laws
+    if law.get("type") != "linearElastic":
+        raise ValueError(f"mechanical law {law.get('type')!r} is not supported (linearElastic)")
+    if str(d.get("planeStress", "no")) in ("yes", "on", "true"):
+        raise ValueError("planeStress is not supported on the device path")
+    if "E" in law and "nu" in law:
+        return dimensioned(law["E"]), dimensioned(law["nu"])
+    mu, K = dimensioned(law["mu"]), dimensioned(law["K"])
+    return 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
+
+
+def read_solid_properties(case_dir):
+    """solidModel type + coefficients with the reference defaults (solidModel.C:1147-1177)"""
+    d = read_dict(os.path.join(case_dir, "constant", "solidProperties"))
+    model = d["solidModel"]
+    coeffs = lookup(d, model + "Coeffs", {}) or {}
+    stab = coeffs.get("stabilisation", {"type": "RhieChow", "scaleFactor": 0.1})
+    if stab.get("type", "RhieChow") != "RhieChow":
+        raise ValueError("only the RhieChow stabilisation is on the device path")
+    if str(coeffs.get("relaxationMethod", "fixed")) != "fixed":
+        raise ValueError("field relaxation is not on the device path")
+    if str(coeffs.get("predictor", "no")) in ("yes", "on", "true"):
+        raise ValueError("the predictor is not on the device path")
+    opts = dict(nCorrectors=int(coeffs.get("nCorrectors", 10000)),
+                solutionTolerance=float(coeffs.get("solutionTolerance", 1e-6)),
+                alternativeTolerance=float(coeffs.get("alternativeTolerance", 1e-7)),
+                rhieChowScale=float(stab.get("scaleFactor", 0.1)),
+                interfaceScale=float(stab.get("interfaceScaleFactor", 0.01)))
+    prec = coeffs.get("precision", "fp64")
+    opts["precision"] = 1 if prec == "mixed" else 0
+    return model, opts
+
+
+def read_solver_controls(case_dir, field="D"):
+    """fvSolution solvers entry for D (PCG tolerance / relTol / maxIter)"""
+    d = read_dict(os.path.join(case_dir, "system", "fvSolution"))
+    s = lookup(d.get("solvers", {}), field, {}) or {}
+    return dict(pcgTolerance=float(s.get("tolerance", 1e-6)), pcgRelTol=float(s.get("relTol", 0.0)),
+                pcgMaxIter=int(s.get("maxIter", 1000)))
+
+
+def write_vol_field(path, name, cls, values, boundary=None, dims="[0 1 0 0 0 0 0]"):
+    """ASCII vol*Field with nonuniform internal values; boundary: list of (patchName, values)"""
+    values = np.asarray(values, np.float64)
+    kind = {"volScalarField": "scalar", "volVectorField": "vector", "volSymmTensorField": "symmTensor",
+            "volTensorField": "tensor"}[cls]
+
+    def fmt(v):
+        v = np.atleast_1d(v)
+        return f"{v[0]:.10g}" if kind == "scalar" else "(" + " ".join(f"{x:.10g}" for x in v) + ")"
+
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(_foam_header(cls, name))
+        f.write(f"dimensions      {dims};\n\n")
+        f.write(f"internalField   nonuniform List<{kind}>\n{len(values)}\n(\n")
+        f.writelines(fmt(v) + "\n" for v in values)
+        f.write(")\n;\n\nboundaryField\n{\n")
+        for pname, pv in boundary or []:
+            pv = np.asarray(pv)
+            f.write(f"    {pname}\n    {{\n        type            calculated;\n")
+            f.write(f"        value           nonuniform List<{kind}>\n{len(pv)}\n(\n")
+            f.writelines(fmt(v) + "\n" for v in pv)
+            f.write(")\n;\n    }\n")
+        f.write("}\n")
+
+
+def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
+    """Reads a solids4foam case (polyMesh, 0/D, mechanicalProperties,
+    solidProperties, fvSolution), runs linearGeometryTotalDisplacement on
+    the device and writes D, sigma and sigmaEq to `write_time`."""
+    from . import solid
+    model, opts = read_solid_properties(case_dir)
+    if model not in ("linearGeometryTotalDisplacement", "linearGeometryTotalDisplacementHIP"):
+        raise ValueError(f"solidModel {model!r} is not on the device path")
+    opts.update(read_solver_controls(case_dir))
+    opts.update(solver_overrides or {})
+    mesh = read_polymesh(case_dir)
+    bnd = mesh["boundary"]
+    pm = solid.PolyMesh.from_arrays(mesh["points"], mesh["fOff"], mesh["fVerts"], mesh["owner"], mesh["neighbour"],
+                                    mesh["nCells"], [p["startFace"] for p in bnd], [p["nFaces"] for p in bnd])
+    bcs = read_D_boundary(case_dir, bnd)
+    E, nu = read_linear_elastic(case_dir)
+    patches = [dict(type=b["type"], value=b.get("value"), traction=b.get("traction"), pressure=b.get("pressure"))
+               for b in bcs]
+    ctx = ctx or solid.Context(0)
+    grad_scheme = ""
+    try:
+        fvs = read_dict(os.path.join(case_dir, "system", "fvSchemes"))
+        grad_scheme = str(lookup(fvs.get("gradSchemes", {}), "grad(D)", "") or "")
+    except FileNotFoundError:
+        pass
+    m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
+                                    ls_full_delta="extendedLeastSquares" in grad_scheme)
+    m.evolve()
+    out = os.path.join(case_dir, write_time)
+    write_vol_field(os.path.join(out, "D"), "D", "volVectorField", m.D)
+    write_vol_field(os.path.join(out, "sigma"), "sigma", "volSymmTensorField", m.sigma, dims="[1 -1 -2 0 0 0 0]")
+    write_vol_field(os.path.join(out, "sigmaEq"), "sigmaEq", "volScalarField", m.sigma_eq(),
+                    dims="[1 -1 -2 0 0 0 0]")
+    return m

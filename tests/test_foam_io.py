@@ -1,0 +1,143 @@
+"""OpenFOAM case I/O: dictionary parsing, polyMesh round trip, boundary
+conditions, material and solver controls (CPU), and a case run end to end
+(GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+from solids4foam_amd import foam_io
+
+SOLID_PROPS = """
+FoamFile { version 2.0; format ascii; class dictionary; object solidProperties; }
+// comment
+solidModel     linearGeometryTotalDisplacement;
+"linearGeometryTotalDisplacementCoeffs|unsLinearGeometryCoeffs"
+{
+    nCorrectors     100000;
+    solutionTolerance 1e-07;
+    /* block comment */
+    stabilisation { type RhieChow; scaleFactor 0.2; }
+}
+"""
+FV_SOLUTION = """
+FoamFile { version 2.0; format ascii; class dictionary; object fvSolution; }
+solvers
+{
+    "D|DD" { tolerance 1e-8; relTol 0.1; maxIter 50; solver PCG; preconditioner FDIC; }
+}
+"""
+MECH = """
+FoamFile { version 2.0; format ascii; class dictionary; object mechanicalProperties; }
+planeStress no;
+mechanical
+(
+    steel
+    {
+        type  linearElastic;
+        rho   rho [1 -3 0 0 0 0 0] 7800;
+        E     E [1 -1 -2 0 0 0 0] 200e+09;
+        nu    nu [0 0 0 0 0 0 0] 0.3;
+    }
+);
+"""
+
+
+def _write(d, rel, text):
+    p = os.path.join(d, rel)
+    os.makedirs(os.path.dirname(p), exist_ok=True)
+    open(p, "w").write(text)
+
+
+def _d_file(patches):
+    body = []
+    for name, kind in patches:
+        if kind == "fixed":
+            body.append(f"{name} {{ type fixedDisplacement; displacement uniform (0 0 0); value uniform (0 0 0); }}")
+        elif kind == "load":
+            body.append(f"{name} {{ type solidTraction; traction uniform (0 0 1e7); pressure uniform 0; "
+                        f"value uniform (0 0 0); }}")
+        else:
+            body.append(f"{name} {{ type solidTraction; traction uniform (0 0 0); pressure uniform 0; "
+                        f"value uniform (0 0 0); }}")
+    return ("FoamFile { version 2.0; format ascii; class volVectorField; object D; }\n"
+            "dimensions [0 1 0 0 0 0 0];\ninternalField uniform (0 0 0);\nboundaryField\n{\n"
+            + "\n".join(body) + "\n}\n")
+
+
+PATCHES = [("left", "free"), ("right", "free"), ("down", "free"), ("up", "free"), ("back", "fixed"),
+           ("front", "load")]
+
+
+def _box_case(tmp, n=(5, 4, 3), jitter=0.1):
+    from solids4foam_amd.solid import PolyMesh
+    pm = PolyMesh.box(*n, jitter=jitter, seed=3)
+    a = pm.addressing()
+    patches = [dict(name=PATCHES[i][0], type="patch", startFace=int(a["patchStart"][i]), nFaces=int(a["patchSize"][i]))
+               for i in range(6)]
+    foam_io.write_polymesh(tmp, a["points"], a["fOff"], a["fVerts"], a["owner"], a["neighbour"], patches, pm.nCells)
+    _write(tmp, "constant/solidProperties", SOLID_PROPS)
+    _write(tmp, "constant/mechanicalProperties", MECH)
+    _write(tmp, "system/fvSolution", FV_SOLUTION)
+    _write(tmp, "0/D", _d_file(PATCHES))
+    return pm, a
+
+
+def test_parse_dictionaries(tmp_path):
+    _write(str(tmp_path), "constant/solidProperties", SOLID_PROPS)
+    _write(str(tmp_path), "constant/mechanicalProperties", MECH)
+    _write(str(tmp_path), "system/fvSolution", FV_SOLUTION)
+    model, opts = foam_io.read_solid_properties(str(tmp_path))
+    assert model == "linearGeometryTotalDisplacement"
+    assert opts["nCorrectors"] == 100000 and opts["solutionTolerance"] == 1e-7
+    assert opts["alternativeTolerance"] == 1e-7 and opts["rhieChowScale"] == 0.2
+    assert foam_io.read_linear_elastic(str(tmp_path)) == (200e9, 0.3)
+    sc = foam_io.read_solver_controls(str(tmp_path))
+    assert sc == dict(pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
+
+
+def test_polymesh_and_bc_roundtrip(tmp_path):
+    pm, a = _box_case(str(tmp_path))
+    m = foam_io.read_polymesh(str(tmp_path))
+    assert m["nCells"] == pm.nCells
+    assert np.array_equal(m["points"], a["points"])  # %.17g round trips doubles
+    assert np.array_equal(m["fOff"], a["fOff"]) and np.array_equal(m["fVerts"], a["fVerts"])
+    assert np.array_equal(m["owner"], a["owner"]) and np.array_equal(m["neighbour"], a["neighbour"])
+    assert [b["startFace"] for b in m["boundary"]] == list(a["patchStart"])
+    bcs = foam_io.read_D_boundary(str(tmp_path), m["boundary"])
+    assert [b["type"] for b in bcs] == ["solidTraction"] * 4 + ["fixedDisplacement", "solidTraction"]
+    assert np.all(bcs[5]["traction"] == [0, 0, 1e7]) and bcs[5]["traction"].shape == (m["boundary"][5]["nFaces"], 3)
+
+
+def test_unsupported_bc_is_rejected(tmp_path):
+    pm, a = _box_case(str(tmp_path))
+    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement", "type solidSymmetry"))
+    m = foam_io.read_polymesh(str(tmp_path))
+    with pytest.raises(ValueError, match="not supported"):
+        foam_io.read_D_boundary(str(tmp_path), m["boundary"])
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/tutorials"), reason="reference tutorials not present")
+def test_parse_reference_tutorial_dictionaries():
+    case = "/root/reference/tutorials/solids/linearElasticity/cantilever2d/segregatedCantilever2d"
+    model, opts = foam_io.read_solid_properties(case)
+    assert model == "linearGeometryTotalDisplacement" and opts["nCorrectors"] == 100000
+    assert foam_io.read_solver_controls(case) == dict(pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
+    assert foam_io.read_linear_elastic(case) == (200e9, 0.3)
+
+
+@pytest.mark.gpu
+def test_run_case_end_to_end(tmp_path):
+    from solids4foam_amd import cases, Context
+    pm, a = _box_case(str(tmp_path), n=(6, 6, 6), jitter=0.1)
+    ctx = Context(0)
+    m = foam_io.run_case(str(tmp_path), ctx=ctx, solver_overrides=dict(nCorrectors=20000, alternativeTolerance=1e-10,
+                                                                         solutionTolerance=1e-10))
+    assert m.report["converged"]
+    D = foam_io._field_value(foam_io.read_dict(os.path.join(str(tmp_path), "1", "D"))["internalField"], pm.nCells, 3)
+    assert np.allclose(D, m.D, rtol=1e-9, atol=1e-20)
+    # same as the box generator's solve with the same settings
+    _, mesh = cases.cube_tension(ctx, n=6, jitter=0.1, seed=3)
+    ref = mesh.solve(nCorrectors=20000, alternativeTolerance=1e-10, solutionTolerance=1e-10, rhieChowScale=0.2,
+                     pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
+    assert np.linalg.norm(ref["D"] - m.D) / np.linalg.norm(ref["D"]) < 1e-9
