@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--calib-bytes", type=int, default=1 << 30,
+                    help="size of the plain copy kernel run once before the timed region (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
@@ -123,6 +125,10 @@ def main():
             _abi.check(L.s4f_profile(ctx.h, 0))
         return lat, outs, prof
 
+    if args.calib_bytes:
+        cms = ctypes_copy(L, ctx, args.calib_bytes)
+        print(f"[bench] k_copy {args.calib_bytes} B read + {args.calib_bytes} B written: {cms:.4f} ms "
+              f"({2 * args.calib_bytes / cms / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         mesh.solve(want_sigma=False, precision=1, **opts)
     res = {}
@@ -194,6 +200,13 @@ def main():
     }
     line.update(extra)
     print(json.dumps(line))
+
+
+def ctypes_copy(L, ctx, nbytes):
+    import ctypes
+    ms = ctypes.c_double()
+    L.s4f_calibrate_copy(ctx.h, int(nbytes), ctypes.byref(ms))
+    return ms.value
 
 
 def ctypes_read(L, ctx, kid):

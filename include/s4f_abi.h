@@ -175,6 +175,7 @@ typedef struct {
     double residualvf;       /* relative displacement change of the last outer iteration */
     double seconds;          /* wall time of the solve on the device stream */
     int32_t mixedFallback;   /* mixed precision fell back to FP64 inner sweeps */
+    int32_t warmStarted;     /* the solve started from a resident / supplied field */
 } s4f_solve_report;
 
 /* Fills opts with the reference defaults. */
@@ -233,6 +234,10 @@ enum { S4F_KERNEL_RESIDUAL = 0,   /* div(sigma) + Rhie-Chow face loop */
    sampleEvery-th launch of each kernel is bracketed by an event pair; launches
    that found their solve finished are excluded. */
 s4f_status s4f_profile(s4f_ctx *ctx, int32_t sampleEvery);
+/* Runs a plain streaming copy kernel (k_copy) of `bytes` read + `bytes`
+   written twice and returns the second launch's time: the known-size
+   reference for HBM traffic counters (FETCH_SIZE / WRITE_SIZE). */
+s4f_status s4f_calibrate_copy(s4f_ctx *ctx, int64_t bytes, double *ms);
 /* average duration (ms) of the sampled working launches of `kernel` */
 s4f_status s4f_profile_read(s4f_ctx *ctx, int32_t kernel, double *avgMs, int64_t *samples, int64_t *launches);
 

@@ -763,6 +763,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
     HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
     std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
+    for (int i = 0; i < nInst; ++i) c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
     HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
     Ctl* hctl = nullptr;
@@ -871,6 +872,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             rep->residualvf = cf.rvf;
             rep->seconds = ms * 1e-3;
             rep->mixedFallback = 0;
+            rep->warmStarted = cf.warm;
         }
         io[i].status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
     }
@@ -1410,6 +1412,29 @@ s4f_status s4f_profile(s4f_ctx* ctx, int32_t sampleEvery) {
         if (!ctx) throw InvalidError("null argument");
         if (sampleEvery < 0) throw InvalidError("sampleEvery must be >= 0");
         ctx->prof.reset(sampleEvery);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_calibrate_copy(s4f_ctx* ctx, int64_t bytes, double* ms) {
+    return guarded([&]() -> s4f_status {
+        if (!ctx || bytes < 32) throw InvalidError("bad argument");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const size_t n = (size_t)bytes / 32;
+        DArr<double> a, b;
+        a.alloc(4 * n);
+        b.alloc(4 * n);
+        HIP_CHECK(hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));
+        const int grid = (int)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, ctx->stream, (const double4*)a.p, (double4*)b.p, n);
+        HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, ctx->stream, (const double4*)a.p, (double4*)b.p, n);
+        HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        float t = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        if (ms) *ms = t;
         return S4F_OK;
     });
 }
@@ -2014,6 +2039,7 @@ void serverStep(s4f_server* S) {
         res.rep.innerTotal = cf.innerTotal;
         res.rep.initialResidual = cf.initResMag;
         res.rep.residualvf = cf.rvf;
+        res.rep.warmStarted = cf.warm;
         res.status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
         res.tSubmit = rq.tSubmit;
         res.tStart = rq.tStart;

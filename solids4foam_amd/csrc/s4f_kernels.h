@@ -819,4 +819,9 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_grad(const BlockMap bm, Ctl* 
     }
 }
 
+// plain streaming copy (traffic-counter calibration: known bytes moved)
+__global__ __launch_bounds__(kBlock) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) b[i] = a[i];
+}
+
 }  // namespace s4f

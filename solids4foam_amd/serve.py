@@ -49,7 +49,7 @@ class Server:
         st = lib().s4f_server_result(self.h, int(rid), ptr(D), ptr(sig), ctypes.byref(rep), ptr(tm))
         check(st, allow=(_abi.S4F_OK, _abi.S4F_WARN_MAXITER, _abi.S4F_ERR_DIVERGED))
         return dict(id=rid, D=D, sigma=sig, nOuter=rep.nOuter, converged=bool(rep.converged),
-                    innerTotal=int(rep.innerTotal), status=st, t_submit=tm[0], t_start=tm[1], t_finish=tm[2],
+                    innerTotal=int(rep.innerTotal), warm_started=bool(rep.warmStarted), status=st, t_submit=tm[0], t_start=tm[1], t_finish=tm[2],
                     latency=tm[2] - tm[0])
 
     def stats(self):
