@@ -46,7 +46,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def cpu_baseline(n, jitter, seed, nOuter, iters):
+def cpu_baseline(n, jitter, seed, nOuter, iters, reltol, tol):
     """Oracle (single-threaded C restatement, PCG + DIC relTol 0.1) on the same
     mesh: seconds for `iters` outer iterations after the first, scaled by the
     GPU FP64 outer-iteration count to solves/s."""
@@ -57,13 +57,14 @@ def cpu_baseline(n, jitter, seed, nOuter, iters):
     bv[om.patch_faces(5)] = (0.0, 0.0, 1e7)
     oc = oracle.Case(om, 200e9, 0.3, [1, 1, 1, 1, 0, 1], bv)
     setup = time.time() - t0
-    oc.time_iterations(1)  # iteration 0 (zero residual) and warm-up
-    sec = oc.time_iterations(iters)
+    kw = dict(pcgRelTol=reltol, pcgTol=1e-9, solTol=tol, altTol=tol)
+    oc.time_iterations(1, **kw)  # iteration 0 (zero residual) and warm-up
+    sec = oc.time_iterations(iters, **kw)
     per_iter = sec / iters
     cell_iter_s = om.nCells / per_iter
     return dict(value=1.0 / (per_iter * nOuter), unit="solves/s", cores=1, kind="port",
                 cell_iter_per_s=cell_iter_s,
-                sample=f"oracle (C restatement, 1 thread, PCG+DIC relTol 0.1) on the same {n}^3 mesh: "
+                sample=f"oracle (C restatement, 1 thread, PCG+DIC relTol {reltol}) on the same {n}^3 mesh: "
                        f"{iters} outer iteration(s) timed ({sec:.1f} s, setup {setup:.1f} s untimed); "
                        f"solves/s = 1/(s_per_outer_iter x {nOuter} outer iterations of the GPU FP64 solve)")
 
@@ -78,7 +79,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--reltol", type=float, default=0.1)
-    ap.add_argument("--profile-every", type=int, default=16)
+    ap.add_argument("--replay", type=int, default=40, help="back-to-back launches per kernel for the roofline")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
@@ -107,8 +108,6 @@ def main():
     L = _abi.lib()
 
     def run(precision, steps, profile):
-        if profile:
-            _abi.check(L.s4f_profile(ctx.h, args.profile_every))
         lat, outs = [], []
         for _ in range(steps):
             t = time.perf_counter()
@@ -118,11 +117,11 @@ def main():
             print(f"[bench] precision={precision} nOuter={out['nOuter']} inner={out['innerTotal']} "
                   f"converged={out['converged']} {lat[-1]:.2f}s", file=sys.stderr, flush=True)
         prof = {}
-        if profile:
+        if profile and args.replay > 0:
+            # each kernel replayed back to back on the solve's final state
             for k, kid in _abi.KERNELS.items():
-                avg, ns, nl = ctypes_read(L, ctx, kid)
-                prof[k] = dict(avg_ms=avg, samples=ns, launches=nl)
-            _abi.check(L.s4f_profile(ctx.h, 0))
+                n = args.replay if k in ("spmv", "pcg_update", "finalize") else max(4, args.replay // 4)
+                prof[k] = dict(avg_ms=ctypes_kernel_time(L, mesh, kid, precision, n), launches=n)
         return lat, outs, prof
 
     if args.calib_bytes:
@@ -144,18 +143,16 @@ def main():
     value = args.steps / total
     nOuter = outs[-1]["nOuter"]
     kern = {}
+    pcg_it = outs[-1]["pcgIterations"]
+    per_solve = {"spmv": pcg_it, "pcg_update": pcg_it, "finalize": 2 * pcg_it, "residual": nOuter, "grad": nOuter}
     for k, p in prof.items():
-        if p["samples"] == 0:
-            continue
-        if k == "finalize":
-            kern[k] = dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"], samples=p["samples"],
-                           total_ms_est=round(p["avg_ms"] * p["launches"], 1))
-            continue
-        b = algorithmic_bytes(k, C, F, B, T)
-        kern[k] = dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"], samples=p["samples"],
-                       alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1),
-                       total_ms_est=round(p["avg_ms"] * p["launches"], 1))
-    dom = max((k for k in kern if k != "finalize"), key=lambda k: kern[k]["total_ms_est"]) if kern else None
+        e = dict(avg_ms=round(p["avg_ms"], 5), replay_launches=p["launches"], launches_per_solve=per_solve[k],
+                 ms_per_solve_est=round(p["avg_ms"] * per_solve[k], 1))
+        if k != "finalize":
+            b = algorithmic_bytes(k, C, F, B, T)
+            e.update(alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1))
+        kern[k] = e
+    dom = max((k for k in kern if k != "finalize"), key=lambda k: kern[k]["ms_per_solve_est"]) if kern else None
     roofline = None
     if dom:
         a = kern[dom]["gbps"]
@@ -173,7 +170,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         n64 = extra.get("fp64_nOuter", nOuter)
-        cpu = cpu_baseline(args.n, args.jitter, args.seed, n64, args.cpu_iters)
+        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.n, args.jitter, args.seed, n64, args.cpu_iters, args.reltol, args.tol)
     line = {
         "metric": "converged solves/s & cell-iter/s at 1e-6 err vs FP64; p50/p99 solve latency",
         "value": round(value, 6),
@@ -192,6 +190,7 @@ def main():
                    "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"diagonal, relTol {args.reltol}"},
         "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
         "nOuter": nOuter,
+        "pcg_iterations": pcg_it,
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
         "roofline": roofline,
@@ -209,13 +208,12 @@ def ctypes_copy(L, ctx, nbytes):
     return ms.value
 
 
-def ctypes_read(L, ctx, kid):
+def ctypes_kernel_time(L, mesh, kid, precision, n):
     import ctypes
-    avg = ctypes.c_double()
-    ns = ctypes.c_int64()
-    nl = ctypes.c_int64()
-    L.s4f_profile_read(ctx.h, kid, ctypes.byref(avg), ctypes.byref(ns), ctypes.byref(nl))
-    return avg.value, ns.value, nl.value
+    from solids4foam_amd import _abi
+    ms = ctypes.c_double()
+    _abi.check(L.s4f_kernel_time(mesh.h, int(kid), int(precision), int(n), ctypes.byref(ms)))
+    return ms.value
 
 
 if __name__ == "__main__":

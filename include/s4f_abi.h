@@ -176,6 +176,7 @@ typedef struct {
     double seconds;          /* wall time of the solve on the device stream */
     int32_t mixedFallback;   /* mixed precision fell back to FP64 inner sweeps */
     int32_t warmStarted;     /* the solve started from a resident / supplied field */
+    int64_t pcgIterations;   /* PCG iterations (3-RHS sweeps) summed over outer iterations */
 } s4f_solve_report;
 
 /* Fills opts with the reference defaults. */
@@ -223,23 +224,23 @@ s4f_status s4f_matrix(s4f_mesh *mesh, double *diag, double *faceCoeff);
 /* measurement (no counterpart in the reference: its timing is the      */
 /* per-time-step ExecutionTime/ClockTime print, solids4Foam.C:68-72)    */
 /* ------------------------------------------------------------------ */
-/* kernel ids for s4f_profile_read */
+/* kernel ids for s4f_kernel_time */
 enum { S4F_KERNEL_RESIDUAL = 0,   /* div(sigma) + Rhie-Chow face loop */
        S4F_KERNEL_SPMV = 1,       /* PCG direction update fused with the 3-RHS SpMV */
        S4F_KERNEL_PCG_UPDATE = 2, /* PCG solution / residual update */
        S4F_KERNEL_FINALIZE = 3,   /* one-workgroup reduction finalize (alpha) */
        S4F_KERNEL_GRAD = 4 };     /* boundary update + least-squares gradient */
-/* Enables (sampleEvery > 0) or disables (0) sampled hipEvent timing of the
-   solver kernels on the context's stream and resets the counters: every
-   sampleEvery-th launch of each kernel is bracketed by an event pair; launches
-   that found their solve finished are excluded. */
-s4f_status s4f_profile(s4f_ctx *ctx, int32_t sampleEvery);
+/* Replays `kernel` back to back `launches` times on the mesh's current state
+   (e.g. right after a solve) with all three components active, on the
+   context's stream between one pair of hipEvents, and returns the average
+   duration in ms.  Launch-by-launch events would perturb the caches the
+   kernels rely on; this is the steady per-launch time. */
+s4f_status s4f_kernel_time(s4f_mesh *mesh, int32_t kernel, int32_t precision, int32_t launches, double *avgMs);
 /* Runs a plain streaming copy kernel (k_copy) of `bytes` read + `bytes`
    written twice and returns the second launch's time: the known-size
    reference for HBM traffic counters (FETCH_SIZE / WRITE_SIZE). */
 s4f_status s4f_calibrate_copy(s4f_ctx *ctx, int64_t bytes, double *ms);
-/* average duration (ms) of the sampled working launches of `kernel` */
-s4f_status s4f_profile_read(s4f_ctx *ctx, int32_t kernel, double *avgMs, int64_t *samples, int64_t *launches);
+
 
 /* ------------------------------------------------------------------ */
 /* batching: many meshes packed into one sliced-ELL structure          */

@@ -48,7 +48,7 @@ class SolveOpts(ctypes.Structure):
 class SolveReport(ctypes.Structure):
     _fields_ = [("nOuter", ctypes.c_int32), ("converged", ctypes.c_int32), ("innerTotal", ctypes.c_int64),
                 ("initialResidual", ctypes.c_double), ("residualvf", ctypes.c_double), ("seconds", ctypes.c_double),
-                ("mixedFallback", ctypes.c_int32), ("warmStarted", ctypes.c_int32)]
+                ("mixedFallback", ctypes.c_int32), ("warmStarted", ctypes.c_int32), ("pcgIterations", ctypes.c_int64)]
 
 
 class MeshDesc(ctypes.Structure):
@@ -70,7 +70,7 @@ EXPORTED = [
     "s4f_material_set", "s4f_bc_set", "s4f_solve_opts_default", "s4f_solve", "s4f_solve_history",
     "s4f_grad", "s4f_sigma", "s4f_residual", "s4f_spmv3", "s4f_pcg3", "s4f_matrix",
     "s4f_batch_create", "s4f_batch_add", "s4f_batch_solve", "s4f_batch_destroy",
-    "s4f_profile", "s4f_profile_read", "s4f_calibrate_copy",
+    "s4f_kernel_time", "s4f_calibrate_copy",
     "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
 ]
@@ -125,9 +125,8 @@ def lib():
         L.s4f_batch_add.argtypes = [vp, vp, ip]
         L.s4f_batch_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), vp, vp, ctypes.POINTER(SolveReport)]
         L.s4f_batch_destroy.argtypes = [vp]
-        L.s4f_profile.argtypes = [vp, ctypes.c_int32]
+        L.s4f_kernel_time.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]
         L.s4f_calibrate_copy.argtypes = [vp, ctypes.c_int64, vp]
-        L.s4f_profile_read.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         L.s4f_server_create.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(SolveOpts),
                                         ctypes.POINTER(vp)]
         L.s4f_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]

@@ -82,64 +82,6 @@ struct DArr {
 
 using namespace s4f;
 
-// Sampled per-kernel timing: every `every`-th launch of a kernel is bracketed
-// by a hipEvent pair on the library stream.  Samples are resolved after the
-// host synchronises; launches that found their solve already finished
-// (early-exit no-ops) are discarded so the average is that of working launches.
-namespace s4f {
-enum KernelId { K_RESIDUAL = 0, K_SPMV = 1, K_UPDATE = 2, K_FINALIZE = 3, K_GRAD = 4, K_COUNT = 5 };
-struct Profiler {
-    int every = 0;
-    struct Sample { int kernel; hipEvent_t a, b; int inner; int outer; };
-    std::vector<Sample> pending;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
-    long long launches[K_COUNT] = {0, 0, 0, 0, 0};
-    long long samples[K_COUNT] = {0, 0, 0, 0, 0};
-    double ms[K_COUNT] = {0, 0, 0, 0, 0};
-    std::pair<hipEvent_t, hipEvent_t> take() {
-        if (!pool.empty()) { auto e = pool.back(); pool.pop_back(); return e; }
-        hipEvent_t a, b;
-        HIP_CHECK(hipEventCreate(&a));
-        HIP_CHECK(hipEventCreate(&b));
-        return {a, b};
-    }
-    // returns the sample index or -1
-    int begin(int k, hipStream_t s, int inner, int outer) {
-        const long long n = launches[k]++;
-        if (every <= 0 || n % every) return -1;
-        auto e = take();
-        HIP_CHECK(hipEventRecord(e.first, s));
-        pending.push_back({k, e.first, e.second, inner, outer});
-        return (int)pending.size() - 1;
-    }
-    void end(int idx, hipStream_t s) {
-        if (idx >= 0) HIP_CHECK(hipEventRecord(pending[idx].b, s));
-    }
-    // after a stream synchronisation: keep samples of launches that did work
-    template <typename Pred>
-    void resolve(Pred didWork) {
-        for (auto& x : pending) {
-            if (didWork(x)) {
-                float t = 0.f;
-                HIP_CHECK(hipEventElapsedTime(&t, x.a, x.b));
-                ms[x.kernel] += t;
-                samples[x.kernel]++;
-            }
-            pool.push_back({x.a, x.b});
-        }
-        pending.clear();
-    }
-    void reset(int ev) {
-        every = ev;
-        for (int k = 0; k < K_COUNT; ++k) { launches[k] = 0; samples[k] = 0; ms[k] = 0; }
-    }
-    ~Profiler() {
-        for (auto& x : pending) { (void)hipEventDestroy(x.a); (void)hipEventDestroy(x.b); }
-        for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-    }
-};
-}  // namespace s4f
-
 // A context lives until it has been destroyed AND its last mesh/batch is
 // gone: destroying it earlier only marks it, so handles never outlive it.
 struct s4f_ctx {
@@ -149,7 +91,6 @@ struct s4f_ctx {
     int nCU = 0;
     int children = 0;
     bool destroyPending = false;
-    s4f::Profiler prof;
 };
 
 namespace {
@@ -705,19 +646,13 @@ void allocState(s4f_mesh* M) {
 // residual update, convergence + beta.  v.p / v.p2 swap roles every iteration.
 template <typename T>
 void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, int nInst, PcgVecs<T>& v, Ctl* ctl, double* part, int n,
-                    hipStream_t s, Profiler* pf = nullptr, int inner0 = 0, int outer = 0) {
+                    hipStream_t s) {
     for (int i = 0; i < n; ++i) {
-        int x = pf ? pf->begin(K_SPMV, s, inner0 + i, outer) : -1;
         hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
                            v.p2, v.q, ctl, part, 1);
-        if (pf) pf->end(x, s);
-        x = pf ? pf->begin(K_FINALIZE, s, inner0 + i, outer) : -1;
         hipLaunchKernelGGL(k_fin_spmv, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
-        if (pf) pf->end(x, s);
-        x = pf ? pf->begin(K_UPDATE, s, inner0 + i, outer) : -1;
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
                            (const T*)v.q, (const Ctl*)ctl, part);
-        if (pf) pf->end(x, s);
         hipLaunchKernelGGL(k_fin_update, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         std::swap(v.p, v.p2);
     }
@@ -779,17 +714,15 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
                        st.hist.p, 1);
     HIP_CHECK(hipGetLastError());
     int chunk = 4;
-    Profiler* pf = (ctx->prof.every > 0 && nInst == 1) ? &ctx->prof : nullptr;
+    long long pcgIters = 0;
     for (int it = 0;; ++it) {
-        int xr = pf ? pf->begin(K_RESIDUAL, s, 0, it) : -1;
         hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
                            (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
-        if (pf) pf->end(xr, s);
         hipLaunchKernelGGL(k_fin_residual, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
         bool allDone = false;
         int total = 0;
         for (;;) {
-            launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s, pf, total, it);
+            launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s);
             total += chunk;
             chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
             HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -803,27 +736,16 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             if (allDone || !anyInner) break;
             if (total > o.pcgMaxIter + 2) break;  // maxIter deactivates every component before this
         }
-        if (pf) {
-            // launches beyond the iterations the PCG ran (or of a finished solve) did no work
-            const int ran = allDone ? -1 : std::max({hctl[0].nIter[0], hctl[0].nIter[1], hctl[0].nIter[2]});
-            pf->resolve([&](const Profiler::Sample& x) {
-                if (x.outer != it) return x.outer < it;  // k_grad of the previous iteration
-                if (ran < 0) return false;
-                if (x.kernel == K_RESIDUAL) return true;
-                return x.inner < ran;  // spmv, alpha finalize and update of working iterations
-            });
-        }
         if (allDone) break;
         int used = 1;
         for (int i = 0; i < nInst; ++i)
             if (!hctl[i].outerDone) used = std::max({used, hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});
+        pcgIters += std::max({hctl[0].nIter[0], hctl[0].nIter[1], hctl[0].nIter[2]});
         // next outer iteration: launch just under the last count, then top up
         chunk = std::max(1, used - 1);
-        int xg = pf ? pf->begin(K_GRAD, s, 0, it) : -1;
         hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
                            st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
                            0);
-        if (pf) pf->end(xg, s);
         hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
                            st.hist.p, 0);
         HIP_CHECK(hipGetLastError());
@@ -873,6 +795,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             rep->seconds = ms * 1e-3;
             rep->mixedFallback = 0;
             rep->warmStarted = cf.warm;
+            rep->pcgIterations = nInst == 1 ? pcgIters : 0;
         }
         io[i].status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
     }
@@ -1407,15 +1330,6 @@ s4f_status s4f_matrix(s4f_mesh* M, double* diag, double* faceCoeff) {
     return S4F_OK;
 }
 
-s4f_status s4f_profile(s4f_ctx* ctx, int32_t sampleEvery) {
-    return guarded([&]() -> s4f_status {
-        if (!ctx) throw InvalidError("null argument");
-        if (sampleEvery < 0) throw InvalidError("sampleEvery must be >= 0");
-        ctx->prof.reset(sampleEvery);
-        return S4F_OK;
-    });
-}
-
 s4f_status s4f_calibrate_copy(s4f_ctx* ctx, int64_t bytes, double* ms) {
     return guarded([&]() -> s4f_status {
         if (!ctx || bytes < 32) throw InvalidError("bad argument");
@@ -1439,13 +1353,75 @@ s4f_status s4f_calibrate_copy(s4f_ctx* ctx, int64_t bytes, double* ms) {
     });
 }
 
-s4f_status s4f_profile_read(s4f_ctx* ctx, int32_t kernel, double* avgMs, int64_t* samples, int64_t* launches) {
-    if (!ctx || kernel < 0 || kernel >= K_COUNT) return fail(S4F_ERR_INVALID, "bad kernel id");
-    const Profiler& p = ctx->prof;
-    if (avgMs) *avgMs = p.samples[kernel] ? p.ms[kernel] / p.samples[kernel] : 0.0;
-    if (samples) *samples = p.samples[kernel];
-    if (launches) *launches = p.launches[kernel];
-    return S4F_OK;
+}  // extern "C"
+
+// Back-to-back replay of one solver kernel on the mesh's current state (all
+// three components active), timed by one event pair around `launches`
+// launches on the library stream: the kernel's steady duration without the
+// perturbation per-launch events cause.
+template <typename T>
+double replayT(s4f_mesh* M, int kernel, int launches) {
+    hipStream_t s = M->ctx->stream;
+    allocState(M);
+    SolveState& st = M->st;
+    PcgVecs<T> v = pcgVecs<T>(st, M->h.C);
+    const DevMesh dm = M->dm();
+    const BlockMap bm = M->bmap();
+    Ctl c;
+    std::memset(&c, 0, sizeof(c));
+    for (int j = 0; j < 3; ++j) { c.active[j] = 1; c.alpha[j] = 0.5; c.beta[j] = 0.5; c.rho[j] = 1.0; c.normFactor[j] = 1.0; }
+    DArr<Ctl> rc;
+    rc.alloc(1);
+    HIP_CHECK(hipMemcpyAsync(rc.p, &c, sizeof(Ctl), hipMemcpyHostToDevice, s));
+    auto launch = [&]() {
+        switch (kernel) {
+            case S4F_KERNEL_RESIDUAL:
+                hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
+                                   (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, rc.p, st.part.p);
+                break;
+            case S4F_KERNEL_SPMV:
+                hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                                   (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1);
+                std::swap(v.p, v.p2);
+                break;
+            case S4F_KERNEL_PCG_UPDATE:
+                hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p, v.r,
+                                   (const T*)v.q, (const Ctl*)rc.p, st.part.p);
+                break;
+            case S4F_KERNEL_FINALIZE:
+                hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kFinBlock), 0, s, bm, rc.p, (const double*)st.part.p);
+                break;
+            case S4F_KERNEL_GRAD:
+                hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x,
+                                   st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, (const double*)nullptr,
+                                   rc.p, st.part.p, (double*)nullptr, 0);
+                break;
+            default:
+                throw InvalidError("unknown kernel id");
+        }
+    };
+    launch();  // warm
+    HIP_CHECK(hipEventRecord(M->ctx->ev0, s));
+    for (int i = 0; i < launches; ++i) launch();
+    HIP_CHECK(hipEventRecord(M->ctx->ev1, s));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventSynchronize(M->ctx->ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, M->ctx->ev0, M->ctx->ev1));
+    return ms / launches;
+}
+
+extern "C" {
+
+s4f_status s4f_kernel_time(s4f_mesh* M, int32_t kernel, int32_t precision, int32_t launches, double* avgMs) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !avgMs || launches < 1) throw InvalidError("bad argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        *avgMs = precision == S4F_PRECISION_MIXED ? replayT<float>(M, kernel, launches)
+                                                  : replayT<double>(M, kernel, launches);
+        return S4F_OK;
+    });
 }
 
 struct s4f_batch {

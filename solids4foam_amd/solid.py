@@ -147,7 +147,7 @@ class SolverMesh:
         check(st, allow=(_abi.S4F_OK, _abi.S4F_WARN_MAXITER))
         out = dict(D=D, sigma=sig, gradD=G, Db=Db, nOuter=rep.nOuter, converged=bool(rep.converged),
                    innerTotal=int(rep.innerTotal), initialResidual=rep.initialResidual, residualvf=rep.residualvf,
-                   seconds=rep.seconds, status=st)
+                   seconds=rep.seconds, pcgIterations=int(rep.pcgIterations), status=st)
         return out
 
     def history(self, cap=100000):
