@@ -63,7 +63,7 @@ enum {
 /* inner preconditioner (fvSolution "preconditioner") */
 enum {
     S4F_PRECOND_DIAGONAL = 0,  /* OpenFOAM "diagonal" (Jacobi) */
-    S4F_PRECOND_CHEBYSHEV = 1  /* Chebyshev polynomial of the Jacobi-scaled matrix */
+    S4F_PRECOND_GAMG = 2       /* aggregation multigrid V(1,1) cycle (PCG with preconditioner GAMG) */
 };
 
 /* ------------------------------------------------------------------ */
@@ -162,7 +162,7 @@ typedef struct {
     double pcgRelTol;              /* default 0.1 */
     int32_t pcgMaxIter;            /* default 1000 */
     int32_t preconditioner;        /* S4F_PRECOND_* */
-    int32_t chebyshevDegree;       /* polynomial degree when preconditioner = CHEBYSHEV */
+    int32_t reserved0;             /* zero */
     int32_t precision;             /* S4F_PRECISION_* */
     int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
 } s4f_solve_opts;

@@ -25,7 +25,7 @@ PATCH_TRACTION = 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECOND_DIAGONAL = 0
-PRECOND_CHEBYSHEV = 1
+PRECOND_GAMG = 2
 
 _STATUS_NAMES = {0: "OK", 1: "ERR_INVALID", 2: "ERR_HIP", 3: "ERR_OOM", 4: "ERR_DIVERGED", 5: "WARN_MAXITER",
                  6: "ERR_ADMISSION", 7: "ERR_NOT_READY"}
@@ -42,7 +42,7 @@ class SolveOpts(ctypes.Structure):
                 ("alternativeTolerance", ctypes.c_double), ("rhieChowScale", ctypes.c_double),
                 ("interfaceScale", ctypes.c_double), ("pcgTolerance", ctypes.c_double),
                 ("pcgRelTol", ctypes.c_double), ("pcgMaxIter", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
-                ("chebyshevDegree", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32)]
+                ("reserved0", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

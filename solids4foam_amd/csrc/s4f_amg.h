@@ -1,0 +1,325 @@
+// s4f_amg.h -- aggregation multigrid V-cycle used as the PCG preconditioner
+// (the device counterpart of running OpenFOAM's PCG with `preconditioner GAMG`).
+//
+// Hierarchy: plain (piecewise-constant) aggregation of strongly connected
+// cells, Galerkin coarse operators A_c = P^T A P, built once per matrix on the
+// host.  Cycle: V(1,1) with damped Jacobi smoothing (omega = 2/3) and an
+// exact solve on the coarsest level (explicit inverse, n <= kCoarseMax), so
+// the cycle is a fixed symmetric positive definite operator and CG stays CG.
+//
+// Launch shape per level (vectors hold 3 components, SoA, stride = the
+// level's row count; converged components are skipped):
+//   down: x = w D^-1 b and r = b - A x in one pass (the neighbours' x is
+//         recomputed from their b and 1/diag), then the restriction
+//         b_c[I] = sum_{i in I} r[i] (members gathered, no atomics)
+//   up:   prolongation x += e[agg] fused into the post-smoothing sweep (the
+//         neighbours' corrected x recomputed on the fly)
+// Level 0 is the solver mesh's own sliced-ELL matrix; coarser levels have
+// their own sliced-ELL arrays.
+#pragma once
+
+#include "s4f_kernels.h"
+
+namespace s4f {
+
+constexpr int kCoarseMax = 1024;  // largest level solved by its explicit inverse
+
+// device view of one coarse level
+struct MgLevel {
+    int n;             // rows
+    int nSlices;
+    const int* sliceStart;
+    const int* sliceWidth;
+    const int* col;    // >= 0 neighbour row, kPad empty
+    const float* a32;  // off-diagonal magnitude (entry = -a)
+    const double* a64;
+    const float* diag32;
+    const double* diag64;
+    const float* invD32;
+    const double* invD64;
+    const int* agg;    // row -> row of the next coarser level (nullptr on the coarsest)
+    const int* memOff; // (coarse side) row -> first member of the finer level
+    const int* mem;    // members (rows of the finer level)
+};
+
+template <typename T> struct MgSel;
+template <> struct MgSel<double> {
+    __device__ static const double* a(const MgLevel& L) { return L.a64; }
+    __device__ static const double* d(const MgLevel& L) { return L.diag64; }
+    __device__ static const double* iD(const MgLevel& L) { return L.invD64; }
+};
+template <> struct MgSel<float> {
+    __device__ static const float* a(const MgLevel& L) { return L.a32; }
+    __device__ static const float* d(const MgLevel& L) { return L.diag32; }
+    __device__ static const float* iD(const MgLevel& L) { return L.invD32; }
+};
+
+__device__ __forceinline__ bool mgSkip(const Ctl* c) { return c->outerDone || c->innerDone; }
+
+// ---------------------------------------------------------------------------
+// level 0 (the mesh's own matrix, BlockMap iteration)
+// ---------------------------------------------------------------------------
+
+// pre-smoothing from zero and the residual for the restriction:
+// x = w D^-1 b, r = b - A x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
+                                                     T* __restrict__ x, T* __restrict__ r, double omega,
+                                                     const Ctl* __restrict__ ctl) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const Ctl* c = ctl + inst;
+    if (mgSkip(c)) return;
+    const int C = m.nCells;
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ dg = MatSel<T>::diag(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const T w = (T)omega;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        const int e0 = m.sliceStart[s];
+        const int width = m.sliceWidth[s];
+        const T iD = invD[P];
+        T xP[3], y[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            xP[j] = act[j] ? w * iD * b[(size_t)j * C + P] : (T)0;
+            y[j] = dg[P] * xP[j];
+        }
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int n = m.col[e];
+            if (n >= 0) {
+                const T a = A[e];
+                const T iDn = invD[n];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (act[j]) y[j] -= a * (w * iDn * b[(size_t)j * C + n]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * C + P;
+            x[i] = xP[j];
+            r[i] = b[i] - y[j];
+        }
+    }
+}
+
+// prolongation + post-smoothing: xf = x + e[agg] (e == nullptr: no coarse
+// correction), z = (1 - w) xf + w D^-1 (b + sum a xf_N), with partial z.b
+// (b is the PCG residual: z.b = rho of the preconditioned residual)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
+                                                   const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
+                                                   const T* __restrict__ e, T* __restrict__ z, double omega,
+                                                   const Ctl* __restrict__ ctl, double* __restrict__ part) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const Ctl* c = ctl + inst;
+    if (mgSkip(c)) return;
+    const int C = m.nCells;
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const T w = (T)omega, w1 = (T)(1.0 - omega);
+    auto xf = [&](int j, int cell) -> T {
+        const T v = x[(size_t)j * C + cell];
+        return e ? v + e[(size_t)j * nCoarse + agg[cell]] : v;
+    };
+    double acc[3] = {0, 0, 0};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        const int e0 = m.sliceStart[s];
+        const int width = m.sliceWidth[s];
+        T sum[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sum[j] = act[j] ? b[(size_t)j * C + P] : (T)0;
+        for (int k = 0; k < width; ++k) {
+            const int en = e0 + k * kSlice + lane;
+            const int n = m.col[en];
+            if (n >= 0) {
+                const T a = A[en];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (act[j]) sum[j] += a * xf(j, n);
+            }
+        }
+        const T iD = invD[P];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * C + P;
+            const T zv = w1 * xf(j, P) + w * iD * sum[j];
+            z[i] = zv;
+            acc[j] += (double)zv * (double)b[i];
+        }
+    }
+    storeBlockPartials<3>(acc, 0u, part + (size_t)lb * kMaxPartials);
+}
+
+// ---------------------------------------------------------------------------
+// coarse levels (grid-stride over the slices of one level)
+// ---------------------------------------------------------------------------
+
+// restriction: bc[I] = sum over members i of I of r[i]   (r of the finer level)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const T* __restrict__ r, int nFine,
+                                                        T* __restrict__ bc, const Ctl* __restrict__ ctl) {
+    if (mgSkip(ctl)) return;
+    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
+    for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
+        T s[3] = {(T)0, (T)0, (T)0};
+        for (int k = Lc.memOff[I]; k < Lc.memOff[I + 1]; ++k) {
+            const int i = Lc.mem[k];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (act[j]) s[j] += r[(size_t)j * nFine + i];
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (act[j]) bc[(size_t)j * Lc.n + I] = s[j];
+    }
+}
+
+// down on a coarse level: x = w D^-1 b, r = b - A x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __restrict__ b, T* __restrict__ x,
+                                                    T* __restrict__ r, double omega, const Ctl* __restrict__ ctl) {
+    if (mgSkip(ctl)) return;
+    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
+    const T* __restrict__ A = MgSel<T>::a(L);
+    const T* __restrict__ iDg = MgSel<T>::iD(L);
+    const T* __restrict__ dg = MgSel<T>::d(L);
+    const T w = (T)omega;
+    const int n = L.n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wavesTotal = gridDim.x * (kBlock / 64);
+    for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
+        const int P = s * kSlice + lane;
+        if (P >= n) continue;
+        const int e0 = L.sliceStart[s];
+        const int width = L.sliceWidth[s];
+        const T iD = iDg[P];
+        T xP[3], y[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            xP[j] = act[j] ? w * iD * b[(size_t)j * n + P] : (T)0;
+            y[j] = dg[P] * xP[j];
+        }
+        for (int k = 0; k < width; ++k) {
+            const int e = e0 + k * kSlice + lane;
+            const int c = L.col[e];
+            if (c >= 0) {
+                const T a = A[e];
+                const T iDc = iDg[c];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (act[j]) y[j] -= a * (w * iDc * b[(size_t)j * n + c]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * n + P;
+            x[i] = xP[j];
+            r[i] = b[i] - y[j];
+        }
+    }
+}
+
+// up on a coarse level: xf = x + e[agg], out = (1 - w) xf + w D^-1 (b + sum a xf_N)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
+                                                  int nCoarse, const T* __restrict__ e, T* __restrict__ out,
+                                                  double omega, const Ctl* __restrict__ ctl) {
+    if (mgSkip(ctl)) return;
+    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
+    const T* __restrict__ A = MgSel<T>::a(L);
+    const T* __restrict__ iDg = MgSel<T>::iD(L);
+    const T w = (T)omega, w1 = (T)(1.0 - omega);
+    const int n = L.n;
+    const int* __restrict__ agg = L.agg;
+    auto xf = [&](int j, int row) -> T { return x[(size_t)j * n + row] + e[(size_t)j * nCoarse + agg[row]]; };
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wavesTotal = gridDim.x * (kBlock / 64);
+    for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
+        const int P = s * kSlice + lane;
+        if (P >= n) continue;
+        const int e0 = L.sliceStart[s];
+        const int width = L.sliceWidth[s];
+        T sum[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sum[j] = act[j] ? b[(size_t)j * n + P] : (T)0;
+        for (int k = 0; k < width; ++k) {
+            const int en = e0 + k * kSlice + lane;
+            const int c = L.col[en];
+            if (c >= 0) {
+                const T a = A[en];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (act[j]) sum[j] += a * xf(j, c);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (act[j]) out[(size_t)j * n + P] = w1 * xf(j, P) + w * iDg[P] * sum[j];
+    }
+}
+
+// coarsest level: x = A^-1 b with the explicit (symmetric) inverse, one
+// wavefront per row
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_dense(int n, const T* __restrict__ Ainv, const T* __restrict__ b,
+                                                     T* __restrict__ x, const Ctl* __restrict__ ctl) {
+    if (mgSkip(ctl)) return;
+    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
+    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc[3] = {0, 0, 0};
+    const T* __restrict__ a = Ainv + (size_t)row * n;
+    for (int c = lane; c < n; c += 64) {
+        const double v = (double)a[c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (act[j]) acc[j] += v * (double)b[(size_t)j * n + c];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[j] = waveReduce<false>(acc[j]);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (act[j]) x[(size_t)j * n + row] = (T)acc[j];
+    }
+}
+
+// rho' = z.r finalize for a V-cycle-preconditioned PCG: beta = rho'/rho
+// (first iteration: no beta), rho = rho'
+__global__ __launch_bounds__(kFinBlock) void k_fin_rho(const BlockMap bm, Ctl* __restrict__ ctlAll,
+                                                       const double* __restrict__ part) {
+    const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
+    Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone || ctl->innerDone) return;
+    __shared__ double tot[3];
+    finReduce<3>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < 3; ++j) {
+            if (!ctl->active[j]) continue;
+            ctl->beta[j] = ctl->firstInner ? 0.0 : tot[j] / ctl->rho[j];
+            ctl->rho[j] = tot[j];
+        }
+    }
+}
+
+}  // namespace s4f

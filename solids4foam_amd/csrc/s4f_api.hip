@@ -25,6 +25,7 @@
 #include "s4f_device.h"
 #include "s4f_internal.h"
 #include "s4f_kernels.h"
+#include "s4f_amg.h"
 
 namespace s4f {
 
@@ -137,8 +138,8 @@ struct HostMesh {
 
 struct SolveState {
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
-    DArr<double> r64, p64, q64, x64, pb64;
-    DArr<float> r32, p32, q32, x32, pb32;
+    DArr<double> r64, p64, q64, x64, pb64, z64;
+    DArr<float> r32, p32, q32, x32, pb32, z32;
     DArr<Ctl> ctl;
     DArr<double> part, hist;
     int histCap = 0;
@@ -146,9 +147,42 @@ struct SolveState {
     int lastNOuter = 0;
 };
 
+// Aggregation multigrid hierarchy of a mesh's matrix M (host build + device arrays)
+struct AmgHostLevel {
+    int n = 0, nSlices = 0;
+    std::vector<int> sliceStart, sliceWidth, col;
+    std::vector<double> a, diag;
+    std::vector<int> agg;          // rows -> rows of the next coarser level (empty on the coarsest)
+    std::vector<int> memOff, mem;  // rows -> member rows of the finer level (empty on level 0)
+};
+struct AmgDevLevel {
+    int n = 0, nSlices = 0;
+    DArr<int> sliceStart, sliceWidth, col, agg, memOff, mem;
+    DArr<double> a64, d64, iD64;
+    DArr<float> a32, d32, iD32;
+    DArr<double> x64, b64, r64, xs64, inv64;  // inv: explicit inverse (coarsest level only)
+    DArr<float> x32, b32, r32, xs32, inv32;
+    MgLevel view() const {
+        MgLevel L{};
+        L.n = n; L.nSlices = nSlices;
+        L.sliceStart = sliceStart.p; L.sliceWidth = sliceWidth.p; L.col = col.p;
+        L.a32 = a32.p; L.a64 = a64.p; L.diag32 = d32.p; L.diag64 = d64.p; L.invD32 = iD32.p; L.invD64 = iD64.p;
+        L.agg = agg.p; L.memOff = memOff.p; L.mem = mem.p;
+        return L;
+    }
+};
+struct Amg {
+    unsigned gen = ~0u;  // material generation the hierarchy was built for
+    std::vector<AmgHostLevel> h;
+    std::vector<std::unique_ptr<AmgDevLevel>> d;
+    double omega = 2.0 / 3.0;
+    double buildSeconds = 0.0;
+};
+
 struct s4f_mesh {
     s4f_ctx* ctx = nullptr;
     HostMesh h;
+    Amg amg;
     // device geometry / addressing
     DArr<int> sliceStart, sliceWidth, col, bCell, bType;
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
@@ -597,11 +631,17 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
     return c;
 }
 
+void singleMeshOnly(const s4f_solve_opts& o, const char* entry) {
+    if (o.preconditioner == S4F_PRECOND_GAMG)
+        throw InvalidError(std::string(entry) + ": the multigrid preconditioner is available in s4f_solve only");
+}
+
 void checkOpts(const s4f_solve_opts& o) {
     if (o.nCorrectors < 1) throw InvalidError("nCorrectors must be >= 1");
     if (o.pcgMaxIter < 0) throw InvalidError("maxIter must be >= 0");
     if (o.precision != S4F_PRECISION_FP64 && o.precision != S4F_PRECISION_MIXED) throw InvalidError("unknown precision");
-    if (o.preconditioner != S4F_PRECOND_DIAGONAL) throw InvalidError("preconditioner not available in this build");
+    if (o.preconditioner != S4F_PRECOND_DIAGONAL && o.preconditioner != S4F_PRECOND_GAMG)
+        throw InvalidError("unknown preconditioner");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -611,6 +651,7 @@ constexpr int kHistCap = 100000;
 template <typename T>
 struct PcgVecs {
     T *r, *p, *q, *x, *p2;  // p and p2 alternate as old/new search direction
+    T* z = nullptr;         // preconditioned residual (V-cycle output)
 };
 
 template <typename T>
@@ -619,13 +660,15 @@ template <>
 PcgVecs<double> pcgVecs<double>(SolveState& st, int C) {
     st.r64.alloc(3 * (size_t)C); st.p64.alloc(3 * (size_t)C); st.q64.alloc(3 * (size_t)C); st.x64.alloc(3 * (size_t)C);
     st.pb64.alloc(3 * (size_t)C);
-    return {st.r64.p, st.p64.p, st.q64.p, st.x64.p, st.pb64.p};
+    st.z64.alloc(3 * (size_t)C);
+    return {st.r64.p, st.p64.p, st.q64.p, st.x64.p, st.pb64.p, st.z64.p};
 }
 template <>
 PcgVecs<float> pcgVecs<float>(SolveState& st, int C) {
     st.r32.alloc(3 * (size_t)C); st.p32.alloc(3 * (size_t)C); st.q32.alloc(3 * (size_t)C); st.x32.alloc(3 * (size_t)C);
     st.pb32.alloc(3 * (size_t)C);
-    return {st.r32.p, st.p32.p, st.q32.p, st.x32.p, st.pb32.p};
+    st.z32.alloc(3 * (size_t)C);
+    return {st.r32.p, st.p32.p, st.q32.p, st.x32.p, st.pb32.p, st.z32.p};
 }
 
 void allocState(s4f_mesh* M) {
@@ -649,11 +692,329 @@ void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, int nInst, PcgVecs<T>
                     hipStream_t s) {
     for (int i = 0; i < n; ++i) {
         hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
-                           v.p2, v.q, ctl, part, 1);
+                           v.p2, v.q, ctl, part, 1, (const T*)nullptr);
         hipLaunchKernelGGL(k_fin_spmv, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
                            (const T*)v.q, (const Ctl*)ctl, part);
         hipLaunchKernelGGL(k_fin_update, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        std::swap(v.p, v.p2);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// aggregation multigrid: host build
+// ---------------------------------------------------------------------------
+template <typename F>
+inline void forRow(const std::vector<int>& ss, const std::vector<int>& sw, const std::vector<int>& col, int i, F&& f) {
+    const int sI = i / kSlice, l = i % kSlice;
+    for (int k = 0; k < sw[sI]; ++k) {
+        const int e = ss[sI] + k * kSlice + l;
+        if (col[e] >= 0) f(e, col[e]);
+    }
+}
+
+// Greedy aggregation on strong connections (a_ij >= theta max_k a_ik):
+// seed aggregates from rows whose strong neighbours are all free, attach the
+// rest to their strongest aggregated neighbour, leftovers become singletons.
+int aggregate(const AmgHostLevel& L, std::vector<int>& agg) {
+    const double theta = 0.25;
+    const int n = L.n;
+    agg.assign(n, -1);
+    std::vector<double> amax(n, 0.0);
+    for (int i = 0; i < n; ++i) forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int) { amax[i] = std::max(amax[i], L.a[e]); });
+    int nc = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free = true;
+        forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
+            if (L.a[e] >= theta * amax[i] && agg[j] >= 0) free = false;
+        });
+        if (!free) continue;
+        agg[i] = nc;
+        forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
+            if (L.a[e] >= theta * amax[i]) agg[j] = nc;
+        });
+        ++nc;
+    }
+    std::vector<int> joined(n, -1);
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        double best = -1.0;
+        int to = -1;
+        forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
+            if (agg[j] >= 0 && L.a[e] >= theta * amax[i] && L.a[e] > best) { best = L.a[e]; to = agg[j]; }
+        });
+        joined[i] = to;
+    }
+    for (int i = 0; i < n; ++i)
+        if (agg[i] < 0 && joined[i] >= 0) agg[i] = joined[i];
+    for (int i = 0; i < n; ++i)
+        if (agg[i] < 0) agg[i] = nc++;
+    return nc;
+}
+
+// Galerkin product with piecewise-constant prolongation: coarse diagonal
+// = sum of member diagonals minus intra-aggregate couplings, off-diagonal
+// magnitudes summed per coarse pair; stored as sliced ELL sorted by column.
+void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHostLevel& Cl) {
+    Cl = AmgHostLevel();
+    Cl.n = nc;
+    Cl.memOff.assign(nc + 1, 0);
+    for (int i = 0; i < F.n; ++i) Cl.memOff[agg[i] + 1]++;
+    for (int I = 0; I < nc; ++I) Cl.memOff[I + 1] += Cl.memOff[I];
+    Cl.mem.resize(F.n);
+    {
+        std::vector<int> fill(Cl.memOff.begin(), Cl.memOff.end() - 1);
+        for (int i = 0; i < F.n; ++i) Cl.mem[fill[agg[i]]++] = i;
+    }
+    Cl.diag.assign(nc, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> rows(nc);
+    for (int I = 0; I < nc; ++I) {
+        std::vector<std::pair<int, double>>& r = rows[I];
+        for (int k = Cl.memOff[I]; k < Cl.memOff[I + 1]; ++k) {
+            const int i = Cl.mem[k];
+            Cl.diag[I] += F.diag[i];
+            forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) {
+                const int J = agg[j];
+                if (J == I) Cl.diag[I] -= F.a[e];
+                else r.push_back({J, F.a[e]});
+            });
+        }
+        std::sort(r.begin(), r.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+            return x.first < y.first;
+        });
+        size_t w = 0;
+        for (size_t k = 0; k < r.size(); ++k) {
+            if (w > 0 && r[w - 1].first == r[k].first) r[w - 1].second += r[k].second;
+            else r[w++] = r[k];
+        }
+        r.resize(w);
+    }
+    Cl.nSlices = (nc + kSlice - 1) / kSlice;
+    Cl.sliceStart.resize((size_t)Cl.nSlices + 1);
+    Cl.sliceWidth.resize(Cl.nSlices);
+    size_t E = 0;
+    for (int sI = 0; sI < Cl.nSlices; ++sI) {
+        Cl.sliceStart[sI] = (int)E;
+        int wmax = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            const int I = sI * kSlice + l;
+            if (I < nc) wmax = std::max(wmax, (int)rows[I].size());
+        }
+        Cl.sliceWidth[sI] = wmax;
+        E += (size_t)wmax * kSlice;
+    }
+    Cl.sliceStart[Cl.nSlices] = (int)E;
+    Cl.col.assign(E, kPad);
+    Cl.a.assign(E, 0.0);
+    for (int I = 0; I < nc; ++I) {
+        const int sI = I / kSlice, l = I % kSlice;
+        for (size_t k = 0; k < rows[I].size(); ++k) {
+            const size_t e = (size_t)Cl.sliceStart[sI] + k * kSlice + l;
+            Cl.col[e] = rows[I][k].first;
+            Cl.a[e] = rows[I][k].second;
+        }
+    }
+}
+
+// Explicit inverse of an SPD matrix (row-major n x n) by Cholesky:
+// A = L L^T, A^-1 = L^-T L^-1.  Returns false on a non-positive pivot.
+bool spdInverse(const AmgHostLevel& H, std::vector<double>& inv) {
+    const int n = H.n;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        A[(size_t)i * n + i] = H.diag[i];
+        forRow(H.sliceStart, H.sliceWidth, H.col, i, [&](int e, int j) { A[(size_t)i * n + j] -= H.a[e]; });
+    }
+    for (int j = 0; j < n; ++j) {  // in-place Cholesky, lower triangle
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = v / d;
+        }
+    }
+    std::vector<double> Li((size_t)n * n, 0.0);  // L^-1, lower triangular
+    for (int j = 0; j < n; ++j) {
+        Li[(size_t)j * n + j] = 1.0 / A[(size_t)j * n + j];
+        for (int i = j + 1; i < n; ++i) {
+            double v = 0.0;
+            for (int k = j; k < i; ++k) v -= A[(size_t)i * n + k] * Li[(size_t)k * n + j];
+            Li[(size_t)i * n + j] = v / A[(size_t)i * n + i];
+        }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+            for (int k = i; k < n; ++k) v += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
+            inv[(size_t)i * n + j] = v;
+            inv[(size_t)j * n + i] = v;
+        }
+    return true;
+}
+
+void buildAmg(s4f_mesh* M) {
+    Amg& A = M->amg;
+    if (A.gen == M->generation && !A.d.empty()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    const HostMesh& h = M->h;
+    if (h.instCell0.size() != 1) throw InvalidError("the multigrid preconditioner takes a single mesh");
+    A.h.clear();
+    A.d.clear();
+    AmgHostLevel L0;
+    L0.n = h.C; L0.nSlices = h.nSlices;
+    L0.sliceStart = h.sliceStart; L0.sliceWidth = h.sliceWidth; L0.col = h.col;
+    L0.a = h.eA; L0.diag = h.diag;
+    A.h.push_back(std::move(L0));
+    // coarsen until the level fits the explicit inverse; at least once above
+    // one slice of rows
+    while (A.h.back().n > kCoarseMax || (A.h.size() == 1 && A.h.back().n > kSlice)) {
+        std::vector<int> agg;
+        const int nc = aggregate(A.h.back(), agg);
+        if (nc >= A.h.back().n) break;
+        AmgHostLevel Cl;
+        galerkin(A.h.back(), agg, nc, Cl);
+        A.h.back().agg = std::move(agg);
+        A.h.push_back(std::move(Cl));
+        if (A.h.size() > 24) break;
+    }
+    std::vector<double> inv;
+    if (A.h.size() > 1) {
+        if (A.h.back().n > kCoarseMax) throw InvalidError("multigrid coarsening stalled above the coarsest-level size");
+        if (!spdInverse(A.h.back(), inv))
+            throw InvalidError("multigrid coarsest matrix is not positive definite (no fixed boundary?)");
+    }
+    hipStream_t s = M->ctx->stream;
+    for (size_t l = 0; l < A.h.size(); ++l) {
+        const AmgHostLevel& H = A.h[l];
+        auto D = std::make_unique<AmgDevLevel>();
+        D->n = H.n;
+        D->nSlices = H.nSlices;
+        if (!H.agg.empty()) D->agg.upload(H.agg, s);
+        if (l > 0) {
+            D->sliceStart.upload(H.sliceStart, s);
+            D->sliceWidth.upload(H.sliceWidth, s);
+            D->col.upload(H.col, s);
+            D->a64.upload(H.a, s);
+            D->d64.upload(H.diag, s);
+            std::vector<double> iD(H.n);
+            std::vector<float> a32(H.a.size()), d32(H.n), iD32(H.n);
+            for (int i = 0; i < H.n; ++i) { iD[i] = 1.0 / H.diag[i]; d32[i] = (float)H.diag[i]; iD32[i] = 1.0f / d32[i]; }
+            for (size_t e = 0; e < H.a.size(); ++e) a32[e] = (float)H.a[e];
+            D->iD64.upload(iD, s);
+            D->a32.upload(a32, s);
+            D->d32.upload(d32, s);
+            D->iD32.upload(iD32, s);
+            D->memOff.upload(H.memOff, s);
+            D->mem.upload(H.mem, s);
+        }
+        if (l + 1 == A.h.size() && l > 0) {
+            std::vector<float> inv32(inv.size());
+            for (size_t k = 0; k < inv.size(); ++k) inv32[k] = (float)inv[k];
+            D->inv64.upload(inv, s);
+            D->inv32.upload(inv32, s);
+        }
+        A.d.push_back(std::move(D));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    A.gen = M->generation;
+    A.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T>
+struct MgVec;
+template <>
+struct MgVec<double> {
+    static double* x(AmgDevLevel& L) { L.x64.alloc(3 * (size_t)L.n); return L.x64.p; }
+    static double* b(AmgDevLevel& L) { L.b64.alloc(3 * (size_t)L.n); return L.b64.p; }
+    static double* r(AmgDevLevel& L) { L.r64.alloc(3 * (size_t)L.n); return L.r64.p; }
+    static double* xs(AmgDevLevel& L) { L.xs64.alloc(3 * (size_t)L.n); return L.xs64.p; }
+    static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
+    static void swapX(AmgDevLevel& L) { std::swap(L.x64.p, L.xs64.p); }
+};
+template <>
+struct MgVec<float> {
+    static float* x(AmgDevLevel& L) { L.x32.alloc(3 * (size_t)L.n); return L.x32.p; }
+    static float* b(AmgDevLevel& L) { L.b32.alloc(3 * (size_t)L.n); return L.b32.p; }
+    static float* r(AmgDevLevel& L) { L.r32.alloc(3 * (size_t)L.n); return L.r32.p; }
+    static float* xs(AmgDevLevel& L) { L.xs32.alloc(3 * (size_t)L.n); return L.xs32.p; }
+    static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
+    static void swapX(AmgDevLevel& L) { std::swap(L.x32.p, L.xs32.p); }
+};
+
+inline int gridRows(int n) { return std::max(1, std::min(8192, (n + kBlock - 1) / kBlock)); }
+inline int gridSlices(int nSlices) { return std::max(1, std::min(8192, (nSlices + kBlock / 64 - 1) / (kBlock / 64))); }
+
+// z = V(r): one V(1,1) cycle on the 3-RHS residual, then k_fin_rho
+template <typename T>
+void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
+                  hipStream_t s) {
+    Amg& A = M->amg;
+    const int nl = (int)A.d.size();
+    const double w = A.omega;
+    const int C = M->h.C;
+    AmgDevLevel& D0 = *A.d[0];
+    T* x0 = MgVec<T>::x(D0);
+    T* r0 = MgVec<T>::r(D0);
+    hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, x0, r0, w,
+                       (const Ctl*)ctl);
+    if (nl > 1) {
+        // down: restrict the finer level's residual, smooth + residual
+        for (int l = 1; l < nl; ++l) {
+            AmgDevLevel& L = *A.d[l];
+            AmgDevLevel& Lf = *A.d[l - 1];
+            const T* rf = l == 1 ? (const T*)r0 : (const T*)MgVec<T>::r(Lf);
+            T* b = MgVec<T>::b(L);
+            hipLaunchKernelGGL((k_mg_restrict<T>), dim3(gridRows(L.n)), dim3(kBlock), 0, s, L.view(), rf, Lf.n, b,
+                               (const Ctl*)ctl);
+            if (l < nl - 1)
+                hipLaunchKernelGGL((k_mg_down<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
+                                   (const T*)b, MgVec<T>::x(L), MgVec<T>::r(L), w, (const Ctl*)ctl);
+        }
+        // coarsest: exact solve
+        {
+            AmgDevLevel& L = *A.d[nl - 1];
+            hipLaunchKernelGGL((k_mg_dense<T>), dim3((L.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.n,
+                               MgVec<T>::inv(L), (const T*)MgVec<T>::b(L), MgVec<T>::x(L), (const Ctl*)ctl);
+        }
+        // up: prolongate the coarser correction into the post-smoothing sweep
+        for (int l = nl - 2; l >= 1; --l) {
+            AmgDevLevel& L = *A.d[l];
+            AmgDevLevel& Lc = *A.d[l + 1];
+            hipLaunchKernelGGL((k_mg_up<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
+                               (const T*)MgVec<T>::b(L), (const T*)MgVec<T>::x(L), Lc.n, (const T*)MgVec<T>::x(Lc),
+                               MgVec<T>::xs(L), w, (const Ctl*)ctl);
+            MgVec<T>::swapX(L);
+        }
+    }
+    const T* e1 = nl > 1 ? (const T*)MgVec<T>::x(*A.d[1]) : nullptr;
+    const int n1 = nl > 1 ? A.d[1]->n : 0;
+    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
+                       (const int*)D0.agg.p, n1, e1, v.z, w, (const Ctl*)ctl, part);
+    hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+    HIP_CHECK(hipGetLastError());
+    (void)C;
+}
+
+// PCG iterations with the V-cycle: spmv(p = z + beta p), alpha, update,
+// convergence, z = V(r), beta
+template <typename T>
+void launchPcgChunkMg(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
+                      int n, hipStream_t s) {
+    for (int i = 0; i < n; ++i) {
+        hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
+                           v.p2, v.q, ctl, part, 1, (const T*)v.z);
+        hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
+                           (const T*)v.q, (const Ctl*)ctl, part);
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        launchVcycle<T>(M, dm, bm, v, ctl, part, s);
         std::swap(v.p, v.p2);
     }
     HIP_CHECK(hipGetLastError());
@@ -681,6 +1042,11 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     SolveState& st = M->st;
     PcgVecs<T> v = pcgVecs<T>(st, C);
     applyStabilisationScale(M, o.rhieChowScale, o.interfaceScale);
+    const bool mg = o.preconditioner == S4F_PRECOND_GAMG;
+    if (mg) {
+        if (nInst != 1) throw InvalidError("the multigrid preconditioner takes a single mesh");
+        buildAmg(M);
+    }
     const DevMesh dm = M->dm();
     const BlockMap bm = M->bmap();
     // initial and old-time fields
@@ -698,7 +1064,10 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
     HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
     std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
-    for (int i = 0; i < nInst; ++i) c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
+    for (int i = 0; i < nInst; ++i) {
+        c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
+        c0[i].pcMode = mg ? 1 : 0;
+    }
     HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
     Ctl* hctl = nullptr;
@@ -719,10 +1088,12 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
                            (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
         hipLaunchKernelGGL(k_fin_residual, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
+        if (mg) launchVcycle<T>(M, dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
         bool allDone = false;
         int total = 0;
         for (;;) {
-            launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s);
+            if (mg) launchPcgChunkMg<T>(M, dm, bm, v, st.ctl.p, st.part.p, chunk, s);
+            else launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s);
             total += chunk;
             chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
             HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -1109,7 +1480,6 @@ void s4f_solve_opts_default(s4f_solve_opts* o) {
     o->pcgRelTol = 0.1;
     o->pcgMaxIter = 1000;
     o->preconditioner = S4F_PRECOND_DIAGONAL;
-    o->chebyshevDegree = 4;
     o->precision = S4F_PRECISION_FP64;
     o->warmStart = 0;
 }
@@ -1251,7 +1621,7 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
             std::vector<float> tf(t.begin(), t.end());
             HIP_CHECK(hipMemcpyAsync(v.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL((k_spmvp<float>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const float*)nullptr,
-                               (const float*)v.p, (float*)nullptr, v.q, st.ctl.p, st.part.p, 0);
+                               (const float*)v.p, (float*)nullptr, v.q, st.ctl.p, st.part.p, 0, (const float*)nullptr);
             HIP_CHECK(hipGetLastError());
             std::vector<float> yf(3 * (size_t)C);
             HIP_CHECK(hipMemcpyAsync(yf.data(), v.q, yf.size() * 4, hipMemcpyDeviceToHost, s));
@@ -1262,7 +1632,7 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
             PcgVecs<double> v = pcgVecs<double>(st, C);
             HIP_CHECK(hipMemcpyAsync(v.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL((k_spmvp<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)nullptr,
-                               (const double*)v.p, (double*)nullptr, v.q, st.ctl.p, st.part.p, 0);
+                               (const double*)v.p, (double*)nullptr, v.q, st.ctl.p, st.part.p, 0, (const double*)nullptr);
             HIP_CHECK(hipGetLastError());
             soaToAos(download(st.q64, 3 * (size_t)C, s), C, 3, y);
         }
@@ -1276,6 +1646,7 @@ s4f_status s4f_pcg3(s4f_mesh* M, const s4f_solve_opts* o, const double* b, const
         if (!M || !o || !b || !normFactor || !x) throw InvalidError("null argument");
         if (!M->materialSet) throw InvalidError("material not set");
         checkOpts(*o);
+        singleMeshOnly(*o, "s4f_pcg3");
         HIP_CHECK(hipSetDevice(M->ctx->device));
         hipStream_t s = M->ctx->stream;
         const int C = M->h.C;
@@ -1381,7 +1752,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 break;
             case S4F_KERNEL_SPMV:
                 hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1);
+                                   (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1, (const T*)nullptr);
                 std::swap(v.p, v.p2);
                 break;
             case S4F_KERNEL_PCG_UPDATE:
@@ -1462,6 +1833,7 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
         if (!b || !opts) throw InvalidError("null argument");
         if (b->members.empty()) throw InvalidError("empty batch");
         checkOpts(*opts);
+        singleMeshOnly(*opts, "s4f_batch_solve");
         HIP_CHECK(hipSetDevice(b->ctx->device));
         if (b->dirty || !b->packed) {
             std::vector<const HostMesh*> hs;
@@ -2085,6 +2457,7 @@ s4f_status s4f_server_create(s4f_ctx* ctx, int64_t budgetBytes, int32_t maxSlots
         if (budgetBytes < (1 << 20)) throw InvalidError("budget must be at least 1 MiB");
         if (maxSlots < 1 || maxSlots > 4096) throw InvalidError("maxSlots must be in 1..4096");
         checkOpts(*opts);
+        singleMeshOnly(*opts, "s4f_server_create");
         HIP_CHECK(hipSetDevice(ctx->device));
         auto S = std::make_unique<s4f_server>();
         S->ctx = ctx;

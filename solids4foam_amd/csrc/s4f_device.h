@@ -73,7 +73,7 @@ struct Ctl {
     unsigned counter[8];
     // settings
     double pcgTol, pcgRelTol, solTol, altTol;
-    int pcgMaxIter, nCorr, histCap, pad2;
+    int pcgMaxIter, nCorr, histCap, pcMode;  // pcMode 1: z comes from the V-cycle (k_fin_rho sets beta)
 };
 
 }  // namespace s4f

@@ -255,7 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMap bm, const T* __restrict__ r,
                                                   const T* __restrict__ pOld, T* __restrict__ pNew, T* __restrict__ q,
-                                                  Ctl* __restrict__ ctlAll, double* __restrict__ part, int reduce) {
+                                                  Ctl* __restrict__ ctlAll, double* __restrict__ part, int reduce,
+                                                  const T* __restrict__ zArr) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
     const Ctl* ctl = ctlAll + inst;
@@ -269,10 +270,11 @@ __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMa
                          reduce ? ctl->active[2] != 0 : true};
     const T be[3] = {reduce ? (T)ctl->beta[0] : (T)0, reduce ? (T)ctl->beta[1] : (T)0, reduce ? (T)ctl->beta[2] : (T)0};
     // direction component of cell c
+    // z = D^-1 r (diagonal preconditioner) or the V-cycle output
     auto dir = [&](int j, int c) -> T {
         const size_t i = (size_t)j * C + c;
         if (!reduce) return pOld[i];
-        const T z = invD[c] * r[i];
+        const T z = zArr ? zArr[i] : invD[c] * r[i];
         return first ? z : z + be[j] * pOld[i];
     };
     double acc[3] = {0, 0, 0};
@@ -765,8 +767,10 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_update(const BlockMap bm, Ctl
             ctl->finalRes[j] = fr;
             const bool conv = fr < ctl->pcgTol || (ctl->pcgRelTol > 1e-20 && fr < ctl->pcgRelTol * ctl->initRes[j]);
             const int act = !conv && ctl->nIter[j] < ctl->pcgMaxIter;
-            ctl->beta[j] = tot[j] / ctl->rho[j];
-            ctl->rho[j] = tot[j];
+            if (ctl->pcMode == 0) {  // diagonal preconditioner: z.r came with the update
+                ctl->beta[j] = tot[j] / ctl->rho[j];
+                ctl->rho[j] = tot[j];
+            }
             ctl->active[j] = act;
             any |= act;
         }

@@ -129,6 +129,41 @@ def test_mixed_precision_solve_error(ctx, name, n, kw):
     assert eD < 1e-6 and eS < 1e-6
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,n,kw", [
+    ("cube16_jitter", (16, 16, 16), dict(jitter=0.15, seed=2)),
+    ("tet4_jitter", (4, 4, 4), dict(jitter=0.15, seed=5, tets=True, ls_full_delta=True)),
+    ("cfg1_cantilever", (50, 10, 10), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
+])
+def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precision):
+    # the preconditioner changes the path, not the fixed point: same converged
+    # field as the oracle (which uses DIC), in fewer PCG iterations
+    from solids4foam_amd import PRECOND_GAMG, sigma_eq
+    print(f"[gamg solve] {name} precision={precision}", flush=True)
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    pm, m, _, _ = _product_mesh(ctx, n, **kw)
+    out = m.solve(precision=precision, preconditioner=PRECOND_GAMG, **TIGHT)
+    jac = m.solve(precision=precision, **TIGHT)
+    eD, eS = rel_l2(out["D"], g["D"]), rel_l2(sigma_eq(out["sigma"]), sigma_eq(g["sigma"]))
+    print(f"  nOuter={out['nOuter']} pcg={out['pcgIterations']} (diagonal {jac['pcgIterations']}) "
+          f"err_D={eD:.2e} err_vonMises={eS:.2e}", flush=True)
+    assert out["converged"]
+    tol = 1e-8 if precision == 0 else 1e-6
+    assert eD < tol and eS < tol
+    assert out["pcgIterations"] < jac["pcgIterations"]
+
+
+def test_multigrid_rejected_outside_single_solve(ctx):
+    from solids4foam_amd import PRECOND_GAMG, _abi
+    from solids4foam_amd.batch import Batch
+    pm, m, _, _ = _product_mesh(ctx, (5, 5, 5), jitter=0.1, seed=3)
+    b = Batch(ctx)
+    b.add(m)
+    with pytest.raises(_abi.S4FError) as e:
+        b.solve(preconditioner=PRECOND_GAMG)
+    assert e.value.status == _abi.S4F_ERR_INVALID
+
+
 def test_first_iterations_follow_oracle(ctx, oracle_mod):
     # same preconditioner (diagonal) on both sides: the first outer iterations
     # reproduce the oracle's residual history
