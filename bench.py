@@ -35,6 +35,10 @@ def algorithmic_bytes(kernel, C, F, B, T):
         return C * 19 * T
     if kernel == "finalize":        # one workgroup: partial rows of 3 values
         return 0
+    if kernel == "mg0_down":        # b, 1/diag, diag; x, r writes; a + col per ELL entry
+        return C * 11 * T + F * 2 * (T + 4)
+    if kernel == "mg0_up":          # b, x, 1/diag, agg; z write; a + col per ELL entry (coarse e cache-resident)
+        return C * (10 * T + 4) + F * 2 * (T + 4)
     if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
         return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
     if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w
@@ -79,12 +83,16 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--reltol", type=float, default=0.1)
+    ap.add_argument("--pcg-maxiter", type=int, default=1000, help="PCG iteration cap per outer iteration")
     ap.add_argument("--precond", choices=["diagonal", "gamg"], default="diagonal",
                     help="PCG preconditioner of the device solves")
     ap.add_argument("--replay", type=int, default=40, help="back-to-back launches per kernel for the roofline")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="no timed solves: one outer iteration for state, then only the kernel replays "
+                         "(the run that rocprofv3 profiles)")
     ap.add_argument("--calib-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run once before the timed region (counter calibration)")
     args = ap.parse_args()
@@ -106,7 +114,7 @@ def main():
     print(f"[bench] mesh {args.n}^3: C={C} F={F} B={B} ELL entries={info['ellEntries']} "
           f"device={info['deviceBytes'] / 2**30:.2f} GiB setup={setup_s:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
-                nCorrectors=100000,
+                nCorrectors=100000, pcgMaxIter=args.pcg_maxiter,
                 preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
     L = _abi.lib()
 
@@ -119,13 +127,34 @@ def main():
             outs.append(out)
             print(f"[bench] precision={precision} nOuter={out['nOuter']} inner={out['innerTotal']} "
                   f"converged={out['converged']} {lat[-1]:.2f}s", file=sys.stderr, flush=True)
-        prof = {}
-        if profile and args.replay > 0:
-            # each kernel replayed back to back on the solve's final state
-            for k, kid in _abi.KERNELS.items():
-                n = args.replay if k in ("spmv", "pcg_update", "finalize") else max(4, args.replay // 4)
-                prof[k] = dict(avg_ms=ctypes_kernel_time(L, mesh, kid, precision, n), launches=n)
+        prof = replay_all(precision) if profile else {}
         return lat, outs, prof
+
+    def replay_all(precision):
+        # each kernel replayed back to back on the mesh's current state
+        prof = {}
+        if args.replay <= 0:
+            return prof
+        ks = dict(_abi.KERNELS)
+        if args.precond == "gamg":
+            ks.update(_abi.MG_KERNELS)
+        for k, kid in ks.items():
+            n = args.replay if k not in ("residual", "grad") else max(4, args.replay // 4)
+            prof[k] = dict(avg_ms=ctypes_kernel_time(L, mesh, kid, precision, n), launches=n)
+            print(f"[bench] replay {k}: {prof[k]['avg_ms']:.4f} ms x {n}", file=sys.stderr, flush=True)
+        return prof
+
+    if args.kernels_only:
+        precision = 0 if args.only == "fp64" else 1
+        mesh.solve(want_sigma=False, precision=precision, **dict(opts, nCorrectors=1, pcgMaxIter=2))
+        prof = replay_all(precision)
+        T = 8 if precision == 0 else 4
+        out = {k: dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"],
+                       gbps=(round(algorithmic_bytes(k, C, F, B, T) / (p["avg_ms"] * 1e-3) / 1e9, 1)
+                             if k not in ("finalize", "vcycle") else None)) for k, p in prof.items()}
+        print(json.dumps(dict(mode="kernels-only", cells=C, precision="fp64" if precision == 0 else "mixed",
+                              kernels=out)))
+        return
 
     if args.calib_bytes:
         cms = ctypes_copy(L, ctx, args.calib_bytes)
@@ -155,15 +184,18 @@ def main():
     nOuter = outs[-1]["nOuter"]
     kern = {}
     pcg_it = outs[-1]["pcgIterations"]
-    per_solve = {"spmv": pcg_it, "pcg_update": pcg_it, "finalize": 2 * pcg_it, "residual": nOuter, "grad": nOuter}
+    mg = args.precond == "gamg"
+    per_solve = {"spmv": pcg_it, "pcg_update": pcg_it, "finalize": (3 if mg else 2) * pcg_it, "residual": nOuter,
+                 "grad": nOuter, "mg0_down": pcg_it + nOuter, "mg0_up": pcg_it + nOuter, "vcycle": pcg_it + nOuter}
     for k, p in prof.items():
         e = dict(avg_ms=round(p["avg_ms"], 5), replay_launches=p["launches"], launches_per_solve=per_solve[k],
                  ms_per_solve_est=round(p["avg_ms"] * per_solve[k], 1))
-        if k != "finalize":
+        if k not in ("finalize", "vcycle"):
             b = algorithmic_bytes(k, C, F, B, T)
             e.update(alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1))
         kern[k] = e
-    dom = max((k for k in kern if k != "finalize"), key=lambda k: kern[k]["ms_per_solve_est"]) if kern else None
+    dom = max((k for k in kern if k not in ("finalize", "vcycle")), key=lambda k: kern[k]["ms_per_solve_est"]) \
+        if kern else None
     roofline = None
     if dom:
         a = kern[dom]["gbps"]
@@ -198,7 +230,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cube {args.n}^3 hex, jitter {args.jitter} seed {args.seed}, z-min fixed, "
                                f"z-max traction 1e7 Pa, linearElastic E=200GPa nu=0.3",
-                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}"},
+                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}, maxIter {args.pcg_maxiter}"},
         "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
         "nOuter": nOuter,
         "pcg_iterations": pcg_it,

@@ -229,7 +229,10 @@ enum { S4F_KERNEL_RESIDUAL = 0,   /* div(sigma) + Rhie-Chow face loop */
        S4F_KERNEL_SPMV = 1,       /* PCG direction update fused with the 3-RHS SpMV */
        S4F_KERNEL_PCG_UPDATE = 2, /* PCG solution / residual update */
        S4F_KERNEL_FINALIZE = 3,   /* one-workgroup reduction finalize (alpha) */
-       S4F_KERNEL_GRAD = 4 };     /* boundary update + least-squares gradient */
+       S4F_KERNEL_GRAD = 4,       /* boundary update + least-squares gradient */
+       S4F_KERNEL_MG0_DOWN = 5,   /* multigrid: fine-level smoothing + residual */
+       S4F_KERNEL_MG0_UP = 6,     /* multigrid: fine-level prolongation + smoothing */
+       S4F_KERNEL_VCYCLE = 7 };   /* multigrid: every launch of one V-cycle */
 /* Replays `kernel` back to back `launches` times on the mesh's current state
    (e.g. right after a solve) with all three components active, on the
    context's stream between one pair of hipEvents, and returns the average

@@ -81,6 +81,7 @@ class Request(ctypes.Structure):
                 ("warmStart", ctypes.c_int32)]
 
 KERNELS = {"residual": 0, "spmv": 1, "pcg_update": 2, "finalize": 3, "grad": 4}
+MG_KERNELS = {"mg0_down": 5, "mg0_up": 6, "vcycle": 7}
 
 
 def lib():

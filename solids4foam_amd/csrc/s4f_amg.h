@@ -60,6 +60,90 @@ __device__ __forceinline__ bool mgSkip(const Ctl* c) { return c->outerDone || c-
 // level 0 (the mesh's own matrix, BlockMap iteration)
 // ---------------------------------------------------------------------------
 
+// Level-0 rows with all three components active and a compile-time slot
+// count W: every load is issued unconditionally (invalid slots gather the
+// row's own cell with a zero coefficient) so a wave's gathers overlap.
+template <typename T, int W>
+__device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
+                                              T* __restrict__ x, T* __restrict__ r, T w) {
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    int nb[W];
+    T a[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int e = e0 + k * kSlice + lane;
+        nb[k] = m.col[e];
+        a[k] = A[e];
+    }
+    const T iD = invD[P], dP = MatSel<T>::diag(m)[P];
+    T bP[3], xP[3], y[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        bP[j] = b[(size_t)j * C + P];
+        xP[j] = w * iD * bP[j];
+        y[j] = dP * xP[j];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const bool v = nb[k] >= 0;
+        const int c = v ? nb[k] : P;
+        const T av = v ? a[k] : (T)0;
+        const T wi = w * invD[c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) y[j] -= av * (wi * b[(size_t)j * C + c]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const size_t i = (size_t)j * C + P;
+        x[i] = xP[j];
+        r[i] = bP[j] - y[j];
+    }
+}
+
+template <typename T, int W, bool COARSE>
+__device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
+                                            const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
+                                            const T* __restrict__ e, T* __restrict__ z, T w, T w1, double* acc) {
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    int nb[W];
+    T a[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int en = e0 + k * kSlice + lane;
+        nb[k] = m.col[en];
+        a[k] = A[en];
+    }
+    auto xf = [&](int j, int c, int I) -> T {
+        const T v = x[(size_t)j * C + c];
+        return COARSE ? v + e[(size_t)j * nCoarse + I] : v;
+    };
+    const int IP = COARSE ? agg[P] : 0;
+    T bP[3], sum[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        bP[j] = b[(size_t)j * C + P];
+        sum[j] = bP[j];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const bool v = nb[k] >= 0;
+        const int c = v ? nb[k] : P;
+        const T av = v ? a[k] : (T)0;
+        const int I = COARSE ? agg[c] : 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, c, I);
+    }
+    const T iD = invD[P];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const T zv = w1 * xf(j, P, IP) + w * iD * sum[j];
+        z[(size_t)j * C + P] = zv;
+        acc[j] += (double)zv * (double)bP[j];
+    }
+}
+
 // pre-smoothing from zero and the residual for the restriction:
 // x = w D^-1 b, r = b - A x
 template <typename T>
@@ -75,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
     const T* __restrict__ dg = MatSel<T>::diag(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const bool all = act[0] && act[1] && act[2];
     const T w = (T)omega;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s1 = bm.s1[lb];
@@ -84,6 +169,14 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
         if (P >= cEnd) continue;
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
+        if (all && width == 6) {
+            mg0DownRowAll<T, 6>(m, P, e0, lane, C, b, x, r, w);
+            continue;
+        }
+        if (all && width == 4) {
+            mg0DownRowAll<T, 4>(m, P, e0, lane, C, b, x, r, w);
+            continue;
+        }
         const T iD = invD[P];
         T xP[3], y[3];
 #pragma unroll
@@ -128,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const bool all = act[0] && act[1] && act[2];
     const T w = (T)omega, w1 = (T)(1.0 - omega);
     auto xf = [&](int j, int cell) -> T {
         const T v = x[(size_t)j * C + cell];
@@ -142,6 +236,16 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
         if (P >= cEnd) continue;
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
+        if (all && width == 6) {
+            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            continue;
+        }
+        if (all && width == 4) {
+            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            continue;
+        }
         T sum[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) sum[j] = act[j] ? b[(size_t)j * C + P] : (T)0;
@@ -192,6 +296,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
     }
 }
 
+// Coarse-level rows are wider (Galerkin stencils) and of varying width:
+// slots are processed 8 at a time with every load unconditional (slots past
+// the row's width or empty read an in-bounds slot and the row's own value
+// with a zero coefficient), all three components computed, active ones stored.
+constexpr int kMgChunk = 8;
+
 // down on a coarse level: x = w D^-1 b, r = b - A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __restrict__ b, T* __restrict__ x,
@@ -211,21 +321,31 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
         const T iD = iDg[P];
-        T xP[3], y[3];
+        T bP[3], xP[3], y[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            xP[j] = act[j] ? w * iD * b[(size_t)j * n + P] : (T)0;
+            bP[j] = b[(size_t)j * n + P];
+            xP[j] = w * iD * bP[j];
             y[j] = dg[P] * xP[j];
         }
-        for (int k = 0; k < width; ++k) {
-            const int e = e0 + k * kSlice + lane;
-            const int c = L.col[e];
-            if (c >= 0) {
-                const T a = A[e];
-                const T iDc = iDg[c];
+        for (int k0 = 0; k0 < width; k0 += kMgChunk) {
+            int nb[kMgChunk];
+            T a[kMgChunk];
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if (act[j]) y[j] -= a * (w * iDc * b[(size_t)j * n + c]);
+            for (int u = 0; u < kMgChunk; ++u) {
+                const int k = k0 + u < width ? k0 + u : width - 1;
+                const int e = e0 + k * kSlice + lane;
+                nb[u] = k0 + u < width ? L.col[e] : -1;
+                a[u] = A[e];
+            }
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u) {
+                const bool v = nb[u] >= 0;
+                const int c = v ? nb[u] : P;
+                const T av = v ? a[u] : (T)0;
+                const T wi = w * iDg[c];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) y[j] -= av * (wi * b[(size_t)j * n + c]);
             }
         }
 #pragma unroll
@@ -233,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
             if (!act[j]) continue;
             const size_t i = (size_t)j * n + P;
             x[i] = xP[j];
-            r[i] = b[i] - y[j];
+            r[i] = bP[j] - y[j];
         }
     }
 }
@@ -250,7 +370,6 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
     const T w = (T)omega, w1 = (T)(1.0 - omega);
     const int n = L.n;
     const int* __restrict__ agg = L.agg;
-    auto xf = [&](int j, int row) -> T { return x[(size_t)j * n + row] + e[(size_t)j * nCoarse + agg[row]]; };
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wavesTotal = gridDim.x * (kBlock / 64);
     for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
@@ -258,22 +377,36 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
         if (P >= n) continue;
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
-        T sum[3];
+        const int IP = agg[P];
+        T sum[3], xP[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] = act[j] ? b[(size_t)j * n + P] : (T)0;
-        for (int k = 0; k < width; ++k) {
-            const int en = e0 + k * kSlice + lane;
-            const int c = L.col[en];
-            if (c >= 0) {
-                const T a = A[en];
+        for (int j = 0; j < 3; ++j) {
+            sum[j] = b[(size_t)j * n + P];
+            xP[j] = x[(size_t)j * n + P] + e[(size_t)j * nCoarse + IP];
+        }
+        for (int k0 = 0; k0 < width; k0 += kMgChunk) {
+            int nb[kMgChunk];
+            T a[kMgChunk];
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if (act[j]) sum[j] += a * xf(j, c);
+            for (int u = 0; u < kMgChunk; ++u) {
+                const int k = k0 + u < width ? k0 + u : width - 1;
+                const int en = e0 + k * kSlice + lane;
+                nb[u] = k0 + u < width ? L.col[en] : -1;
+                a[u] = A[en];
+            }
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u) {
+                const bool v = nb[u] >= 0;
+                const int c = v ? nb[u] : P;
+                const T av = v ? a[u] : (T)0;
+                const int I = agg[c];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sum[j] += av * (x[(size_t)j * n + c] + e[(size_t)j * nCoarse + I]);
             }
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            if (act[j]) out[(size_t)j * n + P] = w1 * xf(j, P) + w * iDg[P] * sum[j];
+            if (act[j]) out[(size_t)j * n + P] = w1 * xP[j] + w * iDg[P] * sum[j];
     }
 }
 

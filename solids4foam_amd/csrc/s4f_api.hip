@@ -691,7 +691,7 @@ template <typename T>
 void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, int nInst, PcgVecs<T>& v, Ctl* ctl, double* part, int n,
                     hipStream_t s) {
     for (int i = 0; i < n; ++i) {
-        hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
+        hipLaunchKernelGGL((k_spmvp<T, false>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
                            v.p2, v.q, ctl, part, 1, (const T*)nullptr);
         hipLaunchKernelGGL(k_fin_spmv, dim3(nInst), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
@@ -1008,7 +1008,7 @@ template <typename T>
 void launchPcgChunkMg(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
                       int n, hipStream_t s) {
     for (int i = 0; i < n; ++i) {
-        hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
+        hipLaunchKernelGGL((k_spmvp<T, true>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
                            v.p2, v.q, ctl, part, 1, (const T*)v.z);
         hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
@@ -1620,7 +1620,7 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
             PcgVecs<float> v = pcgVecs<float>(st, C);
             std::vector<float> tf(t.begin(), t.end());
             HIP_CHECK(hipMemcpyAsync(v.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_spmvp<float>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const float*)nullptr,
+            hipLaunchKernelGGL((k_spmvp<float, false>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const float*)nullptr,
                                (const float*)v.p, (float*)nullptr, v.q, st.ctl.p, st.part.p, 0, (const float*)nullptr);
             HIP_CHECK(hipGetLastError());
             std::vector<float> yf(3 * (size_t)C);
@@ -1631,7 +1631,7 @@ s4f_status s4f_spmv3(s4f_mesh* M, const double* x, double* y, int32_t precision)
         } else {
             PcgVecs<double> v = pcgVecs<double>(st, C);
             HIP_CHECK(hipMemcpyAsync(v.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL((k_spmvp<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)nullptr,
+            hipLaunchKernelGGL((k_spmvp<double, false>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)nullptr,
                                (const double*)v.p, (double*)nullptr, v.q, st.ctl.p, st.part.p, 0, (const double*)nullptr);
             HIP_CHECK(hipGetLastError());
             soaToAos(download(st.q64, 3 * (size_t)C, s), C, 3, y);
@@ -1751,7 +1751,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                                    (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, rc.p, st.part.p);
                 break;
             case S4F_KERNEL_SPMV:
-                hipLaunchKernelGGL((k_spmvp<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                hipLaunchKernelGGL((k_spmvp<T, false>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
                                    (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1, (const T*)nullptr);
                 std::swap(v.p, v.p2);
                 break;
@@ -1767,10 +1767,40 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                                    st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, (const double*)nullptr,
                                    rc.p, st.part.p, (double*)nullptr, 0);
                 break;
+            case S4F_KERNEL_MG0_DOWN: {
+                AmgDevLevel& D0 = *M->amg.d[0];
+                hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                                   MgVec<T>::x(D0), MgVec<T>::r(D0), M->amg.omega, (const Ctl*)rc.p);
+                break;
+            }
+            case S4F_KERNEL_MG0_UP: {
+                AmgDevLevel& D0 = *M->amg.d[0];
+                const bool two = M->amg.d.size() > 1;
+                hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                                   (const T*)MgVec<T>::x(D0), (const int*)D0.agg.p, two ? M->amg.d[1]->n : 0,
+                                   two ? (const T*)MgVec<T>::x(*M->amg.d[1]) : (const T*)nullptr, v.z, M->amg.omega,
+                                   (const Ctl*)rc.p, st.part.p);
+                break;
+            }
+            case S4F_KERNEL_VCYCLE:
+                launchVcycle<T>(M, dm, bm, v, rc.p, st.part.p, s);
+                break;
             default:
                 throw InvalidError("unknown kernel id");
         }
     };
+    if (kernel >= S4F_KERNEL_MG0_DOWN) {
+        if (M->h.instCell0.size() != 1) throw InvalidError("multigrid kernels take a single mesh");
+        buildAmg(M);
+        for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
+            AmgDevLevel& L = *M->amg.d[l];
+            const size_t n3 = 3 * (size_t)L.n;
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::x(L), 0, n3 * sizeof(T), s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::xs(L), 0, n3 * sizeof(T), s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::b(L), 0, n3 * sizeof(T), s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::r(L), 0, n3 * sizeof(T), s));
+        }
+    }
     launch();  // warm
     HIP_CHECK(hipEventRecord(M->ctx->ev0, s));
     for (int i = 0; i < launches; ++i) launch();

@@ -246,13 +246,63 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
     storeBlockPartials<12>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
+// Row of the fused direction update + SpMV with all three components active
+// and a compile-time slot count W: every load of the row is issued without a
+// branch (invalid slots gather the row's own cell and contribute zero), so
+// the wave keeps all of its gathers in flight at once.
+template <typename T, int W, bool MG>
+__device__ __forceinline__ void spmvpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ r,
+                                            const T* __restrict__ pOld, const T* __restrict__ zArr,
+                                            T* __restrict__ pNew, T* __restrict__ q, int first, const T* be,
+                                            double* acc) {
+    const T* __restrict__ A = MatSel<T>::a(m);
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    int nb[W];
+    T a[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int e = e0 + k * kSlice + lane;
+        nb[k] = m.col[e];
+        a[k] = A[e];
+    }
+    auto dir = [&](int j, int c, T iDc) -> T {
+        const size_t i = (size_t)j * C + c;
+        const T z = MG ? zArr[i] : iDc * r[i];
+        const T d = z + be[j] * pOld[i];
+        return first ? z : d;
+    };
+    const T iDP = MG ? (T)0 : invD[P];
+    T pP[3], y[3];
+    const T dP = MatSel<T>::diag(m)[P];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        pP[j] = dir(j, P, iDP);
+        y[j] = dP * pP[j];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const bool v = nb[k] >= 0;
+        const int c = v ? nb[k] : P;
+        const T av = v ? a[k] : (T)0;
+        const T iDc = MG ? (T)0 : invD[c];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) y[j] -= av * dir(j, c, iDc);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const size_t i = (size_t)j * C + P;
+        pNew[i] = pP[j];
+        q[i] = y[j];
+        acc[j] += (double)pP[j] * (double)y[j];
+    }
+}
+
 // ---------------------------------------------------------------------------
-// k_spmvp: p' = z + beta p (z = r/diag; p' = z on the first PCG iteration),
-// recomputed for the row and for every neighbour it gathers, and q = M p'
-// (3 RHS sharing M); partial p'.q.  Components already converged are skipped.
-// With reduce == 0 it is the plain product q = M pOld (no direction update).
+// k_spmvp: p = z + beta p (z = D^-1 r, or the multigrid output when MG),
+// q = M p for the three components; partial p.q per component.  With
+// reduce = 0 it is a plain q = M p (s4f_spmv3).
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool MG>
 __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMap bm, const T* __restrict__ r,
                                                   const T* __restrict__ pOld, T* __restrict__ pNew, T* __restrict__ q,
                                                   Ctl* __restrict__ ctlAll, double* __restrict__ part, int reduce,
@@ -268,13 +318,13 @@ __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMa
     const int first = reduce ? ctl->firstInner : 0;
     const bool act[3] = {reduce ? ctl->active[0] != 0 : true, reduce ? ctl->active[1] != 0 : true,
                          reduce ? ctl->active[2] != 0 : true};
+    const bool all = reduce && act[0] && act[1] && act[2];
     const T be[3] = {reduce ? (T)ctl->beta[0] : (T)0, reduce ? (T)ctl->beta[1] : (T)0, reduce ? (T)ctl->beta[2] : (T)0};
     // direction component of cell c
-    // z = D^-1 r (diagonal preconditioner) or the V-cycle output
     auto dir = [&](int j, int c) -> T {
         const size_t i = (size_t)j * C + c;
         if (!reduce) return pOld[i];
-        const T z = zArr ? zArr[i] : invD[c] * r[i];
+        const T z = MG ? zArr[i] : invD[c] * r[i];
         return first ? z : z + be[j] * pOld[i];
     };
     double acc[3] = {0, 0, 0};
@@ -286,6 +336,14 @@ __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMa
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
+        if (all && width == 6) {
+            spmvpRowAll<T, 6, MG>(m, P, e0, lane, C, r, pOld, zArr, pNew, q, first, be, acc);
+            continue;
+        }
+        if (all && width == 4) {
+            spmvpRowAll<T, 4, MG>(m, P, e0, lane, C, r, pOld, zArr, pNew, q, first, be, acc);
+            continue;
+        }
         T pP[3], y[3];
         const T d = dg[P];
 #pragma unroll
