@@ -165,6 +165,7 @@ typedef struct {
     int32_t reserved0;             /* zero */
     int32_t precision;             /* S4F_PRECISION_* */
     int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
+    double mgCoarseScale;          /* GAMG: scale of the coarse-grid correction, in (0, 2]; 0 = 1.8 */
 } s4f_solve_opts;
 
 typedef struct {

@@ -23,6 +23,7 @@
 namespace s4f {
 
 constexpr int kCoarseMax = 1024;  // largest level solved by its explicit inverse
+constexpr double kMgCoarseScaleDefault = 1.8;  // coarse-correction scale when the caller gives 0
 
 // device view of one coarse level
 struct MgLevel {
@@ -104,7 +105,8 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
 template <typename T, int W, bool COARSE>
 __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
                                             const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
-                                            const T* __restrict__ e, T* __restrict__ z, T w, T w1, double* acc) {
+                                            const T* __restrict__ e, T* __restrict__ z, T w, T w1, T al,
+                                            double* acc) {
     const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     int nb[W];
@@ -117,7 +119,7 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int
     }
     auto xf = [&](int j, int c, int I) -> T {
         const T v = x[(size_t)j * C + c];
-        return COARSE ? v + e[(size_t)j * nCoarse + I] : v;
+        return COARSE ? v + al * e[(size_t)j * nCoarse + I] : v;
     };
     const int IP = COARSE ? agg[P] : 0;
     T bP[3], sum[3];
@@ -212,7 +214,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
                                                    const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
                                                    const T* __restrict__ e, T* __restrict__ z, double omega,
-                                                   const Ctl* __restrict__ ctl, double* __restrict__ part) {
+                                                   double alpha, const Ctl* __restrict__ ctl,
+                                                   double* __restrict__ part) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
     const Ctl* c = ctl + inst;
@@ -222,10 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
     const bool all = act[0] && act[1] && act[2];
-    const T w = (T)omega, w1 = (T)(1.0 - omega);
+    const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
     auto xf = [&](int j, int cell) -> T {
         const T v = x[(size_t)j * C + cell];
-        return e ? v + e[(size_t)j * nCoarse + agg[cell]] : v;
+        return e ? v + al * e[(size_t)j * nCoarse + agg[cell]] : v;
     };
     double acc[3] = {0, 0, 0};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -237,13 +240,13 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (all && width == 6) {
-            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
-            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
             continue;
         }
         if (all && width == 4) {
-            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
-            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, acc);
+            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
             continue;
         }
         T sum[3];
@@ -284,11 +287,23 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
     const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
         T s[3] = {(T)0, (T)0, (T)0};
-        for (int k = Lc.memOff[I]; k < Lc.memOff[I + 1]; ++k) {
-            const int i = Lc.mem[k];
+        const int k1 = Lc.memOff[I + 1];
+        // members 8 at a time, loads unconditional (past the end: the last
+        // member again, weight zero), in member order
+        for (int k0 = Lc.memOff[I]; k0 < k1; k0 += 8) {
+            int mi[8];
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                if (act[j]) s[j] += r[(size_t)j * nFine + i];
+            for (int u = 0; u < 8; ++u) mi[u] = Lc.mem[k0 + u < k1 ? k0 + u : k1 - 1];
+            T rv[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) rv[u][j] = r[(size_t)j * nFine + mi[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (k0 + u < k1) s[j] += rv[u][j];
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j)
@@ -362,12 +377,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
                                                   int nCoarse, const T* __restrict__ e, T* __restrict__ out,
-                                                  double omega, const Ctl* __restrict__ ctl) {
+                                                  double omega, double alpha, const Ctl* __restrict__ ctl) {
     if (mgSkip(ctl)) return;
     const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     const T* __restrict__ A = MgSel<T>::a(L);
     const T* __restrict__ iDg = MgSel<T>::iD(L);
-    const T w = (T)omega, w1 = (T)(1.0 - omega);
+    const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
     const int n = L.n;
     const int* __restrict__ agg = L.agg;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -382,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             sum[j] = b[(size_t)j * n + P];
-            xP[j] = x[(size_t)j * n + P] + e[(size_t)j * nCoarse + IP];
+            xP[j] = x[(size_t)j * n + P] + al * e[(size_t)j * nCoarse + IP];
         }
         for (int k0 = 0; k0 < width; k0 += kMgChunk) {
             int nb[kMgChunk];
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
                 const T av = v ? a[u] : (T)0;
                 const int I = agg[c];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) sum[j] += av * (x[(size_t)j * n + c] + e[(size_t)j * nCoarse + I]);
+                for (int j = 0; j < 3; ++j) sum[j] += av * (x[(size_t)j * n + c] + al * e[(size_t)j * nCoarse + I]);
             }
         }
 #pragma unroll

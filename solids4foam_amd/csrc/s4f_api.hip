@@ -176,6 +176,7 @@ struct Amg {
     std::vector<AmgHostLevel> h;
     std::vector<std::unique_ptr<AmgDevLevel>> d;
     double omega = 2.0 / 3.0;
+    double alpha = 1.0;  // coarse-correction scale (set per solve)
     double buildSeconds = 0.0;
 };
 
@@ -642,6 +643,7 @@ void checkOpts(const s4f_solve_opts& o) {
     if (o.precision != S4F_PRECISION_FP64 && o.precision != S4F_PRECISION_MIXED) throw InvalidError("unknown precision");
     if (o.preconditioner != S4F_PRECOND_DIAGONAL && o.preconditioner != S4F_PRECOND_GAMG)
         throw InvalidError("unknown preconditioner");
+    if (!(o.mgCoarseScale >= 0.0 && o.mgCoarseScale <= 2.0)) throw InvalidError("mgCoarseScale must be in [0, 2]");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -989,14 +991,14 @@ void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>
             AmgDevLevel& Lc = *A.d[l + 1];
             hipLaunchKernelGGL((k_mg_up<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
                                (const T*)MgVec<T>::b(L), (const T*)MgVec<T>::x(L), Lc.n, (const T*)MgVec<T>::x(Lc),
-                               MgVec<T>::xs(L), w, (const Ctl*)ctl);
+                               MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
             MgVec<T>::swapX(L);
         }
     }
     const T* e1 = nl > 1 ? (const T*)MgVec<T>::x(*A.d[1]) : nullptr;
     const int n1 = nl > 1 ? A.d[1]->n : 0;
     hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
-                       (const int*)D0.agg.p, n1, e1, v.z, w, (const Ctl*)ctl, part);
+                       (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
     hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
     (void)C;
@@ -1046,6 +1048,10 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     if (mg) {
         if (nInst != 1) throw InvalidError("the multigrid preconditioner takes a single mesh");
         buildAmg(M);
+        // over-correction of the piecewise-constant coarse correction (plain
+        // aggregation under-represents smooth modes); any scale in (0, 2]
+        // keeps the cycle symmetric positive definite
+        M->amg.alpha = o.mgCoarseScale > 0.0 ? o.mgCoarseScale : kMgCoarseScaleDefault;
     }
     const DevMesh dm = M->dm();
     const BlockMap bm = M->bmap();
@@ -1779,7 +1785,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
                                    (const T*)MgVec<T>::x(D0), (const int*)D0.agg.p, two ? M->amg.d[1]->n : 0,
                                    two ? (const T*)MgVec<T>::x(*M->amg.d[1]) : (const T*)nullptr, v.z, M->amg.omega,
-                                   (const Ctl*)rc.p, st.part.p);
+                                   M->amg.alpha, (const Ctl*)rc.p, st.part.p);
                 break;
             }
             case S4F_KERNEL_VCYCLE:

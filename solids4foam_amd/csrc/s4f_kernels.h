@@ -146,6 +146,55 @@ template <> struct MatSel<float> {
     __device__ static const float* invDiag(const DevMesh& m) { return m.invDiag32; }
 };
 
+// Internal-face slots of a k_residual row with compile-time slot count W,
+// branch-free: a boundary or empty slot gathers the row's own cell and has
+// zero area, Rhie-Chow and matrix coefficients, so it adds exactly zero.
+// Rows list internal faces before boundary faces, so running the boundary
+// slots afterwards keeps the summation order of the slot loop.
+template <int W>
+__device__ __forceinline__ void residualInternalW(const DevMesh& m, int P, int e0, int lane, int C,
+                                                  const double* __restrict__ D, const double* __restrict__ G,
+                                                  const double* DP, const double* GP, const double* CP,
+                                                  const double* sP, double* div, double* rc, double* off,
+                                                  double& asum, bool& hasB) {
+    const size_t E = m.nEntries;
+#pragma unroll 1
+    for (int k = 0; k < W; ++k) {
+        const int e = e0 + k * kSlice + lane;
+        const int n = m.col[e];
+        const bool v = n >= 0;
+        hasB |= (!v && n != kPad);
+        const int c = v ? n : P;
+        const double S0r = m.eS[e], S1r = m.eS[E + e], S2r = m.eS[2 * E + e];
+        const double w = m.eW[e], cfr = m.eC[e], ar = m.eA[e];
+        const double S0 = v ? S0r : 0.0, S1 = v ? S1r : 0.0, S2 = v ? S2r : 0.0;
+        const double cf = v ? cfr : 0.0, a = v ? ar : 0.0;
+        double GN[9], sN[6], DN[3], d[3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) GN[q] = G[(size_t)q * C + c];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { DN[q] = D[(size_t)q * C + c]; d[q] = m.cC[(size_t)q * C + c] - CP[q]; }
+        hooke(GN, cellMu(m, c), cellK(m, c), sN);
+        double sf[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sf[q] = w * (sP[q] - sN[q]) + sN[q];
+        div[0] += S0 * sf[0] + S1 * sf[1] + S2 * sf[2];
+        div[1] += S0 * sf[1] + S1 * sf[3] + S2 * sf[4];
+        div[2] += S0 * sf[2] + S1 * sf[4] + S2 * sf[5];
+        double Gf[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Gf[q] = w * (GP[q] - GN[q]) + GN[q];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double dG = d[0] * Gf[j] + d[1] * Gf[3 + j] + d[2] * Gf[6 + j];
+            rc[j] += cf * ((DN[j] - DP[j]) - dG);
+        }
+        asum += a;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) off[j] += a * DN[j];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_residual: R = div(sigma) + RhieChow (integrated), r = R, and per-workgroup
 // partials of sum|R|, the two normFactor sums and z.r (k_fin_residual).
@@ -182,10 +231,22 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
         const double muP = cellMu(m, P), KP = cellK(m, P);
         hooke(GP, muP, KP, sP);
         double div[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, off[3] = {0, 0, 0}, asum = 0.0;
-        for (int k = 0; k < width; ++k) {
+        bool hasB = false;
+        int kInt = width;  // internal slots left to the general loop below
+        if (width == 6) {
+            residualInternalW<6>(m, P, e0, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
+            kInt = 0;
+        } else if (width == 4) {
+            residualInternalW<4>(m, P, e0, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
+            kInt = 0;
+        } else {
+            hasB = true;
+        }
+        for (int k = 0; hasB && k < width; ++k) {
             const int e = e0 + k * kSlice + lane;
             const int n = m.col[e];
             if (n >= 0) {
+                if (k >= kInt) continue;  // already summed by the fixed-width pass
                 const double S0 = m.eS[e], S1 = m.eS[m.nEntries + e],
                              S2 = m.eS[2 * m.nEntries + e];
                 const double w = m.eW[e];
@@ -417,6 +478,39 @@ __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockM
     storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
 }
 
+// Internal-face slots of the least-squares gradient sum with compile-time
+// slot count W, branch-free (boundary and empty slots gather the row's own
+// cell with zero weight); boundary slots follow in the general loop, in slot
+// order, as rows list internal faces first.
+template <typename T, int W, bool HASX>
+__device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, int lane, int C,
+                                              const double* __restrict__ D, const T* __restrict__ x,
+                                              const double* DnP, const double* CP, double* A, bool& hasB) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int e = e0 + k * kSlice + lane;
+        const int n = m.col[e];
+        const bool v = n >= 0;
+        hasB |= (!v && n != kPad);
+        const int c = v ? n : P;
+        double d[3], dv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            d[q] = m.cC[(size_t)q * C + c] - CP[q];
+            const double DN = HASX ? D[(size_t)q * C + c] + (double)x[(size_t)q * C + c] : D[(size_t)q * C + c];
+            dv[q] = DN - DnP[q];
+        }
+        const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double rr = v ? 1.0 / dd : 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double di = d[i] * rr;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) A[3 * i + j] += di * dv[j];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_grad: D' = D + x; boundary values D_b (solidTraction evaluate /
 // fixedDisplacement); least-squares grad(D') with the boundary correction;
@@ -463,10 +557,24 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         double Gold[9];
         bool haveG = false;
         int kb = width;  // first boundary slot
-        for (int k = 0; k < width; ++k) {
+        bool hasB = false;
+        int kInt = width;  // internal slots left to the general loop below
+        if (width == 6) {
+            if (x) gradInternalW<T, 6, true>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            else gradInternalW<T, 6, false>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            kInt = 0;
+        } else if (width == 4) {
+            if (x) gradInternalW<T, 4, true>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            else gradInternalW<T, 4, false>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            kInt = 0;
+        } else {
+            hasB = true;
+        }
+        for (int k = 0; hasB && k < width; ++k) {
             const int e = e0 + k * kSlice + lane;
             const int n = m.col[e];
             if (n >= 0) {
+                if (k >= kInt) continue;  // already summed by the fixed-width pass
                 double d[3], dv[3];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
