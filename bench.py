@@ -100,9 +100,13 @@ def main():
                     help="size of the plain copy kernel run once before the timed region (counter calibration)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
                     help="summarise rocprofv3 --pmc SQLite outputs (no GPU work) and exit")
+    ap.add_argument("--pmc-json", help="with --pmc-summary: also write the per-kernel figures as JSON")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_cube200_mixed.json"),
+                    help="committed rocprof counter summary that fills roofline.traffic")
     args = ap.parse_args()
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, args.calib_bytes)
+        pmc_summary(args.pmc_summary, args.calib_bytes, args.pmc_json,
+                    dict(workload="cube 200^3 hex jitter 0.15 seed 2", precision=args.only))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
@@ -207,8 +211,13 @@ def main():
     roofline = None
     if dom:
         a = kern[dom]["gbps"]
+        traffic = None
+        if args.n == 200 and args.jitter == 0.15 and args.seed == 2:
+            traffic = load_traffic(args.traffic_json, dom, head)
         roofline = dict(bound="hbm", achieved=a, peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(a / HBM_PEAK_GBPS, 4),
-                        traffic=None, kernel=dom, alg_bytes_per_launch=kern[dom]["alg_bytes"])
+                        traffic=traffic, kernel=dom, alg_bytes_per_launch=kern[dom]["alg_bytes"])
+        if traffic:
+            roofline["traffic_source"] = os.path.relpath(args.traffic_json, ROOT)
     extra = {}
     if "mixed" in res and "fp64" in res:
         lat64, outs64, _ = res["fp64"]
@@ -254,7 +263,32 @@ def main():
     print(json.dumps(line))
 
 
-def pmc_summary(paths, copy_bytes):
+# bench kernel name -> kernel tag in the rocprof summaries
+PMC_TAGS = {"residual": "k_residual", "spmv": "k_spmvp", "pcg_update": "k_update", "grad": "k_grad",
+            "mg0_down": "k_mg0_down", "mg0_up": "k_mg0_up"}
+
+
+def load_traffic(path, kernel, precision):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled) of
+    `kernel` from a committed rocprof summary (JSON written by --pmc-json),
+    or None."""
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    tag = PMC_TAGS.get(kernel)
+    if not tag:
+        return None
+    suffix = "<f>" if precision == "mixed" else "<d>"
+    for name in (tag + suffix, tag):
+        e = d.get("kernels", {}).get(name)
+        if e and e.get("FETCH_SIZE") is not None and e.get("WRITE_SIZE") is not None:
+            return int(round((e["FETCH_SIZE"] + e["WRITE_SIZE"]) * 1e6))
+    return None
+
+
+def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
     """Per-kernel averages of rocprofv3 --pmc counters (FETCH_SIZE / WRITE_SIZE,
     KiB per dispatch) from its SQLite output, each scaled by what the same
     counter reads for the plain copy kernel k_copy of known size (copy_bytes
@@ -295,10 +329,17 @@ def pmc_summary(paths, copy_bytes):
     print(f"copy-kernel scale factors (true KiB / counter KiB, {copy_bytes} B read + written): "
           + ", ".join(f"{c}={f:.3f}" for c, f in scale.items() if f))
     print(f"{'kernel':22s} {'counter':11s} {'avg KiB':>12s} {'scaled MB':>10s} {'dispatches':>10s} {'avg us':>9s}")
+    out = {}
     for k in sorted(rows):
         for c, (v, n, d) in sorted(rows[k].items()):
-            sc = v * scale.get(c, 1.0) * 1024 / 1e6 if scale.get(c) else float("nan")
-            print(f"{k:22s} {c:11s} {v:12.1f} {sc:10.1f} {n:10d} {d:9.2f}")
+            sc = v * scale[c] * 1024 / 1e6 if scale.get(c) else None
+            print(f"{k:22s} {c:11s} {v:12.1f} {sc if sc is not None else float('nan'):10.1f} {n:10d} {d:9.2f}")
+            out.setdefault(k, {})[c] = round(sc, 2) if sc is not None else None
+            out[k]["avg_us"] = round(d, 2)
+    if json_out:
+        with open(json_out, "w") as f:
+            json.dump(dict(meta or {}, scale=scale, unit="MB per dispatch (copy-kernel scaled)", kernels=out), f,
+                      indent=1, sort_keys=True)
 
 
 def ctypes_copy(L, ctx, nbytes):
