@@ -166,6 +166,10 @@ typedef struct {
     int32_t precision;             /* S4F_PRECISION_* */
     int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
     double mgCoarseScale;          /* GAMG: scale of the coarse-grid correction, in (0, 2]; 0 = 1.8 */
+    /* fvSolution relaxationFactors { fields { D a; } } with relaxationMethod
+       fixed (solidModel::relaxField, solidModel.C:837-841): D = D_prev + a (D - D_prev)
+       after each solve; 0 or 1 = no relaxation */
+    double relaxD;
 } s4f_solve_opts;
 
 typedef struct {

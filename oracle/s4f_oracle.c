@@ -73,6 +73,7 @@ typedef struct {
     double interfaceScale;  /* stabilisation interfaceScaleFactor */
     int historyCap;         /* capacity of the history arrays below */
     double *histInitRes, *histRvf; int *histInner;
+    double relaxD;          /* fvSolution relaxationFactors { fields { D } }: 0 or 1 = none */
 } or_opts;
 
 typedef struct {
@@ -988,7 +989,32 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
         for (int i = 0; i < C; ++i) c->D[3*i+q] = psi[i];
     }
     free(rD); free(psi); free(src); free(Ap); free(work);
+    const int relax = o->relaxD > 0.0 && o->relaxD != 1.0;
+    double *DbPrev = NULL;
+    if (relax) {                                            /* boundary values of D.prevIter() */
+        DbPrev = malloc(sizeof(double)*3*c->nB);
+        memcpy(DbPrev, c->Db, sizeof(double)*3*c->nB);
+    }
     or_correct_boundary(c);                                 /* psi.correctBoundaryConditions() */
+    if (relax) {
+        /* relaxField, fixed method (solidModel.C:837-841): D.relax(), i.e.
+           D == prevIter + a (D - prevIter) on cells and boundary values;
+           fixedDisplacement snGrad is evaluated afterwards from the relaxed
+           cell value (fixedDisplacementFvPatchVectorField.C:315-319) */
+        const double a = o->relaxD;
+        for (int i = 0; i < 3*C; ++i) c->D[i] = c->Dprev[i] + a*(c->D[i] - c->Dprev[i]);
+        for (int b = 0; b < c->nB; ++b) {
+            int P = bCell(c, b);
+            if (c->bType[b] == OR_TRACTION) {
+                for (int q = 0; q < 3; ++q) c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+            } else {
+                double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
+                for (int q = 0; q < 3; ++q)
+                    c->snb[3*b+q] = (c->Db[3*b+q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
+            }
+        }
+        free(DbPrev);
+    }
     or_grad(c);                                             /* mechanical().grad(D, gradD) */
     or_sigma(c);                                            /* mechanical().correct(sigma) */
     /* solidModel::converged (solidModelTemplates.C:26-188), total approach */

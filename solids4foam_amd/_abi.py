@@ -43,7 +43,7 @@ class SolveOpts(ctypes.Structure):
                 ("interfaceScale", ctypes.c_double), ("pcgTolerance", ctypes.c_double),
                 ("pcgRelTol", ctypes.c_double), ("pcgMaxIter", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
                 ("reserved0", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
-                ("mgCoarseScale", ctypes.c_double)]
+                ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double)]
 
 
 class SolveReport(ctypes.Structure):

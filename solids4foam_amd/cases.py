@@ -43,3 +43,37 @@ def cube_tension(ctx, n=200, jitter=0.15, seed=2, t=1e7, E=E_STEEL, nu=NU_STEEL,
 
 SMALL_OPTS = dict(pcgTolerance=1e-9, pcgRelTol=0.1, solutionTolerance=1e-7, alternativeTolerance=1e-7,
                   nCorrectors=500)
+
+
+def box_sweep_specs(count, seed, nmin=8, nmax=64, jitter=0.1, h=0.01):
+    """Specifications of the batched-sweep boxes: n_x, n_y, n_z uniform in
+    nmin..nmax (uneven sizes), spacing h in every direction, interior
+    vertices jittered +/- jitter*h, E log-normal (median 100 GPa, sigma 0.5),
+    nu uniform 0.2-0.45, x-max traction of magnitude uniform 1e5-1e7 Pa in a
+    uniformly random direction; x-min fixed, other faces traction-free."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = rng.integers(nmin, nmax + 1, size=3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        specs.append(dict(n=tuple(int(v) for v in n), L=tuple(float(v * h) for v in n), jitter=jitter,
+                          seed=int(rng.integers(1 << 30)), E=float(100e9 * np.exp(0.5 * rng.normal())),
+                          nu=float(rng.uniform(0.2, 0.45)), traction=tuple(float(v) for v in
+                                                                            rng.uniform(1e5, 1e7) * d)))
+    return specs
+
+
+def box_case(ctx, spec, traction=None):
+    """Mesh of one sweep box and its per-boundary-face values (patch order):
+    returns (PolyMesh, SolverMesh, bc_values (nBoundaryFaces, 3))."""
+    pm = PolyMesh.box(*spec["n"], spec["L"], spec["jitter"], spec["seed"], False)
+    types = [FIX, TRAC, TRAC, TRAC, TRAC, TRAC]
+    m = SolverMesh(ctx, pm, types)
+    m.set_material(spec["E"], spec["nu"])
+    t = spec["traction"] if traction is None else traction
+    bv = np.zeros((pm.nBoundaryFaces, 3))
+    bv[pm.patch_faces(1)] = t
+    for p in range(6):
+        m.set_bc(p, bv[pm.patch_faces(p)])
+    return pm, m, bv

@@ -629,6 +629,7 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
     c.pcgMaxIter = o.pcgMaxIter;
     c.nCorr = o.nCorrectors;
     c.histCap = histCap;
+    c.relaxD = (o.relaxD > 0.0) ? o.relaxD : 1.0;
     return c;
 }
 
@@ -644,6 +645,7 @@ void checkOpts(const s4f_solve_opts& o) {
     if (o.preconditioner != S4F_PRECOND_DIAGONAL && o.preconditioner != S4F_PRECOND_GAMG)
         throw InvalidError("unknown preconditioner");
     if (!(o.mgCoarseScale >= 0.0 && o.mgCoarseScale <= 2.0)) throw InvalidError("mgCoarseScale must be in [0, 2]");
+    if (!(o.relaxD >= 0.0 && o.relaxD <= 1.0)) throw InvalidError("relaxD must be in [0, 1]");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }

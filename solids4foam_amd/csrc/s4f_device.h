@@ -74,6 +74,7 @@ struct Ctl {
     // settings
     double pcgTol, pcgRelTol, solTol, altTol;
     int pcgMaxIter, nCorr, histCap, pcMode;  // pcMode 1: z comes from the V-cycle (k_fin_rho sets beta)
+    double relaxD;                           // fixed field under-relaxation factor (1: none)
 };
 
 }  // namespace s4f

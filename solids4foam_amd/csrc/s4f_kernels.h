@@ -485,7 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockM
 template <typename T, int W, bool HASX>
 __device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, int lane, int C,
                                               const double* __restrict__ D, const T* __restrict__ x,
-                                              const double* DnP, const double* CP, double* A, bool& hasB) {
+                                              double ra, const double* DnP, const double* CP, double* A,
+                                              bool& hasB) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
@@ -497,7 +498,7 @@ __device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, i
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             d[q] = m.cC[(size_t)q * C + c] - CP[q];
-            const double DN = HASX ? D[(size_t)q * C + c] + (double)x[(size_t)q * C + c] : D[(size_t)q * C + c];
+            const double DN = HASX ? D[(size_t)q * C + c] + ra * (double)x[(size_t)q * C + c] : D[(size_t)q * C + c];
             dv[q] = DN - DnP[q];
         }
         const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -533,6 +534,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
     // current field and the buffer receiving D' (per-instance ping-pong)
     const double* __restrict__ D = ctl->dpar ? D1 : D0;
+    const bool relax = mode == 0 && ctl->relaxD != 1.0;
+    const double ra = relax ? ctl->relaxD : 1.0;  // x scaled by exactly 1.0 without relaxation
     double* __restrict__ Dn = (mode == 0) ? (ctl->dpar ? D0 : D1) : nullptr;
     const int C = m.nCells, B = m.nBoundary;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // max|D'-D|, max|D'-Dold|, max|D'|, sum D'
@@ -544,11 +547,15 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
-        double DP[3], DnP[3], CP[3];
+        // DuP: the solve's update D + x; DnP: after the fixed field
+        // under-relaxation D_prev + a (DuP - D_prev) (a = 1: the same)
+        double DP[3], DnP[3], DuP[3], CP[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             DP[k] = D[(size_t)k * C + P];
-            DnP[k] = x ? DP[k] + (double)x[(size_t)k * C + P] : DP[k];
+            const double xv = x ? (double)x[(size_t)k * C + P] : 0.0;
+            DuP[k] = DP[k] + xv;
+            DnP[k] = DP[k] + ra * xv;
             CP[k] = m.cC[(size_t)k * C + P];
         }
         double A[9];
@@ -560,12 +567,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         bool hasB = false;
         int kInt = width;  // internal slots left to the general loop below
         if (width == 6) {
-            if (x) gradInternalW<T, 6, true>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
-            else gradInternalW<T, 6, false>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            if (x) gradInternalW<T, 6, true>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
+            else gradInternalW<T, 6, false>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
             kInt = 0;
         } else if (width == 4) {
-            if (x) gradInternalW<T, 4, true>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
-            else gradInternalW<T, 4, false>(m, P, e0, lane, C, D, x, DnP, CP, A, hasB);
+            if (x) gradInternalW<T, 4, true>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
+            else gradInternalW<T, 4, false>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
             kInt = 0;
         } else {
             hasB = true;
@@ -579,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     d[q] = m.cC[(size_t)q * C + n] - CP[q];
-                    const double DN = x ? D[(size_t)q * C + n] + (double)x[(size_t)q * C + n] : D[(size_t)q * C + n];
+                    const double DN = x ? D[(size_t)q * C + n] + ra * (double)x[(size_t)q * C + n] : D[(size_t)q * C + n];
                     dv[q] = DN - DnP[q];
                 }
                 const double rr = 1.0 / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -611,7 +618,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
                         double sn;
                         if (trac) {
                             const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
-                            Dbv[j] = DnP[j] + kG + gv / dl;
+                            Dbv[j] = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
+                            if (relax) {                      // D.relax() on the boundary values too
+                                const double Dbp = Db[(size_t)j * B + b];
+                                Dbv[j] = Dbp + ra * (Dbv[j] - Dbp);
+                            }
                             sn = gv;
                         } else {
                             Dbv[j] = m.bVal[(size_t)j * B + b];

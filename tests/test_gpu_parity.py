@@ -180,6 +180,43 @@ def test_first_iterations_follow_oracle(ctx, oracle_mod):
     assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-9
 
 
+def _nu_box(ctx, oracle_mod, n, nu):
+    from solids4foam_amd import PolyMesh, SolverMesh
+    om = oracle_mod.Mesh.box(n, n, n, (0.1, 0.1, 0.1), 0.1, 3)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(1)] = (1e6, 5e5, 0.0)
+    oc = oracle_mod.Case(om, 100e9, nu, [FIXED] + [TRACTION] * 5, bv)
+    pm = PolyMesh.box(n, n, n, L=(0.1, 0.1, 0.1), jitter=0.1, seed=3)
+    m = SolverMesh(ctx, pm, [FIXED] + [TRACTION] * 5)
+    m.set_material(100e9, nu)
+    for p in range(6):
+        m.set_bc(p, bv[pm.patch_faces(p)])
+    return oc, m
+
+
+def test_field_relaxation_follows_oracle(ctx, oracle_mod):
+    # fixed under-relaxation of D (relaxField): the first outer iterations
+    # reproduce the oracle's history, and at nu = 0.45 (where the unrelaxed
+    # loop oscillates) the relaxed solve converges to the oracle's field
+    oc, m = _nu_box(ctx, oracle_mod, 6, 0.45)
+    ro = oc.solve(history=10, nCorr=6, precond=oracle_mod.PRECOND_DIAGONAL, relaxD=0.7)
+    out = m.solve(nCorrectors=6, relaxD=0.7)
+    h = m.history()
+    assert np.allclose(h["initRes"][:6], ro["history"]["initRes"][:6], rtol=1e-8, atol=1e-14)
+    assert np.allclose(h["rvf"][:6], ro["history"]["rvf"][:6], rtol=1e-8)
+    assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-9
+    oc, m = _nu_box(ctx, oracle_mod, 6, 0.45)
+    kw = dict(solTol=1e-11, altTol=1e-11, nCorr=50000)
+    ro = oc.solve(precond=oracle_mod.PRECOND_DIC, relaxD=0.7, **kw)
+    for precision in (0, 1):
+        out = m.solve(nCorrectors=50000, solutionTolerance=1e-11, alternativeTolerance=1e-11, relaxD=0.7,
+                      precision=precision)
+        print(f"  relaxed nu=0.45 precision={precision}: nOuter={out['nOuter']} (oracle {ro['nOuter']}) "
+              f"err={rel_l2(out['D'], oc.get('D').reshape(-1, 3)):.2e}", flush=True)
+        assert ro["converged"] and out["converged"]
+        assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < (1e-8 if precision == 0 else 1e-6)
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)

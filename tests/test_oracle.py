@@ -283,3 +283,35 @@ def test_pcg_matches_dense_solve(oracle_mod):
         D1 = c3.get("D").reshape(-1, 3)
         exact = D0 + np.linalg.solve(M, R)
         assert np.abs(D1 - exact).max() < 1e-10 * np.abs(exact).max()
+
+
+def _box_nu(oracle_mod, n, nu, jitter=0.1, seed=3):
+    om = oracle_mod.Mesh.box(n, n, n, (0.1, 0.1, 0.1), jitter, seed)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(1)] = (1e6, 5e5, 0.0)
+    return om, oracle_mod.Case(om, 100e9, nu, [FIXED] + [TRACTION] * 5, bv)
+
+
+def test_field_relaxation_keeps_fixed_point(oracle_mod):
+    # relaxField 'fixed' (solidModel.C:837-841) changes the path, not the
+    # converged field: nu = 0.3 reaches the same D with and without it
+    _, a = _box_nu(oracle_mod, 6, 0.3)
+    _, b = _box_nu(oracle_mod, 6, 0.3)
+    kw = dict(nCorr=20000, solTol=1e-11, altTol=1e-11, precond=oracle_mod.PRECOND_DIC)
+    ra, rb = a.solve(**kw), b.solve(relaxD=0.7, **kw)
+    assert ra["converged"] and rb["converged"] and rb["nOuter"] > ra["nOuter"]
+    Da, Db = a.get("D"), b.get("D")
+    assert np.linalg.norm(Da - Db) / np.linalg.norm(Da) < 1e-8
+
+
+def test_field_relaxation_restores_convergence_high_nu(oracle_mod):
+    # the segregated loop oscillates without relaxation at nu = 0.45 and
+    # converges with D relaxed by 0.7
+    _, a = _box_nu(oracle_mod, 6, 0.45)
+    ra = a.solve(nCorr=400, solTol=1e-9, altTol=1e-9, precond=oracle_mod.PRECOND_DIC)
+    assert not ra["converged"] and ra["initRes"] > 0.1
+    _, b = _box_nu(oracle_mod, 6, 0.45)
+    rb = b.solve(nCorr=20000, solTol=1e-9, altTol=1e-9, precond=oracle_mod.PRECOND_DIC, relaxD=0.7)
+    assert rb["converged"]
+    D = b.get("D").reshape(-1, 3)
+    assert np.all(np.isfinite(D)) and np.abs(D).max() > 0
