@@ -103,9 +103,10 @@ def main():
     ap.add_argument("--pmc-json", help="with --pmc-summary: also write the per-kernel figures as JSON")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_cube200_mixed.json"),
                     help="committed rocprof counter summary that fills roofline.traffic")
-    ap.add_argument("--config", choices=["cube", "batch", "serve"], default="cube",
+    ap.add_argument("--config", choices=["cube", "batch", "serve", "tets"], default="cube",
                     help="cube: the 200^3 solve (default); batch: the batched sweep of uneven boxes; "
-                         "serve: a Poisson request stream through the continuous-batching server")
+                         "serve: a Poisson request stream through the continuous-batching server; "
+                         "tets: the n^3 x 6 tetrahedra cube with soft spherical inclusions")
     ap.add_argument("--count", type=int, default=256, help="batch: number of boxes")
     ap.add_argument("--nmax", type=int, default=64, help="batch/serve: largest cells per direction")
     ap.add_argument("--sample", type=int, default=8, help="batch/serve: boxes solved one at a time for the baseline")
@@ -115,6 +116,9 @@ def main():
     ap.add_argument("--slots", type=int, default=64, help="serve: in-flight solves")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serve: HBM arena")
     ap.add_argument("--sweep-tol", type=float, default=1e-9, help="batch/serve: outer tolerance")
+    ap.add_argument("--relax", type=float, default=0.7,
+                    help="batch/serve: fixed under-relaxation of D (the sweep reaches nu = 0.45, where the "
+                         "unrelaxed segregated loop oscillates)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.calib_bytes, args.pmc_json,
@@ -133,6 +137,8 @@ def main():
         return run_batch(args, ctx)
     if args.config == "serve":
         return run_serve(args, ctx)
+    if args.config == "tets":
+        return run_tets(args, ctx)
     t0 = time.time()
     pm, mesh = cases.cube_tension(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
     setup_s = time.time() - t0
@@ -377,14 +383,14 @@ def _sequential(ctx, cases_, specs, opts, label):
 
 def run_batch(args, ctx):
     """Batched sweep: `count` uneven boxes packed into one structure and solved
-    together (mixed precision, diagonal-preconditioned PCG, each instance
-    stopping on its own convergence), against plain FP64 solves of a sample of
-    the same boxes one at a time."""
+    together (mixed precision, PCG with the block-diagonal multigrid V-cycle,
+    each instance stopping on its own convergence), against plain FP64 solves
+    of a sample of the same boxes one at a time."""
     from solids4foam_amd import cases
     from solids4foam_amd.batch import Batch
     specs = cases.box_sweep_specs(args.count, args.seed, 8, args.nmax)
     opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000)
+                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax)
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
     b = Batch(ctx)
@@ -393,9 +399,12 @@ def run_batch(args, ctx):
     cells = sum(m.nCells for _, m, _ in meshes)
     print(f"[bench] batch of {len(specs)} boxes, {cells} cells, set-up {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
-    b.solve(want_sigma=False, precision=1, **dict(opts, nCorrectors=1))  # pack + first launch outside the clock
+    from solids4foam_amd import _abi
+    bopts = dict(opts, preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
+    # pack, multigrid hierarchy and first launch outside the clock
+    b.solve(want_sigma=False, precision=1, **dict(bopts, nCorrectors=1))
     t = time.perf_counter()
-    res = b.solve(want_sigma=False, precision=1, **opts)
+    res = b.solve(want_sigma=False, precision=1, **bopts)
     T = time.perf_counter() - t
     outer = np.array([r["nOuter"] for r in res])
     nconv = sum(r["converged"] for r in res)
@@ -413,7 +422,8 @@ def run_batch(args, ctx):
         "config": {"workload": f"{len(specs)} boxes n in 8..{args.nmax} per direction, jitter 0.1, E log-normal "
                                f"(median 100 GPa, sigma 0.5), nu U(0.2,0.45), x-max traction U(1e5,1e7) Pa random "
                                f"direction, seed {args.seed}", "cells": int(cells), "outer_tolerance": args.sweep_tol,
-                   "pcg": f"diagonal, relTol {args.reltol} (batched); gamg (sequential)"},
+                   "pcg": f"{args.precond}, relTol {args.reltol} (batched); gamg (sequential)",
+                   "relaxD": args.relax},
         "batch_seconds": round(T, 3), "converged": int(nconv), "outer_median": float(np.median(outer)),
         "outer_max": int(outer.max()), "cell_iter_per_s": round(float(sum(m.nCells * r["nOuter"] for (_, m, _), r in
                                                                          zip(meshes, res)) / T), 1),
@@ -455,7 +465,7 @@ def run_serve(args, ctx):
             u += 1
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
     opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000)
+                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax)
     srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1, **opts)
     got, ids = {}, {}
     start = time.perf_counter()
@@ -491,13 +501,58 @@ def run_serve(args, ctx):
         "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, boxes n in 8..{args.nmax}, "
                                f"{args.repeat:.0%} repeat an earlier mesh with a new load (warm start), seed "
                                f"{args.seed}", "slots": args.slots, "budget_GiB": args.budget_gib,
-                   "outer_tolerance": args.sweep_tol, "pcg": f"diagonal, relTol {args.reltol}"},
+                   "outer_tolerance": args.sweep_tol, "pcg": f"diagonal, relTol {args.reltol}",
+                   "relaxD": args.relax},
         "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
         "converged": int(conv), "warm_started": int(warm),
         "sequential_fp64_solves_per_s": round(seq_rate, 4), "sequential_sample": len(secs),
         "speedup_vs_sequential_fp64": round(value / seq_rate, 2),
         "max_err_D_vs_fp64": max(errs) if errs else None,
+    }
+    print(json.dumps(line))
+
+
+def run_tets(args, ctx):
+    """Heterogeneous tetrahedral case: mixed-precision solve (value) and FP64
+    solve of the same settings, displacement and von Mises errors."""
+    import solids4foam_amd as s4f
+    from solids4foam_amd import _abi, cases
+    t0 = time.time()
+    pm, mesh, frac, nInc = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
+    C = mesh.nCells
+    print(f"[bench] tets {args.n}^3 x 6: C={C}, {nInc} inclusions, {frac:.3f} of cells soft, set-up "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
+                nCorrectors=200000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale,
+                preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
+    t = time.perf_counter()
+    mesh.solve(want_sigma=False, precision=1, **dict(opts, nCorrectors=1))
+    print(f"[bench] hierarchy + 1 outer iteration {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    res = {}
+    for name, prec in (("mixed", 1), ("fp64", 0)):
+        t = time.perf_counter()
+        out = mesh.solve(want_sigma=True, precision=prec, **opts)
+        res[name] = (time.perf_counter() - t, out)
+        print(f"[bench] {name}: nOuter={out['nOuter']} converged={out['converged']} {res[name][0]:.2f}s",
+              file=sys.stderr, flush=True)
+    (tm, om), (tf, of) = res["mixed"], res["fp64"]
+    line = {
+        "metric": "converged solves/s & cell-iter/s at 1e-6 err vs FP64 (heterogeneous tetrahedra)",
+        "value": round(1.0 / tm, 6), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"cube {args.n}^3 hex jitter {args.jitter} seed {args.seed} split into 6 tets; "
+                               f"E 200 GPa with {nInc} spherical inclusions (r 0.02-0.06, 10% volume) at 20 GPa; "
+                               f"nu 0.3; x-min fixed, x-max traction (0,5e6,0) Pa", "cells": int(C),
+                   "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}",
+                   "gradient": "extendedLeastSquares 0 (full delta)"},
+        "ms_per_step": round(1e3 * tm, 1), "nOuter": om["nOuter"], "pcg_iterations": om["pcgIterations"],
+        "cell_iter_per_s": round(C * om["nOuter"] / tm, 1),
+        "fp64_solves_per_s": round(1.0 / tf, 6), "fp64_nOuter": of["nOuter"],
+        "mixed_over_fp64": round(tf / tm, 3),
+        "err_D_vs_fp64": rel_l2(om["D"], of["D"]),
+        "err_vonMises_vs_fp64": rel_l2(s4f.sigma_eq(om["sigma"]), s4f.sigma_eq(of["sigma"])),
+        "soft_cell_fraction": round(frac, 4),
     }
     print(json.dumps(line))
 

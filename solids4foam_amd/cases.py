@@ -77,3 +77,60 @@ def box_case(ctx, spec, traction=None):
     for p in range(6):
         m.set_bc(p, bv[pm.patch_faces(p)])
     return pm, m, bv
+
+
+def inclusions(seed, volume_fraction=0.1, rmin=0.02, rmax=0.06):
+    """Spherical inclusions in the unit cube: centres uniform in the cube,
+    radii uniform in [rmin, rmax], added until their summed volume reaches
+    the volume fraction.  Returns (centres (k, 3), radii (k,))."""
+    rng = np.random.default_rng(seed)
+    cs, rs, vol = [], [], 0.0
+    while vol < volume_fraction:
+        r = rng.uniform(rmin, rmax)
+        cs.append(rng.uniform(0.0, 1.0, size=3))
+        rs.append(r)
+        vol += 4.0 / 3.0 * np.pi * r ** 3
+    return np.array(cs), np.array(rs)
+
+
+def inside_spheres_box(C, n, centres, radii, per_hex=1):
+    """mask of cell centres C inside any sphere, for cells of the generated
+    unit-cube box (hex (i, j, k) -> cells per_hex*(i + n j + n^2 k) + 0..per_hex-1,
+    centres within one spacing of their hex): each sphere tests only the
+    cells of the hexes its bounding box (plus one hex) covers"""
+    out = np.zeros(len(C), bool)
+    h = 1.0 / n
+    sub = np.arange(per_hex)
+    for c, r in zip(centres, radii):
+        lo = np.maximum(np.floor((c - r) / h).astype(int) - 1, 0)
+        hi = np.minimum(np.ceil((c + r) / h).astype(int) + 1, n - 1)
+        i = np.arange(lo[0], hi[0] + 1)
+        j = np.arange(lo[1], hi[1] + 1)
+        k = np.arange(lo[2], hi[2] + 1)
+        hexes = (i[None, None, :] + n * j[None, :, None] + n * n * k[:, None, None]).ravel()
+        cells = (per_hex * hexes[:, None] + sub[None, :]).ravel()
+        d = C[cells] - c
+        out[cells[np.einsum("ij,ij->i", d, d) <= r * r]] = True
+    return out
+
+
+def tet_inclusions(ctx, n=200, jitter=0.15, seed=2, inc_seed=3, E_matrix=200e9, E_inc=20e9, nu=0.3,
+                   traction=(0.0, 5e6, 0.0)):
+    """The jittered n^3 cube split into 6 tetrahedra per hex (48M cells at
+    n = 200), per-cell E: E_inc for cells whose centroid lies in a seeded
+    spherical inclusion (10 % volume fraction), E_matrix elsewhere; x-min
+    fixed, x-max shear traction, other faces traction-free.  Tetrahedra use
+    the full-delta least-squares gradient (extendedLeastSquares 0): with the
+    normal-delta leastSquares the segregated loop diverges on tets."""
+    pm = PolyMesh.box(n, n, n, (1.0, 1.0, 1.0), jitter, seed, True)
+    types = [FIX, TRAC, TRAC, TRAC, TRAC, TRAC]
+    m = SolverMesh(ctx, pm, types, ls_full_delta=True)
+    cen, rad = inclusions(inc_seed)
+    inc = inside_spheres_box(pm.cell_centres(), n, cen, rad, per_hex=6)
+    E = np.where(inc, E_inc, E_matrix)
+    m.set_material(E, nu)
+    m.set_bc(0, (0.0, 0.0, 0.0))
+    m.set_bc(1, traction)
+    for p in range(2, 6):
+        m.set_bc(p, (0.0, 0.0, 0.0))
+    return pm, m, float(inc.mean()), len(rad)

@@ -4,8 +4,12 @@
 // Hierarchy: plain (piecewise-constant) aggregation of strongly connected
 // cells, Galerkin coarse operators A_c = P^T A P, built once per matrix on the
 // host.  Cycle: V(1,1) with damped Jacobi smoothing (omega = 2/3) and an
-// exact solve on the coarsest level (explicit inverse, n <= kCoarseMax), so
-// the cycle is a fixed symmetric positive definite operator and CG stays CG.
+// exact solve on the coarsest level (explicit inverse of each instance's
+// block, <= kCoarseBlock rows), so the cycle is a fixed symmetric positive
+// definite operator and CG stays CG.  A batch of meshes is one block-diagonal
+// matrix: aggregates never cross instances, every level keeps each
+// instance's rows contiguous, and each row looks up its instance's control
+// block (converged instances and components are skipped).
 //
 // Launch shape per level (vectors hold 3 components, SoA, stride = the
 // level's row count; converged components are skipped):
@@ -22,7 +26,7 @@
 
 namespace s4f {
 
-constexpr int kCoarseMax = 1024;  // largest level solved by its explicit inverse
+constexpr int kCoarseBlock = 256;  // rows per instance on the coarsest level (explicit inverse)
 constexpr double kMgCoarseScaleDefault = 1.8;  // coarse-correction scale when the caller gives 0
 
 // device view of one coarse level
@@ -41,6 +45,11 @@ struct MgLevel {
     const int* agg;    // row -> row of the next coarser level (nullptr on the coarsest)
     const int* memOff; // (coarse side) row -> first member of the finer level
     const int* mem;    // members (rows of the finer level)
+    const int* rowInst;  // row -> instance
+    const int* blkRow0;  // coarsest level, per instance: first row of its block
+    const int* blkN;     //   rows of its block
+    const int* invOff;   //   offset of its explicit inverse (blkN^2 entries, row-major)
+    const int* pass;     // per instance: passes through this level unchanged (no smoothing, no scaling)
 };
 
 template <typename T> struct MgSel;
@@ -283,9 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const T* __restrict__ r, int nFine,
                                                         T* __restrict__ bc, const Ctl* __restrict__ ctl) {
-    if (mgSkip(ctl)) return;
-    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
+        const Ctl* c = ctl + Lc.rowInst[I];
+        if (mgSkip(c)) continue;
+        const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
         T s[3] = {(T)0, (T)0, (T)0};
         const int k1 = Lc.memOff[I + 1];
         // members 8 at a time, loads unconditional (past the end: the last
@@ -321,8 +331,6 @@ constexpr int kMgChunk = 8;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __restrict__ b, T* __restrict__ x,
                                                     T* __restrict__ r, double omega, const Ctl* __restrict__ ctl) {
-    if (mgSkip(ctl)) return;
-    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     const T* __restrict__ A = MgSel<T>::a(L);
     const T* __restrict__ iDg = MgSel<T>::iD(L);
     const T* __restrict__ dg = MgSel<T>::d(L);
@@ -333,6 +341,16 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
     for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
         const int P = s * kSlice + lane;
         if (P >= n) continue;
+        const int inst = L.rowInst[P];
+        const Ctl* cP = ctl + inst;
+        if (mgSkip(cP)) continue;
+        const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
+        if (L.pass[inst]) {  // already coarsest for this instance: hand b down unchanged
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (act[j]) r[(size_t)j * n + P] = b[(size_t)j * n + P];
+            continue;
+        }
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
         const T iD = iDg[P];
@@ -378,8 +396,6 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
                                                   int nCoarse, const T* __restrict__ e, T* __restrict__ out,
                                                   double omega, double alpha, const Ctl* __restrict__ ctl) {
-    if (mgSkip(ctl)) return;
-    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
     const T* __restrict__ A = MgSel<T>::a(L);
     const T* __restrict__ iDg = MgSel<T>::iD(L);
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
@@ -390,9 +406,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
     for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
         const int P = s * kSlice + lane;
         if (P >= n) continue;
+        const int inst = L.rowInst[P];
+        const Ctl* cP = ctl + inst;
+        if (mgSkip(cP)) continue;
+        const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
+        const int IP = agg[P];
+        if (L.pass[inst]) {  // already coarsest for this instance: the coarser solution unchanged
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (act[j]) out[(size_t)j * n + P] = e[(size_t)j * nCoarse + IP];
+            continue;
+        }
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
-        const int IP = agg[P];
         T sum[3], xP[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -425,23 +451,28 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
     }
 }
 
-// coarsest level: x = A^-1 b with the explicit (symmetric) inverse, one
-// wavefront per row
+// coarsest level: x = A^-1 b with the explicit (symmetric) inverse of the
+// row's instance block, one wavefront per row
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_dense(int n, const T* __restrict__ Ainv, const T* __restrict__ b,
-                                                     T* __restrict__ x, const Ctl* __restrict__ ctl) {
-    if (mgSkip(ctl)) return;
-    const bool act[3] = {ctl->active[0] != 0, ctl->active[1] != 0, ctl->active[2] != 0};
+__global__ __launch_bounds__(kBlock) void k_mg_dense(const MgLevel L, const T* __restrict__ Ainv,
+                                                     const T* __restrict__ b, T* __restrict__ x,
+                                                     const Ctl* __restrict__ ctl) {
     const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+    if (row >= L.n) return;
+    const int inst = L.rowInst[row];
+    const Ctl* c = ctl + inst;
+    if (mgSkip(c)) return;
+    const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const int r0 = L.blkRow0[inst], nb = L.blkN[inst];
+    const int n = L.n;
     double acc[3] = {0, 0, 0};
-    const T* __restrict__ a = Ainv + (size_t)row * n;
-    for (int c = lane; c < n; c += 64) {
-        const double v = (double)a[c];
+    const T* __restrict__ a = Ainv + (size_t)L.invOff[inst] + (size_t)(row - r0) * nb;
+    for (int k = lane; k < nb; k += 64) {
+        const double v = (double)a[k];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            if (act[j]) acc[j] += v * (double)b[(size_t)j * n + c];
+            if (act[j]) acc[j] += v * (double)b[(size_t)j * n + r0 + k];
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[j] = waveReduce<false>(acc[j]);

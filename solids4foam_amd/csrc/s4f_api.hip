@@ -152,15 +152,19 @@ struct AmgHostLevel {
     int n = 0, nSlices = 0;
     std::vector<int> sliceStart, sliceWidth, col;
     std::vector<double> a, diag;
-    std::vector<int> agg;          // rows -> rows of the next coarser level (empty on the coarsest)
+    std::vector<int> agg;          // rows -> rows of the next coarser level (-1: padding row; empty on the coarsest)
     std::vector<int> memOff, mem;  // rows -> member rows of the finer level (empty on level 0)
+    std::vector<int> inst0, instN; // per instance: first row and row count (instances are contiguous)
+    std::vector<int> rowInst;      // row -> instance (-1: padding row)
+    std::vector<int> instPass;     // per instance: 1 if it passes unchanged to the next level (already coarsest)
 };
 struct AmgDevLevel {
     int n = 0, nSlices = 0;
-    DArr<int> sliceStart, sliceWidth, col, agg, memOff, mem;
+    DArr<int> sliceStart, sliceWidth, col, agg, memOff, mem, rowInst, instPass;
+    DArr<int> blkRow0, blkN, invOff;          // coarsest level: per-instance dense blocks
     DArr<double> a64, d64, iD64;
     DArr<float> a32, d32, iD32;
-    DArr<double> x64, b64, r64, xs64, inv64;  // inv: explicit inverse (coarsest level only)
+    DArr<double> x64, b64, r64, xs64, inv64;  // inv: explicit inverses (coarsest level only)
     DArr<float> x32, b32, r32, xs32, inv32;
     MgLevel view() const {
         MgLevel L{};
@@ -168,6 +172,7 @@ struct AmgDevLevel {
         L.sliceStart = sliceStart.p; L.sliceWidth = sliceWidth.p; L.col = col.p;
         L.a32 = a32.p; L.a64 = a64.p; L.diag32 = d32.p; L.diag64 = d64.p; L.invD32 = iD32.p; L.invD64 = iD64.p;
         L.agg = agg.p; L.memOff = memOff.p; L.mem = mem.p;
+        L.rowInst = rowInst.p; L.blkRow0 = blkRow0.p; L.blkN = blkN.p; L.invOff = invOff.p; L.pass = instPass.p;
         return L;
     }
 };
@@ -635,7 +640,8 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
 
 void singleMeshOnly(const s4f_solve_opts& o, const char* entry) {
     if (o.preconditioner == S4F_PRECOND_GAMG)
-        throw InvalidError(std::string(entry) + ": the multigrid preconditioner is available in s4f_solve only");
+        throw InvalidError(std::string(entry) + ": the multigrid preconditioner is available in s4f_solve and "
+                           "s4f_batch_solve only");
 }
 
 void checkOpts(const s4f_solve_opts& o) {
@@ -718,44 +724,45 @@ inline void forRow(const std::vector<int>& ss, const std::vector<int>& sw, const
     }
 }
 
-// Greedy aggregation on strong connections (a_ij >= theta max_k a_ik):
-// seed aggregates from rows whose strong neighbours are all free, attach the
-// rest to their strongest aggregated neighbour, leftovers become singletons.
-int aggregate(const AmgHostLevel& L, std::vector<int>& agg) {
+// Greedy aggregation on strong connections (a_ij >= theta max_k a_ik) of the
+// rows [r0, r1) (one instance; no connection leaves it): seed aggregates from
+// rows whose strong neighbours are all free, attach the rest to their
+// strongest aggregated neighbour, leftovers become singletons.  Aggregates
+// are numbered from nc0 in seed order; returns the count.
+int aggregate(const AmgHostLevel& L, std::vector<int>& agg, int r0, int r1, int nc0) {
     const double theta = 0.25;
-    const int n = L.n;
-    agg.assign(n, -1);
-    std::vector<double> amax(n, 0.0);
-    for (int i = 0; i < n; ++i) forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int) { amax[i] = std::max(amax[i], L.a[e]); });
-    int nc = 0;
-    for (int i = 0; i < n; ++i) {
+    std::vector<double> amax(r1 - r0, 0.0);
+    for (int i = r0; i < r1; ++i)
+        forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int) { amax[i - r0] = std::max(amax[i - r0], L.a[e]); });
+    int nc = nc0;
+    for (int i = r0; i < r1; ++i) {
         if (agg[i] >= 0) continue;
         bool free = true;
         forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
-            if (L.a[e] >= theta * amax[i] && agg[j] >= 0) free = false;
+            if (L.a[e] >= theta * amax[i - r0] && agg[j] >= 0) free = false;
         });
         if (!free) continue;
         agg[i] = nc;
         forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
-            if (L.a[e] >= theta * amax[i]) agg[j] = nc;
+            if (L.a[e] >= theta * amax[i - r0]) agg[j] = nc;
         });
         ++nc;
     }
-    std::vector<int> joined(n, -1);
-    for (int i = 0; i < n; ++i) {
+    std::vector<int> joined(r1 - r0, -1);
+    for (int i = r0; i < r1; ++i) {
         if (agg[i] >= 0) continue;
         double best = -1.0;
         int to = -1;
         forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int j) {
-            if (agg[j] >= 0 && L.a[e] >= theta * amax[i] && L.a[e] > best) { best = L.a[e]; to = agg[j]; }
+            if (agg[j] >= 0 && L.a[e] >= theta * amax[i - r0] && L.a[e] > best) { best = L.a[e]; to = agg[j]; }
         });
-        joined[i] = to;
+        joined[i - r0] = to;
     }
-    for (int i = 0; i < n; ++i)
-        if (agg[i] < 0 && joined[i] >= 0) agg[i] = joined[i];
-    for (int i = 0; i < n; ++i)
+    for (int i = r0; i < r1; ++i)
+        if (agg[i] < 0 && joined[i - r0] >= 0) agg[i] = joined[i - r0];
+    for (int i = r0; i < r1; ++i)
         if (agg[i] < 0) agg[i] = nc++;
-    return nc;
+    return nc - nc0;
 }
 
 // Galerkin product with piecewise-constant prolongation: coarse diagonal
@@ -765,12 +772,14 @@ void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHos
     Cl = AmgHostLevel();
     Cl.n = nc;
     Cl.memOff.assign(nc + 1, 0);
-    for (int i = 0; i < F.n; ++i) Cl.memOff[agg[i] + 1]++;
+    for (int i = 0; i < F.n; ++i)
+        if (agg[i] >= 0) Cl.memOff[agg[i] + 1]++;
     for (int I = 0; I < nc; ++I) Cl.memOff[I + 1] += Cl.memOff[I];
-    Cl.mem.resize(F.n);
+    Cl.mem.resize(Cl.memOff[nc]);
     {
         std::vector<int> fill(Cl.memOff.begin(), Cl.memOff.end() - 1);
-        for (int i = 0; i < F.n; ++i) Cl.mem[fill[agg[i]]++] = i;
+        for (int i = 0; i < F.n; ++i)
+            if (agg[i] >= 0) Cl.mem[fill[agg[i]]++] = i;
     }
     Cl.diag.assign(nc, 0.0);
     std::vector<std::vector<std::pair<int, double>>> rows(nc);
@@ -824,12 +833,12 @@ void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHos
 
 // Explicit inverse of an SPD matrix (row-major n x n) by Cholesky:
 // A = L L^T, A^-1 = L^-T L^-1.  Returns false on a non-positive pivot.
-bool spdInverse(const AmgHostLevel& H, std::vector<double>& inv) {
-    const int n = H.n;
+bool spdInverse(const AmgHostLevel& H, int r0, int n, std::vector<double>& inv) {
     std::vector<double> A((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) {
-        A[(size_t)i * n + i] = H.diag[i];
-        forRow(H.sliceStart, H.sliceWidth, H.col, i, [&](int e, int j) { A[(size_t)i * n + j] -= H.a[e]; });
+        A[(size_t)i * n + i] = H.diag[r0 + i];
+        forRow(H.sliceStart, H.sliceWidth, H.col, r0 + i,
+               [&](int e, int j) { A[(size_t)i * n + (j - r0)] -= H.a[e]; });
     }
     for (int j = 0; j < n; ++j) {  // in-place Cholesky, lower triangle
         double d = A[(size_t)j * n + j];
@@ -868,31 +877,68 @@ void buildAmg(s4f_mesh* M) {
     if (A.gen == M->generation && !A.d.empty()) return;
     const auto t0 = std::chrono::steady_clock::now();
     const HostMesh& h = M->h;
-    if (h.instCell0.size() != 1) throw InvalidError("the multigrid preconditioner takes a single mesh");
+    const int nInst = (int)h.instCell0.size();
     A.h.clear();
     A.d.clear();
     AmgHostLevel L0;
     L0.n = h.C; L0.nSlices = h.nSlices;
     L0.sliceStart = h.sliceStart; L0.sliceWidth = h.sliceWidth; L0.col = h.col;
     L0.a = h.eA; L0.diag = h.diag;
+    L0.inst0 = h.instCell0; L0.instN = h.instNCell;
+    L0.rowInst.assign(h.C, -1);
+    for (int i = 0; i < nInst; ++i)
+        for (int r = h.instCell0[i]; r < h.instCell0[i] + h.instNCell[i]; ++r) L0.rowInst[r] = i;
     A.h.push_back(std::move(L0));
-    // coarsen until the level fits the explicit inverse; at least once above
-    // one slice of rows
-    while (A.h.back().n > kCoarseMax || (A.h.size() == 1 && A.h.back().n > kSlice)) {
-        std::vector<int> agg;
-        const int nc = aggregate(A.h.back(), agg);
-        if (nc >= A.h.back().n) break;
+    // coarsen every instance until its rows fit one dense block; an instance
+    // already there passes to the next level unchanged (identity aggregation)
+    // and its rows skip the smoothing on the way, so its cycle is the one it
+    // has alone.  At least two levels: level 0 is smoothed, the coarsest solved.
+    for (;;) {
+        AmgHostLevel& F = A.h.back();
+        const int big = *std::max_element(F.instN.begin(), F.instN.end());
+        if (A.h.size() > 1 && big <= kCoarseBlock) break;
+        if (A.h.size() > 24) throw InvalidError("multigrid hierarchy deeper than 24 levels");
+        std::vector<int> agg(F.n, -1);
         AmgHostLevel Cl;
-        galerkin(A.h.back(), agg, nc, Cl);
+        F.instPass.assign(nInst, 0);
+        int nc = 0;
+        for (int i = 0; i < nInst; ++i) {
+            const int r0 = F.inst0[i], n = F.instN[i];
+            Cl.inst0.push_back(nc);
+            int k;
+            if (n <= kCoarseBlock) {
+                for (int r = 0; r < n; ++r) agg[r0 + r] = nc + r;
+                k = n;
+                F.instPass[i] = 1;
+            } else {
+                k = aggregate(F, agg, r0, r0 + n, nc);
+                if (k >= n) throw InvalidError("multigrid coarsening stalled");
+            }
+            Cl.instN.push_back(k);
+            nc += k;
+        }
+        AmgHostLevel G;
+        galerkin(F, agg, nc, G);
+        G.inst0 = std::move(Cl.inst0);
+        G.instN = std::move(Cl.instN);
+        G.rowInst.assign(nc, -1);
+        for (int i = 0; i < nInst; ++i)
+            for (int r = G.inst0[i]; r < G.inst0[i] + G.instN[i]; ++r) G.rowInst[r] = i;
         A.h.back().agg = std::move(agg);
-        A.h.push_back(std::move(Cl));
-        if (A.h.size() > 24) break;
+        A.h.push_back(std::move(G));
     }
+    // coarsest: one dense inverse per instance block
     std::vector<double> inv;
+    std::vector<int> invOff(nInst, 0);
     if (A.h.size() > 1) {
-        if (A.h.back().n > kCoarseMax) throw InvalidError("multigrid coarsening stalled above the coarsest-level size");
-        if (!spdInverse(A.h.back(), inv))
-            throw InvalidError("multigrid coarsest matrix is not positive definite (no fixed boundary?)");
+        const AmgHostLevel& H = A.h.back();
+        for (int i = 0; i < nInst; ++i) {
+            invOff[i] = (int)inv.size();
+            std::vector<double> blk;
+            if (H.instN[i] > 0 && !spdInverse(H, H.inst0[i], H.instN[i], blk))
+                throw InvalidError("multigrid coarsest matrix is not positive definite (no fixed boundary?)");
+            inv.insert(inv.end(), blk.begin(), blk.end());
+        }
     }
     hipStream_t s = M->ctx->stream;
     for (size_t l = 0; l < A.h.size(); ++l) {
@@ -917,12 +963,17 @@ void buildAmg(s4f_mesh* M) {
             D->iD32.upload(iD32, s);
             D->memOff.upload(H.memOff, s);
             D->mem.upload(H.mem, s);
+            D->rowInst.upload(H.rowInst, s);
+            if (!H.instPass.empty()) D->instPass.upload(H.instPass, s);
         }
         if (l + 1 == A.h.size() && l > 0) {
             std::vector<float> inv32(inv.size());
             for (size_t k = 0; k < inv.size(); ++k) inv32[k] = (float)inv[k];
             D->inv64.upload(inv, s);
             D->inv32.upload(inv32, s);
+            D->blkRow0.upload(H.inst0, s);
+            D->blkN.upload(H.instN, s);
+            D->invOff.upload(invOff, s);
         }
         A.d.push_back(std::move(D));
     }
@@ -984,8 +1035,8 @@ void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>
         // coarsest: exact solve
         {
             AmgDevLevel& L = *A.d[nl - 1];
-            hipLaunchKernelGGL((k_mg_dense<T>), dim3((L.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s, L.n,
-                               MgVec<T>::inv(L), (const T*)MgVec<T>::b(L), MgVec<T>::x(L), (const Ctl*)ctl);
+            hipLaunchKernelGGL((k_mg_dense<T>), dim3((L.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s,
+                               L.view(), MgVec<T>::inv(L), (const T*)MgVec<T>::b(L), MgVec<T>::x(L), (const Ctl*)ctl);
         }
         // up: prolongate the coarser correction into the post-smoothing sweep
         for (int l = nl - 2; l >= 1; --l) {
@@ -1001,7 +1052,7 @@ void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>
     const int n1 = nl > 1 ? A.d[1]->n : 0;
     hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
                        (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
-    hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+    hipLaunchKernelGGL(k_fin_rho, dim3(M->h.instCell0.size()), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
     (void)C;
 }
@@ -1014,10 +1065,11 @@ void launchPcgChunkMg(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVec
     for (int i = 0; i < n; ++i) {
         hipLaunchKernelGGL((k_spmvp<T, true>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
                            v.p2, v.q, ctl, part, 1, (const T*)v.z);
-        hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        const dim3 nI(M->h.instCell0.size());
+        hipLaunchKernelGGL(k_fin_spmv, nI, dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
                            (const T*)v.q, (const Ctl*)ctl, part);
-        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+        hipLaunchKernelGGL(k_fin_update, nI, dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         launchVcycle<T>(M, dm, bm, v, ctl, part, s);
         std::swap(v.p, v.p2);
     }
@@ -1048,7 +1100,6 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     applyStabilisationScale(M, o.rhieChowScale, o.interfaceScale);
     const bool mg = o.preconditioner == S4F_PRECOND_GAMG;
     if (mg) {
-        if (nInst != 1) throw InvalidError("the multigrid preconditioner takes a single mesh");
         buildAmg(M);
         // over-correction of the piecewise-constant coarse correction (plain
         // aggregation under-represents smooth modes); any scale in (0, 2]
@@ -1798,7 +1849,6 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
         }
     };
     if (kernel >= S4F_KERNEL_MG0_DOWN) {
-        if (M->h.instCell0.size() != 1) throw InvalidError("multigrid kernels take a single mesh");
         buildAmg(M);
         for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
             AmgDevLevel& L = *M->amg.d[l];
@@ -1871,7 +1921,6 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
         if (!b || !opts) throw InvalidError("null argument");
         if (b->members.empty()) throw InvalidError("empty batch");
         checkOpts(*opts);
-        singleMeshOnly(*opts, "s4f_batch_solve");
         HIP_CHECK(hipSetDevice(b->ctx->device));
         if (b->dirty || !b->packed) {
             std::vector<const HostMesh*> hs;

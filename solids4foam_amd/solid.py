@@ -44,6 +44,7 @@ class PolyMesh:
         self.nBoundaryFaces = self.nFaces - self.nInternalFaces
         self._addr = None
         self._geom = None
+        self._patch_sizes = None  # known without the addressing for generated boxes
 
     @classmethod
     def box(cls, nx, ny, nz, L=(1.0, 1.0, 1.0), jitter=0.0, seed=0, tets=False):
@@ -51,7 +52,13 @@ class PolyMesh:
         Lv = f64(L)
         check(lib().s4f_polymesh_box(int(nx), int(ny), int(nz), ptr(Lv), float(jitter), int(seed), int(bool(tets)),
                                      ctypes.byref(h)))
-        return cls(h)
+        pm = cls(h)
+        # blockMesh patch order x-min, x-max, y-min, y-max, z-min, z-max; a
+        # tet split cuts every boundary quad into two triangles
+        k = 2 if tets else 1
+        pm._patch_sizes = [k * ny * nz, k * ny * nz, k * nx * nz, k * nx * nz, k * nx * ny, k * nx * ny]
+        assert sum(pm._patch_sizes) == pm.nBoundaryFaces
+        return pm
 
     @classmethod
     def from_arrays(cls, points, face_offsets, face_verts, owner, neighbour, n_cells, patch_start, patch_size):
@@ -82,7 +89,18 @@ class PolyMesh:
             self._geom = g
         return self._geom
 
+    def cell_centres(self):
+        """cell centres only (nCells, 3), without the face geometry"""
+        if self._geom is not None:
+            return self._geom["C"]
+        C = np.zeros((self.nCells, 3))
+        check(lib().s4f_polymesh_geometry(self.h, None, None, ptr(C), None))
+        return C
+
     def patch_faces(self, p):
+        if self._patch_sizes is not None:
+            s = sum(self._patch_sizes[:p])
+            return slice(s, s + self._patch_sizes[p])
         a = self.addressing()
         s = int(a["patchStart"][p]) - self.nInternalFaces
         return slice(s, s + int(a["patchSize"][p]))

@@ -153,31 +153,38 @@ def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precis
     assert out["pcgIterations"] < jac["pcgIterations"]
 
 
-def test_multigrid_rejected_outside_single_solve(ctx):
-    from solids4foam_amd import PRECOND_GAMG, _abi
+def test_multigrid_batch_equals_alone(ctx):
+    # a batch is one block-diagonal matrix: each instance's hierarchy, cycle
+    # and reductions are the ones it has alone (an instance already at its
+    # coarsest size passes through the remaining levels untouched)
+    from solids4foam_amd import PRECOND_GAMG
     from solids4foam_amd.batch import Batch
-    pm, m, _, _ = _product_mesh(ctx, (5, 5, 5), jitter=0.1, seed=3)
-    b = Batch(ctx)
-    b.add(m)
+    print("[gamg batch]", flush=True)
+    specs = [((9, 7, 6), dict(jitter=0.1, seed=1)), ((30, 6, 6), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
+             ((5, 5, 5), dict(jitter=0.1, seed=3)), ((20, 11, 13), dict(jitter=0.1, seed=4)),
+             ((24, 24, 12), dict(jitter=0.1, seed=5))]
+    meshes = [_product_mesh(ctx, n, **kw) for n, kw in specs]
+    for precision in (0, 1):
+        opts = dict(solutionTolerance=1e-10, alternativeTolerance=1e-10, nCorrectors=50000,
+                    preconditioner=PRECOND_GAMG, precision=precision)
+        alone = [m.solve(**opts) for _, m, _, _ in meshes]
+        b = Batch(ctx)
+        for _, m, _, _ in meshes:
+            b.add(m)
+        res = b.solve(**opts)
+        for a, r in zip(alone, res):
+            assert r["converged"] and a["converged"]
+            assert r["nOuter"] == a["nOuter"] and r["innerTotal"] == a["innerTotal"]
+            assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
+        b.close()
+
+
+def test_multigrid_rejected_by_server(ctx):
+    from solids4foam_amd import PRECOND_GAMG, _abi
+    from solids4foam_amd.serve import Server
     with pytest.raises(_abi.S4FError) as e:
-        b.solve(preconditioner=PRECOND_GAMG)
+        Server(ctx, 64 << 20, max_slots=4, preconditioner=PRECOND_GAMG)
     assert e.value.status == _abi.S4F_ERR_INVALID
-
-
-def test_first_iterations_follow_oracle(ctx, oracle_mod):
-    # same preconditioner (diagonal) on both sides: the first outer iterations
-    # reproduce the oracle's residual history
-    n = (8, 8, 8)
-    om, oc = _oracle_case(oracle_mod, n, jitter=0.15, seed=4)
-    pm, m, _, _ = _product_mesh(ctx, n, jitter=0.15, seed=4)
-    ro = oc.solve(history=10, nCorr=4, precond=oracle_mod.PRECOND_DIAGONAL)
-    out = m.solve(nCorrectors=4)
-    h = m.history()
-    assert out["nOuter"] == 4
-    assert np.allclose(h["initRes"][:4], ro["history"]["initRes"][:4], rtol=1e-8, atol=1e-14)
-    assert np.array_equal(h["inner"][:4], ro["history"]["inner"][:4])
-    assert np.allclose(h["rvf"][:4], ro["history"]["rvf"][:4], rtol=1e-8)
-    assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-9
 
 
 def _nu_box(ctx, oracle_mod, n, nu):
@@ -205,16 +212,20 @@ def test_field_relaxation_follows_oracle(ctx, oracle_mod):
     assert np.allclose(h["initRes"][:6], ro["history"]["initRes"][:6], rtol=1e-8, atol=1e-14)
     assert np.allclose(h["rvf"][:6], ro["history"]["rvf"][:6], rtol=1e-8)
     assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-9
+    # converge both sides tightly (a PCG tolerance of 1e-9 stops the outer loop
+    # where the normalised residual drops below it, ~1e-8 apart between paths)
     oc, m = _nu_box(ctx, oracle_mod, 6, 0.45)
-    kw = dict(solTol=1e-11, altTol=1e-11, nCorr=50000)
+    kw = dict(solTol=1e-14, altTol=1e-14, nCorr=100000, pcgTol=1e-15)
     ro = oc.solve(precond=oracle_mod.PRECOND_DIC, relaxD=0.7, **kw)
+    Do = oc.get("D").reshape(-1, 3)
     for precision in (0, 1):
-        out = m.solve(nCorrectors=50000, solutionTolerance=1e-11, alternativeTolerance=1e-11, relaxD=0.7,
-                      precision=precision)
+        out = m.solve(nCorrectors=100000, solutionTolerance=1e-14, alternativeTolerance=1e-14, relaxD=0.7,
+                      precision=precision, pcgTolerance=1e-15)
+        err = rel_l2(out["D"], Do)
         print(f"  relaxed nu=0.45 precision={precision}: nOuter={out['nOuter']} (oracle {ro['nOuter']}) "
-              f"err={rel_l2(out['D'], oc.get('D').reshape(-1, 3)):.2e}", flush=True)
+              f"err={err:.2e}", flush=True)
         assert ro["converged"] and out["converged"]
-        assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < (1e-8 if precision == 0 else 1e-6)
+        assert err < (1e-8 if precision == 0 else 1e-6)
 
 
 def test_solve_is_bitwise_deterministic(ctx):
