@@ -104,6 +104,9 @@ s4f_status s4f_polymesh_addressing(const s4f_polymesh *pm, double *points, int32
 /* OpenFOAM primitiveMesh geometry: face centres Cf, area vectors Sf (nFaces*3),
    cell centres C (nCells*3), volumes V (nCells). */
 s4f_status s4f_polymesh_geometry(const s4f_polymesh *pm, double *Cf, double *Sf, double *C, double *V);
+/* pointCells (primitiveMesh::pointCells): offsets (nPoints+1) and the cells of
+   each point in ascending order; cells = NULL fills only the offsets. */
+s4f_status s4f_polymesh_point_cells(const s4f_polymesh *pm, int32_t *offsets, int32_t *cells);
 
 /* ------------------------------------------------------------------ */
 /* solver mesh: addressing + geometry on the device                    */
@@ -197,6 +200,19 @@ s4f_status s4f_solve(s4f_mesh *mesh, const s4f_solve_opts *opts, double *D_inout
                      s4f_solve_report *report);
 /* Per-outer-iteration history of the last solve: initRes, rvf (doubles) and
    inner iterations (int32) for the first `cap` iterations. */
+/* Post-loop fields of the last s4f_solve on this mesh (device-resident D, gradD):
+   solidModel::writeFields (solidModel.C:1936-1962): epsilon = symm(gradD)
+   (C*6, XX XY XZ YY YZ ZZ), epsilonEq = sqrt(2/3 |dev epsilon|^2), sigmaEq =
+   sqrt(3/2 |dev sigma|^2); any output may be NULL. */
+s4f_status s4f_strain_fields(s4f_mesh *mesh, double *epsilon, double *epsilonEq, double *sigmaEq);
+/* pointD of the last s4f_solve (linGeomTotalDispSolid.C:212-214 ->
+   mechanicalModel::interpolate, mechanicalModel.C:829-845 ->
+   enhancedVolPointInterpolation with gradient, enhancedVolPointInterpolate.C:350-420):
+   inverse-distance weighted, gradient-extrapolated cell values over each
+   point's cells.  points (nPoints*3), pointCells CSR (s4f_polymesh_point_cells). */
+s4f_status s4f_point_displacement(s4f_mesh *mesh, int32_t nPoints, const double *points,
+                                  const int32_t *pointCellOffsets, const int32_t *pointCells, double *pointD);
+
 s4f_status s4f_solve_history(s4f_mesh *mesh, int32_t cap, double *initRes, double *rvf,
                              int32_t *inner, int32_t *count);
 

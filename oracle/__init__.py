@@ -48,6 +48,7 @@ def lib():
         L.or_case_create.restype = vp
         L.or_case_create.argtypes = [vp, dp, dp, ip, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.or_case_free.argtypes = [vp]
+        L.or_point_interpolate.argtypes = [vp, dp, dp, dp]
         L.or_field_size.restype = ctypes.c_long
         L.or_field_size.argtypes = [vp, ctypes.c_char_p]
         L.or_get_field.argtypes = [vp, ctypes.c_char_p, dp]
@@ -232,6 +233,14 @@ class Case:
     def time_iterations(self, n, **kw):
         o = self._opts(default_opts(**kw))
         return lib().or_time_iterations(self.h, ctypes.byref(o), int(n))
+
+    def point_interpolate(self, D, G):
+        """pointD from cell D (C, 3) and gradD (C, 9)"""
+        D = np.ascontiguousarray(D, np.float64).reshape(-1)
+        G = np.ascontiguousarray(G, np.float64).reshape(-1)
+        out = np.zeros(self.mesh.nPoints * 3)
+        lib().or_point_interpolate(self.h, _dp(D), _dp(G), _dp(out))
+        return out.reshape(-1, 3)
 
     def reference_source(self, q):
         out = np.zeros(self.mesh.nCells)

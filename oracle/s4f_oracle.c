@@ -965,6 +965,52 @@ void or_init(or_case *c, const double *D0)
     c->iCorr = 0;
 }
 
+/* pointD (linGeomTotalDispSolid.C:212-214 -> mechanicalModel::interpolate,
+   mechanicalModel.C:829-845 -> enhancedVolPointInterpolation::interpolate with
+   the gradient, enhancedVolPointInterpolate.C:350-420): for every point p,
+   sum over its cells c (ascending) of w (D_c + (x_p - C_c).G_c), w = 1/|x_p - C_c|,
+   divided by the sum of w.  D (C*3) and G (C*9) are given; out (P*3). */
+void or_point_interpolate(const or_case *c, const double *D, const double *G, double *out)
+{
+    const or_mesh *m = c->m;
+    const int P = m->nPoints;
+    int *cnt = calloc((size_t)P + 1, sizeof(int));
+    for (int f = 0; f < m->nFaces; ++f)
+        for (int k = m->fOff[f]; k < m->fOff[f+1]; ++k) cnt[m->fVerts[k] + 1] += (f < m->nInternalFaces) ? 2 : 1;
+    for (int p = 0; p < P; ++p) cnt[p+1] += cnt[p];
+    int *lst = malloc(sizeof(int)*(size_t)(cnt[P] > 0 ? cnt[P] : 1));
+    int *fill = malloc(sizeof(int)*(size_t)(P > 0 ? P : 1));
+    for (int p = 0; p < P; ++p) fill[p] = cnt[p];
+    for (int f = 0; f < m->nFaces; ++f)
+        for (int k = m->fOff[f]; k < m->fOff[f+1]; ++k) {
+            int p = m->fVerts[k];
+            lst[fill[p]++] = m->owner[f];
+            if (f < m->nInternalFaces) lst[fill[p]++] = m->neighbour[f];
+        }
+    for (int p = 0; p < P; ++p) {
+        int a = cnt[p], b = cnt[p+1];
+        /* insertion sort (a point has few cells), then skip repeats */
+        for (int i = a + 1; i < b; ++i) {
+            int v = lst[i], j = i - 1;
+            while (j >= a && lst[j] > v) { lst[j+1] = lst[j]; --j; }
+            lst[j+1] = v;
+        }
+        double acc[3] = {0, 0, 0}, sw = 0.0;
+        const double *x = m->points + 3*(size_t)p;
+        for (int i = a; i < b; ++i) {
+            if (i > a && lst[i] == lst[i-1]) continue;
+            int cc = lst[i];
+            double d[3] = {x[0] - m->C[3*cc], x[1] - m->C[3*cc+1], x[2] - m->C[3*cc+2]};
+            double w = 1.0/sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
+            for (int j = 0; j < 3; ++j)
+                acc[j] += w*(D[3*cc+j] + d[0]*G[9*cc+j] + d[1]*G[9*cc+3+j] + d[2]*G[9*cc+6+j]);
+            sw += w;
+        }
+        for (int j = 0; j < 3; ++j) out[3*(size_t)p+j] = sw > 1e-300 ? acc[j]/sw : acc[j];
+    }
+    free(cnt); free(lst); free(fill);
+}
+
 /* One outer iteration of linGeomTotalDispSolid::evolve (:138-209).
    Returns 1 if converged per solidModel::converged. */
 int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut, int *innerOut)

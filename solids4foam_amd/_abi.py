@@ -74,6 +74,7 @@ EXPORTED = [
     "s4f_kernel_time", "s4f_calibrate_copy",
     "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
+    "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
 ]
 
 

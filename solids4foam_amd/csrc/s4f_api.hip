@@ -145,6 +145,7 @@ struct SolveState {
     int histCap = 0;
     std::vector<double> hHist;
     int lastNOuter = 0;
+    int lastDpar = -1;  // buffer (0: D0, 1: D1) holding the last solve's field; -1: no solve yet
 };
 
 // Aggregation multigrid hierarchy of a mesh's matrix M (host build + device arrays)
@@ -1202,6 +1203,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         hDb = download(st.Db, 3 * (size_t)h.B, s);
     st.hHist = download(st.hist, st.hist.n, s);
     st.lastNOuter = hctl[0].nOuter;
+    st.lastDpar = nInst == 1 ? hctl[0].dpar : -1;
     for (int i = 0; i < nInst; ++i) {
         const Ctl& cf = hctl[i];
         const int c0i = h.instCell0[i], n = h.instNCell[i], b0 = h.instB0[i], nb = h.instNB[i];
@@ -1881,6 +1883,91 @@ s4f_status s4f_kernel_time(s4f_mesh* M, int32_t kernel, int32_t precision, int32
                                                   : replayT<double>(M, kernel, launches);
         return S4F_OK;
     });
+}
+
+s4f_status s4f_strain_fields(s4f_mesh* M, double* epsilon, double* epsilonEq, double* sigmaEq) {
+    return guarded([&]() -> s4f_status {
+        if (!M) throw InvalidError("null argument");
+        if (M->st.lastDpar < 0) throw InvalidError("no converged field on the device: call s4f_solve first");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        const int C = M->h.C;
+        DArr<double> e, ee, se;
+        if (epsilon) e.alloc(6 * (size_t)C);
+        if (epsilonEq) ee.alloc(C);
+        if (sigmaEq) se.alloc(C);
+        hipLaunchKernelGGL(k_strain_fields, dim3(std::max(1, std::min(4096, (C + kBlock - 1) / kBlock))), dim3(kBlock),
+                           0, s, M->dm(), (const double*)M->st.G.p, e.p, ee.p, se.p);
+        HIP_CHECK(hipGetLastError());
+        if (epsilon) {
+            std::vector<double> h = download(e, 6 * (size_t)C, s);
+            for (int c = 0; c < C; ++c)
+                for (int q = 0; q < 6; ++q) epsilon[6 * (size_t)c + q] = h[(size_t)q * C + c];
+        }
+        if (epsilonEq) {
+            std::vector<double> h = download(ee, C, s);
+            std::memcpy(epsilonEq, h.data(), C * sizeof(double));
+        }
+        if (sigmaEq) {
+            std::vector<double> h = download(se, C, s);
+            std::memcpy(sigmaEq, h.data(), C * sizeof(double));
+        }
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_point_displacement(s4f_mesh* M, int32_t nPoints, const double* points, const int32_t* pointCellOffsets,
+                                  const int32_t* pointCells, double* pointD) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !points || !pointCellOffsets || !pointCells || !pointD || nPoints < 0)
+            throw InvalidError("null argument");
+        if (M->st.lastDpar < 0) throw InvalidError("no converged field on the device: call s4f_solve first");
+        const int C = M->h.C;
+        const int nnz = pointCellOffsets[nPoints];
+        if (pointCellOffsets[0] != 0 || nnz < 0) throw InvalidError("bad point-cell offsets");
+        for (int p = 0; p < nPoints; ++p)
+            if (pointCellOffsets[p + 1] < pointCellOffsets[p]) throw InvalidError("bad point-cell offsets");
+        for (int k = 0; k < nnz; ++k)
+            if (pointCells[k] < 0 || pointCells[k] >= C) throw InvalidError("point-cell index out of range");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        DArr<double> pts, out;
+        DArr<int> off, cells;
+        pts.upload(std::vector<double>(points, points + 3 * (size_t)nPoints), s);
+        off.upload(std::vector<int>(pointCellOffsets, pointCellOffsets + nPoints + 1), s);
+        cells.upload(std::vector<int>(pointCells, pointCells + nnz), s);
+        out.alloc(3 * (size_t)std::max(nPoints, 1));
+        const double* D = M->st.lastDpar ? M->st.D1.p : M->st.D0.p;
+        hipLaunchKernelGGL(k_point_interpolate, dim3(std::max(1, std::min(4096, (nPoints + kBlock - 1) / kBlock))),
+                           dim3(kBlock), 0, s, M->dm(), D, (const double*)M->st.G.p, nPoints, (const double*)pts.p,
+                           (const int*)off.p, (const int*)cells.p, out.p);
+        HIP_CHECK(hipGetLastError());
+        std::vector<double> h = download(out, 3 * (size_t)nPoints, s);
+        std::memcpy(pointD, h.data(), 3 * (size_t)nPoints * sizeof(double));
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_polymesh_point_cells(const s4f_polymesh* p, int32_t* offsets, int32_t* cells) {
+    if (!p || !offsets) return fail(S4F_ERR_INVALID, "null argument");
+    const PolyMesh& m = *p->pm;
+    // cells of each point through the faces of each cell, ascending cell order
+    std::vector<std::vector<int>> pc(m.nPoints);
+    for (int f = 0; f < m.nFaces; ++f)
+        for (int k = m.fOff[f]; k < m.fOff[f + 1]; ++k) {
+            const int pt = m.fVerts[k];
+            pc[pt].push_back(m.owner[f]);
+            if (f < m.nInternalFaces) pc[pt].push_back(m.neighbour[f]);
+        }
+    offsets[0] = 0;
+    for (int pt = 0; pt < m.nPoints; ++pt) {
+        std::vector<int>& v = pc[pt];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        offsets[pt + 1] = offsets[pt] + (int)v.size();
+        if (cells) std::copy(v.begin(), v.end(), cells + offsets[pt]);
+    }
+    return S4F_OK;
 }
 
 struct s4f_batch {

@@ -728,6 +728,69 @@ __global__ __launch_bounds__(kBlock) void k_sigma(const DevMesh m, const double*
     }
 }
 
+// solidModel::writeFields (solidModel.C:1936-1962): epsilon = symm(gradD)
+// (xx xy xz yy yz zz, SoA), epsilonEq = sqrt(2/3 |dev epsilon|^2),
+// sigmaEq = sqrt(3/2 |dev sigma|^2) with sigma from Hooke's law
+__global__ __launch_bounds__(kBlock) void k_strain_fields(const DevMesh m, const double* __restrict__ G,
+                                                          double* __restrict__ eps, double* __restrict__ epsEq,
+                                                          double* __restrict__ sigEq) {
+    const int C = m.nCells;
+    for (int P = blockIdx.x * kBlock + threadIdx.x; P < C; P += gridDim.x * kBlock) {
+        double GP[9], s[6];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) GP[q] = G[(size_t)q * C + P];
+        const double e[6] = {GP[0], 0.5 * (GP[1] + GP[3]), 0.5 * (GP[2] + GP[6]), GP[4], 0.5 * (GP[5] + GP[7]), GP[8]};
+        if (eps) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) eps[(size_t)q * C + P] = e[q];
+        }
+        // |dev t|^2 of a symmetric tensor (xx xy xz yy yz zz)
+        auto magDev2 = [](const double* t) {
+            const double tr3 = (t[0] + t[3] + t[5]) / 3.0;
+            const double a = t[0] - tr3, b = t[3] - tr3, c = t[5] - tr3;
+            return a * a + b * b + c * c + 2.0 * (t[1] * t[1] + t[2] * t[2] + t[4] * t[4]);
+        };
+        if (epsEq) epsEq[P] = sqrt((2.0 / 3.0) * magDev2(e));
+        if (sigEq) {
+            hooke(GP, cellMu(m, P), cellK(m, P), s);
+            sigEq[P] = sqrt(1.5 * magDev2(s));
+        }
+    }
+}
+
+// mechanicalModel::interpolate -> enhancedVolPointInterpolation with the
+// gradient (mechanicalModel.C:829-845, enhancedVolPointInterpolate.C:350-420):
+// point value = sum_c w (D_c + (x_p - C_c).gradD_c) / sum_c w over the
+// point's cells (ascending cell order), w = 1/|x_p - C_c|
+__global__ __launch_bounds__(kBlock) void k_point_interpolate(const DevMesh m, const double* __restrict__ D,
+                                                              const double* __restrict__ G, int nPoints,
+                                                              const double* __restrict__ pts,
+                                                              const int* __restrict__ pcOff,
+                                                              const int* __restrict__ pcCell,
+                                                              double* __restrict__ pD) {
+    const int C = m.nCells;
+    for (int p = blockIdx.x * kBlock + threadIdx.x; p < nPoints; p += gridDim.x * kBlock) {
+        const double x[3] = {pts[3 * (size_t)p], pts[3 * (size_t)p + 1], pts[3 * (size_t)p + 2]};
+        double acc[3] = {0, 0, 0}, sw = 0.0;
+        for (int k = pcOff[p]; k < pcOff[p + 1]; ++k) {
+            const int c = pcCell[k];
+            double d[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) d[q] = x[q] - m.cC[(size_t)q * C + c];
+            const double w = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double dG = d[0] * G[(size_t)j * C + c] + d[1] * G[(size_t)(3 + j) * C + c] +
+                                  d[2] * G[(size_t)(6 + j) * C + c];
+                acc[j] += w * (D[(size_t)j * C + c] + dG);
+            }
+            sw += w;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) pD[3 * (size_t)p + j] = sw > 1e-300 ? acc[j] / sw : acc[j];
+    }
+}
+
 // von Mises equivalent stress sqrt(3/2 |dev sigma|^2) (solidModel.C:1957-1961)
 __global__ __launch_bounds__(kBlock) void k_sigma_eq(const double* __restrict__ sig, double* __restrict__ eq, int n) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {

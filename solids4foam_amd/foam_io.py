@@ -410,7 +410,8 @@ def write_vol_field(path, name, cls, values, boundary=None, dims="[0 1 0 0 0 0 0
 def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     """Reads a solids4foam case (polyMesh, 0/D, mechanicalProperties,
     solidProperties, fvSolution), runs linearGeometryTotalDisplacement on
-    the device and writes D, sigma and sigmaEq to `write_time`."""
+    the device and writes D, sigma, sigmaEq, epsilon, epsilonEq and pointD
+    to `write_time`."""
     from . import solid
     model, opts = read_solid_properties(case_dir)
     if model not in ("linearGeometryTotalDisplacement", "linearGeometryTotalDisplacementHIP"):
@@ -438,6 +439,23 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     out = os.path.join(case_dir, write_time)
     write_vol_field(os.path.join(out, "D"), "D", "volVectorField", m.D)
     write_vol_field(os.path.join(out, "sigma"), "sigma", "volSymmTensorField", m.sigma, dims="[1 -1 -2 0 0 0 0]")
-    write_vol_field(os.path.join(out, "sigmaEq"), "sigmaEq", "volScalarField", m.sigma_eq(),
+    f = m.write_fields()
+    write_vol_field(os.path.join(out, "sigmaEq"), "sigmaEq", "volScalarField", f["sigmaEq"],
                     dims="[1 -1 -2 0 0 0 0]")
+    write_vol_field(os.path.join(out, "epsilon"), "epsilon", "volSymmTensorField", f["epsilon"], dims="[0 0 0 0 0 0 0]")
+    write_vol_field(os.path.join(out, "epsilonEq"), "epsilonEq", "volScalarField", f["epsilonEq"],
+                    dims="[0 0 0 0 0 0 0]")
+    write_point_field(os.path.join(out, "pointD"), "pointD", f["pointD"])
     return m
+
+
+def write_point_field(path, name, values, dims="[0 1 0 0 0 0 0]"):
+    """ASCII pointVectorField with nonuniform internal values (calculated patches)"""
+    values = np.asarray(values, np.float64)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(_foam_header("pointVectorField", name))
+        f.write(f"dimensions      {dims};\n\n")
+        f.write(f"internalField   nonuniform List<vector>\n{len(values)}\n(\n")
+        f.writelines("(" + " ".join(f"{x:.10g}" for x in v) + ")\n" for v in values)
+        f.write(")\n;\n\nboundaryField\n{\n    \".*\"\n    {\n        type            calculated;\n    }\n}\n")

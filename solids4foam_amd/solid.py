@@ -89,6 +89,14 @@ class PolyMesh:
             self._geom = g
         return self._geom
 
+    def point_cells(self):
+        """pointCells as CSR (offsets (nPoints+1,), cells), ascending cell order"""
+        off = np.zeros(self.nPoints + 1, np.int32)
+        check(lib().s4f_polymesh_point_cells(self.h, ptr(off), None))
+        cells = np.zeros(int(off[-1]), np.int32)
+        check(lib().s4f_polymesh_point_cells(self.h, ptr(off), ptr(cells)))
+        return off, cells
+
     def cell_centres(self):
         """cell centres only (nCells, 3), without the face geometry"""
         if self._geom is not None:
@@ -166,6 +174,24 @@ class SolverMesh:
         out = dict(D=D, sigma=sig, gradD=G, Db=Db, nOuter=rep.nOuter, converged=bool(rep.converged),
                    innerTotal=int(rep.innerTotal), initialResidual=rep.initialResidual, residualvf=rep.residualvf,
                    seconds=rep.seconds, pcgIterations=int(rep.pcgIterations), status=st)
+        return out
+
+    def strain_fields(self):
+        """epsilon (C, 6), epsilonEq (C,), sigmaEq (C,) of the last solve (solidModel::writeFields)"""
+        eps = np.zeros((self.nCells, 6))
+        ee = np.zeros(self.nCells)
+        se = np.zeros(self.nCells)
+        check(lib().s4f_strain_fields(self.h, ptr(eps), ptr(ee), ptr(se)))
+        return dict(epsilon=eps, epsilonEq=ee, sigmaEq=se)
+
+    def point_displacement(self):
+        """pointD (nPoints, 3) of the last solve: inverse-distance weighted,
+        gradient-extrapolated cell values (mechanicalModel::interpolate)"""
+        pm = self.poly
+        pts = pm.addressing()["points"]
+        off, cells = pm.point_cells()
+        out = np.zeros((pm.nPoints, 3))
+        check(lib().s4f_point_displacement(self.h, pm.nPoints, ptr(f64(pts)), ptr(off), ptr(cells), ptr(out)))
         return out
 
     def history(self, cap=100000):
@@ -276,6 +302,14 @@ class LinGeomTotalDispSolid:
 
     def sigma_eq(self):
         return sigma_eq(self.sigma)
+
+    def write_fields(self):
+        """solidModel::writeFields (solidModel.C:1936-1962) plus pointD
+        (linGeomTotalDispSolid.C:212-214): epsilon, epsilonEq, sigmaEq and
+        the point displacement of the last evolve(), computed on the device"""
+        f = self.mesh.strain_fields()
+        f["pointD"] = self.mesh.point_displacement()
+        return f
 
 
 SOLID_MODELS = {LinGeomTotalDispSolid.typeName: LinGeomTotalDispSolid}

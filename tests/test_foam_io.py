@@ -136,6 +136,10 @@ def test_run_case_end_to_end(tmp_path):
     assert m.report["converged"]
     D = foam_io._field_value(foam_io.read_dict(os.path.join(str(tmp_path), "1", "D"))["internalField"], pm.nCells, 3)
     assert np.allclose(D, m.D, rtol=1e-9, atol=1e-20)
+    for name, k in (("pointD", 3), ("epsilonEq", 1), ("epsilon", 6), ("sigmaEq", 1)):
+        n = pm.nPoints if name == "pointD" else pm.nCells
+        v = foam_io._field_value(foam_io.read_dict(os.path.join(str(tmp_path), "1", name))["internalField"], n, k)
+        assert np.all(np.isfinite(v)) and np.abs(v).max() > 0
     # same as the box generator's solve with the same settings
     _, mesh = cases.cube_tension(ctx, n=6, jitter=0.1, seed=3)
     ref = mesh.solve(nCorrectors=20000, alternativeTolerance=1e-10, solutionTolerance=1e-10, rhieChowScale=0.2,

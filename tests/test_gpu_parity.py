@@ -228,6 +228,42 @@ def test_field_relaxation_follows_oracle(ctx, oracle_mod):
         assert err < (1e-8 if precision == 0 else 1e-6)
 
 
+def _eps_eq(e):
+    """sqrt(2/3 |dev e|^2) of symmetric tensors (n, 6: xx xy xz yy yz zz)"""
+    tr = (e[:, 0] + e[:, 3] + e[:, 5]) / 3.0
+    d2 = (e[:, 0] - tr) ** 2 + (e[:, 3] - tr) ** 2 + (e[:, 5] - tr) ** 2 + 2 * (e[:, 1] ** 2 + e[:, 2] ** 2 + e[:, 4] ** 2)
+    return np.sqrt(2.0 / 3.0 * d2)
+
+
+def test_post_loop_fields(ctx, oracle_mod):
+    # pointD (enhancedVolPointInterpolation with gradient) against the oracle
+    # on the device's own converged D and gradD; epsilon = symm(gradD),
+    # epsilonEq and sigmaEq of solidModel::writeFields
+    from solids4foam_amd import sigma_eq
+    for tets in (False, True):
+        n = (7, 6, 5) if not tets else (3, 3, 3)
+        pm, m, pt, bv = _product_mesh(ctx, n, jitter=0.15, seed=3, tets=tets, ls_full_delta=tets)
+        out = m.solve(want_grad=True, solutionTolerance=1e-10, alternativeTolerance=1e-10, nCorrectors=50000)
+        om, oc = _oracle_case(oracle_mod, n, jitter=0.15, seed=3, tets=tets, ls_full_delta=tets)
+        pD = m.point_displacement()
+        pO = oc.point_interpolate(out["D"], out["gradD"])
+        assert np.abs(pD - pO).max() <= 1e-12 * np.abs(pO).max()
+        f = m.strain_fields()
+        G = out["gradD"].reshape(-1, 3, 3)
+        S = 0.5 * (G + G.transpose(0, 2, 1))
+        eps = np.stack([S[:, 0, 0], S[:, 0, 1], S[:, 0, 2], S[:, 1, 1], S[:, 1, 2], S[:, 2, 2]], axis=1)
+        assert np.allclose(f["epsilon"], eps, rtol=1e-13, atol=1e-13 * np.abs(eps).max())
+        assert np.allclose(f["epsilonEq"], _eps_eq(eps), rtol=1e-12)
+        assert np.allclose(f["sigmaEq"], sigma_eq(out["sigma"]), rtol=1e-12)
+
+
+def test_equivalent_strain_closed_form():
+    # uniaxial strain a: epsilonEq = 2|a|/3; engineering shear g: g/sqrt(3)
+    a, g = 3e-4, 2e-4
+    e = np.array([[a, 0, 0, 0, 0, 0], [0, g / 2, 0, 0, 0, 0]])
+    assert np.allclose(_eps_eq(e), [2 * a / 3, g / np.sqrt(3)], rtol=1e-14)
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)

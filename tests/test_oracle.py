@@ -315,3 +315,40 @@ def test_field_relaxation_restores_convergence_high_nu(oracle_mod):
     assert rb["converged"]
     D = b.get("D").reshape(-1, 3)
     assert np.all(np.isfinite(D)) and np.abs(D).max() > 0
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_point_interpolation_linear_exact(oracle_mod, tets):
+    # pointD = sum w (D_c + (x_p - C_c).G_c) / sum w reproduces any linear
+    # field exactly at every point (interior and boundary), whatever the weights
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(3, 3))
+    a0 = rng.normal(size=3)
+    m = oracle_mod.Mesh.box(5, 4, 3, jitter=0.15, seed=9, tets=tets)
+    c = _case(oracle_mod, m, [FIXED] * 6, np.zeros((m.nBoundaryFaces, 3)))
+    D = a0 + m.C @ A
+    G = np.broadcast_to(A.reshape(1, 9), (m.nCells, 9))
+    pD = c.point_interpolate(D, G)
+    assert np.abs(pD - (a0 + m.points @ A)).max() < 1e-12
+
+
+def test_point_interpolation_weights(oracle_mod):
+    # inverse-distance weights: on a uniform orthogonal mesh an interior point
+    # is equidistant from its 8 cells, so pointD is their plain mean (G = 0);
+    # a boundary corner point has exactly one cell
+    m = oracle_mod.Mesh.box(4, 4, 4, (1.0, 1.0, 1.0))
+    c = _case(oracle_mod, m, [FIXED] * 6, np.zeros((m.nBoundaryFaces, 3)))
+    rng = np.random.default_rng(5)
+    D = rng.normal(size=(m.nCells, 3))
+    pD = c.point_interpolate(D, np.zeros((m.nCells, 9)))
+    # blockMesh numbering: point (i, j, k) = i + 5 j + 25 k, cell (i, j, k) = i + 4 j + 16 k
+    p = 2 + 5 * 2 + 25 * 2
+    cells = [i + 4 * j + 16 * k for i in (1, 2) for j in (1, 2) for k in (1, 2)]
+    assert np.allclose(pD[p], D[cells].mean(axis=0), rtol=1e-13, atol=1e-15)
+    assert np.allclose(pD[0], D[0], rtol=1e-14)
+    # jittered: weights are 1/|x_p - C_c|, checked on one interior point
+    mj = oracle_mod.Mesh.box(4, 4, 4, (1.0, 1.0, 1.0), jitter=0.2, seed=3)
+    cj = _case(oracle_mod, mj, [FIXED] * 6, np.zeros((mj.nBoundaryFaces, 3)))
+    pDj = cj.point_interpolate(D, np.zeros((mj.nCells, 9)))
+    w = 1.0 / np.linalg.norm(mj.points[p] - mj.C[cells], axis=1)
+    assert np.allclose(pDj[p], (w[:, None] * D[cells]).sum(0) / w.sum(), rtol=1e-12)
