@@ -47,7 +47,7 @@
 #define OR_VSMALL 1e-300 /* OpenFOAM VSMALL for double */
 #define OR_ROOTVSMALL 1e-150
 
-enum { OR_FIXED = 0, OR_TRACTION = 1 };
+enum { OR_FIXED = 0, OR_TRACTION = 1, OR_EMPTY = 2 };
 enum { OR_PRECOND_DIC = 0, OR_PRECOND_DIAGONAL = 1, OR_PRECOND_NONE = 2 };
 
 typedef struct {
@@ -84,6 +84,7 @@ typedef struct {
 typedef struct {
     const or_mesh *m;
     int nB;                 /* boundary faces */
+    int emptyDir;           /* 2-D: axis normal to the empty patches (not solved), -1 in 3-D */
     int lsFullDelta;        /* 0: OpenFOAM leastSquares (normal patch delta); 1: extendedLeastSquares full delta */
     int *patchType;         /* per patch */
     int *bType;             /* per boundary face */
@@ -537,6 +538,16 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         c->patchType[p] = patchType[p];
         for (int q = 0; q < m->patchSize[p]; ++q) c->bType[m->patchStart[p] - F + q] = patchType[p];
     }
+    /* 2-D (empty patches, [OF] emptyFvPatch): the empty direction is the
+       axis along the empty faces' normal; that component is not solved */
+    c->emptyDir = -1;
+    for (int b = 0; b < B; ++b) if (c->bType[b] == OR_EMPTY) {
+        const double *S = m->Sf + 3*(F+b);
+        int ax = 0;
+        for (int q = 1; q < 3; ++q) if (fabs(S[q]) > fabs(S[ax])) ax = q;
+        c->emptyDir = ax;
+        break;
+    }
     c->bValue = xcalloc((size_t)B*3, sizeof(double)); memcpy(c->bValue, bValue, sizeof(double)*3*B);
     c->bPressure = xcalloc((size_t)B, sizeof(double)); if (bPressure) memcpy(c->bPressure, bPressure, sizeof(double)*B);
 
@@ -590,6 +601,7 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         for (int q = 0; q < 6; ++q) { dd[6*o+q] += wdd[q]; dd[6*n+q] += wdd[q]; }
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;  /* empty faces carry no values */
         int P = bCell(c, b);
         double d[3]; patch_delta(c, b, lsFullDelta, d);
         double md2 = dot3(d, d);
@@ -597,7 +609,15 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         for (int q = 0; q < 6; ++q) dd[6*P+q] += wdd[q];
     }
     double *invDd = xcalloc((size_t)C*6, sizeof(double));
-    for (int i = 0; i < C; ++i) inv_symm(dd + 6*i, invDd + 6*i);
+    /* 2-D: dd has a zero row and column along the empty direction; invert
+       dd + e e (e that axis) and take e e off the inverse again, so the
+       gradient has no component along it */
+    static const int diagIdx[3] = {0, 3, 5};
+    for (int i = 0; i < C; ++i) {
+        if (c->emptyDir >= 0) dd[6*i + diagIdx[c->emptyDir]] += 1.0;
+        inv_symm(dd + 6*i, invDd + 6*i);
+        if (c->emptyDir >= 0) invDd[6*i + diagIdx[c->emptyDir]] -= 1.0;
+    }
     c->lsP = xcalloc((size_t)F*3, sizeof(double)); c->lsN = xcalloc((size_t)F*3, sizeof(double));
     c->lsB = xcalloc((size_t)B*3, sizeof(double));
     for (int f = 0; f < F; ++f) {
@@ -609,6 +629,7 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         symm_dot_v(invDd + 6*n, d, t); for (int q = 0; q < 3; ++q) c->lsN[3*f+q] = -r*t[q];
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         double d[3]; patch_delta(c, b, lsFullDelta, d);
         double r = 1.0/dot3(d, d), t[3];
@@ -688,11 +709,13 @@ void or_grad(or_case *c)
         }
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         double dv[3]; for (int q = 0; q < 3; ++q) dv[q] = c->Db[3*b+q] - c->D[3*P+q];
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->G[9*P + 3*i + j] += c->lsB[3*b+i]*dv[j];
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         const double *n = c->bN + 3*b, *GP = c->G + 9*P;
         double nG[3]; vdotT(n, GP, nG);
@@ -751,6 +774,7 @@ void or_update_traction(or_case *c)
 void or_correct_boundary(or_case *c)
 {
     for (int b = 0; b < c->nB; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
         if (c->bType[b] == OR_TRACTION) {
@@ -798,6 +822,7 @@ void or_residual(or_case *c)
         for (int q = 0; q < 3; ++q) { c->R[3*o+q] += fl[q]; c->R[3*n+q] -= fl[q]; }
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         const double *S = m->Sf + 3*(F+b), *s = c->sigb + 6*b;
         double fl[3] = { S[0]*s[0] + S[1]*s[1] + S[2]*s[2],
@@ -918,6 +943,7 @@ void or_reference_source(or_case *c, int q, double *out)
         out[o] -= fl; out[n] += fl;
     }
     for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
         double gm = c->impK[P]*m->magSf[F+b];
         if (c->bType[b] == OR_TRACTION) {
@@ -1028,6 +1054,7 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
     double *work = malloc(sizeof(double)*4*C);
     double ir[3], fr[3]; int nit[3];
     for (int q = 0; q < 3; ++q) {
+        if (q == c->emptyDir) { ir[q] = 0.0; fr[q] = 0.0; nit[q] = 0; continue; }  /* not a solved component */
         for (int i = 0; i < C; ++i) psi[i] = c->D[3*i+q];
         amul(c, psi, Ap);
         for (int i = 0; i < C; ++i) src[i] = Ap[i] + c->R[3*i+q];

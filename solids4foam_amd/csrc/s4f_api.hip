@@ -134,6 +134,7 @@ struct HostMesh {
     // instances packed in this structure (one for a plain mesh): cells start
     // on a 64-row slice boundary, padding rows carry no faces
     std::vector<int> instCell0, instNCell, instB0, instNB;
+    std::vector<int> instMask;  // per instance: solved components (bit j: component j; 2-D leaves one out)
 };
 
 struct SolveState {
@@ -315,7 +316,8 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
         int f = F;
         for (int p = 0; p < d.nPatches; ++p) {
             if (h.patchStart[p] != f || h.patchSize[p] < 0) throw InvalidError("patches must tile the boundary faces in order");
-            if (h.patchType[p] != S4F_PATCH_FIXED_DISPLACEMENT && h.patchType[p] != S4F_PATCH_TRACTION)
+            if (h.patchType[p] != S4F_PATCH_FIXED_DISPLACEMENT && h.patchType[p] != S4F_PATCH_TRACTION &&
+                h.patchType[p] != S4F_PATCH_EMPTY)
                 throw InvalidError("unsupported patch type");
             f += h.patchSize[p];
         }
@@ -349,6 +351,15 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
     h.bLs.resize(3 * (size_t)B); h.bDelta.resize(B); h.bMagSf.resize(B);
     for (int p = 0; p < d.nPatches; ++p)
         for (int q = 0; q < h.patchSize[p]; ++q) h.bType[h.patchStart[p] - F + q] = h.patchType[p];
+    // 2-D: the axis along the (first) empty face's normal is not solved
+    int emptyDir = -1;
+    for (int b = 0; b < B && emptyDir < 0; ++b)
+        if (h.bType[b] == S4F_PATCH_EMPTY) {
+            const double* S = Sf + 3 * (size_t)(F + b);
+            emptyDir = 0;
+            for (int q = 1; q < 3; ++q)
+                if (std::fabs(S[q]) > std::fabs(S[emptyDir])) emptyDir = q;
+        }
     // least-squares matrices
     std::vector<double> dd6((size_t)C * 6, 0.0);
     auto addDD = [&](int c, const double* v) {
@@ -384,25 +395,36 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
         h.bDelta[b] = 1.0 / std::sqrt(dot3(dn, dn));
         const double* ld = d.lsFullDelta ? dl : dn;
         for (int q = 0; q < 3; ++q) bdl[3 * (size_t)b + q] = ld[q];
+        if (h.bType[b] == S4F_PATCH_EMPTY) {  // empty faces carry no values
+            for (int q = 0; q < 3; ++q) h.bLs[(size_t)q * B + b] = 0.0;
+            continue;
+        }
         addDD(P, ld);
         const double r = 1.0 / dot3(ld, ld);
         for (int q = 0; q < 3; ++q) h.bLs[(size_t)q * B + b] = ld[q] * r;
     }
     h.minv.resize(6 * (size_t)C);
     h.cC.resize(3 * (size_t)C);
+    // 2-D: the LS matrix has a zero row and column along the empty axis:
+    // invert it with a unit on that diagonal entry, then take the unit off
+    // the inverse, so gradients have no component along the axis
+    const int emptyDiag = emptyDir < 0 ? -1 : (emptyDir == 0 ? 0 : (emptyDir == 1 ? 3 : 5));
     for (int c = 0; c < C; ++c) {
-        const double* s = &dd6[6 * (size_t)c];
+        double* s = &dd6[6 * (size_t)c];
+        if (emptyDiag >= 0) s[emptyDiag] += 1.0;
         const double xx = s[0], xy = s[1], xz = s[2], yy = s[3], yz = s[4], zz = s[5];
         const double det = xx * yy * zz + xy * yz * xz + xz * xy * yz - xx * yz * yz - xy * xy * zz - xz * yy * xz;
         if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) throw InvalidError("singular least-squares matrix in cell " + std::to_string(c));
-        const double inv[6] = {(yy * zz - yz * yz) / det, (xz * yz - xy * zz) / det, (xy * yz - xz * yy) / det,
-                               (xx * zz - xz * xz) / det, (xy * xz - xx * yz) / det, (xx * yy - xy * xy) / det};
+        double inv[6] = {(yy * zz - yz * yz) / det, (xz * yz - xy * zz) / det, (xy * yz - xz * yy) / det,
+                         (xx * zz - xz * xz) / det, (xy * xz - xx * yz) / det, (xx * yy - xy * xy) / det};
+        if (emptyDiag >= 0) inv[emptyDiag] -= 1.0;
         for (int q = 0; q < 6; ++q) h.minv[(size_t)q * C + c] = inv[q];
         for (int q = 0; q < 3; ++q) h.cC[(size_t)q * C + c] = Cc[3 * (size_t)c + q];
     }
     // sliced ELL: row lists in face order, internal first then boundary
     std::vector<int> cnt(C, 0);
-    for (int f = 0; f < NF; ++f) cnt[h.owner[f]]++;
+    for (int f = 0; f < NF; ++f)
+        if (f < F || h.bType[f - F] != S4F_PATCH_EMPTY) cnt[h.owner[f]]++;
     for (int f = 0; f < F; ++f) cnt[h.neighbour[f]]++;
     h.nSlices = (C + kSlice - 1) / kSlice;
     h.sliceStart.resize((size_t)h.nSlices + 1);
@@ -447,6 +469,10 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
     }
     for (int b = 0; b < B; ++b) {
         const int f = F + b, P = h.owner[f];
+        if (h.bType[b] == S4F_PATCH_EMPTY) {  // no row slot: no kernel touches an empty face
+            h.entB[b] = -1;
+            continue;
+        }
         const int e = slot(P);
         h.col[e] = -1 - b;
         h.entB[b] = e;
@@ -458,6 +484,7 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
     h.instNCell = {C};
     h.instB0 = {0};
     h.instNB = {B};
+    h.instMask = {emptyDir < 0 ? 7 : (7 & ~(1 << emptyDir))};
 }
 
 void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCell) {
@@ -636,6 +663,7 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
     c.nCorr = o.nCorrectors;
     c.histCap = histCap;
     c.relaxD = (o.relaxD > 0.0) ? o.relaxD : 1.0;
+    c.solveMask = 7;
     return c;
 }
 
@@ -1127,6 +1155,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     for (int i = 0; i < nInst; ++i) {
         c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
         c0[i].pcMode = mg ? 1 : 0;
+        c0[i].solveMask = M->h.instMask.empty() ? 7 : M->h.instMask[i];
     }
     HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
@@ -1250,6 +1279,7 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
         P.instNCell.push_back(m->C);
         P.instB0.push_back((int)B);
         P.instNB.push_back(m->B);
+        P.instMask.push_back(m->instMask.empty() ? 7 : m->instMask[0]);
         C += (long long)m->nSlices * kSlice;
         B += m->B;
         E += (long long)m->col.size();
@@ -2439,6 +2469,7 @@ void serverAdmit(s4f_server* S) {
         placeLoads(S, rq, pl);
         Ctl c = makeCtl(S->opts, 0);
         c.dpar = warm ? pl.dpar : 0;
+        c.solveMask = rq.mesh->h.instMask.empty() ? 7 : rq.mesh->h.instMask[0];
         c.warm = warm;
         HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
         HIP_CHECK(hipMemcpy(S->ctl.p + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice));

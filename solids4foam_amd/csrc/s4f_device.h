@@ -75,6 +75,7 @@ struct Ctl {
     double pcgTol, pcgRelTol, solTol, altTol;
     int pcgMaxIter, nCorr, histCap, pcMode;  // pcMode 1: z comes from the V-cycle (k_fin_rho sets beta)
     double relaxD;                           // fixed field under-relaxation factor (1: none)
+    int solveMask, pad1;                     // solved components (bit j); 2-D leaves the empty axis out
 };
 
 }  // namespace s4f

@@ -923,8 +923,9 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, C
         int any = 0;
         for (int j = 0; j < 3; ++j) {
             const double nf = tot[3 + j] + tot[6 + j] + 1e-20;
+            const bool solved = (ctl->solveMask >> j) & 1;  // fvMatrix::solveSegregated valid components
             ctl->normFactor[j] = nf;
-            ctl->initRes[j] = tot[j] / nf;
+            ctl->initRes[j] = solved ? tot[j] / nf : 0.0;
             ctl->finalRes[j] = ctl->initRes[j];
             mag2 += ctl->initRes[j] * ctl->initRes[j];
             ctl->rho[j] = tot[9 + j];
@@ -932,7 +933,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, C
             ctl->alpha[j] = 0.0;
             ctl->beta[j] = 0.0;
             // iterate unless already converged (finalRes = initRes < tolerance)
-            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
+            const int act = solved && (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
             ctl->active[j] = act;
             any |= act;
         }
@@ -952,11 +953,12 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_pcg_start(const BlockMap bm, 
     if (threadIdx.x == 0) {
         int any = 0;
         for (int j = 0; j < 3; ++j) {
-            ctl->initRes[j] = tot[j] / ctl->normFactor[j];
+            const bool solved = (ctl->solveMask >> j) & 1;
+            ctl->initRes[j] = solved ? tot[j] / ctl->normFactor[j] : 0.0;
             ctl->finalRes[j] = ctl->initRes[j];
             ctl->rho[j] = tot[3 + j];
             ctl->nIter[j] = 0;
-            const int act = (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
+            const int act = solved && (ctl->pcgMaxIter > 0) && !(ctl->initRes[j] < ctl->pcgTol);
             ctl->active[j] = act;
             any |= act;
         }

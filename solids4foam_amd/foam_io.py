@@ -304,11 +304,14 @@ def _field_value(v, n, width):
     raise ValueError(f"unsupported field value {v!r}")
 
 
-SUPPORTED_D_BCS = ("fixedDisplacement", "solidTraction")
+SUPPORTED_D_BCS = ("fixedDisplacement", "linearSpatialDisplacement", "solidTraction", "empty")
 
 
-def read_D_boundary(case_dir, boundary, time="0"):
-    """per patch: dict(type, value (n,3) | traction (n,3), pressure (n,))"""
+def read_D_boundary(case_dir, boundary, time="0", face_centres=None):
+    """per patch: dict(type, value (n,3) | traction (n,3), pressure (n,)).
+    linearSpatialDisplacement (linearSpatialDisplacementFvPatchVectorField.C:96-106)
+    is a fixedDisplacement with value a + (b & Cf); it needs the boundary face
+    centres (face_centres: (nBoundaryFaces, 3) in patch order)."""
     d = read_dict(os.path.join(case_dir, time, "D"))
     bf = d["boundaryField"]
     out = []
@@ -321,6 +324,16 @@ def read_D_boundary(case_dir, boundary, time="0"):
         if t == "fixedDisplacement":
             disp = e.get("displacement", e.get("value", ["uniform", [0, 0, 0]]))
             out.append(dict(type=t, value=_field_value(disp, n, 3)))
+        elif t == "linearSpatialDisplacement":
+            if face_centres is None:
+                raise ValueError(f"patch {p['name']}: linearSpatialDisplacement needs the face centres")
+            a = np.asarray(e["a"], np.float64).reshape(3)
+            b = np.asarray(e["b"], np.float64).reshape(3, 3)
+            s0 = p["startFace"] - boundary[0]["startFace"]
+            Cf = np.asarray(face_centres)[s0:s0 + n]
+            out.append(dict(type="fixedDisplacement", value=a + Cf @ b.T))
+        elif t == "empty":
+            out.append(dict(type="empty"))
         elif t == "solidTraction":
             tr = e.get("traction", ["uniform", [0, 0, 0]])
             pr = e.get("pressure", ["uniform", 0])
@@ -422,7 +435,8 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     bnd = mesh["boundary"]
     pm = solid.PolyMesh.from_arrays(mesh["points"], mesh["fOff"], mesh["fVerts"], mesh["owner"], mesh["neighbour"],
                                     mesh["nCells"], [p["startFace"] for p in bnd], [p["nFaces"] for p in bnd])
-    bcs = read_D_boundary(case_dir, bnd)
+    Cf = pm.geometry()["Cf"][pm.nInternalFaces:]
+    bcs = read_D_boundary(case_dir, bnd, face_centres=Cf)
     E, nu = read_linear_elastic(case_dir)
     patches = [dict(type=b["type"], value=b.get("value"), traction=b.get("traction"), pressure=b.get("pressure"))
                for b in bcs]

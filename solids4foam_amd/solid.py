@@ -269,23 +269,22 @@ class LinGeomTotalDispSolid:
     typeName = "linearGeometryTotalDisplacement"
 
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False):
-        # patches: list of dicts {type: "fixedDisplacement"|"solidTraction", value|traction, pressure}
+        # patches: list of dicts {type: "fixedDisplacement"|"solidTraction"|"empty", value|traction, pressure}
+        kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
+                 "empty": _abi.PATCH_EMPTY}
         types = []
         for p in patches:
             t = p["type"]
-            if t == "fixedDisplacement":
-                types.append(_abi.PATCH_FIXED_DISPLACEMENT)
-            elif t == "solidTraction":
-                types.append(_abi.PATCH_TRACTION)
-            else:
+            if t not in kinds:
                 raise ValueError(f"Unsupported boundary condition type {t!r} "
-                                 "(valid: fixedDisplacement, solidTraction)")
+                                 "(valid: fixedDisplacement, solidTraction, empty)")
+            types.append(kinds[t])
         self.mesh = SolverMesh(ctx, polymesh, types, ls_full_delta)
         self.mesh.set_material(E, nu)
         for i, p in enumerate(patches):
             if p["type"] == "fixedDisplacement":
                 self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))
-            else:
+            elif p["type"] == "solidTraction":
                 self.mesh.set_bc(i, p.get("traction", (0.0, 0.0, 0.0)), p.get("pressure"))
         self.opts = dict(coeffs or {})
         self.opts.update(solver or {})

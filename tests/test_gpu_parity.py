@@ -264,6 +264,37 @@ def test_equivalent_strain_closed_form():
     assert np.allclose(_eps_eq(e), [2 * a / 3, g / np.sqrt(3)], rtol=1e-14)
 
 
+def test_reference_patch_test_2d(ctx, oracle_mod):
+    # tutorials/solids/linearElasticity/patchTest (2-D: empty front/back,
+    # linearSpatialDisplacement outer patch): the README's exact uniform strain
+    # (exx 2e-6, eyy 6e-6, exy 4e-6) and the oracle's field, in both precisions
+    from solids4foam_amd import PATCH_EMPTY, PolyMesh, SolverMesh
+    from tests import _blockmesh as bm
+    a = bm.hex_blocks(bm.PATCH_TEST_POINTS, bm.PATCH_TEST_HEXES, bm.PATCH_TEST_PATCHES)
+    pm = PolyMesh.from_arrays(a["points"], a["fOff"], a["fVerts"], a["owner"], a["neighbour"], a["nCells"],
+                              a["patchStart"], a["patchSize"])
+    m = SolverMesh(ctx, pm, [FIXED, PATCH_EMPTY], ls_full_delta=True)
+    m.set_material(200e9, 0.3)
+    Cf = pm.geometry()["Cf"][pm.nInternalFaces:]
+    nb = int(a["patchSize"][0])
+    m.set_bc(0, bm.linear_spatial(Cf[:nb], bm.PATCH_TEST_A, bm.PATCH_TEST_B))
+    om = oracle_mod.Mesh.from_arrays(a["points"], a["fOff"], a["fVerts"], a["owner"], a["neighbour"], a["nCells"],
+                                     a["patchStart"], a["patchSize"])
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[:nb] = bm.linear_spatial(Cf[:nb], bm.PATCH_TEST_A, bm.PATCH_TEST_B)
+    oc = oracle_mod.Case(om, 200e9, 0.3, [FIXED, 2], bv, ls_full_delta=True)
+    oc.solve(nCorr=1000, solTol=1e-14, altTol=1e-14, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+    for precision in (0, 1):
+        out = m.solve(precision=precision, want_grad=True, nCorrectors=1000, solutionTolerance=1e-14,
+                      alternativeTolerance=1e-14, pcgTolerance=1e-15)
+        assert out["converged"]
+        G = out["gradD"].reshape(-1, 3, 3)
+        assert np.allclose(G[:, 0, 0], 2e-6, rtol=1e-8) and np.allclose(G[:, 1, 1], 6e-6, rtol=1e-8)
+        assert np.allclose(0.5 * (G[:, 0, 1] + G[:, 1, 0]), 4e-6, rtol=1e-8)
+        assert np.abs(G[:, 2, :]).max() < 1e-20 and np.abs(out["D"][:, 2]).max() == 0.0
+        assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-10
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)
