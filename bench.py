@@ -466,7 +466,9 @@ def run_serve(args, ctx):
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
     opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
                 pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax)
-    srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1, **opts)
+    from solids4foam_amd import _abi
+    srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1,
+                 preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL, **opts)
     got, ids = {}, {}
     start = time.perf_counter()
     k = 0
@@ -501,7 +503,7 @@ def run_serve(args, ctx):
         "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, boxes n in 8..{args.nmax}, "
                                f"{args.repeat:.0%} repeat an earlier mesh with a new load (warm start), seed "
                                f"{args.seed}", "slots": args.slots, "budget_GiB": args.budget_gib,
-                   "outer_tolerance": args.sweep_tol, "pcg": f"diagonal, relTol {args.reltol}",
+                   "outer_tolerance": args.sweep_tol, "pcg": f"{args.precond}, relTol {args.reltol}",
                    "relaxD": args.relax},
         "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 3),

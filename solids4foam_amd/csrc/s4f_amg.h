@@ -293,6 +293,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const T* __restrict__ r, int nFine,
                                                         T* __restrict__ bc, const Ctl* __restrict__ ctl) {
     for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
+        if (Lc.rowInst[I] < 0) continue;  // padding row
         const Ctl* c = ctl + Lc.rowInst[I];
         if (mgSkip(c)) continue;
         const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
@@ -342,6 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
         const int P = s * kSlice + lane;
         if (P >= n) continue;
         const int inst = L.rowInst[P];
+        if (inst < 0) continue;  // padding row
         const Ctl* cP = ctl + inst;
         if (mgSkip(cP)) continue;
         const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
@@ -407,6 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
         const int P = s * kSlice + lane;
         if (P >= n) continue;
         const int inst = L.rowInst[P];
+        if (inst < 0) continue;  // padding row
         const Ctl* cP = ctl + inst;
         if (mgSkip(cP)) continue;
         const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
@@ -461,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(const MgLevel L, const T* _
     const int lane = threadIdx.x & 63;
     if (row >= L.n) return;
     const int inst = L.rowInst[row];
+    if (inst < 0) return;  // padding row
     const Ctl* c = ctl + inst;
     if (mgSkip(c)) return;
     const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
@@ -499,6 +503,42 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_rho(const BlockMap bm, Ctl* _
             ctl->rho[j] = tot[j];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// packing of per-mesh hierarchies into one block-diagonal hierarchy (server)
+// ---------------------------------------------------------------------------
+
+// int segment: mode 0 dst = src + add; 1 dst = src >= 0 ? src + add : src;
+// 2 dst = add + k; 3 dst = add
+struct SegInt {
+    const int* src;
+    int* dst;
+    int n, add, mode, pad;
+};
+// 32-bit word copy segment
+struct SegWords {
+    const unsigned* src;
+    unsigned* dst;
+    long long n;
+};
+
+__global__ __launch_bounds__(kBlock) void k_seg_int(const SegInt* __restrict__ segs) {
+    const SegInt g = segs[blockIdx.y];
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < g.n; k += gridDim.x * kBlock) {
+        int v;
+        if (g.mode == 0) v = g.src[k] + g.add;
+        else if (g.mode == 1) { v = g.src[k]; if (v >= 0) v += g.add; }
+        else if (g.mode == 2) v = g.add + k;
+        else v = g.add;
+        g.dst[k] = v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_words(const SegWords* __restrict__ segs) {
+    const SegWords g = segs[blockIdx.y];
+    for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < g.n; k += (long long)gridDim.x * kBlock)
+        g.dst[k] = g.src[k];
 }
 
 }  // namespace s4f

@@ -669,8 +669,8 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
 
 void singleMeshOnly(const s4f_solve_opts& o, const char* entry) {
     if (o.preconditioner == S4F_PRECOND_GAMG)
-        throw InvalidError(std::string(entry) + ": the multigrid preconditioner is available in s4f_solve and "
-                           "s4f_batch_solve only");
+        throw InvalidError(std::string(entry) + ": the multigrid preconditioner is not available here "
+                           "(s4f_solve, s4f_batch_solve and the server have it)");
 }
 
 void checkOpts(const s4f_solve_opts& o) {
@@ -1020,7 +1020,7 @@ struct MgVec<double> {
     static double* r(AmgDevLevel& L) { L.r64.alloc(3 * (size_t)L.n); return L.r64.p; }
     static double* xs(AmgDevLevel& L) { L.xs64.alloc(3 * (size_t)L.n); return L.xs64.p; }
     static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
-    static void swapX(AmgDevLevel& L) { std::swap(L.x64.p, L.xs64.p); }
+    static void swapX(AmgDevLevel& L) { std::swap(L.x64.p, L.xs64.p); std::swap(L.x64.n, L.xs64.n); }
 };
 template <>
 struct MgVec<float> {
@@ -1029,20 +1029,35 @@ struct MgVec<float> {
     static float* r(AmgDevLevel& L) { L.r32.alloc(3 * (size_t)L.n); return L.r32.p; }
     static float* xs(AmgDevLevel& L) { L.xs32.alloc(3 * (size_t)L.n); return L.xs32.p; }
     static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
-    static void swapX(AmgDevLevel& L) { std::swap(L.x32.p, L.xs32.p); }
+    static void swapX(AmgDevLevel& L) { std::swap(L.x32.p, L.xs32.p); std::swap(L.x32.n, L.xs32.n); }
 };
 
 inline int gridRows(int n) { return std::max(1, std::min(8192, (n + kBlock - 1) / kBlock)); }
 inline int gridSlices(int nSlices) { return std::max(1, std::min(8192, (nSlices + kBlock / 64 - 1) / (kBlock / 64))); }
 
+// the levels a V-cycle runs on: a mesh's own hierarchy, or the server's
+// packed hierarchy of its in-flight requests
+struct MgRun {
+    std::vector<std::unique_ptr<AmgDevLevel>>* levels;
+    double omega, alpha;
+    int nFin;  // finalize workgroups (instances)
+};
+
+MgRun meshRun(s4f_mesh* M) {
+    return MgRun{&M->amg.d, M->amg.omega, M->amg.alpha, (int)M->h.instCell0.size()};
+}
+
 // z = V(r): one V(1,1) cycle on the 3-RHS residual, then k_fin_rho
 template <typename T>
-void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
+void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
                   hipStream_t s) {
-    Amg& A = M->amg;
+    std::vector<std::unique_ptr<AmgDevLevel>>& Ad = *R.levels;
+    struct {
+        std::vector<std::unique_ptr<AmgDevLevel>>& d;
+        double alpha;
+    } A{Ad, R.alpha};
     const int nl = (int)A.d.size();
-    const double w = A.omega;
-    const int C = M->h.C;
+    const double w = R.omega;
     AmgDevLevel& D0 = *A.d[0];
     T* x0 = MgVec<T>::x(D0);
     T* r0 = MgVec<T>::r(D0);
@@ -1081,25 +1096,24 @@ void launchVcycle(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>
     const int n1 = nl > 1 ? A.d[1]->n : 0;
     hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
                        (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
-    hipLaunchKernelGGL(k_fin_rho, dim3(M->h.instCell0.size()), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+    hipLaunchKernelGGL(k_fin_rho, dim3(R.nFin), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
-    (void)C;
 }
 
 // PCG iterations with the V-cycle: spmv(p = z + beta p), alpha, update,
 // convergence, z = V(r), beta
 template <typename T>
-void launchPcgChunkMg(s4f_mesh* M, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
+void launchPcgChunkMg(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v, Ctl* ctl, double* part,
                       int n, hipStream_t s) {
     for (int i = 0; i < n; ++i) {
         hipLaunchKernelGGL((k_spmvp<T, true>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)v.p,
                            v.p2, v.q, ctl, part, 1, (const T*)v.z);
-        const dim3 nI(M->h.instCell0.size());
+        const dim3 nI(R.nFin);
         hipLaunchKernelGGL(k_fin_spmv, nI, dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
         hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p2, v.r,
                            (const T*)v.q, (const Ctl*)ctl, part);
         hipLaunchKernelGGL(k_fin_update, nI, dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
-        launchVcycle<T>(M, dm, bm, v, ctl, part, s);
+        launchVcycle<T>(R, dm, bm, v, ctl, part, s);
         std::swap(v.p, v.p2);
     }
     HIP_CHECK(hipGetLastError());
@@ -1177,11 +1191,11 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
                            (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
         hipLaunchKernelGGL(k_fin_residual, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
-        if (mg) launchVcycle<T>(M, dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
+        if (mg) launchVcycle<T>(meshRun(M), dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
         bool allDone = false;
         int total = 0;
         for (;;) {
-            if (mg) launchPcgChunkMg<T>(M, dm, bm, v, st.ctl.p, st.part.p, chunk, s);
+            if (mg) launchPcgChunkMg<T>(meshRun(M), dm, bm, v, st.ctl.p, st.part.p, chunk, s);
             else launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s);
             total += chunk;
             chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
@@ -1874,7 +1888,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 break;
             }
             case S4F_KERNEL_VCYCLE:
-                launchVcycle<T>(M, dm, bm, v, rc.p, st.part.p, s);
+                launchVcycle<T>(meshRun(M), dm, bm, v, rc.p, st.part.p, s);
                 break;
             default:
                 throw InvalidError("unknown kernel id");
@@ -2178,9 +2192,16 @@ struct s4f_server {
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
     DArr<float> eA32, diag32, invDiag32;
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
-    DArr<double> r64, p64, pb64, q64, x64;
-    DArr<float> r32, p32, pb32, q32, x32;
+    DArr<double> r64, p64, pb64, q64, x64, z64;
+    DArr<float> r32, p32, pb32, q32, x32, z32;
     DArr<Ctl> ctl;
+    // GAMG: packed block-diagonal hierarchy of the in-flight requests (level 0
+    // is the arena: agg per arena cell; levels >= 1 packed per request, rebuilt
+    // on the device from each mesh's own hierarchy whenever the set changes)
+    bool mgOn = false;
+    std::vector<std::unique_ptr<AmgDevLevel>> mg;
+    DArr<SegInt> segI;
+    DArr<SegWords> segW;
     DArr<double> part;
     DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, finList;
     DArr<int> nbmInst, nbmS0, nbmS1, nfinList;  // init map of newly admitted instances
@@ -2238,10 +2259,19 @@ void serverAllocate(s4f_server* S) {
     S->bLs.alloc(3 * Bc); S->bVal.alloc(3 * Bc); S->bPres.alloc(Bc);
     S->D0.alloc(3 * Cc); S->D1.alloc(3 * Cc); S->Dold.alloc(3 * Cc); S->G.alloc(9 * Cc); S->sig.alloc(6 * Cc);
     S->Db.alloc(3 * Bc); S->g.alloc(3 * Bc); S->snb.alloc(3 * Bc); S->Gb.alloc(9 * Bc); S->sigb.alloc(6 * Bc);
+    S->mgOn = S->opts.preconditioner == S4F_PRECOND_GAMG;
+    if (S->mgOn) {
+        S->mg.clear();
+        S->mg.push_back(std::make_unique<AmgDevLevel>());
+        S->mg[0]->n = (int)Cc;
+        S->mg[0]->agg.alloc(Cc);
+    }
     if (S->opts.precision == S4F_PRECISION_MIXED) {
         S->r32.alloc(3 * Cc); S->p32.alloc(3 * Cc); S->pb32.alloc(3 * Cc); S->q32.alloc(3 * Cc); S->x32.alloc(3 * Cc);
+        if (S->mgOn) S->z32.alloc(3 * Cc);
     } else {
         S->r64.alloc(3 * Cc); S->p64.alloc(3 * Cc); S->pb64.alloc(3 * Cc); S->q64.alloc(3 * Cc); S->x64.alloc(3 * Cc);
+        if (S->mgOn) S->z64.alloc(3 * Cc);
     }
     S->ctl.alloc(S->maxSlots);
     std::vector<Ctl> zero(S->maxSlots);
@@ -2426,9 +2456,13 @@ void buildTiles(const s4f_server* S, const std::vector<int>& slots, std::vector<
 template <typename T>
 PcgVecs<T> serverVecs(s4f_server* S);
 template <>
-PcgVecs<double> serverVecs<double>(s4f_server* S) { return {S->r64.p, S->p64.p, S->q64.p, S->x64.p, S->pb64.p}; }
+PcgVecs<double> serverVecs<double>(s4f_server* S) {
+    return {S->r64.p, S->p64.p, S->q64.p, S->x64.p, S->pb64.p, S->z64.p};
+}
 template <>
-PcgVecs<float> serverVecs<float>(s4f_server* S) { return {S->r32.p, S->p32.p, S->q32.p, S->x32.p, S->pb32.p}; }
+PcgVecs<float> serverVecs<float>(s4f_server* S) {
+    return {S->r32.p, S->p32.p, S->q32.p, S->x32.p, S->pb32.p, S->z32.p};
+}
 
 BlockMap makeMap(const s4f_server* S, const int* inst, const int* s0, const int* s1, const int* fin, int nBlocks) {
     BlockMap b{};
@@ -2470,6 +2504,7 @@ void serverAdmit(s4f_server* S) {
         Ctl c = makeCtl(S->opts, 0);
         c.dpar = warm ? pl.dpar : 0;
         c.solveMask = rq.mesh->h.instMask.empty() ? 7 : rq.mesh->h.instMask[0];
+        c.pcMode = S->mgOn ? 1 : 0;
         c.warm = warm;
         HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
         HIP_CHECK(hipMemcpy(S->ctl.p + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice));
@@ -2505,10 +2540,175 @@ void serverAdmit(s4f_server* S) {
     S->nTiles = -1;  // active map must be rebuilt
 }
 
+template <typename D>
+void ensureCap(DArr<D>& a, size_t n) {
+    if (a.n < n || !a.p) a.alloc(std::max<size_t>(n, 1));
+}
+
+// Packs the in-flight requests' own hierarchies into one block-diagonal
+// hierarchy: per level, each request's rows start on a slice boundary (its
+// slices, entries and member lists copied with index offsets); a request
+// whose hierarchy is shallower passes through the deeper levels (identity
+// aggregation, marked in instPass) to its dense block on the coarsest level.
+// All copies are segmented device-to-device kernels.
+void serverPackMg(s4f_server* S, const std::vector<int>& slots) {
+    hipStream_t s = S->ctx->stream;
+    const bool f32 = S->opts.precision == S4F_PRECISION_MIXED;
+    const int nI = (int)slots.size();
+    std::vector<s4f_mesh*> ms(nI);
+    std::vector<int> c0(nI), dk(nI);
+    int D = 2;
+    for (int k = 0; k < nI; ++k) {
+        const Request& rq = S->live[S->slotReq[slots[k]]];
+        ms[k] = rq.mesh;
+        c0[k] = (int)S->places[rq.place].c0;
+        dk[k] = (int)ms[k]->amg.h.size();
+        if (dk[k] < 2 || (int)ms[k]->amg.d.size() != dk[k]) throw InvalidError("request without a multigrid hierarchy");
+        D = std::max(D, dk[k]);
+    }
+    while ((int)S->mg.size() > D) S->mg.pop_back();
+    while ((int)S->mg.size() < D) S->mg.push_back(std::make_unique<AmgDevLevel>());
+    auto rowsOf = [&](int k, int l) { return ms[k]->amg.h[std::min(l, dk[k] - 1)].n; };
+    std::vector<std::vector<int>> R(D, std::vector<int>(nI, 0)), Sl = R, E = R, Mo = R;
+    std::vector<int> Rt(D, 0), St(D, 0), Et(D, 0), Mt(D, 0);
+    for (int l = 1; l < D; ++l)
+        for (int k = 0; k < nI; ++k) {
+            const bool real = l < dk[k];
+            const int rows = rowsOf(k, l), slices = (rows + kSlice - 1) / kSlice;
+            const AmgHostLevel& H = ms[k]->amg.h[std::min(l, dk[k] - 1)];
+            const int ents = real ? H.sliceStart[H.nSlices] : 0;
+            const int mem = real ? (int)H.mem.size() : rows;
+            R[l][k] = Rt[l]; Sl[l][k] = St[l]; E[l][k] = Et[l]; Mo[l][k] = Mt[l];
+            Rt[l] += slices * kSlice; St[l] += slices; Et[l] += ents; Mt[l] += mem;
+        }
+    std::vector<SegInt> si;
+    std::vector<SegWords> sw;
+    auto addI = [&](const int* src, int* dst, int n, int add, int mode) {
+        if (n > 0) si.push_back(SegInt{src, dst, n, add, mode, 0});
+    };
+    auto addW = [&](const void* src, void* dst, long long nWords) {
+        if (nWords > 0) sw.push_back(SegWords{(const unsigned*)src, (unsigned*)dst, nWords});
+    };
+    std::vector<int> blkRow0(S->maxSlots, 0), blkN(S->maxSlots, 0), invOff(S->maxSlots, 0);
+    long long invTotal = 0;
+    for (int k = 0; k < nI; ++k) {
+        const int nc = rowsOf(k, D - 1);
+        blkRow0[slots[k]] = R[D - 1][k];
+        blkN[slots[k]] = nc;
+        invOff[slots[k]] = (int)invTotal;
+        invTotal += (long long)nc * nc;
+    }
+    if (invTotal > INT32_MAX) throw InvalidError("coarsest blocks too large");
+    for (int l = 1; l < D; ++l) {
+        AmgDevLevel& L = *S->mg[l];
+        L.n = Rt[l];
+        L.nSlices = St[l];
+        ensureCap(L.sliceStart, (size_t)St[l] + 1);
+        ensureCap(L.sliceWidth, St[l]);
+        ensureCap(L.col, Et[l]);
+        if (f32) { ensureCap(L.a32, Et[l]); ensureCap(L.d32, Rt[l]); ensureCap(L.iD32, Rt[l]); }
+        else { ensureCap(L.a64, Et[l]); ensureCap(L.d64, Rt[l]); ensureCap(L.iD64, Rt[l]); }
+        ensureCap(L.memOff, (size_t)Rt[l] + 1);
+        ensureCap(L.mem, Mt[l]);
+        ensureCap(L.rowInst, Rt[l]);
+        if (l < D - 1) ensureCap(L.agg, Rt[l]);
+        std::vector<int> pass(S->maxSlots, 0);
+        for (int k = 0; k < nI; ++k) {
+            const bool real = l < dk[k];
+            const int rows = rowsOf(k, l), slices = (rows + kSlice - 1) / kSlice;
+            const int finerOff = l == 1 ? c0[k] : R[l - 1][k];
+            pass[slots[k]] = l >= dk[k] - 1 ? 1 : 0;
+            if (real) {
+                const AmgDevLevel& Dl = *ms[k]->amg.d[l];
+                const AmgHostLevel& H = ms[k]->amg.h[l];
+                const int ents = H.sliceStart[H.nSlices];
+                addI(Dl.sliceStart.p, L.sliceStart.p + Sl[l][k], slices, E[l][k], 0);
+                addI(Dl.sliceWidth.p, L.sliceWidth.p + Sl[l][k], slices, 0, 0);
+                addI(Dl.col.p, L.col.p + E[l][k], ents, R[l][k], 1);
+                if (f32) {
+                    addW(Dl.a32.p, L.a32.p + E[l][k], ents);
+                    addW(Dl.d32.p, L.d32.p + R[l][k], rows);
+                    addW(Dl.iD32.p, L.iD32.p + R[l][k], rows);
+                } else {
+                    addW(Dl.a64.p, L.a64.p + E[l][k], 2LL * ents);
+                    addW(Dl.d64.p, L.d64.p + R[l][k], 2LL * rows);
+                    addW(Dl.iD64.p, L.iD64.p + R[l][k], 2LL * rows);
+                }
+                addI(Dl.memOff.p, L.memOff.p + R[l][k], rows, Mo[l][k], 0);
+                addI(Dl.mem.p, L.mem.p + Mo[l][k], (int)H.mem.size(), finerOff, 0);
+                if (l < D - 1) {
+                    if (l < dk[k] - 1) addI(Dl.agg.p, L.agg.p + R[l][k], rows, R[l + 1][k], 0);
+                    else addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+                }
+            } else {  // pass-through rows: no matrix, identity members and aggregation
+                addI(nullptr, L.sliceStart.p + Sl[l][k], slices, E[l][k], 3);
+                addI(nullptr, L.sliceWidth.p + Sl[l][k], slices, 0, 3);
+                addI(nullptr, L.memOff.p + R[l][k], rows, Mo[l][k], 2);
+                addI(nullptr, L.mem.p + Mo[l][k], rows, finerOff, 2);
+                if (l < D - 1) addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+            }
+            const int memN = real ? (int)ms[k]->amg.h[l].mem.size() : rows;
+            addI(nullptr, L.rowInst.p + R[l][k], rows, slots[k], 3);
+            addI(nullptr, L.rowInst.p + R[l][k] + rows, slices * kSlice - rows, -1, 3);
+            addI(nullptr, L.memOff.p + R[l][k] + rows, slices * kSlice - rows, Mo[l][k] + memN, 3);
+        }
+        addI(nullptr, L.sliceStart.p + St[l], 1, Et[l], 3);
+        addI(nullptr, L.memOff.p + Rt[l], 1, Mt[l], 3);
+        L.instPass.upload(pass, s);
+        if (l == D - 1) {
+            L.blkRow0.upload(blkRow0, s);
+            L.blkN.upload(blkN, s);
+            L.invOff.upload(invOff, s);
+            if (f32) ensureCap(L.inv32, (size_t)invTotal);
+            else ensureCap(L.inv64, (size_t)invTotal);
+            for (int k = 0; k < nI; ++k) {
+                const AmgDevLevel& Dc = *ms[k]->amg.d[dk[k] - 1];
+                const long long nn = (long long)blkN[slots[k]] * blkN[slots[k]];
+                if (f32) addW(Dc.inv32.p, L.inv32.p + invOff[slots[k]], nn);
+                else addW(Dc.inv64.p, L.inv64.p + invOff[slots[k]], 2 * nn);
+            }
+        }
+    }
+    // level 0: arena cell -> packed level-1 row
+    AmgDevLevel& L0 = *S->mg[0];
+    for (int k = 0; k < nI; ++k)
+        addI(ms[k]->amg.d[0]->agg.p, L0.agg.p + c0[k], ms[k]->amg.h[0].n, R[1][k], 0);
+    if (!si.empty()) {
+        S->segI.upload(si, s);
+        int maxN = 0;
+        for (const SegInt& g : si) maxN = std::max(maxN, g.n);
+        for (size_t a = 0; a < si.size(); a += 65535) {
+            const int nseg = (int)std::min<size_t>(65535, si.size() - a);
+            hipLaunchKernelGGL(k_seg_int, dim3(std::max(1, std::min(1024, (maxN + kBlock - 1) / kBlock)), nseg),
+                               dim3(kBlock), 0, s, (const SegInt*)(S->segI.p + a));
+        }
+    }
+    if (!sw.empty()) {
+        S->segW.upload(sw, s);
+        long long maxN = 0;
+        for (const SegWords& g : sw) maxN = std::max(maxN, g.n);
+        for (size_t a = 0; a < sw.size(); a += 65535) {
+            const int nseg = (int)std::min<size_t>(65535, sw.size() - a);
+            hipLaunchKernelGGL(k_seg_words,
+                               dim3((unsigned)std::max<long long>(1, std::min<long long>(1024, (maxN + kBlock - 1) / kBlock)), nseg),
+                               dim3(kBlock), 0, s, (const SegWords*)(S->segW.p + a));
+        }
+    }
+    HIP_CHECK(hipGetLastError());
+    // the host tables above are read by the copies: finish them before they go
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+MgRun serverRun(s4f_server* S) {
+    return MgRun{&S->mg, 2.0 / 3.0, S->opts.mgCoarseScale > 0.0 ? S->opts.mgCoarseScale : kMgCoarseScaleDefault,
+                 S->nActive};
+}
+
 void serverRebuildMap(s4f_server* S) {
     std::vector<int> slots;
     for (int i = 0; i < S->maxSlots; ++i)
         if (S->slotReq[i] >= 0) slots.push_back(i);
+    if (S->mgOn && !slots.empty()) serverPackMg(S, slots);
     std::vector<int> inst, s0, s1;
     buildTiles(S, slots, inst, s0, s1, S->hiBlk0, S->hiNBlk);
     hipStream_t s = S->ctx->stream;
@@ -2535,9 +2735,11 @@ void serverStep(s4f_server* S) {
     hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)S->D0.p,
                        (const double*)S->D1.p, S->G.p, S->sigb.p, v.r, S->ctl.p, S->part.p);
     hipLaunchKernelGGL(k_fin_residual, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p);
+    if (S->mgOn) launchVcycle<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, s);  // z0 = V(r0), rho0
     int total = 0, chunk = S->chunk;
     for (;;) {
-        launchPcgChunk<T>(dm, bm, nA, v, S->ctl.p, S->part.p, chunk, s);
+        if (S->mgOn) launchPcgChunkMg<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, chunk, s);
+        else launchPcgChunk<T>(dm, bm, nA, v, S->ctl.p, S->part.p, chunk, s);
         total += chunk;
         chunk = std::max(2, chunk / 4);
         HIP_CHECK(hipMemcpyAsync(S->hctl, S->ctl.p, S->maxSlots * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -2662,7 +2864,6 @@ s4f_status s4f_server_create(s4f_ctx* ctx, int64_t budgetBytes, int32_t maxSlots
         if (budgetBytes < (1 << 20)) throw InvalidError("budget must be at least 1 MiB");
         if (maxSlots < 1 || maxSlots > 4096) throw InvalidError("maxSlots must be in 1..4096");
         checkOpts(*opts);
-        singleMeshOnly(*opts, "s4f_server_create");
         HIP_CHECK(hipSetDevice(ctx->device));
         auto S = std::make_unique<s4f_server>();
         S->ctx = ctx;
@@ -2690,6 +2891,10 @@ s4f_status s4f_server_submit(s4f_server* S, const s4f_request* rq, int64_t* id) 
         const HostMesh& h = m->h;
         if ((long long)h.nSlices * kSlice > S->Ccap || (long long)h.col.size() > S->Ecap || h.B > S->Bcap)
             return fail(S4F_ERR_ADMISSION, "request footprint exceeds the server's HBM budget");
+        if (S->mgOn) {  // the request's own hierarchy (cached on the mesh), packed by the server thread
+            HIP_CHECK(hipSetDevice(S->ctx->device));
+            buildAmg(m);
+        }
         Request r;
         r.mesh = m;
         r.warm = rq->warmStart;

@@ -179,14 +179,6 @@ def test_multigrid_batch_equals_alone(ctx):
         b.close()
 
 
-def test_multigrid_rejected_by_server(ctx):
-    from solids4foam_amd import PRECOND_GAMG, _abi
-    from solids4foam_amd.serve import Server
-    with pytest.raises(_abi.S4FError) as e:
-        Server(ctx, 64 << 20, max_slots=4, preconditioner=PRECOND_GAMG)
-    assert e.value.status == _abi.S4F_ERR_INVALID
-
-
 def _nu_box(ctx, oracle_mod, n, nu):
     from solids4foam_amd import PolyMesh, SolverMesh
     om = oracle_mod.Mesh.box(n, n, n, (0.1, 0.1, 0.1), 0.1, 3)

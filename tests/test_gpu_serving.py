@@ -65,6 +65,35 @@ def test_server_results_equal_alone(ctx, precision):
         assert r["latency"] > 0
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_server_multigrid_equals_alone(ctx, precision):
+    # the server packs each request's own multigrid hierarchy into one
+    # block-diagonal hierarchy (shallower ones pass through to their dense
+    # block): every result is bitwise the multigrid solve alone
+    from solids4foam_amd import PRECOND_GAMG
+    from solids4foam_amd.serve import Server
+    print(f"[serving] multigrid equal-alone precision={precision}", flush=True)
+    meshes = [_mesh(ctx, n, **kw) for n, kw in _specs()]
+    opts = dict(OPTS, preconditioner=PRECOND_GAMG)
+    alone = [m.solve(precision=precision, **opts) for m, _ in meshes]
+    srv = Server(ctx, 256 << 20, max_slots=4, precision=precision, **opts)
+    ids = []
+    for m, bv in meshes:  # 4 slots for 6 requests: the packed set changes as they retire
+        ids.append(srv.submit(m, bv))
+    got = {}
+    t0 = time.time()
+    while len(got) < len(ids) and time.time() - t0 < 120:
+        rid = srv.poll(100)
+        if rid is not None:
+            got[rid] = srv.result(rid)
+    srv.close()
+    assert len(got) == len(ids)
+    for rid, a in zip(ids, alone):
+        r = got[rid]
+        assert r["converged"] and r["nOuter"] == a["nOuter"] and r["innerTotal"] == a["innerTotal"]
+        assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
+
+
 def test_server_admission_by_footprint(ctx):
     from solids4foam_amd import S4FError, _abi
     from solids4foam_amd.serve import Server
