@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--slots", type=int, default=64, help="serve: in-flight solves")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serve: HBM arena")
     ap.add_argument("--sweep-tol", type=float, default=1e-9, help="batch/serve: outer tolerance")
-    ap.add_argument("--relax", type=float, default=0.7,
+    ap.add_argument("--relax", type=float, default=0.6,
                     help="batch/serve: fixed under-relaxation of D (the sweep reaches nu = 0.45, where the "
                          "unrelaxed segregated loop oscillates)")
     args = ap.parse_args()
@@ -486,10 +486,15 @@ def run_serve(args, ctx):
     srv.close()
     lat = np.array([got[i]["latency"] for i in range(len(reqs))])
     conv = sum(got[i]["converged"] for i in range(len(reqs)))
+    for i in range(len(reqs)):
+        if not got[i]["converged"]:
+            j, _, w = reqs[i]
+            print(f"[bench] request {i} (mesh {j} {specs[j]['n']} nu {specs[j]['nu']:.3f}, warm {w}) not converged: "
+                  f"status {got[i]['status']} nOuter {got[i]['nOuter']}", file=sys.stderr, flush=True)
     warm = sum(got[i]["warm_started"] for i in range(len(reqs)))
     first = min(got[i]["t_submit"] for i in range(len(reqs)))
     last = max(got[i]["t_finish"] for i in range(len(reqs)))
-    value = len(reqs) / (last - first)
+    value = conv / (last - first)  # converged solves per second of the stream
     print(f"[bench] served {len(reqs)} in {wall:.1f}s: {value:.2f} solves/s p50 {np.percentile(lat, 50):.2f}s "
           f"p99 {np.percentile(lat, 99):.2f}s converged {conv} warm {warm}", file=sys.stderr, flush=True)
     idx = [i for i in range(len(reqs)) if not reqs[i][2]][:args.sample]
