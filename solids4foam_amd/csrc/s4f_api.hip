@@ -198,6 +198,8 @@ struct s4f_mesh {
     // block map (single instance)
     DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell;
     int nBlocks = 0, slicesPerBlock = 0;
+    std::vector<int> tInst, tS0, tS1, tBlk0, tNBlk;  // host copy of the workgroup tiles
+    DArr<int> aInst, aS0, aS1, aBlk0, aNBlk, aFin;   // tiles of the instances still iterating (batches)
     bool materialSet = false;
     unsigned generation = 0;  // bumped when material data changes (serving cache key)
     double rcScale = 0.1, ifScale = 0.01;  // stabilisation scales baked into eC
@@ -626,6 +628,7 @@ void uploadGeometry(s4f_mesh* M) {
     }
     M->slicesPerBlock = spb;
     M->nBlocks = (int)inst.size();
+    M->tInst = inst; M->tS0 = s0; M->tS1 = s1; M->tBlk0 = blk0; M->tNBlk = nblk;
     M->bmInst.upload(inst, s);
     M->bmS0.upload(s0, s);
     M->bmS1.upload(s1, s);
@@ -1150,7 +1153,10 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         M->amg.alpha = o.mgCoarseScale > 0.0 ? o.mgCoarseScale : kMgCoarseScaleDefault;
     }
     const DevMesh dm = M->dm();
-    const BlockMap bm = M->bmap();
+    const BlockMap bmFull = M->bmap();
+    BlockMap bm = bmFull;  // narrowed to the iterating instances as a batch's instances finish
+    int nFin = nInst;
+    MgRun mgRun = meshRun(M);
     // initial and old-time fields
     std::vector<double> D0(3 * (size_t)C, 0.0);
     bool warm = false;
@@ -1185,18 +1191,41 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
                        st.hist.p, 1);
     HIP_CHECK(hipGetLastError());
+    // Batches: when instances finish, launch only the tiles of those still
+    // iterating.  Each instance keeps its own tiles (and the order of its
+    // partials), so its result does not change.
+    auto narrow = [&](const std::vector<int>& live) {
+        std::vector<int> inst, s0, s1, blk0(nInst, 0), nblk(nInst, 0);
+        for (int i : live) {
+            blk0[i] = (int)inst.size();
+            for (int b = M->tBlk0[i]; b < M->tBlk0[i] + M->tNBlk[i]; ++b) {
+                inst.push_back(M->tInst[b]);
+                s0.push_back(M->tS0[b]);
+                s1.push_back(M->tS1[b]);
+            }
+            nblk[i] = (int)inst.size() - blk0[i];
+        }
+        M->aInst.upload(inst, s); M->aS0.upload(s0, s); M->aS1.upload(s1, s);
+        M->aBlk0.upload(blk0, s); M->aNBlk.upload(nblk, s); M->aFin.upload(live, s);
+        bm = bmFull;
+        bm.inst = M->aInst.p; bm.s0 = M->aS0.p; bm.s1 = M->aS1.p; bm.iBlk0 = M->aBlk0.p; bm.iNBlk = M->aNBlk.p;
+        bm.finInst = M->aFin.p;
+        bm.nBlocks = (int)inst.size();
+        nFin = (int)live.size();
+        mgRun.nFin = nFin;
+    };
     int chunk = 4;
     long long pcgIters = 0;
     for (int it = 0;; ++it) {
         hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
                            (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
-        hipLaunchKernelGGL(k_fin_residual, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
-        if (mg) launchVcycle<T>(meshRun(M), dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
+        hipLaunchKernelGGL(k_fin_residual, dim3(nFin), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
+        if (mg) launchVcycle<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
         bool allDone = false;
         int total = 0;
         for (;;) {
-            if (mg) launchPcgChunkMg<T>(meshRun(M), dm, bm, v, st.ctl.p, st.part.p, chunk, s);
-            else launchPcgChunk<T>(dm, bm, nInst, v, st.ctl.p, st.part.p, chunk, s);
+            if (mg) launchPcgChunkMg<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, chunk, s);
+            else launchPcgChunk<T>(dm, bm, nFin, v, st.ctl.p, st.part.p, chunk, s);
             total += chunk;
             chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
             HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -1220,10 +1249,19 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
                            st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
                            0);
-        hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
+        hipLaunchKernelGGL(k_fin_grad, dim3(nFin), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
                            st.hist.p, 0);
         HIP_CHECK(hipGetLastError());
         if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
+        if (nInst > 1) {  // instances that finished at the previous outer iteration leave the launch
+            std::vector<int> live;
+            for (int i = 0; i < nInst; ++i)
+                if (!hctl[i].outerDone) live.push_back(i);
+            if ((int)live.size() < nFin && !live.empty()) {
+                HIP_CHECK(hipStreamSynchronize(s));  // queued kernels still read the current tile arrays
+                narrow(live);
+            }
+        }
     }
     HIP_CHECK(hipEventRecord(ctx->ev1, s));
     HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
