@@ -26,7 +26,7 @@
 
 namespace s4f {
 
-constexpr int kCoarseBlock = 256;  // rows per instance on the coarsest level (explicit inverse)
+constexpr int kCoarseBlock = 1024;  // rows per instance on the coarsest level (explicit inverse)
 constexpr double kMgCoarseScaleDefault = 1.8;  // coarse-correction scale when the caller gives 0
 
 // device view of one coarse level

@@ -884,23 +884,32 @@ bool spdInverse(const AmgHostLevel& H, int r0, int n, std::vector<double>& inv) 
             A[(size_t)i * n + j] = v / d;
         }
     }
-    std::vector<double> Li((size_t)n * n, 0.0);  // L^-1, lower triangular
-    for (int j = 0; j < n; ++j) {
-        Li[(size_t)j * n + j] = 1.0 / A[(size_t)j * n + j];
-        for (int i = j + 1; i < n; ++i) {
-            double v = 0.0;
-            for (int k = j; k < i; ++k) v -= A[(size_t)i * n + k] * Li[(size_t)k * n + j];
-            Li[(size_t)i * n + j] = v / A[(size_t)i * n + i];
+    // W = L^-1 row by row: W_i = (e_i - sum_{k<i} L_ik W_k) / L_ii (contiguous row updates)
+    std::vector<double> W((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double* wi = &W[(size_t)i * n];
+        wi[i] = 1.0;
+        for (int k = 0; k < i; ++k) {
+            const double l = A[(size_t)i * n + k];
+            const double* wk = &W[(size_t)k * n];
+            for (int j = 0; j <= k; ++j) wi[j] -= l * wk[j];
+        }
+        const double rd = 1.0 / A[(size_t)i * n + i];
+        for (int j = 0; j <= i; ++j) wi[j] *= rd;
+    }
+    // A^-1 = W^T W, accumulated by rows of W (lower triangle, then mirrored)
+    inv.assign((size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        const double* wk = &W[(size_t)k * n];
+        for (int i = 0; i <= k; ++i) {
+            const double a = wk[i];
+            if (a == 0.0) continue;
+            double* row = &inv[(size_t)i * n];
+            for (int j = 0; j <= i; ++j) row[j] += a * wk[j];
         }
     }
-    inv.assign((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double v = 0.0;
-            for (int k = i; k < n; ++k) v += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
-            inv[(size_t)i * n + j] = v;
-            inv[(size_t)j * n + i] = v;
-        }
+        for (int j = 0; j < i; ++j) inv[(size_t)j * n + i] = inv[(size_t)i * n + j];
     return true;
 }
 
