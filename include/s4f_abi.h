@@ -52,8 +52,9 @@ typedef struct s4f_server s4f_server;
 enum {
     S4F_PATCH_FIXED_DISPLACEMENT = 0, /* fixedDisplacementFvPatchVectorField.C:258-356 */
     S4F_PATCH_TRACTION = 1,           /* solidTractionFvPatchVectorField.C:321-463 */
-    S4F_PATCH_EMPTY = 2               /* [OF] emptyFvPatch: 2-D; its faces carry no values and the
+    S4F_PATCH_EMPTY = 2,              /* [OF] emptyFvPatch: 2-D; its faces carry no values and the
                                          component along their normal is not solved */
+    S4F_PATCH_SYMMETRY = 3            /* solidSymmetryFvPatchVectorField.C:147-250 (first order) */
 };
 
 /* precision of the inner linear solve */

@@ -47,7 +47,7 @@
 #define OR_VSMALL 1e-300 /* OpenFOAM VSMALL for double */
 #define OR_ROOTVSMALL 1e-150
 
-enum { OR_FIXED = 0, OR_TRACTION = 1, OR_EMPTY = 2 };
+enum { OR_FIXED = 0, OR_TRACTION = 1, OR_EMPTY = 2, OR_SYMMETRY = 3 };
 enum { OR_PRECOND_DIC = 0, OR_PRECOND_DIAGONAL = 1, OR_PRECOND_NONE = 2 };
 
 typedef struct {
@@ -660,7 +660,11 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         c->diag[m->owner[f]] += a;
         c->diag[m->neighbour[f]] += a;
     }
-    for (int b = 0; b < B; ++b) if (c->bType[b] == OR_FIXED) {
+    for (int b = 0; b < B; ++b) if (c->bType[b] == OR_FIXED || c->bType[b] == OR_SYMMETRY) {
+        /* symmetry: [OF] basicSymmetry's implicit part is per component
+           (n_j^2); one matrix serves the three components here, so the full
+           coefficient is taken (M only steers the defect correction; the
+           residual carries the exact condition) */
         int P = bCell(c, b);
         c->diag[P] += c->impK[P]*m->magSf[F+b]*c->bDelta[b];
     }
@@ -781,6 +785,17 @@ void or_correct_boundary(or_case *c)
             for (int q = 0; q < 3; ++q) {
                 c->Db[3*b+q] = c->D[3*P+q] + kG[q] + c->g[3*b+q]/c->bDelta[b];
                 c->snb[3*b+q] = c->g[3*b+q];
+            }
+        } else if (c->bType[b] == OR_SYMMETRY) {
+            /* solidSymmetry, first order (solidSymmetryFvPatchVectorField.C:147-250):
+               DP = D_P + k & G_P with k = (I - nn) & patch delta, and the patch
+               delta is the normal one ([OF.com] fvPatch::delta), so k = 0;
+               value (DP + (I - 2nn) DP)/2 = DP - n (n.DP), snGrad -n (n.DP) deltaCoeffs */
+            const double *n = c->bN + 3*b, *DP = c->D + 3*P;
+            double nD = dot3(n, DP);
+            for (int q = 0; q < 3; ++q) {
+                c->Db[3*b+q] = DP[q] - n[q]*nD;
+                c->snb[3*b+q] = -n[q]*nD*c->bDelta[b];
             }
         } else {
             for (int q = 0; q < 3; ++q) {
@@ -943,7 +958,7 @@ void or_reference_source(or_case *c, int q, double *out)
         out[o] -= fl; out[n] += fl;
     }
     for (int b = 0; b < B; ++b) {
-        if (c->bType[b] == OR_EMPTY) continue;
+        if (c->bType[b] == OR_EMPTY || c->bType[b] == OR_SYMMETRY) continue;  /* literal form: fixed and traction */
         int P = bCell(c, b);
         double gm = c->impK[P]*m->magSf[F+b];
         if (c->bType[b] == OR_TRACTION) {
@@ -1078,8 +1093,18 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
         for (int i = 0; i < 3*C; ++i) c->D[i] = c->Dprev[i] + a*(c->D[i] - c->Dprev[i]);
         for (int b = 0; b < c->nB; ++b) {
             int P = bCell(c, b);
+            if (c->bType[b] == OR_EMPTY) continue;
             if (c->bType[b] == OR_TRACTION) {
                 for (int q = 0; q < 3; ++q) c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+            } else if (c->bType[b] == OR_SYMMETRY) {
+                /* boundary values relaxed; solidSymmetry::snGrad is evaluated
+                   from the relaxed cell value */
+                const double *n = c->bN + 3*b, *DP = c->D + 3*P;
+                double nD = dot3(n, DP);
+                for (int q = 0; q < 3; ++q) {
+                    c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+                    c->snb[3*b+q] = -n[q]*nD*c->bDelta[b];
+                }
             } else {
                 double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
                 for (int q = 0; q < 3; ++q)

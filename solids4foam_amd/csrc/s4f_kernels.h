@@ -612,11 +612,26 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
                     const double k0 = m.bKv[b], k1 = m.bKv[(size_t)B + b], k2 = m.bKv[2 * (size_t)B + b];
                     const double dl = m.bDelta[b];
                     const bool trac = m.bType[b] == 1;
+                    const bool sym = m.bType[b] == 3;
+                    // solidSymmetry (first order, normal patch delta so no k correction):
+                    // value D_P - n (n.D_P) after the solve (relaxed like every boundary
+                    // value), snGrad -n (n.D_P) deltaCoeffs from the relaxed cell value
+                    const double nb0 = m.bN[b], nb1 = m.bN[(size_t)B + b], nb2 = m.bN[2 * (size_t)B + b];
+                    const double nDu = nb0 * DuP[0] + nb1 * DuP[1] + nb2 * DuP[2];
+                    const double nDn = nb0 * DnP[0] + nb1 * DnP[1] + nb2 * DnP[2];
+                    const double nbv[3] = {nb0, nb1, nb2};
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const double kG = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
                         double sn;
-                        if (trac) {
+                        if (sym) {
+                            Dbv[j] = DuP[j] - nbv[j] * nDu;
+                            if (relax) {
+                                const double Dbp = Db[(size_t)j * B + b];
+                                Dbv[j] = Dbp + ra * (Dbv[j] - Dbp);
+                            }
+                            sn = -nbv[j] * nDn * dl;
+                        } else if (trac) {
                             const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
                             Dbv[j] = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
                             if (relax) {                      // D.relax() on the boundary values too

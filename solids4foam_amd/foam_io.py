@@ -304,7 +304,7 @@ def _field_value(v, n, width):
     raise ValueError(f"unsupported field value {v!r}")
 
 
-SUPPORTED_D_BCS = ("fixedDisplacement", "linearSpatialDisplacement", "solidTraction", "empty")
+SUPPORTED_D_BCS = ("fixedDisplacement", "linearSpatialDisplacement", "solidTraction", "solidSymmetry", "empty")
 
 
 def read_D_boundary(case_dir, boundary, time="0", face_centres=None):
@@ -334,6 +334,10 @@ def read_D_boundary(case_dir, boundary, time="0", face_centres=None):
             out.append(dict(type="fixedDisplacement", value=a + Cf @ b.T))
         elif t == "empty":
             out.append(dict(type="empty"))
+        elif t == "solidSymmetry":
+            if str(e.get("secondOrder", "no")) in ("yes", "on", "true"):
+                raise ValueError(f"patch {p['name']}: solidSymmetry secondOrder is not supported on the device path")
+            out.append(dict(type="solidSymmetry"))
         elif t == "solidTraction":
             tr = e.get("traction", ["uniform", [0, 0, 0]])
             pr = e.get("pressure", ["uniform", 0])

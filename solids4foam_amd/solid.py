@@ -271,13 +271,13 @@ class LinGeomTotalDispSolid:
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False):
         # patches: list of dicts {type: "fixedDisplacement"|"solidTraction"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
-                 "empty": _abi.PATCH_EMPTY}
+                 "empty": _abi.PATCH_EMPTY, "solidSymmetry": _abi.PATCH_SYMMETRY}
         types = []
         for p in patches:
             t = p["type"]
             if t not in kinds:
                 raise ValueError(f"Unsupported boundary condition type {t!r} "
-                                 "(valid: fixedDisplacement, solidTraction, empty)")
+                                 "(valid: fixedDisplacement, solidTraction, solidSymmetry, empty)")
             types.append(kinds[t])
         self.mesh = SolverMesh(ctx, polymesh, types, ls_full_delta)
         self.mesh.set_material(E, nu)

@@ -111,7 +111,8 @@ def test_polymesh_and_bc_roundtrip(tmp_path):
 
 def test_unsupported_bc_is_rejected(tmp_path):
     pm, a = _box_case(str(tmp_path))
-    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement", "type solidSymmetry"))
+    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement",
+                                                           "type fixedDisplacementZeroShear"))
     m = foam_io.read_polymesh(str(tmp_path))
     with pytest.raises(ValueError, match="not supported"):
         foam_io.read_D_boundary(str(tmp_path), m["boundary"])

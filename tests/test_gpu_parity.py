@@ -287,6 +287,38 @@ def test_reference_patch_test_2d(ctx, oracle_mod):
         assert rel_l2(out["D"], oc.get("D").reshape(-1, 3)) < 1e-10
 
 
+def test_symmetry_planes(ctx, oracle_mod):
+    # solidSymmetry on the three min faces, normal traction on z-max: the
+    # orthogonal mesh reproduces the exact uniaxial-stress field; on a jittered
+    # mesh the device field matches the oracle's
+    from solids4foam_amd import PATCH_SYMMETRY, PolyMesh, SolverMesh
+    E, nu, t = 200e9, 0.3, 1e7
+    types = [PATCH_SYMMETRY, TRACTION, PATCH_SYMMETRY, TRACTION, PATCH_SYMMETRY, TRACTION]
+    for jitter in (0.0, 0.15):
+        pm = PolyMesh.box(6, 5, 4, L=(1.0, 1.0, 1.0), jitter=jitter, seed=3)
+        m = SolverMesh(ctx, pm, types)
+        m.set_material(E, nu)
+        bv = np.zeros((pm.nBoundaryFaces, 3))
+        bv[pm.patch_faces(5)] = (0.0, 0.0, t)
+        for p in (1, 3, 5):
+            m.set_bc(p, bv[pm.patch_faces(p)])
+        om = oracle_mod.Mesh.box(6, 5, 4, (1.0, 1.0, 1.0), jitter, 3)
+        oc = oracle_mod.Case(om, E, nu, [3, TRACTION, 3, TRACTION, 3, TRACTION], bv)
+        oc.solve(nCorr=50000, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+        Do = oc.get("D").reshape(-1, 3)
+        for precision in (0, 1):
+            out = m.solve(precision=precision, nCorrectors=50000, solutionTolerance=1e-13, alternativeTolerance=1e-13,
+                          pcgTolerance=1e-15)
+            assert out["converged"]
+            err = rel_l2(out["D"], Do)
+            print(f"  symmetry jitter={jitter} precision={precision}: nOuter={out['nOuter']} err={err:.2e}", flush=True)
+            assert err < (1e-8 if precision == 0 else 1e-6)
+            if jitter == 0.0:
+                X = pm.cell_centres()
+                exact = np.stack([-nu * t * X[:, 0] / E, -nu * t * X[:, 1] / E, t * X[:, 2] / E], axis=1)
+                assert np.abs(out["D"] - exact).max() < 1e-7 * np.abs(exact).max()
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)

@@ -389,3 +389,26 @@ def test_reference_patch_test_2d(oracle_mod):
     r2 = c2.solve(nCorr=1000, solTol=1e-14, altTol=1e-14, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
     G2 = c2.get("G").reshape(-1, 3, 3)
     assert np.abs(G2[:, 0, 0] - 2e-6).max() > 1e-9
+
+
+def test_symmetry_planes_uniaxial_exact(oracle_mod):
+    # solidSymmetry on x-min, y-min, z-min, normal traction t on z-max, free
+    # x-max / y-max: the exact solution is uniaxial stress, D = (-nu t x/E,
+    # -nu t y/E, t z/E), which an orthogonal mesh reproduces exactly
+    E, nu, t = 200e9, 0.3, 1e7
+    m = oracle_mod.Mesh.box(6, 5, 4, (1.0, 1.0, 1.0))
+    SYM = 3
+    bv = np.zeros((m.nBoundaryFaces, 3))
+    bv[m.patch_faces(5)] = (0.0, 0.0, t)
+    c = oracle_mod.Case(m, E, nu, [SYM, TRACTION, SYM, TRACTION, SYM, TRACTION], bv)
+    r = c.solve(nCorr=20000, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+    assert r["converged"]
+    D = c.get("D").reshape(-1, 3)
+    X = m.C
+    exact = np.stack([-nu * t * X[:, 0] / E, -nu * t * X[:, 1] / E, t * X[:, 2] / E], axis=1)
+    assert np.abs(D - exact).max() < 1e-8 * np.abs(exact).max()
+    s = c.get("sigma").reshape(-1, 6)
+    assert np.allclose(s[:, 5], t, rtol=1e-7) and np.abs(s[:, :5]).max() < 1e-6 * t
+    # the symmetry value is the tangential part of the cell value: n.D_b = 0
+    Db = c.get("Db").reshape(-1, 3)
+    assert np.abs(Db[m.patch_faces(0), 0]).max() < 1e-20 and np.abs(Db[m.patch_faces(4), 2]).max() < 1e-20
