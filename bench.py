@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--replay", type=int, default=40, help="back-to-back launches per kernel for the roofline")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch every kernel individually (no HIP graphs)")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no timed solves: one outer iteration for state, then only the kernel replays "
@@ -148,7 +149,7 @@ def main():
     print(f"[bench] mesh {args.n}^3: C={C} F={F} B={B} ELL entries={info['ellEntries']} "
           f"device={info['deviceBytes'] / 2**30:.2f} GiB setup={setup_s:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
-                nCorrectors=100000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale,
+                nCorrectors=100000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale, eagerLaunch=int(args.eager),
                 preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
     L = _abi.lib()
 
@@ -390,7 +391,8 @@ def run_batch(args, ctx):
     from solids4foam_amd.batch import Batch
     specs = cases.box_sweep_specs(args.count, args.seed, 8, args.nmax)
     opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax)
+                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax,
+                eagerLaunch=int(args.eager))
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
     b = Batch(ctx)
@@ -465,7 +467,8 @@ def run_serve(args, ctx):
             u += 1
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
     opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax)
+                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax,
+                eagerLaunch=int(args.eager))
     from solids4foam_amd import _abi
     srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1,
                  preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL, **opts)
@@ -531,7 +534,7 @@ def run_tets(args, ctx):
     print(f"[bench] tets {args.n}^3 x 6: C={C}, {nInc} inclusions, {frac:.3f} of cells soft, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
-                nCorrectors=200000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale,
+                nCorrectors=200000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale, eagerLaunch=int(args.eager),
                 preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
     t = time.perf_counter()
     mesh.solve(want_sigma=False, precision=1, **dict(opts, nCorrectors=1))

@@ -168,7 +168,8 @@ typedef struct {
     double pcgRelTol;              /* default 0.1 */
     int32_t pcgMaxIter;            /* default 1000 */
     int32_t preconditioner;        /* S4F_PRECOND_* */
-    int32_t reserved0;             /* zero */
+    int32_t eagerLaunch;           /* 0: the per-iteration launch sequences are captured in HIP graphs
+                                      and replayed; 1: every kernel is launched individually */
     int32_t precision;             /* S4F_PRECISION_* */
     int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
     double mgCoarseScale;          /* GAMG: scale of the coarse-grid correction, in (0, 2]; 0 = 1.8 */

@@ -44,7 +44,7 @@ class SolveOpts(ctypes.Structure):
                 ("alternativeTolerance", ctypes.c_double), ("rhieChowScale", ctypes.c_double),
                 ("interfaceScale", ctypes.c_double), ("pcgTolerance", ctypes.c_double),
                 ("pcgRelTol", ctypes.c_double), ("pcgMaxIter", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
+                ("eagerLaunch", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
                 ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double)]
 
 

@@ -88,6 +88,7 @@ using namespace s4f;
 struct s4f_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t capture = nullptr;  // private stream on which launch sequences are captured into graphs
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nCU = 0;
     int children = 0;
@@ -101,6 +102,7 @@ void ctxRelease(s4f_ctx* ctx) {
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->capture) (void)hipStreamDestroy(ctx->capture);
     delete ctx;
 }
 void ctxChildGone(s4f_ctx* ctx) {
@@ -686,6 +688,7 @@ void checkOpts(const s4f_solve_opts& o) {
         throw InvalidError("unknown preconditioner");
     if (!(o.mgCoarseScale >= 0.0 && o.mgCoarseScale <= 2.0)) throw InvalidError("mgCoarseScale must be in [0, 2]");
     if (!(o.relaxD >= 0.0 && o.relaxD <= 1.0)) throw InvalidError("relaxD must be in [0, 1]");
+    if (o.eagerLaunch != 0 && o.eagerLaunch != 1) throw InvalidError("eagerLaunch must be 0 or 1");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -1034,7 +1037,6 @@ struct MgVec<double> {
     static double* r(AmgDevLevel& L) { L.r64.alloc(3 * (size_t)L.n); return L.r64.p; }
     static double* xs(AmgDevLevel& L) { L.xs64.alloc(3 * (size_t)L.n); return L.xs64.p; }
     static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
-    static void swapX(AmgDevLevel& L) { std::swap(L.x64.p, L.xs64.p); std::swap(L.x64.n, L.xs64.n); }
 };
 template <>
 struct MgVec<float> {
@@ -1043,7 +1045,6 @@ struct MgVec<float> {
     static float* r(AmgDevLevel& L) { L.r32.alloc(3 * (size_t)L.n); return L.r32.p; }
     static float* xs(AmgDevLevel& L) { L.xs32.alloc(3 * (size_t)L.n); return L.xs32.p; }
     static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
-    static void swapX(AmgDevLevel& L) { std::swap(L.x32.p, L.xs32.p); std::swap(L.x32.n, L.xs32.n); }
 };
 
 inline int gridRows(int n) { return std::max(1, std::min(8192, (n + kBlock - 1) / kBlock)); }
@@ -1096,17 +1097,24 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
             hipLaunchKernelGGL((k_mg_dense<T>), dim3((L.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s,
                                L.view(), MgVec<T>::inv(L), (const T*)MgVec<T>::b(L), MgVec<T>::x(L), (const Ctl*)ctl);
         }
-        // up: prolongate the coarser correction into the post-smoothing sweep
+        // up: prolongate the coarser correction into the post-smoothing sweep;
+        // a level's result goes to xs (x keeps the pre-smoothed values), the
+        // coarsest level's solution is in its x.  No buffer changes role, so a
+        // captured launch sequence replays with the same pointers.
+        auto corr = [&](int l) -> const T* {
+            AmgDevLevel& Lc = *A.d[l];
+            return l == nl - 1 ? (const T*)MgVec<T>::x(Lc) : (const T*)MgVec<T>::xs(Lc);
+        };
         for (int l = nl - 2; l >= 1; --l) {
             AmgDevLevel& L = *A.d[l];
             AmgDevLevel& Lc = *A.d[l + 1];
             hipLaunchKernelGGL((k_mg_up<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
-                               (const T*)MgVec<T>::b(L), (const T*)MgVec<T>::x(L), Lc.n, (const T*)MgVec<T>::x(Lc),
+                               (const T*)MgVec<T>::b(L), (const T*)MgVec<T>::x(L), Lc.n, corr(l + 1),
                                MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
-            MgVec<T>::swapX(L);
         }
     }
-    const T* e1 = nl > 1 ? (const T*)MgVec<T>::x(*A.d[1]) : nullptr;
+    const T* e1 = nullptr;
+    if (nl > 1) e1 = nl - 1 == 1 ? (const T*)MgVec<T>::x(*A.d[1]) : (const T*)MgVec<T>::xs(*A.d[1]);
     const int n1 = nl > 1 ? A.d[1]->n : 0;
     hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
                        (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
@@ -1132,6 +1140,42 @@ void launchPcgChunkMg(const MgRun& R, const DevMesh& dm, const BlockMap& bm, Pcg
     }
     HIP_CHECK(hipGetLastError());
 }
+
+// An instantiated HIP graph of a captured launch sequence.
+struct GraphExec {
+    hipGraphExec_t e = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec&) = delete;
+    GraphExec& operator=(const GraphExec&) = delete;
+    ~GraphExec() { reset(); }
+    void reset() {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+    // records what f launches on stream `cap` (nothing runs) and instantiates it
+    template <typename F>
+    void capture(hipStream_t cap, F&& f) {
+        reset();
+        HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            f(cap);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamEndCapture(cap, &g));
+        const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) {
+            e = nullptr;
+            HIP_CHECK(err);
+        }
+    }
+    void launch(hipStream_t s) const { HIP_CHECK(hipGraphLaunch(e, s)); }
+};
 
 struct InstanceIO {
     double* D_inout = nullptr;  // AoS nCells*3
@@ -1225,19 +1269,47 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         nFin = (int)live.size();
         mgRun.nFin = nFin;
     };
+    // launch sequences: the explicit terms of an outer iteration (after the
+    // previous one's gradient), and PCG iterations.  After the first outer
+    // iteration (which allocates every buffer) they are captured into HIP
+    // graphs and replayed; PCG iterations go in pairs so the two search-
+    // direction buffers are where the captured launches expect them.
+    auto launchGradResidual = [&](hipStream_t q) {
+        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, q, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
+                           st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
+                           0);
+        hipLaunchKernelGGL(k_fin_grad, dim3(nFin), dim3(kFinBlock), 0, q, bm, st.ctl.p, (const double*)st.part.p,
+                           st.hist.p, 0);
+    };
+    auto launchResidual = [&](hipStream_t q) {
+        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, q, dm, bm, (const double*)st.D0.p,
+                           (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
+        hipLaunchKernelGGL(k_fin_residual, dim3(nFin), dim3(kFinBlock), 0, q, bm, st.ctl.p, (const double*)st.part.p);
+        if (mg) launchVcycle<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, q);  // z0 = V(r0), rho0
+    };
+    auto launchPcg = [&](int n, hipStream_t q) {
+        if (mg) launchPcgChunkMg<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, n, q);
+        else launchPcgChunk<T>(dm, bm, nFin, v, st.ctl.p, st.part.p, n, q);
+    };
+    const bool graphs = o.eagerLaunch == 0;
+    GraphExec gOuter, gPair;
+    bool captured = false;
     int chunk = 4;
     long long pcgIters = 0;
+    launchResidual(s);
+    HIP_CHECK(hipGetLastError());
     for (int it = 0;; ++it) {
-        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)st.D0.p,
-                           (const double*)st.D1.p, st.G.p, st.sigb.p, v.r, st.ctl.p, st.part.p);
-        hipLaunchKernelGGL(k_fin_residual, dim3(nFin), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p);
-        if (mg) launchVcycle<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, s);  // z0 = V(r0), rho0
         bool allDone = false;
         int total = 0;
         for (;;) {
-            if (mg) launchPcgChunkMg<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, chunk, s);
-            else launchPcgChunk<T>(dm, bm, nFin, v, st.ctl.p, st.part.p, chunk, s);
-            total += chunk;
+            if (captured) {
+                const int pairs = (chunk + 1) / 2;
+                for (int k = 0; k < pairs; ++k) gPair.launch(s);
+                total += 2 * pairs;
+            } else {
+                launchPcg(chunk, s);
+                total += chunk;
+            }
             chunk = std::max(2, chunk / 4);  // top-up chunks after an under-estimate
             HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -1257,13 +1329,6 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         pcgIters += std::max({hctl[0].nIter[0], hctl[0].nIter[1], hctl[0].nIter[2]});
         // next outer iteration: launch just under the last count, then top up
         chunk = std::max(1, used - 1);
-        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
-                           st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
-                           0);
-        hipLaunchKernelGGL(k_fin_grad, dim3(nFin), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
-                           st.hist.p, 0);
-        HIP_CHECK(hipGetLastError());
-        if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
         if (nInst > 1) {  // instances that finished at the previous outer iteration leave the launch
             std::vector<int> live;
             for (int i = 0; i < nInst; ++i)
@@ -1271,8 +1336,22 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             if ((int)live.size() < nFin && !live.empty()) {
                 HIP_CHECK(hipStreamSynchronize(s));  // queued kernels still read the current tile arrays
                 narrow(live);
+                captured = false;
             }
         }
+        if (graphs && !captured) {
+            gOuter.capture(ctx->capture, [&](hipStream_t q) { launchGradResidual(q); launchResidual(q); });
+            gPair.capture(ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
+            captured = true;
+        }
+        if (captured) {
+            gOuter.launch(s);
+        } else {
+            launchGradResidual(s);
+            launchResidual(s);
+        }
+        HIP_CHECK(hipGetLastError());
+        if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
     }
     HIP_CHECK(hipEventRecord(ctx->ev1, s));
     HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -1426,6 +1505,7 @@ s4f_status s4f_ctx_create(int device, s4f_ctx** out) {
         auto c = std::make_unique<s4f_ctx>();
         c->device = device;
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&c->ev0));
         HIP_CHECK(hipEventCreate(&c->ev1));
         hipDeviceProp_t prop;

@@ -327,6 +327,41 @@ def test_solve_is_bitwise_deterministic(ctx):
     assert a["nOuter"] == b["nOuter"]
 
 
+def test_graph_replay_equals_eager(ctx):
+    # the per-iteration launch sequences replayed from HIP graphs give the
+    # same bits as launching every kernel individually, for one mesh (each
+    # preconditioner and precision) and for a batch whose launch narrows
+    # (and is captured again) as its instances finish
+    from solids4foam_amd import PRECOND_GAMG
+    from solids4foam_amd.batch import Batch
+    print("[graphs]", flush=True)
+    _, m, _, _ = _product_mesh(ctx, (14, 10, 9), jitter=0.15, seed=8)
+    for pc in (0, PRECOND_GAMG):
+        for precision in (0, 1):
+            opts = dict(solutionTolerance=1e-9, alternativeTolerance=1e-9, preconditioner=pc, precision=precision)
+            e = m.solve(eagerLaunch=1, **opts)
+            g1 = m.solve(**opts)
+            g2 = m.solve(**opts)
+            for g in (g1, g2):
+                assert g["converged"] and g["nOuter"] == e["nOuter"] and g["innerTotal"] == e["innerTotal"]
+                assert np.array_equal(g["D"], e["D"]) and np.array_equal(g["sigma"], e["sigma"])
+    specs = [((9, 7, 6), dict(jitter=0.1, seed=1)), ((16, 4, 4), dict(L=(1.0, 0.25, 0.25), bc="cantilever")),
+             ((5, 5, 5), dict(jitter=0.1, seed=3)), ((20, 11, 3), dict(jitter=0.1, seed=4))]
+    meshes = [_product_mesh(ctx, n, **kw) for n, kw in specs]
+    b = Batch(ctx)
+    for _, bm, _, _ in meshes:
+        b.add(bm)
+    opts = dict(solutionTolerance=1e-10, alternativeTolerance=1e-10, nCorrectors=20000, preconditioner=PRECOND_GAMG,
+                precision=1)
+    e = b.solve(eagerLaunch=1, **opts)
+    g = b.solve(**opts)
+    assert len(set(r["nOuter"] for r in e)) > 1  # instances finish at different outer iterations
+    for a, r in zip(e, g):
+        assert r["converged"] and r["nOuter"] == a["nOuter"] and r["innerTotal"] == a["innerTotal"]
+        assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
+    b.close()
+
+
 def test_batch_equals_alone(ctx):
     from solids4foam_amd.batch import Batch
     print("[batch]", flush=True)
