@@ -37,10 +37,10 @@ def algorithmic_bytes(kernel, C, F, B, T):
         return C * 19 * T
     if kernel == "finalize":        # one workgroup: partial rows of 3 values
         return 0
-    if kernel == "mg0_down":        # b, 1/diag, diag; x, r writes; a + col per ELL entry
-        return C * 11 * T + F * 2 * (T + 4)
-    if kernel == "mg0_up":          # b, x, 1/diag, agg; z write; a + col per ELL entry (coarse e cache-resident)
-        return C * (10 * T + 4) + F * 2 * (T + 4)
+    if kernel == "mg0_down":        # b, 1/diag, diag; r write; a + col per ELL entry
+        return C * 8 * T + F * 2 * (T + 4)
+    if kernel == "mg0_up":          # b, 1/diag, agg; z write; a + col per ELL entry (coarse e cache-resident)
+        return C * (7 * T + 4) + F * 2 * (T + 4)
     if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
         return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
     if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w

@@ -66,6 +66,12 @@ template <> struct MgSel<float> {
 
 __device__ __forceinline__ bool mgSkip(const Ctl* c) { return c->outerDone || c->innerDone; }
 
+// a product rounded on its own (never fused into a following add), so a
+// value recomputed from its inputs has the bits of the one computed before
+template <typename T> __device__ __forceinline__ T mulRn(T a, T b);
+template <> __device__ __forceinline__ float mulRn<float>(float a, float b) { return __fmul_rn(a, b); }
+template <> __device__ __forceinline__ double mulRn<double>(double a, double b) { return __dmul_rn(a, b); }
+
 // ---------------------------------------------------------------------------
 // level 0 (the mesh's own matrix, BlockMap iteration)
 // ---------------------------------------------------------------------------
@@ -75,7 +81,7 @@ __device__ __forceinline__ bool mgSkip(const Ctl* c) { return c->outerDone || c-
 // row's own cell with a zero coefficient) so a wave's gathers overlap.
 template <typename T, int W>
 __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
-                                              T* __restrict__ x, T* __restrict__ r, T w) {
+                                              T* __restrict__ r, T w) {
     const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     int nb[W];
@@ -104,16 +110,12 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
         for (int j = 0; j < 3; ++j) y[j] -= av * (wi * b[(size_t)j * C + c]);
     }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        const size_t i = (size_t)j * C + P;
-        x[i] = xP[j];
-        r[i] = bP[j] - y[j];
-    }
+    for (int j = 0; j < 3; ++j) r[(size_t)j * C + P] = bP[j] - y[j];
 }
 
 template <typename T, int W, bool COARSE>
 __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
-                                            const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
+                                            const int* __restrict__ agg, int nCoarse,
                                             const T* __restrict__ e, T* __restrict__ z, T w, T w1, T al,
                                             double* acc) {
     const T* __restrict__ A = MatSel<T>::a(m);
@@ -126,11 +128,14 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int
         nb[k] = m.col[en];
         a[k] = A[en];
     }
-    auto xf = [&](int j, int c, int I) -> T {
-        const T v = x[(size_t)j * C + c];
+    // pre-smoothed value from zero, recomputed exactly as k_mg0_down forms it
+    auto xf = [&](int j, T wi, T bv, int I) -> T {
+        const T v = mulRn(wi, bv);
         return COARSE ? v + al * e[(size_t)j * nCoarse + I] : v;
     };
     const int IP = COARSE ? agg[P] : 0;
+    const T iD = invD[P];
+    const T wP = w * iD;
     T bP[3], sum[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -143,23 +148,23 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int
         const int c = v ? nb[k] : P;
         const T av = v ? a[k] : (T)0;
         const int I = COARSE ? agg[c] : 0;
+        const T wc = w * invD[c];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, c, I);
+        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, wc, b[(size_t)j * C + c], I);
     }
-    const T iD = invD[P];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T zv = w1 * xf(j, P, IP) + w * iD * sum[j];
+        const T zv = w1 * xf(j, wP, bP[j], IP) + w * iD * sum[j];
         z[(size_t)j * C + P] = zv;
         acc[j] += (double)zv * (double)bP[j];
     }
 }
 
 // pre-smoothing from zero and the residual for the restriction:
-// x = w D^-1 b, r = b - A x
+// x = w D^-1 b (not stored: k_mg0_up forms it again from b), r = b - A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
-                                                     T* __restrict__ x, T* __restrict__ r, double omega,
+                                                     T* __restrict__ r, double omega,
                                                      const Ctl* __restrict__ ctl) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
@@ -181,11 +186,11 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (all && width == 6) {
-            mg0DownRowAll<T, 6>(m, P, e0, lane, C, b, x, r, w);
+            mg0DownRowAll<T, 6>(m, P, e0, lane, C, b, r, w);
             continue;
         }
         if (all && width == 4) {
-            mg0DownRowAll<T, 4>(m, P, e0, lane, C, b, x, r, w);
+            mg0DownRowAll<T, 4>(m, P, e0, lane, C, b, r, w);
             continue;
         }
         const T iD = invD[P];
@@ -210,18 +215,18 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
         for (int j = 0; j < 3; ++j) {
             if (!act[j]) continue;
             const size_t i = (size_t)j * C + P;
-            x[i] = xP[j];
             r[i] = b[i] - y[j];
         }
     }
 }
 
-// prolongation + post-smoothing: xf = x + e[agg] (e == nullptr: no coarse
-// correction), z = (1 - w) xf + w D^-1 (b + sum a xf_N), with partial z.b
+// prolongation + post-smoothing: xf = x + e[agg] with x = w D^-1 b the
+// pre-smoothed value (e == nullptr: no coarse correction),
+// z = (1 - w) xf + w D^-1 (b + sum a xf_N), with partial z.b
 // (b is the PCG residual: z.b = rho of the preconditioned residual)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
-                                                   const T* __restrict__ x, const int* __restrict__ agg, int nCoarse,
+                                                   const int* __restrict__ agg, int nCoarse,
                                                    const T* __restrict__ e, T* __restrict__ z, double omega,
                                                    double alpha, const Ctl* __restrict__ ctl,
                                                    double* __restrict__ part) {
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     const bool all = act[0] && act[1] && act[2];
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
     auto xf = [&](int j, int cell) -> T {
-        const T v = x[(size_t)j * C + cell];
+        const T v = mulRn(w * invD[cell], b[(size_t)j * C + cell]);
         return e ? v + al * e[(size_t)j * nCoarse + agg[cell]] : v;
     };
     double acc[3] = {0, 0, 0};
@@ -249,13 +254,13 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (all && width == 6) {
-            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
+            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
             continue;
         }
         if (all && width == 4) {
-            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, x, agg, nCoarse, e, z, w, w1, al, acc);
+            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
             continue;
         }
         T sum[3];

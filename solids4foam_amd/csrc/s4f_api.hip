@@ -198,8 +198,8 @@ struct s4f_mesh {
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
     DArr<float> eA32, diag32, invDiag32;
     // block map (single instance)
-    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell;
-    int nBlocks = 0, slicesPerBlock = 0;
+    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, iB0, iNB;
+    int nBlocks = 0, slicesPerBlock = 0, maxInstNB = 0;
     std::vector<int> tInst, tS0, tS1, tBlk0, tNBlk;  // host copy of the workgroup tiles
     DArr<int> aInst, aS0, aS1, aBlk0, aNBlk, aFin;   // tiles of the instances still iterating (batches)
     bool materialSet = false;
@@ -265,6 +265,8 @@ BlockMap s4f_mesh::bmap() const {
     b.iNBlk = iNBlk.p;
     b.iCell0 = iCell0.p;
     b.iNCell = iNCell.p;
+    b.iB0 = iB0.p;
+    b.iNB = iNB.p;
     b.finInst = nullptr;
     b.nBlocks = nBlocks;
     return b;
@@ -640,6 +642,9 @@ void uploadGeometry(s4f_mesh* M) {
     M->iNBlk.upload(nblk, s);
     M->iCell0.upload(h.instCell0, s);
     M->iNCell.upload(h.instNCell, s);
+    M->iB0.upload(h.instB0, s);
+    M->iNB.upload(h.instNB, s);
+    M->maxInstNB = h.instNB.empty() ? 0 : *std::max_element(h.instNB.begin(), h.instNB.end());
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1074,9 +1079,8 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
     const int nl = (int)A.d.size();
     const double w = R.omega;
     AmgDevLevel& D0 = *A.d[0];
-    T* x0 = MgVec<T>::x(D0);
     T* r0 = MgVec<T>::r(D0);
-    hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, x0, r0, w,
+    hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, r0, w,
                        (const Ctl*)ctl);
     if (nl > 1) {
         // down: restrict the finer level's residual, smooth + residual
@@ -1116,7 +1120,7 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
     const T* e1 = nullptr;
     if (nl > 1) e1 = nl - 1 == 1 ? (const T*)MgVec<T>::x(*A.d[1]) : (const T*)MgVec<T>::xs(*A.d[1]);
     const int n1 = nl > 1 ? A.d[1]->n : 0;
-    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)x0,
+    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
                        (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
     hipLaunchKernelGGL(k_fin_rho, dim3(R.nFin), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
@@ -1139,6 +1143,28 @@ void launchPcgChunkMg(const MgRun& R, const DevMesh& dm, const BlockMap& bm, Pcg
         std::swap(v.p, v.p2);
     }
     HIP_CHECK(hipGetLastError());
+}
+
+// The fields the gradient step reads and writes.
+struct GradBufs {
+    double *D0, *D1, *G, *Db, *g, *snb, *Gb, *sigb;
+};
+
+// boundary values, cell gradients, boundary gradients and stress (modes as
+// k_grad); nInst instances (bm.finInst, or 0..nInst-1), at most maxNB
+// boundary faces each
+template <typename T>
+void launchGradStep(const DevMesh& dm, const BlockMap& bm, int nInst, int maxNB, const GradBufs& f, const T* x,
+                    const double* Dold, Ctl* ctl, double* part, int mode, hipStream_t s) {
+    const dim3 fg(std::max(1, std::min(1024, (maxNB + kBlock - 1) / kBlock)), std::max(1, nInst));
+    if (mode != 2)
+        hipLaunchKernelGGL((k_grad_bnd_pre<T>), fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.D0, x,
+                           (const double*)f.D1, (const double*)f.G, f.Db, (const double*)f.g, f.snb,
+                           (const Ctl*)ctl, mode);
+    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, f.D0, x, f.D1, f.G,
+                       (const double*)f.Db, Dold, ctl, part, mode);
+    hipLaunchKernelGGL(k_grad_bnd_post, fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.G, (const double*)f.snb,
+                       f.Gb, f.sigb, f.g, (const Ctl*)ctl, mode);
 }
 
 // An instantiated HIP graph of a captured launch sequence.
@@ -1240,9 +1266,8 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     const double* Dold = warm ? st.Dold.p : nullptr;
     HIP_CHECK(hipEventRecord(ctx->ev0, s));
     // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
-    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)nullptr,
-                       st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p,
-                       st.hist.p, 1);
+    const GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+    launchGradStep<T>(dm, bm, nInst, M->maxInstNB, gbuf, nullptr, Dold, st.ctl.p, st.part.p, 1, s);
     hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
                        st.hist.p, 1);
     HIP_CHECK(hipGetLastError());
@@ -1275,9 +1300,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     // graphs and replayed; PCG iterations go in pairs so the two search-
     // direction buffers are where the captured launches expect them.
     auto launchGradResidual = [&](hipStream_t q) {
-        hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, q, dm, bm, st.D0.p, (const T*)v.x, st.D1.p,
-                           st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, Dold, st.ctl.p, st.part.p, st.hist.p,
-                           0);
+        launchGradStep<T>(dm, bm, nFin, M->maxInstNB, gbuf, (const T*)v.x, Dold, st.ctl.p, st.part.p, 0, q);
         hipLaunchKernelGGL(k_fin_grad, dim3(nFin), dim3(kFinBlock), 0, q, bm, st.ctl.p, (const double*)st.part.p,
                            st.hist.p, 0);
     };
@@ -1774,9 +1797,9 @@ s4f_status s4f_grad(s4f_mesh* M, const double* D, const double* Db, const double
         HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
         const DevMesh dm = M->dm();
         const BlockMap bm = M->bmap();
-        hipLaunchKernelGGL((k_grad<double>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p,
-                           (const double*)nullptr, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p,
-                           st.sigb.p, (const double*)nullptr, st.ctl.p, st.part.p, (double*)nullptr, 2);
+        const GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+        launchGradStep<double>(dm, bm, (int)M->h.instCell0.size(), M->maxInstNB, gbuf, nullptr, nullptr, st.ctl.p,
+                               st.part.p, 2, s);
         HIP_CHECK(hipGetLastError());
         soaToAos(download(st.G, 9 * (size_t)C, s), C, 9, gradD);
         if (gradDb && B) soaToAos(download(st.Gb, 9 * (size_t)B, s), B, 9, gradDb);
@@ -1997,21 +2020,21 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kFinBlock), 0, s, bm, rc.p, (const double*)st.part.p);
                 break;
             case S4F_KERNEL_GRAD:
-                hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, st.D0.p, (const T*)v.x,
-                                   st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p, (const double*)nullptr,
-                                   rc.p, st.part.p, (double*)nullptr, 0);
+                launchGradStep<T>(dm, bm, 1, M->maxInstNB,
+                                  GradBufs{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p},
+                                  (const T*)v.x, nullptr, rc.p, st.part.p, 0, s);
                 break;
             case S4F_KERNEL_MG0_DOWN: {
                 AmgDevLevel& D0 = *M->amg.d[0];
                 hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   MgVec<T>::x(D0), MgVec<T>::r(D0), M->amg.omega, (const Ctl*)rc.p);
+                                   MgVec<T>::r(D0), M->amg.omega, (const Ctl*)rc.p);
                 break;
             }
             case S4F_KERNEL_MG0_UP: {
                 AmgDevLevel& D0 = *M->amg.d[0];
                 const bool two = M->amg.d.size() > 1;
                 hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   (const T*)MgVec<T>::x(D0), (const int*)D0.agg.p, two ? M->amg.d[1]->n : 0,
+                                   (const int*)D0.agg.p, two ? M->amg.d[1]->n : 0,
                                    two ? (const T*)MgVec<T>::x(*M->amg.d[1]) : (const T*)nullptr, v.z, M->amg.omega,
                                    M->amg.alpha, (const Ctl*)rc.p, st.part.p);
                 break;
@@ -2028,8 +2051,10 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
         for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
             AmgDevLevel& L = *M->amg.d[l];
             const size_t n3 = 3 * (size_t)L.n;
-            HIP_CHECK(hipMemsetAsync(MgVec<T>::x(L), 0, n3 * sizeof(T), s));
-            HIP_CHECK(hipMemsetAsync(MgVec<T>::xs(L), 0, n3 * sizeof(T), s));
+            if (l > 0) {  // the fine level keeps only the residual
+                HIP_CHECK(hipMemsetAsync(MgVec<T>::x(L), 0, n3 * sizeof(T), s));
+                HIP_CHECK(hipMemsetAsync(MgVec<T>::xs(L), 0, n3 * sizeof(T), s));
+            }
             HIP_CHECK(hipMemsetAsync(MgVec<T>::b(L), 0, n3 * sizeof(T), s));
             HIP_CHECK(hipMemsetAsync(MgVec<T>::r(L), 0, n3 * sizeof(T), s));
         }
@@ -2332,9 +2357,9 @@ struct s4f_server {
     DArr<SegInt> segI;
     DArr<SegWords> segW;
     DArr<double> part;
-    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, finList;
+    DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, iB0, iNB, finList;
     DArr<int> nbmInst, nbmS0, nbmS1, nfinList;  // init map of newly admitted instances
-    std::vector<int> hiBlk0, hiNBlk, hiCell0, hiNCell;
+    std::vector<int> hiBlk0, hiNBlk, hiCell0, hiNCell, hiB0, hiNB;
     int nTiles = 0, nActive = 0, chunk = 4;
     Ctl* hctl = nullptr;
     // bookkeeping (server thread only)
@@ -2410,7 +2435,9 @@ void serverAllocate(s4f_server* S) {
     HIP_CHECK(hipHostMalloc((void**)&S->hctl, S->maxSlots * sizeof(Ctl), hipHostMallocDefault));
     S->hiBlk0.assign(S->maxSlots, 0); S->hiNBlk.assign(S->maxSlots, 0);
     S->hiCell0.assign(S->maxSlots, 0); S->hiNCell.assign(S->maxSlots, 0);
+    S->hiB0.assign(S->maxSlots, 0); S->hiNB.assign(S->maxSlots, 0);
     S->iBlk0.alloc(S->maxSlots); S->iNBlk.alloc(S->maxSlots); S->iCell0.alloc(S->maxSlots); S->iNCell.alloc(S->maxSlots);
+    S->iB0.alloc(S->maxSlots); S->iNB.alloc(S->maxSlots);
     S->slotReq.assign(S->maxSlots, -1);
 }
 
@@ -2593,10 +2620,13 @@ PcgVecs<float> serverVecs<float>(s4f_server* S) {
     return {S->r32.p, S->p32.p, S->q32.p, S->x32.p, S->pb32.p, S->z32.p};
 }
 
+int serverMaxNB(const s4f_server* S) { return *std::max_element(S->hiNB.begin(), S->hiNB.end()); }
+
 BlockMap makeMap(const s4f_server* S, const int* inst, const int* s0, const int* s1, const int* fin, int nBlocks) {
     BlockMap b{};
     b.inst = inst; b.s0 = s0; b.s1 = s1;
     b.iBlk0 = S->iBlk0.p; b.iNBlk = S->iNBlk.p; b.iCell0 = S->iCell0.p; b.iNCell = S->iNCell.p;
+    b.iB0 = S->iB0.p; b.iNB = S->iNB.p;
     b.finInst = fin;
     b.nBlocks = nBlocks;
     return b;
@@ -2644,6 +2674,8 @@ void serverAdmit(s4f_server* S) {
         HIP_CHECK(hipGetLastError());
         S->hiCell0[slot] = (int)pl.c0;
         S->hiNCell[slot] = n;
+        S->hiB0[slot] = (int)pl.b0;
+        S->hiNB[slot] = (int)pl.nB;
         pl.hasField = false;
         S->live.push_back(rq);
         S->slotReq[slot] = (int)S->live.size() - 1;
@@ -2657,12 +2689,13 @@ void serverAdmit(s4f_server* S) {
     S->nbmInst.upload(inst, s); S->nbmS0.upload(s0, s); S->nbmS1.upload(s1, s); S->nfinList.upload(newSlots, s);
     S->iBlk0.upload(S->hiBlk0, s); S->iNBlk.upload(S->hiNBlk, s);
     S->iCell0.upload(S->hiCell0, s); S->iNCell.upload(S->hiNCell, s);
+    S->iB0.upload(S->hiB0, s); S->iNB.upload(S->hiNB, s);
     S->part.alloc(std::max<size_t>(S->part.n, inst.size() * kMaxPartials));
     const BlockMap bm = makeMap(S, S->nbmInst.p, S->nbmS0.p, S->nbmS1.p, S->nfinList.p, (int)inst.size());
     const DevMesh dm = serverDm(S);
-    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, S->D0.p, (const T*)nullptr, S->D1.p,
-                       S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p, (const double*)S->Dold.p, S->ctl.p,
-                       S->part.p, (double*)nullptr, 1);
+    launchGradStep<T>(dm, bm, (int)newSlots.size(), serverMaxNB(S),
+                      GradBufs{S->D0.p, S->D1.p, S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p}, nullptr,
+                      (const double*)S->Dold.p, S->ctl.p, S->part.p, 1, s);
     hipLaunchKernelGGL(k_fin_grad, dim3((int)newSlots.size()), dim3(kFinBlock), 0, s, bm, S->ctl.p,
                        (const double*)S->part.p, (double*)nullptr, 1);
     HIP_CHECK(hipGetLastError());
@@ -2882,9 +2915,9 @@ void serverStep(s4f_server* S) {
     for (int sl : slots)
         if (!S->hctl[sl].outerDone) used = std::max({used, S->hctl[sl].nIter[0], S->hctl[sl].nIter[1], S->hctl[sl].nIter[2]});
     S->chunk = std::max(1, used - 1);
-    hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, S->D0.p, (const T*)v.x, S->D1.p, S->G.p,
-                       S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p, (const double*)S->Dold.p, S->ctl.p, S->part.p,
-                       (double*)nullptr, 0);
+    launchGradStep<T>(dm, bm, nA, serverMaxNB(S),
+                      GradBufs{S->D0.p, S->D1.p, S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p},
+                      (const T*)v.x, (const double*)S->Dold.p, S->ctl.p, S->part.p, 0, s);
     hipLaunchKernelGGL(k_fin_grad, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p,
                        (double*)nullptr, 0);
     HIP_CHECK(hipGetLastError());

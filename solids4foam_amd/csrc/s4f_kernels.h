@@ -39,6 +39,8 @@ struct BlockMap {
     const int* iCell0; // per instance: first cell
     const int* iNCell; // per instance: number of cells
     const int* finInst;// instance finalized by finalize-kernel workgroup i (nullptr: i itself)
+    const int* iB0;    // per instance: first boundary face
+    const int* iNB;    // per instance: number of boundary faces
     int nBlocks;
 };
 
@@ -487,7 +489,8 @@ __device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, i
                                               const double* __restrict__ D, const T* __restrict__ x,
                                               double ra, const double* DnP, const double* CP, double* A,
                                               bool& hasB) {
-#pragma unroll
+    // two faces at a time: enough gathers in flight at four waves per SIMD
+#pragma unroll 2
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
         const int n = m.col[e];
@@ -513,20 +516,102 @@ __device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, i
 }
 
 // ---------------------------------------------------------------------------
-// k_grad: D' = D + x; boundary values D_b (solidTraction evaluate /
-// fixedDisplacement); least-squares grad(D') with the boundary correction;
-// boundary stress and the next traction snGrad; outer convergence test.
+// The gradient step of an outer iteration, in three launches:
+//   k_grad_bnd_pre   boundary faces: D' = D + x at the face cell, boundary
+//                    values D_b (solidTraction evaluate / fixedDisplacement /
+//                    solidSymmetry) and the snGrad used by the residual
+//   k_grad           cells: least-squares grad(D') with the boundary values,
+//                    D' stored, partials of the outer convergence test
+//   k_grad_bnd_post  boundary faces: boundary gradient and stress, the next
+//                    traction snGrad
 // mode 0: outer iteration; mode 1: initialisation (x = 0, G_old = 0, g = 0,
-// no convergence test); mode 2: gradient from given D_b/snb (no boundary update).
+// no convergence test); mode 2: gradient from given D_b/snb (no boundary
+// update: k_grad_bnd_pre is not launched, k_grad_bnd_post leaves g).
+// The face kernels run one workgroup row (blockIdx.y) per instance over that
+// instance's boundary faces; empty faces carry no value and are skipped.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int faceInst(const BlockMap& bm) {
+    return bm.finInst ? bm.finInst[blockIdx.y] : (int)blockIdx.y;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const BlockMap bm,
+                                                         const double* __restrict__ D0, const T* __restrict__ x,
+                                                         const double* __restrict__ D1, const double* __restrict__ G,
+                                                         double* __restrict__ Db, const double* __restrict__ g,
+                                                         double* __restrict__ snb, const Ctl* __restrict__ ctlAll,
+                                                         int mode) {
+    const int inst = faceInst(bm);
+    const Ctl* ctl = ctlAll + inst;
+    if (mode == 0 && ctl->outerDone) return;
+    if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
+    const double* __restrict__ D = ctl->dpar ? D1 : D0;
+    const bool relax = mode == 0 && ctl->relaxD != 1.0;
+    const double ra = relax ? ctl->relaxD : 1.0;
+    const int C = m.nCells, B = m.nBoundary;
+    const int b0 = bm.iB0[inst], b1 = b0 + bm.iNB[inst];
+    for (int b = b0 + blockIdx.x * kBlock + threadIdx.x; b < b1; b += gridDim.x * kBlock) {
+        const int type = m.bType[b];
+        if (type == 2) continue;
+        const int P = m.bCell[b];
+        // DuP: the solve's update D + x; DnP: after the fixed field
+        // under-relaxation D_prev + a (DuP - D_prev) (a = 1: the same)
+        double DuP[3], DnP[3], Gold[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double DP = D[(size_t)k * C + P];
+            const double xv = x ? (double)x[(size_t)k * C + P] : 0.0;
+            DuP[k] = DP + xv;
+            DnP[k] = DP + ra * xv;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Gold[q] = (mode == 1) ? 0.0 : G[(size_t)q * C + P];
+        const double k0 = m.bKv[b], k1 = m.bKv[(size_t)B + b], k2 = m.bKv[2 * (size_t)B + b];
+        const double dl = m.bDelta[b];
+        const bool trac = type == 1;
+        const bool sym = type == 3;
+        // solidSymmetry (first order, normal patch delta so no k correction):
+        // value D_P - n (n.D_P) after the solve (relaxed like every boundary
+        // value), snGrad -n (n.D_P) deltaCoeffs from the relaxed cell value
+        const double nb0 = m.bN[b], nb1 = m.bN[(size_t)B + b], nb2 = m.bN[2 * (size_t)B + b];
+        const double nDu = nb0 * DuP[0] + nb1 * DuP[1] + nb2 * DuP[2];
+        const double nDn = nb0 * DnP[0] + nb1 * DnP[1] + nb2 * DnP[2];
+        const double nbv[3] = {nb0, nb1, nb2};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double kG = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
+            double Dbv, sn;
+            if (sym) {
+                Dbv = DuP[j] - nbv[j] * nDu;
+                if (relax) {
+                    const double Dbp = Db[(size_t)j * B + b];
+                    Dbv = Dbp + ra * (Dbv - Dbp);
+                }
+                sn = -nbv[j] * nDn * dl;
+            } else if (trac) {
+                const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
+                Dbv = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
+                if (relax) {                  // D.relax() on the boundary values too
+                    const double Dbp = Db[(size_t)j * B + b];
+                    Dbv = Dbp + ra * (Dbv - Dbp);
+                }
+                sn = gv;
+            } else {
+                Dbv = m.bVal[(size_t)j * B + b];
+                sn = (Dbv - (DnP[j] + kG)) * dl;
+            }
+            Db[(size_t)j * B + b] = Dbv;
+            snb[(size_t)j * B + b] = sn;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap bm, double* __restrict__ D0,
                                                  const T* __restrict__ x, double* __restrict__ D1,
-                                                 double* __restrict__ G, double* __restrict__ Db,
-                                                 double* __restrict__ g, double* __restrict__ snb,
-                                                 double* __restrict__ Gb, double* __restrict__ sigb,
+                                                 double* __restrict__ G, const double* __restrict__ Db,
                                                  const double* __restrict__ Dold, Ctl* __restrict__ ctlAll,
-                                                 double* __restrict__ part, double* __restrict__ hist, int mode) {
+                                                 double* __restrict__ part, int mode) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
     Ctl* ctl = ctlAll + inst;
@@ -547,23 +632,17 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (P >= cEnd) continue;
-        // DuP: the solve's update D + x; DnP: after the fixed field
-        // under-relaxation D_prev + a (DuP - D_prev) (a = 1: the same)
-        double DP[3], DnP[3], DuP[3], CP[3];
+        double DP[3], DnP[3], CP[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             DP[k] = D[(size_t)k * C + P];
             const double xv = x ? (double)x[(size_t)k * C + P] : 0.0;
-            DuP[k] = DP[k] + xv;
             DnP[k] = DP[k] + ra * xv;
             CP[k] = m.cC[(size_t)k * C + P];
         }
         double A[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) A[q] = 0.0;
-        double Gold[9];
-        bool haveG = false;
-        int kb = width;  // first boundary slot
         bool hasB = false;
         int kInt = width;  // internal slots left to the general loop below
         if (width == 6) {
@@ -597,58 +676,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
                     for (int j = 0; j < 3; ++j) A[3 * i + j] += di * dv[j];
                 }
             } else if (n != kPad) {
-                if (kb == width) kb = k;
                 const int b = -1 - n;
+                const double lv[3] = {m.bLs[b], m.bLs[(size_t)B + b], m.bLs[2 * (size_t)B + b]};
                 double Dbv[3];
-                if (mode == 2) {
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) Dbv[q] = Db[(size_t)q * B + b];
-                } else {
-                    if (!haveG) {
-#pragma unroll
-                        for (int q = 0; q < 9; ++q) Gold[q] = (mode == 1) ? 0.0 : G[(size_t)q * C + P];
-                        haveG = true;
-                    }
-                    const double k0 = m.bKv[b], k1 = m.bKv[(size_t)B + b], k2 = m.bKv[2 * (size_t)B + b];
-                    const double dl = m.bDelta[b];
-                    const bool trac = m.bType[b] == 1;
-                    const bool sym = m.bType[b] == 3;
-                    // solidSymmetry (first order, normal patch delta so no k correction):
-                    // value D_P - n (n.D_P) after the solve (relaxed like every boundary
-                    // value), snGrad -n (n.D_P) deltaCoeffs from the relaxed cell value
-                    const double nb0 = m.bN[b], nb1 = m.bN[(size_t)B + b], nb2 = m.bN[2 * (size_t)B + b];
-                    const double nDu = nb0 * DuP[0] + nb1 * DuP[1] + nb2 * DuP[2];
-                    const double nDn = nb0 * DnP[0] + nb1 * DnP[1] + nb2 * DnP[2];
-                    const double nbv[3] = {nb0, nb1, nb2};
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const double kG = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
-                        double sn;
-                        if (sym) {
-                            Dbv[j] = DuP[j] - nbv[j] * nDu;
-                            if (relax) {
-                                const double Dbp = Db[(size_t)j * B + b];
-                                Dbv[j] = Dbp + ra * (Dbv[j] - Dbp);
-                            }
-                            sn = -nbv[j] * nDn * dl;
-                        } else if (trac) {
-                            const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
-                            Dbv[j] = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
-                            if (relax) {                      // D.relax() on the boundary values too
-                                const double Dbp = Db[(size_t)j * B + b];
-                                Dbv[j] = Dbp + ra * (Dbv[j] - Dbp);
-                            }
-                            sn = gv;
-                        } else {
-                            Dbv[j] = m.bVal[(size_t)j * B + b];
-                            sn = (Dbv[j] - (DnP[j] + kG)) * dl;
-                        }
-                        Db[(size_t)j * B + b] = Dbv[j];
-                        snb[(size_t)j * B + b] = sn;
-                    }
-                }
-                const double l0 = m.bLs[b], l1 = m.bLs[(size_t)B + b], l2 = m.bLs[2 * (size_t)B + b];
-                const double lv[3] = {l0, l1, l2};
+                for (int q = 0; q < 3; ++q) Dbv[q] = Db[(size_t)q * B + b];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -657,56 +689,15 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         }
         const double M0 = m.minv[P], M1 = m.minv[(size_t)C + P], M2 = m.minv[2 * (size_t)C + P];
         const double M3 = m.minv[3 * (size_t)C + P], M4 = m.minv[4 * (size_t)C + P], M5 = m.minv[5 * (size_t)C + P];
-        double Gn[9];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            Gn[j] = M0 * A[j] + M1 * A[3 + j] + M2 * A[6 + j];
-            Gn[3 + j] = M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j];
-            Gn[6 + j] = M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j];
+            G[(size_t)j * C + P] = M0 * A[j] + M1 * A[3 + j] + M2 * A[6 + j];
+            G[(size_t)(3 + j) * C + P] = M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j];
+            G[(size_t)(6 + j) * C + P] = M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j];
         }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) G[(size_t)q * C + P] = Gn[q];
         if (Dn) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) Dn[(size_t)q * C + P] = DnP[q];
-        }
-        // boundary gradient, stress and the next traction snGrad
-        if (kb < width) {
-            const double muP = cellMu(m, P), KP = cellK(m, P), ik = cellImpK(m, P);
-            for (int k = kb; k < width; ++k) {
-                const int e = e0 + k * kSlice + lane;
-                const int n = m.col[e];
-                if (n >= 0 || n == kPad) continue;
-                const int b = -1 - n;
-                const double nv[3] = {m.bN[b], m.bN[(size_t)B + b], m.bN[2 * (size_t)B + b]};
-                double Gbv[9], sb[6];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double nG = nv[0] * Gn[j] + nv[1] * Gn[3 + j] + nv[2] * Gn[6 + j];
-                    const double sn = snb[(size_t)j * B + b];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) Gbv[3 * i + j] = Gn[3 * i + j] + nv[i] * (sn - nG);
-                }
-                hooke(Gbv, muP, KP, sb);
-                if (Gb) {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) Gb[(size_t)q * B + b] = Gbv[q];
-                }
-#pragma unroll
-                for (int q = 0; q < 6; ++q) sigb[(size_t)q * B + b] = sb[q];
-                if (mode != 2 && m.bType[b] == 1) {
-                    const double Tt[9] = {sb[0] - ik * Gbv[0], sb[1] - ik * Gbv[1], sb[2] - ik * Gbv[2],
-                                          sb[1] - ik * Gbv[3], sb[3] - ik * Gbv[4], sb[4] - ik * Gbv[5],
-                                          sb[2] - ik * Gbv[6], sb[4] - ik * Gbv[7], sb[5] - ik * Gbv[8]};
-                    const double rik = 1.0 / ik;
-                    const double pr = m.bPres[b];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const double nT = nv[0] * Tt[j] + nv[1] * Tt[3 + j] + nv[2] * Tt[6 + j];
-                        g[(size_t)j * B + b] = ((m.bVal[(size_t)j * B + b] - nv[j] * pr) - nT) * rik;
-                    }
-                }
-            }
         }
         if (mode != 2) {
             double dx2 = 0.0, dold2 = 0.0, dn2 = 0.0;
@@ -726,6 +717,55 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     }
     if (mode == 2) return;
     storeBlockPartials<6>(acc, 7u, part + (size_t)lb * kMaxPartials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_bnd_post(const DevMesh m, const BlockMap bm,
+                                                          const double* __restrict__ G,
+                                                          const double* __restrict__ snb, double* __restrict__ Gb,
+                                                          double* __restrict__ sigb, double* __restrict__ g,
+                                                          const Ctl* __restrict__ ctlAll, int mode) {
+    const int inst = faceInst(bm);
+    const Ctl* ctl = ctlAll + inst;
+    if (mode == 0 && ctl->outerDone) return;
+    const int C = m.nCells, B = m.nBoundary;
+    const int b0 = bm.iB0[inst], b1 = b0 + bm.iNB[inst];
+    for (int b = b0 + blockIdx.x * kBlock + threadIdx.x; b < b1; b += gridDim.x * kBlock) {
+        const int type = m.bType[b];
+        if (type == 2) continue;
+        const int P = m.bCell[b];
+        double Gn[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Gn[q] = G[(size_t)q * C + P];
+        const double nv[3] = {m.bN[b], m.bN[(size_t)B + b], m.bN[2 * (size_t)B + b]};
+        double Gbv[9], sb[6];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double nG = nv[0] * Gn[j] + nv[1] * Gn[3 + j] + nv[2] * Gn[6 + j];
+            const double sn = snb[(size_t)j * B + b];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Gbv[3 * i + j] = Gn[3 * i + j] + nv[i] * (sn - nG);
+        }
+        hooke(Gbv, cellMu(m, P), cellK(m, P), sb);
+        if (Gb) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Gb[(size_t)q * B + b] = Gbv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sigb[(size_t)q * B + b] = sb[q];
+        if (mode != 2 && type == 1) {
+            const double ik = cellImpK(m, P);
+            const double Tt[9] = {sb[0] - ik * Gbv[0], sb[1] - ik * Gbv[1], sb[2] - ik * Gbv[2],
+                                  sb[1] - ik * Gbv[3], sb[3] - ik * Gbv[4], sb[4] - ik * Gbv[5],
+                                  sb[2] - ik * Gbv[6], sb[4] - ik * Gbv[7], sb[5] - ik * Gbv[8]};
+            const double rik = 1.0 / ik;
+            const double pr = m.bPres[b];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double nT = nv[0] * Tt[j] + nv[1] * Tt[3 + j] + nv[2] * Tt[6 + j];
+                g[(size_t)j * B + b] = ((m.bVal[(size_t)j * B + b] - nv[j] * pr) - nT) * rik;
+            }
+        }
+    }
 }
 
 // sigma = Hooke(G) for output (xx xy xz yy yz zz, SoA)
