@@ -54,7 +54,11 @@ enum {
     S4F_PATCH_TRACTION = 1,           /* solidTractionFvPatchVectorField.C:321-463 */
     S4F_PATCH_EMPTY = 2,              /* [OF] emptyFvPatch: 2-D; its faces carry no values and the
                                          component along their normal is not solved */
-    S4F_PATCH_SYMMETRY = 3            /* solidSymmetryFvPatchVectorField.C:147-250 (first order) */
+    S4F_PATCH_SYMMETRY = 3,           /* solidSymmetryFvPatchVectorField.C:147-250 (first order) */
+    S4F_PATCH_ZERO_SHEAR = 4          /* fixedDisplacementZeroShearFvPatchVectorField.C:189-216 over
+                                         solidDirectionMixedFvPatchVectorField.C:154-378 (first order,
+                                         non-orthogonal corrections, no limiter): normal displacement
+                                         prescribed, zero shear traction */
 };
 
 /* precision of the inner linear solve */
@@ -148,8 +152,10 @@ s4f_status s4f_mesh_info(const s4f_mesh *mesh, int64_t out[6]);
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
 s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);
-/* Per-face boundary data of patch `patch`: fixedDisplacement -> displacement
-   (size*3); solidTraction -> traction (size*3) and pressure (size, or NULL = 0). */
+/* Per-face boundary data of patch `patch`: fixedDisplacement and
+   fixedDisplacementZeroShear -> displacement (size*3; only its normal
+   component binds the zero-shear patch); solidTraction -> traction (size*3)
+   and pressure (size, or NULL = 0). */
 s4f_status s4f_bc_set(s4f_mesh *mesh, int32_t patch, const double *value, const double *pressure);
 
 /* ------------------------------------------------------------------ */

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-FIXED, TRACTION = 0, 1
+FIXED, TRACTION, EMPTY, SYMMETRY, ZERO_SHEAR = 0, 1, 2, 3, 4
 PRECOND_DIC, PRECOND_DIAGONAL, PRECOND_NONE = 0, 1, 2
 
 

@@ -47,7 +47,7 @@
 #define OR_VSMALL 1e-300 /* OpenFOAM VSMALL for double */
 #define OR_ROOTVSMALL 1e-150
 
-enum { OR_FIXED = 0, OR_TRACTION = 1, OR_EMPTY = 2, OR_SYMMETRY = 3 };
+enum { OR_FIXED = 0, OR_TRACTION = 1, OR_EMPTY = 2, OR_SYMMETRY = 3, OR_ZERO_SHEAR = 4 };
 enum { OR_PRECOND_DIC = 0, OR_PRECOND_DIAGONAL = 1, OR_PRECOND_NONE = 2 };
 
 typedef struct {
@@ -660,11 +660,13 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         c->diag[m->owner[f]] += a;
         c->diag[m->neighbour[f]] += a;
     }
-    for (int b = 0; b < B; ++b) if (c->bType[b] == OR_FIXED || c->bType[b] == OR_SYMMETRY) {
-        /* symmetry: [OF] basicSymmetry's implicit part is per component
-           (n_j^2); one matrix serves the three components here, so the full
-           coefficient is taken (M only steers the defect correction; the
-           residual carries the exact condition) */
+    for (int b = 0; b < B; ++b)
+        if (c->bType[b] == OR_FIXED || c->bType[b] == OR_SYMMETRY || c->bType[b] == OR_ZERO_SHEAR) {
+        /* symmetry and fixedDisplacementZeroShear: [OF] basicSymmetry's and
+           directionMixed's implicit parts are per component (|n_j|); one
+           matrix serves the three components here, so the full coefficient
+           is taken (M only steers the defect correction; the residual carries
+           the exact condition) */
         int P = bCell(c, b);
         c->diag[P] += c->impK[P]*m->magSf[F+b]*c->bDelta[b];
     }
@@ -751,11 +753,15 @@ void or_sigma(or_case *c)
 }
 
 /* solidTractionFvPatchVectorField::updateCoeffs (:321-395) with
-   linGeomTotalDispSolid::tractionBoundarySnGrad (:235-271), relaxFac 1 */
+   linGeomTotalDispSolid::tractionBoundarySnGrad (:235-271), relaxFac 1.
+   fixedDisplacementZeroShear::updateCoeffs
+   (fixedDisplacementZeroShearFvPatchVectorField.C:189-216) sets its refGrad
+   to the same snGrad for zero traction and zero pressure. */
 void or_update_traction(or_case *c)
 {
-    for (int b = 0; b < c->nB; ++b) if (c->bType[b] == OR_TRACTION) {
+    for (int b = 0; b < c->nB; ++b) if (c->bType[b] == OR_TRACTION || c->bType[b] == OR_ZERO_SHEAR) {
         int P = bCell(c, b);
+        const int zs = c->bType[b] == OR_ZERO_SHEAR;
         const double *n = c->bN + 3*b, *s = c->sigb + 6*b, *Gb = c->Gb + 9*b;
         double ik = c->impK[P], rik = 1.0/ik;
         double T[9] = { s[0] - ik*Gb[0], s[1] - ik*Gb[1], s[2] - ik*Gb[2],
@@ -763,8 +769,21 @@ void or_update_traction(or_case *c)
                         s[2] - ik*Gb[6], s[4] - ik*Gb[7], s[5] - ik*Gb[8] };
         double nT[3]; vdotT(n, T, nT);
         for (int q = 0; q < 3; ++q)
-            c->g[3*b+q] = ((c->bValue[3*b+q] - n[q]*c->bPressure[b]) - nT[q])*rik;
+            c->g[3*b+q] = (zs ? -nT[q] : (c->bValue[3*b+q] - n[q]*c->bPressure[b]) - nT[q])*rik;
     }
+}
+
+/* fixedDisplacementZeroShear value (solidDirectionMixedFvPatchVectorField.C:
+   154-259, nonOrthogonalCorrections on, first order, limitCoeff 1) from the
+   cell value DP: valueFraction nn, refValue the patch displacement, refGrad g;
+   n (n.refValue) + (I - nn).(DP + k & G_P + g/deltaCoeffs) */
+static void zero_shear_value(const or_case *c, int b, const double *DP, const double *kG, double *out)
+{
+    const double *n = c->bN + 3*b, *g = c->g + 3*b, *Dv = c->bValue + 3*b;
+    double gv[3];
+    for (int q = 0; q < 3; ++q) gv[q] = DP[q] + kG[q] + g[q]/c->bDelta[b];
+    double nD = dot3(n, Dv), ng = dot3(n, gv);
+    for (int q = 0; q < 3; ++q) out[q] = n[q]*nD + (gv[q] - n[q]*ng);
 }
 
 /* D.correctBoundaryConditions() after the solve:
@@ -785,6 +804,15 @@ void or_correct_boundary(or_case *c)
             for (int q = 0; q < 3; ++q) {
                 c->Db[3*b+q] = c->D[3*P+q] + kG[q] + c->g[3*b+q]/c->bDelta[b];
                 c->snb[3*b+q] = c->g[3*b+q];
+            }
+        } else if (c->bType[b] == OR_ZERO_SHEAR) {
+            /* value from the solved cell value; snGrad (:262-378) recomputes
+               the same value from the cell value at its call:
+               (value - (D_P + k & G_P)) deltaCoeffs */
+            double v[3]; zero_shear_value(c, b, c->D + 3*P, kG, v);
+            for (int q = 0; q < 3; ++q) {
+                c->Db[3*b+q] = v[q];
+                c->snb[3*b+q] = (v[q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
             }
         } else if (c->bType[b] == OR_SYMMETRY) {
             /* solidSymmetry, first order (solidSymmetryFvPatchVectorField.C:147-250):
@@ -958,7 +986,8 @@ void or_reference_source(or_case *c, int q, double *out)
         out[o] -= fl; out[n] += fl;
     }
     for (int b = 0; b < B; ++b) {
-        if (c->bType[b] == OR_EMPTY || c->bType[b] == OR_SYMMETRY) continue;  /* literal form: fixed and traction */
+        if (c->bType[b] == OR_EMPTY || c->bType[b] == OR_SYMMETRY || c->bType[b] == OR_ZERO_SHEAR)
+            continue;  /* literal form: fixed and traction */
         int P = bCell(c, b);
         double gm = c->impK[P]*m->magSf[F+b];
         if (c->bType[b] == OR_TRACTION) {
@@ -1096,6 +1125,15 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
             if (c->bType[b] == OR_EMPTY) continue;
             if (c->bType[b] == OR_TRACTION) {
                 for (int q = 0; q < 3; ++q) c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+            } else if (c->bType[b] == OR_ZERO_SHEAR) {
+                /* boundary values relaxed; the snGrad is evaluated from the
+                   relaxed cell value */
+                double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
+                double v[3]; zero_shear_value(c, b, c->D + 3*P, kG, v);
+                for (int q = 0; q < 3; ++q) {
+                    c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+                    c->snb[3*b+q] = (v[q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
+                }
             } else if (c->bType[b] == OR_SYMMETRY) {
                 /* boundary values relaxed; solidSymmetry::snGrad is evaluated
                    from the relaxed cell value */

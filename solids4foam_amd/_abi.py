@@ -24,6 +24,7 @@ PATCH_FIXED_DISPLACEMENT = 0
 PATCH_TRACTION = 1
 PATCH_EMPTY = 2
 PATCH_SYMMETRY = 3
+PATCH_ZERO_SHEAR = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECOND_DIAGONAL = 0

@@ -322,8 +322,7 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
         int f = F;
         for (int p = 0; p < d.nPatches; ++p) {
             if (h.patchStart[p] != f || h.patchSize[p] < 0) throw InvalidError("patches must tile the boundary faces in order");
-            if (h.patchType[p] != S4F_PATCH_FIXED_DISPLACEMENT && h.patchType[p] != S4F_PATCH_TRACTION &&
-                h.patchType[p] != S4F_PATCH_EMPTY && h.patchType[p] != S4F_PATCH_SYMMETRY)
+            if (h.patchType[p] < S4F_PATCH_FIXED_DISPLACEMENT || h.patchType[p] > S4F_PATCH_ZERO_SHEAR)
                 throw InvalidError("unsupported patch type");
             f += h.patchSize[p];
         }
@@ -532,9 +531,11 @@ void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCel
         h.diag[nn] += a;
     }
     for (int b = 0; b < B; ++b)
-        // symmetry: the full fixed-value coefficient (one matrix serves the
-        // three components; the residual carries the exact condition)
-        if (h.bType[b] == S4F_PATCH_FIXED_DISPLACEMENT || h.bType[b] == S4F_PATCH_SYMMETRY) {
+        // symmetry and zero shear: the full fixed-value coefficient (their
+        // implicit parts are per component; one matrix serves the three
+        // components, and the residual carries the exact condition)
+        if (h.bType[b] == S4F_PATCH_FIXED_DISPLACEMENT || h.bType[b] == S4F_PATCH_SYMMETRY ||
+            h.bType[b] == S4F_PATCH_ZERO_SHEAR) {
             const int P = h.bCell[b];
             h.diag[P] += ik(P) * h.bMagSf[b] * h.bDelta[b];
         }

@@ -153,8 +153,7 @@ template <> struct MatSel<float> {
 // zero area, Rhie-Chow and matrix coefficients, so it adds exactly zero.
 // Rows list internal faces before boundary faces, so running the boundary
 // slots afterwards keeps the summation order of the slot loop.
-template <int W>
-__device__ __forceinline__ void residualInternalW(const DevMesh& m, int P, int e0, int lane, int C,
+__device__ __forceinline__ void residualInternal(const DevMesh& m, int P, int e0, int W, int lane, int C,
                                                   const double* __restrict__ D, const double* __restrict__ G,
                                                   const double* DP, const double* GP, const double* CP,
                                                   const double* sP, double* div, double* rc, double* off,
@@ -234,50 +233,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
         hooke(GP, muP, KP, sP);
         double div[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, off[3] = {0, 0, 0}, asum = 0.0;
         bool hasB = false;
-        int kInt = width;  // internal slots left to the general loop below
-        if (width == 6) {
-            residualInternalW<6>(m, P, e0, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
-            kInt = 0;
-        } else if (width == 4) {
-            residualInternalW<4>(m, P, e0, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
-            kInt = 0;
-        } else {
-            hasB = true;
-        }
+        residualInternal(m, P, e0, width, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
         for (int k = 0; hasB && k < width; ++k) {
             const int e = e0 + k * kSlice + lane;
             const int n = m.col[e];
-            if (n >= 0) {
-                if (k >= kInt) continue;  // already summed by the fixed-width pass
-                const double S0 = m.eS[e], S1 = m.eS[m.nEntries + e],
-                             S2 = m.eS[2 * m.nEntries + e];
-                const double w = m.eW[e];
-                double GN[9], sN[6], DN[3], d[3];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) GN[q] = G[(size_t)q * C + n];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) { DN[q] = D[(size_t)q * C + n]; d[q] = m.cC[(size_t)q * C + n] - CP[q]; }
-                hooke(GN, cellMu(m, n), cellK(m, n), sN);
-                double sf[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) sf[q] = w * (sP[q] - sN[q]) + sN[q];
-                div[0] += S0 * sf[0] + S1 * sf[1] + S2 * sf[2];
-                div[1] += S0 * sf[1] + S1 * sf[3] + S2 * sf[4];
-                div[2] += S0 * sf[2] + S1 * sf[4] + S2 * sf[5];
-                double Gf[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Gf[q] = w * (GP[q] - GN[q]) + GN[q];
-                const double cf = m.eC[e];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double dG = d[0] * Gf[j] + d[1] * Gf[3 + j] + d[2] * Gf[6 + j];
-                    rc[j] += cf * ((DN[j] - DP[j]) - dG);
-                }
-                const double a = m.eA[e];
-                asum += a;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) off[j] += a * DN[j];
-            } else if (n != kPad) {
+            if (n < 0 && n != kPad) {
                 const int b = -1 - n;
                 const size_t E = m.nEntries;
                 const double S0 = m.eS[e], S1 = m.eS[E + e], S2 = m.eS[2 * E + e];
@@ -570,6 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const 
         const double dl = m.bDelta[b];
         const bool trac = type == 1;
         const bool sym = type == 3;
+        const bool zs = type == 4;
         // solidSymmetry (first order, normal patch delta so no k correction):
         // value D_P - n (n.D_P) after the solve (relaxed like every boundary
         // value), snGrad -n (n.D_P) deltaCoeffs from the relaxed cell value
@@ -577,11 +538,44 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const 
         const double nDu = nb0 * DuP[0] + nb1 * DuP[1] + nb2 * DuP[2];
         const double nDn = nb0 * DnP[0] + nb1 * DnP[1] + nb2 * DnP[2];
         const double nbv[3] = {nb0, nb1, nb2};
+        double kGv[3], gvv[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double kG = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
+            kGv[j] = k0 * Gold[j] + k1 * Gold[3 + j] + k2 * Gold[6 + j];
+            gvv[j] = (mode == 1 || !(trac || zs)) ? 0.0 : g[(size_t)j * B + b];
+        }
+        // fixedDisplacementZeroShear (direction-mixed, valueFraction nn, first
+        // order): n (n.Dbar) + (I - nn).(D_P + k.G_P + g/deltaCoeffs), the value
+        // from the solved cell value, the snGrad from the relaxed one
+        double zu[3] = {0, 0, 0}, zn[3] = {0, 0, 0};
+        if (zs) {
+            double gu[3], gn[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                gu[j] = DuP[j] + kGv[j] + gvv[j] / dl;
+                gn[j] = DnP[j] + kGv[j] + gvv[j] / dl;
+            }
+            const double nD = nb0 * m.bVal[b] + nb1 * m.bVal[(size_t)B + b] + nb2 * m.bVal[2 * (size_t)B + b];
+            const double ngu = nb0 * gu[0] + nb1 * gu[1] + nb2 * gu[2];
+            const double ngn = nb0 * gn[0] + nb1 * gn[1] + nb2 * gn[2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                zu[j] = nbv[j] * nD + (gu[j] - nbv[j] * ngu);
+                zn[j] = nbv[j] * nD + (gn[j] - nbv[j] * ngn);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double kG = kGv[j];
             double Dbv, sn;
-            if (sym) {
+            if (zs) {
+                Dbv = zu[j];
+                if (relax) {
+                    const double Dbp = Db[(size_t)j * B + b];
+                    Dbv = Dbp + ra * (Dbv - Dbp);
+                }
+                sn = (zn[j] - (DnP[j] + kG)) * dl;
+            } else if (sym) {
                 Dbv = DuP[j] - nbv[j] * nDu;
                 if (relax) {
                     const double Dbp = Db[(size_t)j * B + b];
@@ -589,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const 
                 }
                 sn = -nbv[j] * nDn * dl;
             } else if (trac) {
-                const double gv = (mode == 1) ? 0.0 : g[(size_t)j * B + b];
+                const double gv = gvv[j];
                 Dbv = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
                 if (relax) {                  // D.relax() on the boundary values too
                     const double Dbp = Db[(size_t)j * B + b];
@@ -752,17 +746,20 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_post(const DevMesh m, const
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q) sigb[(size_t)q * B + b] = sb[q];
-        if (mode != 2 && type == 1) {
+        if (mode != 2 && (type == 1 || type == 4)) {
+            // fixedDisplacementZeroShear: the traction snGrad for zero
+            // traction and pressure (its shear-free refGrad)
+            const bool zs = type == 4;
             const double ik = cellImpK(m, P);
             const double Tt[9] = {sb[0] - ik * Gbv[0], sb[1] - ik * Gbv[1], sb[2] - ik * Gbv[2],
                                   sb[1] - ik * Gbv[3], sb[3] - ik * Gbv[4], sb[4] - ik * Gbv[5],
                                   sb[2] - ik * Gbv[6], sb[4] - ik * Gbv[7], sb[5] - ik * Gbv[8]};
             const double rik = 1.0 / ik;
-            const double pr = m.bPres[b];
+            const double pr = zs ? 0.0 : m.bPres[b];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const double nT = nv[0] * Tt[j] + nv[1] * Tt[3 + j] + nv[2] * Tt[6 + j];
-                g[(size_t)j * B + b] = ((m.bVal[(size_t)j * B + b] - nv[j] * pr) - nT) * rik;
+                g[(size_t)j * B + b] = (zs ? -nT : (m.bVal[(size_t)j * B + b] - nv[j] * pr) - nT) * rik;
             }
         }
     }

@@ -304,7 +304,8 @@ def _field_value(v, n, width):
     raise ValueError(f"unsupported field value {v!r}")
 
 
-SUPPORTED_D_BCS = ("fixedDisplacement", "linearSpatialDisplacement", "solidTraction", "solidSymmetry", "empty")
+SUPPORTED_D_BCS = ("fixedDisplacement", "fixedDisplacementZeroShear", "linearSpatialDisplacement", "solidTraction",
+                   "solidSymmetry", "empty")
 
 
 def read_D_boundary(case_dir, boundary, time="0", face_centres=None):
@@ -332,6 +333,19 @@ def read_D_boundary(case_dir, boundary, time="0", face_centres=None):
             s0 = p["startFace"] - boundary[0]["startFace"]
             Cf = np.asarray(face_centres)[s0:s0 + n]
             out.append(dict(type="fixedDisplacement", value=a + Cf @ b.T))
+        elif t == "fixedDisplacementZeroShear":
+            # fixedDisplacementZeroShearFvPatchVectorField.C:81-120 (totalDisp from
+            # "value") over solidDirectionMixedFvPatchVectorField.C:79-111 (defaults)
+            on = ("yes", "on", "true")
+            if "displacementSeries" in e:
+                raise ValueError(f"patch {p['name']}: displacementSeries is not supported on the device path")
+            if str(e.get("forceZeroShearGrad", "no")) in on or str(e.get("secondOrder", "no")) in on:
+                raise ValueError(f"patch {p['name']}: forceZeroShearGrad/secondOrder are not supported on the "
+                                 "device path")
+            if str(e.get("nonOrthogonalCorrections", "yes")) not in on or float(e.get("limitCoeff", 1.0)) != 1.0:
+                raise ValueError(f"patch {p['name']}: only nonOrthogonalCorrections yes and limitCoeff 1 are "
+                                 "supported on the device path")
+            out.append(dict(type=t, value=_field_value(e.get("value", ["uniform", [0, 0, 0]]), n, 3)))
         elif t == "empty":
             out.append(dict(type="empty"))
         elif t == "solidSymmetry":

@@ -269,20 +269,23 @@ class LinGeomTotalDispSolid:
     typeName = "linearGeometryTotalDisplacement"
 
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False):
-        # patches: list of dicts {type: "fixedDisplacement"|"solidTraction"|"empty", value|traction, pressure}
+        # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
+        # "solidSymmetry"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
-                 "empty": _abi.PATCH_EMPTY, "solidSymmetry": _abi.PATCH_SYMMETRY}
+                 "empty": _abi.PATCH_EMPTY, "solidSymmetry": _abi.PATCH_SYMMETRY,
+                 "fixedDisplacementZeroShear": _abi.PATCH_ZERO_SHEAR}
         types = []
         for p in patches:
             t = p["type"]
             if t not in kinds:
                 raise ValueError(f"Unsupported boundary condition type {t!r} "
-                                 "(valid: fixedDisplacement, solidTraction, solidSymmetry, empty)")
+                                 "(valid: fixedDisplacement, fixedDisplacementZeroShear, solidTraction, "
+                                 "solidSymmetry, empty)")
             types.append(kinds[t])
         self.mesh = SolverMesh(ctx, polymesh, types, ls_full_delta)
         self.mesh.set_material(E, nu)
         for i, p in enumerate(patches):
-            if p["type"] == "fixedDisplacement":
+            if p["type"] in ("fixedDisplacement", "fixedDisplacementZeroShear"):
                 self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))
             elif p["type"] == "solidTraction":
                 self.mesh.set_bc(i, p.get("traction", (0.0, 0.0, 0.0)), p.get("pressure"))

@@ -111,9 +111,19 @@ def test_polymesh_and_bc_roundtrip(tmp_path):
 
 def test_unsupported_bc_is_rejected(tmp_path):
     pm, a = _box_case(str(tmp_path))
-    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement",
-                                                           "type fixedDisplacementZeroShear"))
     m = foam_io.read_polymesh(str(tmp_path))
+    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement",
+                                                           "type solidContact"))
+    with pytest.raises(ValueError, match="not supported"):
+        foam_io.read_D_boundary(str(tmp_path), m["boundary"])
+    # zero shear is read (value = the prescribed displacement); its options
+    # outside the device path's first-order form are refused
+    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement;",
+                                                           "type fixedDisplacementZeroShear;"))
+    bcs = foam_io.read_D_boundary(str(tmp_path), m["boundary"])
+    assert bcs[4]["type"] == "fixedDisplacementZeroShear" and bcs[4]["value"].shape == (m["boundary"][4]["nFaces"], 3)
+    _write(str(tmp_path), "0/D", _d_file(PATCHES).replace("type fixedDisplacement;",
+                                                           "type fixedDisplacementZeroShear; secondOrder yes;"))
     with pytest.raises(ValueError, match="not supported"):
         foam_io.read_D_boundary(str(tmp_path), m["boundary"])
 

@@ -319,6 +319,43 @@ def test_symmetry_planes(ctx, oracle_mod):
                 assert np.abs(out["D"] - exact).max() < 1e-7 * np.abs(exact).max()
 
 
+def test_zero_shear_patches(ctx, oracle_mod):
+    # fixedDisplacementZeroShear rollers on the three min faces.  (i) z-max
+    # moved by delta through a zero-shear patch (tangential part of its vector
+    # ignored): the exact uniaxial field on a skewed mesh; (ii) an oblique
+    # traction on z-max (shear reaches the rollers): the device field matches
+    # the oracle's, with and without field relaxation
+    from solids4foam_amd import PATCH_ZERO_SHEAR, PolyMesh, SolverMesh
+    E, nu, delta = 200e9, 0.3, 1e-4
+    ZS = PATCH_ZERO_SHEAR
+    for case in ("displacement", "traction"):
+        top = ZS if case == "displacement" else TRACTION
+        types = [ZS, TRACTION, ZS, TRACTION, ZS, top]
+        pm = PolyMesh.box(6, 5, 4, L=(1.0, 1.0, 1.0), jitter=0.15, seed=3)
+        m = SolverMesh(ctx, pm, types, ls_full_delta=True)
+        m.set_material(E, nu)
+        bv = np.zeros((pm.nBoundaryFaces, 3))
+        bv[pm.patch_faces(5)] = (0.3 * delta, -0.2 * delta, delta) if case == "displacement" else (2e6, -1e6, 1e7)
+        for p in range(6):
+            m.set_bc(p, bv[pm.patch_faces(p)])
+        om = oracle_mod.Mesh.box(6, 5, 4, (1.0, 1.0, 1.0), 0.15, 3)
+        oc = oracle_mod.Case(om, E, nu, types, bv, ls_full_delta=True)
+        oc.solve(nCorr=50000, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+        Do = oc.get("D").reshape(-1, 3)
+        for precision, relax in ((0, 0.0), (1, 0.0), (0, 0.6)):
+            out = m.solve(precision=precision, nCorrectors=50000, solutionTolerance=1e-13, alternativeTolerance=1e-13,
+                          pcgTolerance=1e-15, relaxD=relax)
+            assert out["converged"]
+            err = rel_l2(out["D"], Do)
+            print(f"  zero shear {case} precision={precision} relax={relax}: nOuter={out['nOuter']} err={err:.2e}",
+                  flush=True)
+            assert err < (1e-8 if precision == 0 else 1e-6)
+            if case == "displacement" and precision == 0:
+                X = pm.cell_centres()
+                exact = np.stack([-nu * delta * X[:, 0], -nu * delta * X[:, 1], delta * X[:, 2]], axis=1)
+                assert np.abs(out["D"] - exact).max() < 1e-8 * delta
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)

@@ -412,3 +412,62 @@ def test_symmetry_planes_uniaxial_exact(oracle_mod):
     # the symmetry value is the tangential part of the cell value: n.D_b = 0
     Db = c.get("Db").reshape(-1, 3)
     assert np.abs(Db[m.patch_faces(0), 0]).max() < 1e-20 and np.abs(Db[m.patch_faces(4), 2]).max() < 1e-20
+
+
+def _zero_shear_box(oracle_mod, jitter, top, relax=None):
+    # fixedDisplacementZeroShear rollers (zero normal displacement) on x-min,
+    # y-min, z-min; x-max / y-max traction free; z-max either a normal
+    # traction ("traction", t) or a zero-shear patch with prescribed
+    # displacement ("zs", its tangential part must not matter)
+    E, nu = 200e9, 0.3
+    m = oracle_mod.Mesh.box(6, 5, 4, (1.0, 1.0, 1.0), jitter, 3)
+    ZS = oracle_mod.ZERO_SHEAR
+    bv = np.zeros((m.nBoundaryFaces, 3))
+    kind, v = top
+    bv[m.patch_faces(5)] = v
+    types = [ZS, TRACTION, ZS, TRACTION, ZS, TRACTION if kind == "traction" else ZS]
+    # the full boundary delta in the least-squares vectors (extendedLeastSquares)
+    # keeps the gradient of a linear field exact on skewed boundary cells
+    c = oracle_mod.Case(m, E, nu, types, bv, ls_full_delta=jitter > 0)
+    kw = dict(relaxD=relax) if relax else {}
+    r = c.solve(nCorr=50000, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC, **kw)
+    return m, c, r, E, nu
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.15])
+def test_zero_shear_rollers_uniaxial_exact(oracle_mod, jitter):
+    # rollers + normal traction t: uniaxial stress, D = (-nu t x/E, -nu t y/E,
+    # t z/E); linear fields are exact for the corrected scheme on skewed cells
+    t = 1e7
+    m, c, r, E, nu = _zero_shear_box(oracle_mod, jitter, ("traction", (0.0, 0.0, t)))
+    assert r["converged"]
+    X = m.C
+    exact = np.stack([-nu * t * X[:, 0] / E, -nu * t * X[:, 1] / E, t * X[:, 2] / E], axis=1)
+    assert np.abs(c.get("D").reshape(-1, 3) - exact).max() < 1e-8 * np.abs(exact).max()
+    s = c.get("sigma").reshape(-1, 6)
+    assert np.allclose(s[:, 5], t, rtol=1e-7) and np.abs(s[:, :5]).max() < 1e-6 * t
+    # the roller value: zero normal component, the tangential part follows the
+    # cell (not the prescribed zero vector, which a fixed patch would impose)
+    Db = c.get("Db").reshape(-1, 3)
+    f0 = m.patch_faces(0)
+    assert np.abs(Db[f0, 0]).max() < 1e-20
+    assert np.abs(Db[f0, 1]).max() > 1e-6 * t / E
+
+
+def test_zero_shear_prescribed_normal_displacement(oracle_mod):
+    # z-max moved by delta along its normal through a zero-shear patch whose
+    # prescribed vector also has tangential components: only n (n.value)
+    # binds, so the field is uniaxial stress with strain delta:
+    # D = (-nu delta x, -nu delta y, delta z), sigma_zz = E delta
+    delta = 1e-4
+    m, c, r, E, nu = _zero_shear_box(oracle_mod, 0.15, ("zs", (0.3 * delta, -0.2 * delta, delta)))
+    assert r["converged"]
+    X = m.C
+    exact = np.stack([-nu * delta * X[:, 0], -nu * delta * X[:, 1], delta * X[:, 2]], axis=1)
+    assert np.abs(c.get("D").reshape(-1, 3) - exact).max() < 1e-8 * delta
+    s = c.get("sigma").reshape(-1, 6)
+    assert np.allclose(s[:, 5], E * delta, rtol=1e-7)
+    # with the relaxation factor 0.6 the loop reaches the same field
+    m2, c2, r2, _, _ = _zero_shear_box(oracle_mod, 0.15, ("zs", (0.3 * delta, -0.2 * delta, delta)), relax=0.6)
+    assert r2["converged"]
+    assert np.abs(c2.get("D").reshape(-1, 3) - exact).max() < 1e-8 * delta
