@@ -61,6 +61,9 @@ enum {
                                          prescribed, zero shear traction */
 };
 
+/* field relaxation method */
+enum { S4F_RELAX_FIXED = 0, S4F_RELAX_AITKEN = 1 };
+
 /* precision of the inner linear solve */
 enum {
     S4F_PRECISION_FP64 = 0,  /* plain path: everything FP64 */
@@ -179,10 +182,15 @@ typedef struct {
     int32_t precision;             /* S4F_PRECISION_* */
     int32_t warmStart;             /* 1: D_inout holds the initial field (old-time field = it) */
     double mgCoarseScale;          /* GAMG: scale of the coarse-grid correction, in (0, 2]; 0 = 1.8 */
-    /* fvSolution relaxationFactors { fields { D a; } } with relaxationMethod
-       fixed (solidModel::relaxField, solidModel.C:837-841): D = D_prev + a (D - D_prev)
-       after each solve; 0 or 1 = no relaxation */
+    /* fvSolution relaxationFactors { fields { D a; } }: with relaxationMethod
+       fixed (solidModel::relaxField, solidModel.C:837-841) D = D_prev + a (D - D_prev)
+       after each solve, 0 or 1 = no relaxation; with Aitken the first factor */
     double relaxD;
+    /* solidProperties relaxationMethod (solidModel.C:1175-1178): S4F_RELAX_FIXED or
+       S4F_RELAX_AITKEN (solidModel.C:842-897: per cell and boundary face,
+       D -= alpha (D_prev - D), alpha updated by Aitken's formula, clipped to [0, 2]) */
+    int32_t relaxationMethod;
+    int32_t reserved1;             /* zero */
 } s4f_solve_opts;
 
 typedef struct {

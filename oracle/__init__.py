@@ -83,7 +83,7 @@ class Opts(ctypes.Structure):
                 ("pcgMaxIter", ctypes.c_int), ("rcScale", ctypes.c_double), ("interfaceScale", ctypes.c_double),
                 ("historyCap", ctypes.c_int), ("histInitRes", ctypes.POINTER(ctypes.c_double)),
                 ("histRvf", ctypes.POINTER(ctypes.c_double)), ("histInner", ctypes.POINTER(ctypes.c_int)),
-                ("relaxD", ctypes.c_double)]
+                ("relaxD", ctypes.c_double), ("relaxAitken", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -148,7 +148,7 @@ class Mesh:
 
 def default_opts(**kw):
     o = dict(nCorr=10000, solTol=1e-6, altTol=1e-7, precond=PRECOND_DIC, pcgTol=1e-9, pcgRelTol=0.1,
-             pcgMaxIter=1000, rcScale=0.1, interfaceScale=0.01, relaxD=0.0)
+             pcgMaxIter=1000, rcScale=0.1, interfaceScale=0.01, relaxD=0.0, relaxAitken=0)
     o.update(kw)
     return o
 

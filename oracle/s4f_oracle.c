@@ -74,6 +74,7 @@ typedef struct {
     int historyCap;         /* capacity of the history arrays below */
     double *histInitRes, *histRvf; int *histInner;
     double relaxD;          /* fvSolution relaxationFactors { fields { D } }: 0 or 1 = none */
+    int relaxAitken;        /* solidProperties relaxationMethod: 0 fixed, 1 Aitken */
 } or_opts;
 
 typedef struct {
@@ -111,6 +112,8 @@ typedef struct {
     double *g;                       /* traction snGrad (solidTraction gradient()), nB*3 */
     double *snb;                     /* boundary snGrad used by the gradient boundary correction, nB*3 */
     double *R;                       /* explicit residual, C*3 (integrated) */
+    double *aAlpha, *aRes;           /* Aitken: aitkenAlpha (C), aitkenResidual (C*3) */
+    double *aAlphaB, *aResB;         /* their boundary values (nB, nB*3) */
     int iCorr;
 } or_case;
 
@@ -676,6 +679,8 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
     c->G = xcalloc((size_t)C*9, sizeof(double)); c->Gb = xcalloc((size_t)B*9, sizeof(double));
     c->sig = xcalloc((size_t)C*6, sizeof(double)); c->sigb = xcalloc((size_t)B*6, sizeof(double));
     c->g = xcalloc((size_t)B*3, sizeof(double)); c->snb = xcalloc((size_t)B*3, sizeof(double));
+    c->aAlpha = xcalloc((size_t)C, sizeof(double)); c->aRes = xcalloc((size_t)C*3, sizeof(double));
+    c->aAlphaB = xcalloc((size_t)B, sizeof(double)); c->aResB = xcalloc((size_t)B*3, sizeof(double));
     c->R = xcalloc((size_t)C*3, sizeof(double));
     return c;
 }
@@ -690,6 +695,7 @@ void or_case_free(or_case *c)
     free(c->diag); free(c->upper);
     free(c->D); free(c->Dprev); free(c->Dold); free(c->Db); free(c->G); free(c->Gb);
     free(c->sig); free(c->sigb); free(c->g); free(c->snb); free(c->R);
+    free(c->aAlpha); free(c->aRes); free(c->aAlphaB); free(c->aResB);
     free(c);
 }
 
@@ -1030,8 +1036,15 @@ void or_init(or_case *c, const double *D0)
     memset(c->g, 0, sizeof(double)*3*B);
     or_correct_boundary(c);
     or_grad(c);
-    memset(c->sig, 0, sizeof(double)*6*C);
-    memset(c->sigb, 0, sizeof(double)*6*B);
+    /* the constructor computes no stress: a cold start keeps the zero sigma
+       field until the first mechanical().correct(), so the first residual
+       and traction update see sigma = 0; a restart (D0 given) reads the
+       stress written with D, i.e. Hooke's law of its gradient */
+    if (D0) or_sigma(c);
+    else {
+        memset(c->sig, 0, sizeof(double)*6*C);
+        memset(c->sigb, 0, sizeof(double)*6*B);
+    }
     c->iCorr = 0;
 }
 
@@ -1106,7 +1119,8 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
         for (int i = 0; i < C; ++i) c->D[3*i+q] = psi[i];
     }
     free(rD); free(psi); free(src); free(Ap); free(work);
-    const int relax = o->relaxD > 0.0 && o->relaxD != 1.0;
+    const int aitken = o->relaxAitken != 0;
+    const int relax = aitken || (o->relaxD > 0.0 && o->relaxD != 1.0);
     double *DbPrev = NULL;
     if (relax) {                                            /* boundary values of D.prevIter() */
         DbPrev = malloc(sizeof(double)*3*c->nB);
@@ -1114,37 +1128,59 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
     }
     or_correct_boundary(c);                                 /* psi.correctBoundaryConditions() */
     if (relax) {
-        /* relaxField, fixed method (solidModel.C:837-841): D.relax(), i.e.
-           D == prevIter + a (D - prevIter) on cells and boundary values;
-           fixedDisplacement snGrad is evaluated afterwards from the relaxed
-           cell value (fixedDisplacementFvPatchVectorField.C:315-319) */
-        const double a = o->relaxD;
-        for (int i = 0; i < 3*C; ++i) c->D[i] = c->Dprev[i] + a*(c->D[i] - c->Dprev[i]);
+        if (aitken) {
+            /* relaxField, Aitken (solidModel.C:842-897); aitkenAlpha is a
+               volScalarField and aitkenResidual a volVectorField, so both act
+               per cell and per boundary face:
+                 r = D.prevIter() - D
+                 alpha = a0 (iCorr 0: the D relaxation factor, or 1), else
+                 alpha = clip(alpha (r_prev . (r_prev - r)) / (|r_prev - r|^2 + SMALL), 0, 2)
+                 D -= alpha r
+               Fixed-value patches ignore the subtraction (fixedValue disables
+               its assignment operators); the other patch values take it. */
+            const double a0 = o->relaxD > 0.0 ? o->relaxD : 1.0;
+            for (int i = 0; i < C + c->nB; ++i) {
+                const int bnd = i >= C, b = i - C;
+                if (bnd && (c->bType[b] == OR_EMPTY || c->bType[b] == OR_FIXED)) continue;
+                double *Dv = bnd ? c->Db + 3*b : c->D + 3*i;
+                const double *Dp = bnd ? DbPrev + 3*b : c->Dprev + 3*i;
+                double *rv = bnd ? c->aResB + 3*b : c->aRes + 3*i;
+                double *al = bnd ? c->aAlphaB + b : c->aAlpha + i;
+                double r[3] = {Dp[0] - Dv[0], Dp[1] - Dv[1], Dp[2] - Dv[2]};
+                double alpha = a0;
+                if (c->iCorr != 0) {
+                    double d[3] = {rv[0] - r[0], rv[1] - r[1], rv[2] - r[2]};
+                    alpha = *al*dot3(rv, d)/(dot3(d, d) + OR_SMALL);
+                    alpha = alpha > 2.0 ? 2.0 : (alpha < 0.0 ? 0.0 : alpha);
+                }
+                *al = alpha;
+                for (int q = 0; q < 3; ++q) { rv[q] = r[q]; Dv[q] -= alpha*r[q]; }
+            }
+        } else {
+            /* relaxField, fixed method (solidModel.C:837-841): D.relax(), i.e.
+               D == prevIter + a (D - prevIter) on cells and boundary values */
+            const double a = o->relaxD;
+            for (int i = 0; i < 3*C; ++i) c->D[i] = c->Dprev[i] + a*(c->D[i] - c->Dprev[i]);
+            for (int b = 0; b < c->nB; ++b) {
+                if (c->bType[b] == OR_EMPTY || c->bType[b] == OR_FIXED) continue;
+                for (int q = 0; q < 3; ++q) c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
+            }
+        }
+        /* the snGrads of fixedDisplacement (fixedDisplacementFvPatchVectorField.C:
+           315-319), solidSymmetry and the zero-shear patch are evaluated
+           afterwards from the relaxed cell value */
         for (int b = 0; b < c->nB; ++b) {
             int P = bCell(c, b);
-            if (c->bType[b] == OR_EMPTY) continue;
-            if (c->bType[b] == OR_TRACTION) {
-                for (int q = 0; q < 3; ++q) c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
-            } else if (c->bType[b] == OR_ZERO_SHEAR) {
-                /* boundary values relaxed; the snGrad is evaluated from the
-                   relaxed cell value */
-                double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
+            if (c->bType[b] == OR_EMPTY || c->bType[b] == OR_TRACTION) continue;
+            double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
+            if (c->bType[b] == OR_ZERO_SHEAR) {
                 double v[3]; zero_shear_value(c, b, c->D + 3*P, kG, v);
-                for (int q = 0; q < 3; ++q) {
-                    c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
-                    c->snb[3*b+q] = (v[q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
-                }
+                for (int q = 0; q < 3; ++q) c->snb[3*b+q] = (v[q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
             } else if (c->bType[b] == OR_SYMMETRY) {
-                /* boundary values relaxed; solidSymmetry::snGrad is evaluated
-                   from the relaxed cell value */
                 const double *n = c->bN + 3*b, *DP = c->D + 3*P;
                 double nD = dot3(n, DP);
-                for (int q = 0; q < 3; ++q) {
-                    c->Db[3*b+q] = DbPrev[3*b+q] + a*(c->Db[3*b+q] - DbPrev[3*b+q]);
-                    c->snb[3*b+q] = -n[q]*nD*c->bDelta[b];
-                }
+                for (int q = 0; q < 3; ++q) c->snb[3*b+q] = -n[q]*nD*c->bDelta[b];
             } else {
-                double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
                 for (int q = 0; q < 3; ++q)
                     c->snb[3*b+q] = (c->Db[3*b+q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
             }
@@ -1219,6 +1255,8 @@ static double *field_ptr(or_case *c, const char *name, size_t *n)
     F_("lsP", c->lsP, 3*F) F_("lsN", c->lsN, 3*F) F_("lsB", c->lsB, 3*B)
     F_("impK", c->impK, C) F_("impKf", c->impKf, F) F_("gammaf", c->gammaf, F)
     F_("mu", c->mu, C) F_("lambda", c->lambda, C) F_("K", c->K, C)
+    F_("aitkenAlpha", c->aAlpha, C) F_("aitkenResidual", c->aRes, 3*C)
+    F_("aitkenAlphab", c->aAlphaB, B) F_("aitkenResidualb", c->aResB, 3*B)
     #undef F_
     *n = 0; return NULL;
 }

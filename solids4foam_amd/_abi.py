@@ -25,6 +25,7 @@ PATCH_TRACTION = 1
 PATCH_EMPTY = 2
 PATCH_SYMMETRY = 3
 PATCH_ZERO_SHEAR = 4
+RELAX_FIXED, RELAX_AITKEN = 0, 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECOND_DIAGONAL = 0
@@ -46,7 +47,8 @@ class SolveOpts(ctypes.Structure):
                 ("interfaceScale", ctypes.c_double), ("pcgTolerance", ctypes.c_double),
                 ("pcgRelTol", ctypes.c_double), ("pcgMaxIter", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
                 ("eagerLaunch", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
-                ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double)]
+                ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double),
+                ("relaxationMethod", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

@@ -141,6 +141,7 @@ struct HostMesh {
 
 struct SolveState {
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
+    DArr<double> aAlpha, aRes, aAlphaB, aResB;  // Aitken relaxation state (cells, boundary faces)
     DArr<double> r64, p64, q64, x64, pb64, z64;
     DArr<float> r32, p32, q32, x32, pb32, z32;
     DArr<Ctl> ctl;
@@ -676,6 +677,7 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
     c.nCorr = o.nCorrectors;
     c.histCap = histCap;
     c.relaxD = (o.relaxD > 0.0) ? o.relaxD : 1.0;
+    c.relaxAitken = o.relaxationMethod == S4F_RELAX_AITKEN ? 1 : 0;
     c.solveMask = 7;
     return c;
 }
@@ -695,6 +697,8 @@ void checkOpts(const s4f_solve_opts& o) {
     if (!(o.mgCoarseScale >= 0.0 && o.mgCoarseScale <= 2.0)) throw InvalidError("mgCoarseScale must be in [0, 2]");
     if (!(o.relaxD >= 0.0 && o.relaxD <= 1.0)) throw InvalidError("relaxD must be in [0, 1]");
     if (o.eagerLaunch != 0 && o.eagerLaunch != 1) throw InvalidError("eagerLaunch must be 0 or 1");
+    if (o.relaxationMethod != S4F_RELAX_FIXED && o.relaxationMethod != S4F_RELAX_AITKEN)
+        throw InvalidError("relaxationMethod must be fixed (0) or Aitken (1)");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -1149,6 +1153,7 @@ void launchPcgChunkMg(const MgRun& R, const DevMesh& dm, const BlockMap& bm, Pcg
 // The fields the gradient step reads and writes.
 struct GradBufs {
     double *D0, *D1, *G, *Db, *g, *snb, *Gb, *sigb;
+    double *aAlpha = nullptr, *aRes = nullptr, *aAlphaB = nullptr, *aResB = nullptr;  // Aitken state
 };
 
 // boundary values, cell gradients, boundary gradients and stress (modes as
@@ -1158,10 +1163,13 @@ template <typename T>
 void launchGradStep(const DevMesh& dm, const BlockMap& bm, int nInst, int maxNB, const GradBufs& f, const T* x,
                     const double* Dold, Ctl* ctl, double* part, int mode, hipStream_t s) {
     const dim3 fg(std::max(1, std::min(1024, (maxNB + kBlock - 1) / kBlock)), std::max(1, nInst));
+    if (mode == 0 && f.aAlpha)
+        hipLaunchKernelGGL((k_aitken<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, f.D0, x, f.D1, f.aAlpha,
+                           f.aRes, (const Ctl*)ctl);
     if (mode != 2)
         hipLaunchKernelGGL((k_grad_bnd_pre<T>), fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.D0, x,
                            (const double*)f.D1, (const double*)f.G, f.Db, (const double*)f.g, f.snb,
-                           (const Ctl*)ctl, mode);
+                           (const Ctl*)ctl, f.aAlphaB, f.aResB, mode);
     hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, f.D0, x, f.D1, f.G,
                        (const double*)f.Db, Dold, ctl, part, mode);
     hipLaunchKernelGGL(k_grad_bnd_post, fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.G, (const double*)f.snb,
@@ -1256,6 +1264,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
     for (int i = 0; i < nInst; ++i) {
         c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
+        c0[i].sigma0 = c0[i].warm ? 0 : 1;  // a cold start's stress field is zero until the first update
         c0[i].pcMode = mg ? 1 : 0;
         c0[i].solveMask = M->h.instMask.empty() ? 7 : M->h.instMask[i];
     }
@@ -1267,7 +1276,12 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     const double* Dold = warm ? st.Dold.p : nullptr;
     HIP_CHECK(hipEventRecord(ctx->ev0, s));
     // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
-    const GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+    GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+    if (o.relaxationMethod == S4F_RELAX_AITKEN) {
+        st.aAlpha.alloc(C); st.aRes.alloc(3 * (size_t)C);
+        st.aAlphaB.alloc(std::max(1, h.B)); st.aResB.alloc(3 * (size_t)std::max(1, h.B));
+        gbuf.aAlpha = st.aAlpha.p; gbuf.aRes = st.aRes.p; gbuf.aAlphaB = st.aAlphaB.p; gbuf.aResB = st.aResB.p;
+    }
     launchGradStep<T>(dm, bm, nInst, M->maxInstNB, gbuf, nullptr, Dold, st.ctl.p, st.part.p, 1, s);
     hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
                        st.hist.p, 1);
@@ -2347,6 +2361,7 @@ struct s4f_server {
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
     DArr<float> eA32, diag32, invDiag32;
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
+    DArr<double> aAlpha, aRes, aAlphaB, aResB;  // Aitken relaxation state
     DArr<double> r64, p64, pb64, q64, x64, z64;
     DArr<float> r32, p32, pb32, q32, x32, z32;
     DArr<Ctl> ctl;
@@ -2394,7 +2409,8 @@ long long footprintBytes(const HostMesh& h) {
 
 void serverAllocate(s4f_server* S) {
     // split the budget over cells, ELL entries (7 per cell) and boundary faces (1 per cell)
-    const double perCell = kCellBytes + 7.0 * kEntryBytes + 1.0 * kBndBytes;
+    const bool aitken = S->opts.relaxationMethod == S4F_RELAX_AITKEN;
+    const double perCell = kCellBytes + 7.0 * kEntryBytes + 1.0 * kBndBytes + (aitken ? 64.0 : 0.0);
     long long C = (long long)(S->budget / perCell);
     C = std::max<long long>(C / kSlice * kSlice, kSlice);
     if (C > INT32_MAX / 16) C = INT32_MAX / 16 / kSlice * kSlice;
@@ -2414,6 +2430,9 @@ void serverAllocate(s4f_server* S) {
     S->bLs.alloc(3 * Bc); S->bVal.alloc(3 * Bc); S->bPres.alloc(Bc);
     S->D0.alloc(3 * Cc); S->D1.alloc(3 * Cc); S->Dold.alloc(3 * Cc); S->G.alloc(9 * Cc); S->sig.alloc(6 * Cc);
     S->Db.alloc(3 * Bc); S->g.alloc(3 * Bc); S->snb.alloc(3 * Bc); S->Gb.alloc(9 * Bc); S->sigb.alloc(6 * Bc);
+    if (aitken) {
+        S->aAlpha.alloc(Cc); S->aRes.alloc(3 * Cc); S->aAlphaB.alloc(Bc); S->aResB.alloc(3 * Bc);
+    }
     S->mgOn = S->opts.preconditioner == S4F_PRECOND_GAMG;
     if (S->mgOn) {
         S->mg.clear();
@@ -2621,6 +2640,12 @@ PcgVecs<float> serverVecs<float>(s4f_server* S) {
     return {S->r32.p, S->p32.p, S->q32.p, S->x32.p, S->pb32.p, S->z32.p};
 }
 
+GradBufs serverGradBufs(s4f_server* S) {
+    GradBufs f{S->D0.p, S->D1.p, S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p};
+    f.aAlpha = S->aAlpha.p; f.aRes = S->aRes.p; f.aAlphaB = S->aAlphaB.p; f.aResB = S->aResB.p;  // null unless Aitken
+    return f;
+}
+
 int serverMaxNB(const s4f_server* S) { return *std::max_element(S->hiNB.begin(), S->hiNB.end()); }
 
 BlockMap makeMap(const s4f_server* S, const int* inst, const int* s0, const int* s1, const int* fin, int nBlocks) {
@@ -2666,6 +2691,7 @@ void serverAdmit(s4f_server* S) {
         c.solveMask = rq.mesh->h.instMask.empty() ? 7 : rq.mesh->h.instMask[0];
         c.pcMode = S->mgOn ? 1 : 0;
         c.warm = warm;
+        c.sigma0 = warm ? 0 : 1;
         HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
         HIP_CHECK(hipMemcpy(S->ctl.p + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice));
         const int n = (int)pl.nCells;
@@ -2695,7 +2721,7 @@ void serverAdmit(s4f_server* S) {
     const BlockMap bm = makeMap(S, S->nbmInst.p, S->nbmS0.p, S->nbmS1.p, S->nfinList.p, (int)inst.size());
     const DevMesh dm = serverDm(S);
     launchGradStep<T>(dm, bm, (int)newSlots.size(), serverMaxNB(S),
-                      GradBufs{S->D0.p, S->D1.p, S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p}, nullptr,
+                      serverGradBufs(S), nullptr,
                       (const double*)S->Dold.p, S->ctl.p, S->part.p, 1, s);
     hipLaunchKernelGGL(k_fin_grad, dim3((int)newSlots.size()), dim3(kFinBlock), 0, s, bm, S->ctl.p,
                        (const double*)S->part.p, (double*)nullptr, 1);
@@ -2917,7 +2943,7 @@ void serverStep(s4f_server* S) {
         if (!S->hctl[sl].outerDone) used = std::max({used, S->hctl[sl].nIter[0], S->hctl[sl].nIter[1], S->hctl[sl].nIter[2]});
     S->chunk = std::max(1, used - 1);
     launchGradStep<T>(dm, bm, nA, serverMaxNB(S),
-                      GradBufs{S->D0.p, S->D1.p, S->G.p, S->Db.p, S->g.p, S->snb.p, S->Gb.p, S->sigb.p},
+                      serverGradBufs(S),
                       (const T*)v.x, (const double*)S->Dold.p, S->ctl.p, S->part.p, 0, s);
     hipLaunchKernelGGL(k_fin_grad, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p,
                        (double*)nullptr, 0);

@@ -62,7 +62,8 @@ struct DevMesh {
 struct Ctl {
     int outerDone, innerDone, converged, diverged;
     int iCorr, nOuter, firstInner, warm;
-    int dpar, pad0;  // which displacement buffer holds the current field
+    int dpar;        // which displacement buffer holds the current field
+    int sigma0;      // cold start: the stress field is zero until the first outer iteration ends
     int active[4];
     int nIter[4];
     double initRes[4], normFactor[4], finalRes[4];
@@ -75,7 +76,8 @@ struct Ctl {
     double pcgTol, pcgRelTol, solTol, altTol;
     int pcgMaxIter, nCorr, histCap, pcMode;  // pcMode 1: z comes from the V-cycle (k_fin_rho sets beta)
     double relaxD;                           // fixed field under-relaxation factor (1: none)
-    int solveMask, pad1;                     // solved components (bit j); 2-D leaves the empty axis out
+    int solveMask;                           // solved components (bit j); 2-D leaves the empty axis out
+    int relaxAitken;                         // relaxationMethod Aitken (relaxD: its first factor)
 };
 
 }  // namespace s4f

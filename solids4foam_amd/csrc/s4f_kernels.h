@@ -156,8 +156,8 @@ template <> struct MatSel<float> {
 __device__ __forceinline__ void residualInternal(const DevMesh& m, int P, int e0, int W, int lane, int C,
                                                   const double* __restrict__ D, const double* __restrict__ G,
                                                   const double* DP, const double* GP, const double* CP,
-                                                  const double* sP, double* div, double* rc, double* off,
-                                                  double& asum, bool& hasB) {
+                                                  const double* sP, double sf, double* div, double* rc,
+                                                  double* off, double& asum, bool& hasB) {
     const size_t E = m.nEntries;
 #pragma unroll 1
     for (int k = 0; k < W; ++k) {
@@ -175,13 +175,13 @@ __device__ __forceinline__ void residualInternal(const DevMesh& m, int P, int e0
         for (int q = 0; q < 9; ++q) GN[q] = G[(size_t)q * C + c];
 #pragma unroll
         for (int q = 0; q < 3; ++q) { DN[q] = D[(size_t)q * C + c]; d[q] = m.cC[(size_t)q * C + c] - CP[q]; }
-        hooke(GN, cellMu(m, c), cellK(m, c), sN);
-        double sf[6];
+        hooke(GN, cellMu(m, c) * sf, cellK(m, c) * sf, sN);
+        double sfc[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sf[q] = w * (sP[q] - sN[q]) + sN[q];
-        div[0] += S0 * sf[0] + S1 * sf[1] + S2 * sf[2];
-        div[1] += S0 * sf[1] + S1 * sf[3] + S2 * sf[4];
-        div[2] += S0 * sf[2] + S1 * sf[4] + S2 * sf[5];
+        for (int q = 0; q < 6; ++q) sfc[q] = w * (sP[q] - sN[q]) + sN[q];
+        div[0] += S0 * sfc[0] + S1 * sfc[1] + S2 * sfc[2];
+        div[1] += S0 * sfc[1] + S1 * sfc[3] + S2 * sfc[4];
+        div[2] += S0 * sfc[2] + S1 * sfc[4] + S2 * sfc[5];
         double Gf[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) Gf[q] = w * (GP[q] - GN[q]) + GN[q];
@@ -212,6 +212,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
     const double* __restrict__ D = ctl->dpar ? D1 : D0;
     const int C = m.nCells;
     const double xb0 = ctl->xbar[0], xb1 = ctl->xbar[1], xb2 = ctl->xbar[2];
+    // a cold start's first residual sees the zero initial stress field (the
+    // constructor computes grad(D) but no sigma, linGeomTotalDispSolid.C:64-84)
+    const double sf = (ctl->sigma0 && ctl->iCorr == 0) ? 0.0 : 1.0;
     double acc[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) acc[i] = 0.0;
@@ -229,11 +232,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
         for (int k = 0; k < 3; ++k) { DP[k] = D[(size_t)k * C + P]; CP[k] = m.cC[(size_t)k * C + P]; }
 #pragma unroll
         for (int k = 0; k < 9; ++k) GP[k] = G[(size_t)k * C + P];
-        const double muP = cellMu(m, P), KP = cellK(m, P);
+        const double muP = cellMu(m, P) * sf, KP = cellK(m, P) * sf;
         hooke(GP, muP, KP, sP);
         double div[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, off[3] = {0, 0, 0}, asum = 0.0;
         bool hasB = false;
-        residualInternal(m, P, e0, width, lane, C, D, G, DP, GP, CP, sP, div, rc, off, asum, hasB);
+        residualInternal(m, P, e0, width, lane, C, D, G, DP, GP, CP, sP, sf, div, rc, off, asum, hasB);
         for (int k = 0; hasB && k < width; ++k) {
             const int e = e0 + k * kSlice + lane;
             const int n = m.col[e];
@@ -494,20 +497,81 @@ __device__ __forceinline__ int faceInst(const BlockMap& bm) {
     return bm.finInst ? bm.finInst[blockIdx.y] : (int)blockIdx.y;
 }
 
+// Aitken factor update of relaxField (solidModel.C:876-892) at one cell or
+// boundary face: r = D_prev - D*, alpha = a0 on the first outer iteration,
+// else clip(alpha (r_prev . (r_prev - r)) / (|r_prev - r|^2 + SMALL), 0, 2);
+// returns alpha and stores it with r (the next r_prev)
+__device__ __forceinline__ double aitkenFactor(const double* r, double* alpha, double* res, size_t i, size_t n,
+                                               bool first, double a0) {
+    double al = a0;
+    if (!first) {
+        const double rp[3] = {res[i], res[n + i], res[2 * n + i]};
+        const double d[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+        al = alpha[i] * (rp[0] * d[0] + rp[1] * d[1] + rp[2] * d[2]) / ((d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + kSmall);
+        al = al > 2.0 ? 2.0 : (al < 0.0 ? 0.0 : al);
+    }
+    alpha[i] = al;
+    res[i] = r[0];
+    res[n + i] = r[1];
+    res[2 * n + i] = r[2];
+    return al;
+}
+
+// relaxField, Aitken (solidModel.C:842-897), on the cells: D' = D* - alpha
+// (D_prev - D*) with D* = D + x the solved field, into the buffer receiving D'
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_aitken(const DevMesh m, const BlockMap bm, double* __restrict__ D0,
+                                                   const T* __restrict__ x, double* __restrict__ D1,
+                                                   double* __restrict__ alpha, double* __restrict__ res,
+                                                   const Ctl* __restrict__ ctlAll) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const Ctl* ctl = ctlAll + inst;
+    if (ctl->outerDone) return;
+    if (ctl->firstInner) x = nullptr;
+    const double* D = ctl->dpar ? D1 : D0;
+    double* Dn = ctl->dpar ? D0 : D1;
+    const bool first = ctl->iCorr == 0;
+    const double a0 = ctl->relaxD;
+    const int C = m.nCells;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        double Ds[3], r[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double DP = D[(size_t)k * C + P];
+            Ds[k] = DP + (x ? (double)x[(size_t)k * C + P] : 0.0);
+            r[k] = DP - Ds[k];
+        }
+        const double al = aitkenFactor(r, alpha, res, P, C, first, a0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Dn[(size_t)k * C + P] = Ds[k] - al * r[k];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const BlockMap bm,
                                                          const double* __restrict__ D0, const T* __restrict__ x,
                                                          const double* __restrict__ D1, const double* __restrict__ G,
                                                          double* __restrict__ Db, const double* __restrict__ g,
                                                          double* __restrict__ snb, const Ctl* __restrict__ ctlAll,
+                                                         double* __restrict__ alphaB, double* __restrict__ resB,
                                                          int mode) {
     const int inst = faceInst(bm);
     const Ctl* ctl = ctlAll + inst;
     if (mode == 0 && ctl->outerDone) return;
     if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
     const double* __restrict__ D = ctl->dpar ? D1 : D0;
-    const bool relax = mode == 0 && ctl->relaxD != 1.0;
+    // Aitken: the relaxed cell values are in the buffer receiving D' (k_aitken)
+    const bool aitken = mode == 0 && ctl->relaxAitken;
+    const double* __restrict__ Dr = ctl->dpar ? D0 : D1;
+    const bool relax = mode == 0 && !aitken && ctl->relaxD != 1.0;
     const double ra = relax ? ctl->relaxD : 1.0;
+    const bool first = ctl->iCorr == 0;
     const int C = m.nCells, B = m.nBoundary;
     const int b0 = bm.iB0[inst], b1 = b0 + bm.iNB[inst];
     for (int b = b0 + blockIdx.x * kBlock + threadIdx.x; b < b1; b += gridDim.x * kBlock) {
@@ -522,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const 
             const double DP = D[(size_t)k * C + P];
             const double xv = x ? (double)x[(size_t)k * C + P] : 0.0;
             DuP[k] = DP + xv;
-            DnP[k] = DP + ra * xv;
+            DnP[k] = aitken ? Dr[(size_t)k * C + P] : DP + ra * xv;
         }
 #pragma unroll
         for (int q = 0; q < 9; ++q) Gold[q] = (mode == 1) ? 0.0 : G[(size_t)q * C + P];
@@ -564,38 +628,44 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_pre(const DevMesh m, const 
                 zn[j] = nbv[j] * nD + (gn[j] - nbv[j] * ngn);
             }
         }
+        // correctBoundaryConditions after the solve, then relaxField on the
+        // patch values (fixed values stay); the snGrads follow the relaxed cell
+        double Dbv[3], sn[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const double kG = kGv[j];
-            double Dbv, sn;
             if (zs) {
-                Dbv = zu[j];
-                if (relax) {
-                    const double Dbp = Db[(size_t)j * B + b];
-                    Dbv = Dbp + ra * (Dbv - Dbp);
-                }
-                sn = (zn[j] - (DnP[j] + kG)) * dl;
+                Dbv[j] = zu[j];
+                sn[j] = (zn[j] - (DnP[j] + kG)) * dl;
             } else if (sym) {
-                Dbv = DuP[j] - nbv[j] * nDu;
-                if (relax) {
-                    const double Dbp = Db[(size_t)j * B + b];
-                    Dbv = Dbp + ra * (Dbv - Dbp);
-                }
-                sn = -nbv[j] * nDn * dl;
+                Dbv[j] = DuP[j] - nbv[j] * nDu;
+                sn[j] = -nbv[j] * nDn * dl;
             } else if (trac) {
-                const double gv = gvv[j];
-                Dbv = DuP[j] + kG + gv / dl;  // correctBoundaryConditions after the solve
-                if (relax) {                  // D.relax() on the boundary values too
-                    const double Dbp = Db[(size_t)j * B + b];
-                    Dbv = Dbp + ra * (Dbv - Dbp);
-                }
-                sn = gv;
+                Dbv[j] = DuP[j] + kG + gvv[j] / dl;
+                sn[j] = gvv[j];
             } else {
-                Dbv = m.bVal[(size_t)j * B + b];
-                sn = (Dbv - (DnP[j] + kG)) * dl;
+                Dbv[j] = m.bVal[(size_t)j * B + b];
+                sn[j] = (Dbv[j] - (DnP[j] + kG)) * dl;
             }
-            Db[(size_t)j * B + b] = Dbv;
-            snb[(size_t)j * B + b] = sn;
+        }
+        if (type != 0 && (relax || aitken)) {
+            double Dbp[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Dbp[j] = Db[(size_t)j * B + b];
+            if (aitken) {
+                const double r[3] = {Dbp[0] - Dbv[0], Dbp[1] - Dbv[1], Dbp[2] - Dbv[2]};
+                const double al = aitkenFactor(r, alphaB, resB, b, B, first, ctl->relaxD);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Dbv[j] -= al * r[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Dbv[j] = Dbp[j] + ra * (Dbv[j] - Dbp[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Db[(size_t)j * B + b] = Dbv[j];
+            snb[(size_t)j * B + b] = sn[j];
         }
     }
 }
@@ -613,9 +683,14 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     if (mode == 0 && ctl->firstInner) x = nullptr;  // no PCG iteration ran: the update is zero
     // current field and the buffer receiving D' (per-instance ping-pong)
     const double* __restrict__ D = ctl->dpar ? D1 : D0;
-    const bool relax = mode == 0 && ctl->relaxD != 1.0;
+    double* Dn = (mode == 0) ? (ctl->dpar ? D0 : D1) : nullptr;
+    // D' = D + a x (fixed relaxation, a = 1 without), or the Aitken-relaxed
+    // field k_aitken already stored in Dn
+    const bool aitken = mode == 0 && ctl->relaxAitken;
+    const bool relax = mode == 0 && !aitken && ctl->relaxD != 1.0;
     const double ra = relax ? ctl->relaxD : 1.0;  // x scaled by exactly 1.0 without relaxation
-    double* __restrict__ Dn = (mode == 0) ? (ctl->dpar ? D0 : D1) : nullptr;
+    const double* Dg = aitken ? Dn : D;
+    const T* xg = aitken ? nullptr : x;
     const int C = m.nCells, B = m.nBoundary;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // max|D'-D|, max|D'-Dold|, max|D'|, sum D'
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -630,8 +705,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             DP[k] = D[(size_t)k * C + P];
-            const double xv = x ? (double)x[(size_t)k * C + P] : 0.0;
-            DnP[k] = DP[k] + ra * xv;
+            const double xv = xg ? (double)xg[(size_t)k * C + P] : 0.0;
+            DnP[k] = aitken ? Dg[(size_t)k * C + P] : DP[k] + ra * xv;
             CP[k] = m.cC[(size_t)k * C + P];
         }
         double A[9];
@@ -640,12 +715,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         bool hasB = false;
         int kInt = width;  // internal slots left to the general loop below
         if (width == 6) {
-            if (x) gradInternalW<T, 6, true>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
-            else gradInternalW<T, 6, false>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
+            if (xg) gradInternalW<T, 6, true>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
+            else gradInternalW<T, 6, false>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
             kInt = 0;
         } else if (width == 4) {
-            if (x) gradInternalW<T, 4, true>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
-            else gradInternalW<T, 4, false>(m, P, e0, lane, C, D, x, ra, DnP, CP, A, hasB);
+            if (xg) gradInternalW<T, 4, true>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
+            else gradInternalW<T, 4, false>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
             kInt = 0;
         } else {
             hasB = true;
@@ -659,7 +734,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     d[q] = m.cC[(size_t)q * C + n] - CP[q];
-                    const double DN = x ? D[(size_t)q * C + n] + ra * (double)x[(size_t)q * C + n] : D[(size_t)q * C + n];
+                    const double DN =
+                        xg ? Dg[(size_t)q * C + n] + ra * (double)xg[(size_t)q * C + n] : Dg[(size_t)q * C + n];
                     dv[q] = DN - DnP[q];
                 }
                 const double rr = 1.0 / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -689,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
             G[(size_t)(3 + j) * C + P] = M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j];
             G[(size_t)(6 + j) * C + P] = M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j];
         }
-        if (Dn) {
+        if (Dn && !aitken) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) Dn[(size_t)q * C + P] = DnP[q];
         }
@@ -740,6 +816,10 @@ __global__ __launch_bounds__(kBlock) void k_grad_bnd_post(const DevMesh m, const
             for (int i = 0; i < 3; ++i) Gbv[3 * i + j] = Gn[3 * i + j] + nv[i] * (sn - nG);
         }
         hooke(Gbv, cellMu(m, P), cellK(m, P), sb);
+        if (mode == 1 && ctl->sigma0) {  // cold start: the stress field is still zero
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sb[q] = 0.0;
+        }
         if (Gb) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) Gb[(size_t)q * B + b] = Gbv[q];

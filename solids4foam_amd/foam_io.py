@@ -390,8 +390,10 @@ def read_solid_properties(case_dir):
     stab = coeffs.get("stabilisation", {"type": "RhieChow", "scaleFactor": 0.1})
     if stab.get("type", "RhieChow") != "RhieChow":
         raise ValueError("only the RhieChow stabilisation is on the device path")
-    if str(coeffs.get("relaxationMethod", "fixed")) != "fixed":
-        raise ValueError("field relaxation is not on the device path")
+    # solidModel::relaxField (solidModel.C:823-906): fixed or Aitken
+    method = str(coeffs.get("relaxationMethod", "fixed"))
+    if method not in ("fixed", "Aitken"):
+        raise ValueError(f"relaxationMethod {method!r} is not on the device path (fixed, Aitken)")
     if str(coeffs.get("predictor", "no")) in ("yes", "on", "true"):
         raise ValueError("the predictor is not on the device path")
     opts = dict(nCorrectors=int(coeffs.get("nCorrectors", 10000)),
@@ -401,15 +403,26 @@ def read_solid_properties(case_dir):
                 interfaceScale=float(stab.get("interfaceScaleFactor", 0.01)))
     prec = coeffs.get("precision", "fp64")
     opts["precision"] = 1 if prec == "mixed" else 0
+    opts["relaxationMethod"] = 1 if method == "Aitken" else 0
     return model, opts
 
 
 def read_solver_controls(case_dir, field="D"):
-    """fvSolution solvers entry for D (PCG tolerance / relTol / maxIter)"""
+    """fvSolution solvers entry for D (PCG tolerance / relTol / maxIter) and,
+    when present, the field relaxation factor of D (relaxationFactors
+    { fields { D a; } } or the older flat relaxationFactors { D a; })"""
     d = read_dict(os.path.join(case_dir, "system", "fvSolution"))
     s = lookup(d.get("solvers", {}), field, {}) or {}
-    return dict(pcgTolerance=float(s.get("tolerance", 1e-6)), pcgRelTol=float(s.get("relTol", 0.0)),
-                pcgMaxIter=int(s.get("maxIter", 1000)))
+    out = dict(pcgTolerance=float(s.get("tolerance", 1e-6)), pcgRelTol=float(s.get("relTol", 0.0)),
+               pcgMaxIter=int(s.get("maxIter", 1000)))
+    rf = d.get("relaxationFactors", {}) or {}
+    fields = rf.get("fields", None)
+    a = lookup(fields, field, None) if isinstance(fields, dict) else None
+    if a is None and isinstance(rf.get(field, None), (int, float, str)):
+        a = rf[field]
+    if a is not None:
+        out["relaxD"] = float(a)
+    return out
 
 
 def write_vol_field(path, name, cls, values, boundary=None, dims="[0 1 0 0 0 0 0]"):

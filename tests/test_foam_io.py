@@ -83,6 +83,22 @@ def _box_case(tmp, n=(5, 4, 3), jitter=0.1):
     return pm, a
 
 
+def test_relaxation_settings(tmp_path):
+    # relaxationMethod (solidProperties) and the D field factor (fvSolution)
+    _write(str(tmp_path), "constant/solidProperties", SOLID_PROPS.replace("nCorrectors     100000;",
+                                                                           "nCorrectors 100000; relaxationMethod Aitken;"))
+    _write(str(tmp_path), "system/fvSolution", FV_SOLUTION + "relaxationFactors { equations { } fields { D 0.7; } }\n")
+    _, opts = foam_io.read_solid_properties(str(tmp_path))
+    assert opts["relaxationMethod"] == 1
+    assert foam_io.read_solver_controls(str(tmp_path))["relaxD"] == 0.7
+    _write(str(tmp_path), "system/fvSolution", FV_SOLUTION + "relaxationFactors { D 0.5; }\n")
+    assert foam_io.read_solver_controls(str(tmp_path))["relaxD"] == 0.5
+    _write(str(tmp_path), "constant/solidProperties", SOLID_PROPS.replace("nCorrectors     100000;",
+                                                                           "relaxationMethod QuasiNewton;"))
+    with pytest.raises(ValueError, match="relaxationMethod"):
+        foam_io.read_solid_properties(str(tmp_path))
+
+
 def test_parse_dictionaries(tmp_path):
     _write(str(tmp_path), "constant/solidProperties", SOLID_PROPS)
     _write(str(tmp_path), "constant/mechanicalProperties", MECH)
@@ -90,7 +106,7 @@ def test_parse_dictionaries(tmp_path):
     model, opts = foam_io.read_solid_properties(str(tmp_path))
     assert model == "linearGeometryTotalDisplacement"
     assert opts["nCorrectors"] == 100000 and opts["solutionTolerance"] == 1e-7
-    assert opts["alternativeTolerance"] == 1e-7 and opts["rhieChowScale"] == 0.2
+    assert opts["alternativeTolerance"] == 1e-7 and opts["rhieChowScale"] == 0.2 and opts["relaxationMethod"] == 0
     assert foam_io.read_linear_elastic(str(tmp_path)) == (200e9, 0.3)
     sc = foam_io.read_solver_controls(str(tmp_path))
     assert sc == dict(pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)

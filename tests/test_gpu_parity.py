@@ -220,6 +220,63 @@ def test_field_relaxation_follows_oracle(ctx, oracle_mod):
         assert err < (1e-8 if precision == 0 else 1e-6)
 
 
+def _moved_box(ctx, oracle_mod, n, nu):
+    # x-min fixed, x-max moved by a fixed displacement, other faces free
+    from solids4foam_amd import PolyMesh, SolverMesh
+    types = [FIXED, FIXED] + [TRACTION] * 4
+    om = oracle_mod.Mesh.box(n, n, n, (0.1, 0.1, 0.1), 0.1, 3)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(1)] = (1e-5, 5e-6, 0.0)
+    oc = oracle_mod.Case(om, 100e9, nu, types, bv)
+    pm = PolyMesh.box(n, n, n, L=(0.1, 0.1, 0.1), jitter=0.1, seed=3)
+    m = SolverMesh(ctx, pm, types)
+    m.set_material(100e9, nu)
+    for p in range(6):
+        m.set_bc(p, bv[pm.patch_faces(p)])
+    return oc, m
+
+
+def test_aitken_relaxation_follows_oracle(ctx, oracle_mod):
+    # relaxationMethod Aitken (per cell and boundary face factors): the first
+    # outer iterations reproduce the oracle's history for a displaced and a
+    # loaded box, both converge to the oracle's field, and a batch of the two
+    # gives each the bits it has alone
+    from solids4foam_amd import _abi
+    from solids4foam_amd.batch import Batch
+    AIT = _abi.RELAX_AITKEN
+    boxes = {"moved": _moved_box(ctx, oracle_mod, 6, 0.3), "loaded": _nu_box(ctx, oracle_mod, 6, 0.3)}
+    for name, (oc, m) in boxes.items():
+        ro = oc.solve(history=10, nCorr=8, precond=oracle_mod.PRECOND_DIAGONAL, relaxAitken=1)
+        m.solve(nCorrectors=8, relaxationMethod=AIT)
+        h = m.history()
+        assert np.allclose(h["initRes"][:8], ro["history"]["initRes"][:8], rtol=1e-8, atol=1e-14), name
+        assert np.allclose(h["rvf"][:8], ro["history"]["rvf"][:8], rtol=1e-8), name
+    for name, (oc, m) in (("moved", _moved_box(ctx, oracle_mod, 6, 0.3)), ("loaded", _nu_box(ctx, oracle_mod, 6, 0.3))):
+        ro = oc.solve(precond=oracle_mod.PRECOND_DIC, relaxAitken=1, solTol=1e-13, altTol=1e-13, nCorr=100000,
+                      pcgTol=1e-15)
+        Do = oc.get("D").reshape(-1, 3)
+        for precision in (0, 1):
+            out = m.solve(nCorrectors=100000, solutionTolerance=1e-13, alternativeTolerance=1e-13,
+                          relaxationMethod=AIT, precision=precision, pcgTolerance=1e-15)
+            err = rel_l2(out["D"], Do)
+            print(f"  Aitken {name} precision={precision}: nOuter={out['nOuter']} (oracle {ro['nOuter']}) "
+                  f"err={err:.2e}", flush=True)
+            assert ro["converged"] and out["converged"]
+            assert err < (1e-8 if precision == 0 else 1e-6)
+    opts = dict(nCorrectors=100000, solutionTolerance=1e-10, alternativeTolerance=1e-10, relaxationMethod=AIT,
+                precision=1)
+    ms = [boxes["moved"][1], boxes["loaded"][1]]
+    alone = [mm.solve(**opts) for mm in ms]
+    b = Batch(ctx)
+    for mm in ms:
+        b.add(mm)
+    res = b.solve(**opts)
+    b.close()
+    for a, r in zip(alone, res):
+        assert r["converged"] and r["nOuter"] == a["nOuter"]
+        assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
+
+
 def _eps_eq(e):
     """sqrt(2/3 |dev e|^2) of symmetric tensors (n, 6: xx xy xz yy yz zz)"""
     tr = (e[:, 0] + e[:, 3] + e[:, 5]) / 3.0

@@ -471,3 +471,59 @@ def test_zero_shear_prescribed_normal_displacement(oracle_mod):
     m2, c2, r2, _, _ = _zero_shear_box(oracle_mod, 0.15, ("zs", (0.3 * delta, -0.2 * delta, delta)), relax=0.6)
     assert r2["converged"]
     assert np.abs(c2.get("D").reshape(-1, 3) - exact).max() < 1e-8 * delta
+
+
+def _aitken_case(oracle_mod, driven):
+    # 6^3 jittered box, x-min fixed; x-max either moved (fixed displacement)
+    # or loaded (traction); other faces traction free
+    m = oracle_mod.Mesh.box(6, 6, 6, (0.1, 0.1, 0.1), 0.1, 3)
+    bv = np.zeros((m.nBoundaryFaces, 3))
+    if driven == "displacement":
+        bv[m.patch_faces(1)] = (1e-5, 5e-6, 0.0)
+        types = [FIXED, FIXED] + [TRACTION] * 4
+    else:
+        bv[m.patch_faces(1)] = (1e6, 5e5, 0.0)
+        types = [FIXED] + [TRACTION] * 5
+    return m, oracle_mod.Case(m, 100e9, 0.3, types, bv)
+
+
+def test_aitken_first_iteration_extrapolates(oracle_mod):
+    # relaxField Aitken subtracts alpha (D_prev - D) from the solved field:
+    # on the first iteration (alpha = the D relaxation factor, or 1) the
+    # relaxed field is D* + a0 (D* - D_prev), with D_prev = 0 here: (1 + a0) D*
+    out = {}
+    for key, kw in (("plain", {}), ("a1", dict(relaxAitken=1)), ("a05", dict(relaxAitken=1, relaxD=0.5))):
+        m, c = _aitken_case(oracle_mod, "displacement")
+        c.solve(nCorr=1, pcgTol=1e-14, pcgRelTol=0.0, precond=oracle_mod.PRECOND_DIC, **kw)
+        out[key] = (c.get("D"), c.get("aitkenAlpha"))
+    D = out["plain"][0]
+    assert np.abs(D).max() > 0
+    assert np.abs(out["a1"][0] - 2.0 * D).max() <= 1e-14 * np.abs(D).max()
+    assert np.abs(out["a05"][0] - 1.5 * D).max() <= 1e-14 * np.abs(D).max()
+    assert np.all(out["a1"][1] == 1.0) and np.all(out["a05"][1] == 0.5)
+
+
+@pytest.mark.parametrize("driven", ["displacement", "traction"])
+def test_aitken_reaches_the_fixed_point(oracle_mod, driven):
+    # the relaxation changes the path, not the converged field; the factors
+    # stay within the clip [0, 2]
+    m, c = _aitken_case(oracle_mod, driven)
+    r0 = c.solve(nCorr=20000, solTol=1e-11, altTol=1e-11, pcgTol=1e-13, precond=oracle_mod.PRECOND_DIC)
+    m, ca = _aitken_case(oracle_mod, driven)
+    ra = ca.solve(nCorr=20000, solTol=1e-11, altTol=1e-11, pcgTol=1e-13, precond=oracle_mod.PRECOND_DIC,
+                  relaxAitken=1)
+    assert r0["converged"] and ra["converged"]
+    D, Da = c.get("D"), ca.get("D")
+    assert np.linalg.norm(Da - D) / np.linalg.norm(D) < 1e-8
+    a, ab = ca.get("aitkenAlpha"), ca.get("aitkenAlphab")
+    assert a.min() >= 0.0 and a.max() <= 2.0 and ab.min() >= 0.0 and ab.max() <= 2.0
+    if driven == "traction":
+        # the first solve leaves the cells at rest (the load reaches them
+        # through the boundary values), so r_prev = 0 at the second iteration
+        # and the multiplicative update keeps the cell factors at zero; the
+        # fixed patch's values are never relaxed (its factor is not set)
+        assert np.all(a == 0.0)
+        m, c1 = _aitken_case(oracle_mod, driven)
+        c1.solve(nCorr=1, pcgTol=1e-13, precond=oracle_mod.PRECOND_DIC, relaxAitken=1)
+        ab1 = c1.get("aitkenAlphab")
+        assert np.all(ab1[m.patch_faces(0)] == 0.0) and np.all(ab1[m.patch_faces(1)] == 1.0)
