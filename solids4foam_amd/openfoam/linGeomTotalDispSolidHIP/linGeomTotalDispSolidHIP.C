@@ -8,6 +8,7 @@
 #include "linearElastic.H"
 #include "solidTractionFvPatchVectorField.H"
 #include "fixedDisplacementFvPatchVectorField.H"
+#include "directionMixedFvPatchFields.H"
 
 #include <vector>
 
@@ -43,19 +44,38 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
     forAll(D().boundaryField(), patchI)
     {
         const word t = D().boundaryField()[patchI].type();
-        if (t == fixedDisplacementFvPatchVectorField::typeName)
+        if
+        (
+            t == fixedDisplacementFvPatchVectorField::typeName
+         || t == "linearSpatialDisplacement"
+        )
         {
+            // linearSpatialDisplacement: its evaluated values are uploaded
             patchType_[patchI] = S4F_PATCH_FIXED_DISPLACEMENT;
         }
         else if (t == solidTractionFvPatchVectorField::typeName)
         {
             patchType_[patchI] = S4F_PATCH_TRACTION;
         }
+        else if (t == "solidSymmetry")
+        {
+            patchType_[patchI] = S4F_PATCH_SYMMETRY;
+        }
+        else if (t == "fixedDisplacementZeroShear")
+        {
+            patchType_[patchI] = S4F_PATCH_ZERO_SHEAR;
+        }
+        else if (t == "empty")
+        {
+            patchType_[patchI] = S4F_PATCH_EMPTY;
+        }
         else
         {
             FatalErrorIn(type() + "::createDeviceMesh()")
                 << "patch " << m.boundary()[patchI].name() << " has type " << t
-                << "; the device path supports fixedDisplacement and solidTraction"
+                << "; the device path supports fixedDisplacement, "
+                << "linearSpatialDisplacement, solidTraction, solidSymmetry, "
+                << "fixedDisplacementZeroShear and empty"
                 << abort(FatalError);
         }
     }
@@ -77,19 +97,20 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
             Cf[3*faceI + d] = m.Cf()[faceI][d];
         }
     }
-    forAll(m.boundary(), patchI)
+    // patches from the polyMesh (an empty fvPatch has size 0, its faces exist)
+    forAll(m.boundaryMesh(), patchI)
     {
-        const fvPatch& p = m.boundary()[patchI];
-        pStart.push_back(p.start());
-        pSize.push_back(p.size());
+        const polyPatch& pp = m.boundaryMesh()[patchI];
+        pStart.push_back(pp.start());
+        pSize.push_back(pp.size());
         pType.push_back(patchType_[patchI]);
-        forAll(p, i)
+        for (label i = 0; i < pp.size(); ++i)
         {
-            const label faceI = p.start() + i;
+            const label faceI = pp.start() + i;
             for (direction d = 0; d < 3; ++d)
             {
-                Sf[3*faceI + d] = m.Sf().boundaryField()[patchI][i][d];
-                Cf[3*faceI + d] = m.Cf().boundaryField()[patchI][i][d];
+                Sf[3*faceI + d] = m.faceAreas()[faceI][d];
+                Cf[3*faceI + d] = m.faceCentres()[faceI][d];
             }
         }
     }
@@ -133,6 +154,10 @@ void linGeomTotalDispSolidHIP::uploadBoundaryValues()
     const fvMesh& m = mesh();
     forAll(m.boundary(), patchI)
     {
+        if (patchType_[patchI] == S4F_PATCH_EMPTY)
+        {
+            continue;  // empty faces carry no values
+        }
         const label n = m.boundary()[patchI].size();
         std::vector<double> v(3*n), p(n, 0.0);
         if (patchType_[patchI] == S4F_PATCH_TRACTION)
@@ -143,6 +168,16 @@ void linGeomTotalDispSolidHIP::uploadBoundaryValues()
             {
                 for (direction d = 0; d < 3; ++d) { v[3*i + d] = bc.traction()[i][d]; }
                 p[i] = bc.pressure()[i];
+            }
+        }
+        else if (patchType_[patchI] == S4F_PATCH_ZERO_SHEAR)
+        {
+            // the prescribed displacement is the direction-mixed refValue
+            const directionMixedFvPatchVectorField& bc =
+                refCast<const directionMixedFvPatchVectorField>(D().boundaryField()[patchI]);
+            forAll(bc, i)
+            {
+                for (direction d = 0; d < 3; ++d) { v[3*i + d] = bc.refValue()[i][d]; }
             }
         }
         else
@@ -182,6 +217,23 @@ s4f_solve_opts linGeomTotalDispSolidHIP::solveOptions() const
     o.precision =
         word(solidModelDict().lookupOrDefault<word>("precision", "fp64")) == "mixed"
       ? S4F_PRECISION_MIXED : S4F_PRECISION_FP64;
+    // solidModel::relaxField (solidModel.C:823-906): fixed or Aitken, with
+    // the fvSolution field relaxation factor of D
+    const word method
+    (
+        solidModelDict().lookupOrDefault<word>("relaxationMethod", "fixed")
+    );
+    if (method != "fixed" && method != "Aitken")
+    {
+        FatalErrorIn(type() + "::solveOptions()")
+            << "relaxationMethod " << method << " is not on the device path"
+            << abort(FatalError);
+    }
+    o.relaxationMethod = method == "Aitken" ? S4F_RELAX_AITKEN : S4F_RELAX_FIXED;
+    if (mesh().relaxField(D().name()))
+    {
+        o.relaxD = mesh().fieldRelaxationFactor(D().name());
+    }
     o.warmStart = 1;
     return o;
 }
