@@ -64,6 +64,9 @@ enum {
 /* field relaxation method */
 enum { S4F_RELAX_FIXED = 0, S4F_RELAX_AITKEN = 1 };
 
+/* d2dt2Schemes of rho*fvm::d2dt2(D) (linGeomTotalDispSolid.C:143) */
+enum { S4F_D2DT2_STEADY = 0, S4F_D2DT2_EULER = 1 };
+
 /* precision of the inner linear solve */
 enum {
     S4F_PRECISION_FP64 = 0,  /* plain path: everything FP64 */
@@ -155,6 +158,20 @@ s4f_status s4f_mesh_info(const s4f_mesh *mesh, int64_t out[6]);
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
 s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);
+
+/* Density (mechanicalProperties rho, linearElastic.C:62-104): perCell ? nCells
+   values : one.  Needed for the inertia term and the body force. */
+s4f_status s4f_density_set(s4f_mesh *mesh, const double *rho, int32_t perCell);
+
+/* Gravitational acceleration g (constant/g; solidModel's rho*g body force in
+   linGeomTotalDispSolid.C:146): 3 values, zero by default. */
+s4f_status s4f_gravity_set(s4f_mesh *mesh, const double *g);
+
+/* Old-time fields of a transient step (S4F_D2DT2_EULER): D.oldTime() and
+   D.oldTime().oldTime() (nCells*3 each, AoS; DoldOld NULL: equal to Dold).
+   The step starts from Dold unless the solve is given a warm-start field, and
+   the convergence test's old-time field is Dold (solidModelTemplates.C:26-188). */
+s4f_status s4f_old_time_set(s4f_mesh *mesh, const double *Dold, const double *DoldOld);
 /* Per-face boundary data of patch `patch`: fixedDisplacement and
    fixedDisplacementZeroShear -> displacement (size*3; only its normal
    component binds the zero-shear patch); solidTraction -> traction (size*3)
@@ -191,6 +208,13 @@ typedef struct {
        D -= alpha (D_prev - D), alpha updated by Aitken's formula, clipped to [0, 2]) */
     int32_t relaxationMethod;
     int32_t reserved1;             /* zero */
+    /* transient: S4F_D2DT2_EULER ([OF] EulerD2dt2Scheme: matrix diagonal
+       coefft rDeltaT2 rho V, source rDeltaT2 rho V (coefft0 D_old - coefft00 D_oldOld))
+       with the time step deltaT and the previous one deltaT0 (0: = deltaT) */
+    int32_t d2dt2;
+    int32_t reserved2;             /* zero */
+    double deltaT;
+    double deltaT0;
 } s4f_solve_opts;
 
 typedef struct {

@@ -62,6 +62,7 @@ def lib():
         L.or_solve.argtypes = [vp, vp, vp]
         L.or_time_iterations.restype = ctypes.c_double
         L.or_time_iterations.argtypes = [vp, vp, ctypes.c_int]
+        L.or_advance_time.argtypes = [vp]
         L.or_reference_source.argtypes = [vp, ctypes.c_int, dp]
         L.or_amul.argtypes = [vp, dp, dp]
         L.or_sigma_eq.argtypes = [vp, dp]
@@ -83,7 +84,8 @@ class Opts(ctypes.Structure):
                 ("pcgMaxIter", ctypes.c_int), ("rcScale", ctypes.c_double), ("interfaceScale", ctypes.c_double),
                 ("historyCap", ctypes.c_int), ("histInitRes", ctypes.POINTER(ctypes.c_double)),
                 ("histRvf", ctypes.POINTER(ctypes.c_double)), ("histInner", ctypes.POINTER(ctypes.c_int)),
-                ("relaxD", ctypes.c_double), ("relaxAitken", ctypes.c_int)]
+                ("relaxD", ctypes.c_double), ("relaxAitken", ctypes.c_int), ("d2dt2", ctypes.c_int),
+                ("deltaT", ctypes.c_double), ("deltaT0", ctypes.c_double), ("g", ctypes.c_double * 3)]
 
 
 class Report(ctypes.Structure):
@@ -148,7 +150,8 @@ class Mesh:
 
 def default_opts(**kw):
     o = dict(nCorr=10000, solTol=1e-6, altTol=1e-7, precond=PRECOND_DIC, pcgTol=1e-9, pcgRelTol=0.1,
-             pcgMaxIter=1000, rcScale=0.1, interfaceScale=0.01, relaxD=0.0, relaxAitken=0)
+             pcgMaxIter=1000, rcScale=0.1, interfaceScale=0.01, relaxD=0.0, relaxAitken=0, d2dt2=0,
+             deltaT=0.0, deltaT0=0.0, g=(0.0, 0.0, 0.0))
     o.update(kw)
     return o
 
@@ -229,6 +232,10 @@ class Case:
             n = min(rep.nOuter, history)
             out["history"] = dict(initRes=hist[0][:n].copy(), rvf=hist[1][:n].copy(), inner=hist[2][:n].copy())
         return out
+
+    def advance_time(self):
+        """new time step: D -> D.oldTime() -> D.oldTime().oldTime()"""
+        lib().or_advance_time(self.h)
 
     def time_iterations(self, n, **kw):
         o = self._opts(default_opts(**kw))

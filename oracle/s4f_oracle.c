@@ -75,6 +75,9 @@ typedef struct {
     double *histInitRes, *histRvf; int *histInner;
     double relaxD;          /* fvSolution relaxationFactors { fields { D } }: 0 or 1 = none */
     int relaxAitken;        /* solidProperties relaxationMethod: 0 fixed, 1 Aitken */
+    int d2dt2;              /* d2dt2Schemes: 0 steadyState, 1 Euler */
+    double deltaT, deltaT0; /* time step and the previous one (Euler d2dt2) */
+    double g[3];            /* gravitational acceleration (constant/g), body force rho g */
 } or_opts;
 
 typedef struct {
@@ -113,6 +116,9 @@ typedef struct {
     double *snb;                     /* boundary snGrad used by the gradient boundary correction, nB*3 */
     double *R;                       /* explicit residual, C*3 (integrated) */
     double *aAlpha, *aRes;           /* Aitken: aitkenAlpha (C), aitkenResidual (C*3) */
+    double *rho;                     /* density per cell (mechanicalProperties rho) */
+    double *Doo;                     /* D.oldTime().oldTime(), C*3 */
+    double *diag0;                   /* the steady matrix diagonal (the inertia term is added per solve) */
     double *aAlphaB, *aResB;         /* their boundary values (nB, nB*3) */
     int iCorr;
 } or_case;
@@ -682,6 +688,8 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
     c->aAlpha = xcalloc((size_t)C, sizeof(double)); c->aRes = xcalloc((size_t)C*3, sizeof(double));
     c->aAlphaB = xcalloc((size_t)B, sizeof(double)); c->aResB = xcalloc((size_t)B*3, sizeof(double));
     c->R = xcalloc((size_t)C*3, sizeof(double));
+    c->rho = xcalloc((size_t)C, sizeof(double)); c->Doo = xcalloc((size_t)C*3, sizeof(double));
+    c->diag0 = xcalloc((size_t)C, sizeof(double)); memcpy(c->diag0, c->diag, sizeof(double)*C);
     return c;
 }
 
@@ -689,6 +697,7 @@ void or_case_free(or_case *c)
 {
     if (!c) return;
     free(c->patchType); free(c->bType); free(c->bValue); free(c->bPressure);
+    free(c->rho); free(c->Doo); free(c->diag0);
     free(c->mu); free(c->lambda); free(c->K); free(c->impK);
     free(c->w); free(c->nonOrthDelta); free(c->corrVec); free(c->bDelta); free(c->bK); free(c->bN);
     free(c->lsP); free(c->lsN); free(c->lsB); free(c->impKf); free(c->gammaf);
@@ -1103,6 +1112,33 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
     memcpy(c->Dprev, c->D, sizeof(double)*3*C);           /* storePrevIter */
     or_update_traction(c);                                  /* BC updateCoeffs at assembly */
     or_residual(c);                                         /* explicit terms */
+    /* rho*fvm::d2dt2(D) on the left and rho*g on the right
+       (linGeomTotalDispSolid.C:141-149).  [OF] EulerD2dt2Scheme::fvmD2dt2:
+       diag = coefft rDeltaT2 V rho, source = rDeltaT2 V rho (coefft0 D.oldTime()
+       - coefft00 D.oldTime().oldTime()), coefft = (dt + dt0)/(2 dt),
+       coefft00 = (dt + dt0)/(2 dt0), coefft0 = coefft + coefft00,
+       rDeltaT2 = 4/(dt + dt0)^2.  In defect form the matrix gains the diagonal
+       and the explicit residual gains source - diag D + rho g V. */
+    {
+        const double *V = m->V;
+        double ct = 0.0, ct0 = 0.0, ct00 = 0.0, r2 = 0.0;
+        if (o->d2dt2) {
+            const double dt = o->deltaT, dt0 = o->deltaT0 > 0.0 ? o->deltaT0 : o->deltaT;
+            ct = (dt + dt0)/(2.0*dt); ct00 = (dt + dt0)/(2.0*dt0); ct0 = ct + ct00;
+            r2 = 4.0/((dt + dt0)*(dt + dt0));
+        }
+        for (int i = 0; i < C; ++i) {
+            const double rv = c->rho[i]*V[i];
+            const double iner = ct*r2*rv;
+            c->diag[i] = c->diag0[i] + iner;
+            for (int q = 0; q < 3; ++q) {
+                double add = rv*o->g[q];
+                if (o->d2dt2)
+                    add += r2*rv*(ct0*c->Dold[3*i+q] - ct00*c->Doo[3*i+q]) - iner*c->D[3*i+q];
+                c->R[3*i+q] += add;
+            }
+        }
+    }
     /* DEqn.solve(): segregated PCG on M x = M D + R, warm start x = D */
     double *rD = malloc(sizeof(double)*C);
     if (o->precond == OR_PRECOND_DIC) dic_rD(c, rD);
@@ -1230,6 +1266,16 @@ int or_solve(or_case *c, const or_opts *o, or_report *rep)
     return conv;
 }
 
+/* start a new time step: the converged field becomes D.oldTime() and the
+   previous old-time field D.oldTime().oldTime(); the loop counter restarts */
+void or_advance_time(or_case *c)
+{
+    const int C = c->m->nCells;
+    memcpy(c->Doo, c->Dold, sizeof(double)*3*C);
+    memcpy(c->Dold, c->D, sizeof(double)*3*C);
+    c->iCorr = 0;
+}
+
 /* timing helper for the CPU baseline: n outer iterations, wall seconds */
 double or_time_iterations(or_case *c, const or_opts *o, int n)
 {
@@ -1256,6 +1302,7 @@ static double *field_ptr(or_case *c, const char *name, size_t *n)
     F_("impK", c->impK, C) F_("impKf", c->impKf, F) F_("gammaf", c->gammaf, F)
     F_("mu", c->mu, C) F_("lambda", c->lambda, C) F_("K", c->K, C)
     F_("aitkenAlpha", c->aAlpha, C) F_("aitkenResidual", c->aRes, 3*C)
+    F_("rho", c->rho, C) F_("Doo", c->Doo, 3*C)
     F_("aitkenAlphab", c->aAlphaB, B) F_("aitkenResidualb", c->aResB, 3*B)
     #undef F_
     *n = 0; return NULL;

@@ -137,6 +137,13 @@ struct HostMesh {
     // on a 64-row slice boundary, padding rows carry no faces
     std::vector<int> instCell0, instNCell, instB0, instNB;
     std::vector<int> instMask;  // per instance: solved components (bit j: component j; 2-D leaves one out)
+    // dynamics and body force (single meshes)
+    std::vector<double> V;        // cell volumes
+    std::vector<double> rho;      // density, C values or 1 (empty: not set)
+    double g[3] = {0.0, 0.0, 0.0};
+    std::vector<double> diagBase; // the steady diagonal (diag may carry the inertia term)
+    std::vector<double> Do, Doo;  // old-time fields (SoA 3C) of a transient step
+    bool hasOld = false;
 };
 
 struct SolveState {
@@ -208,6 +215,11 @@ struct s4f_mesh {
     double rcScale = 0.1, ifScale = 0.01;  // stabilisation scales baked into eC
     std::vector<char> bcSet;
     SolveState st;
+    // momentum-equation sources of the current solve (DevMesh fext / iner)
+    DArr<double> fext, iner;
+    bool srcOn = false, inerOn = false;
+    double inerKey = 0.0;     // coefft rDeltaT2 the device diagonal carries (0: steady)
+    unsigned rhoGen = 0, inerRhoGen = 0;
     DevMesh dm() const;
     BlockMap bmap() const;
 };
@@ -229,6 +241,8 @@ DevMesh s4f_mesh::dm() const {
     m.eC = eC.p;
     m.eA32 = eA32.p;
     m.cC = cC.p;
+    m.fext = srcOn ? fext.p : nullptr;
+    m.iner = inerOn ? iner.p : nullptr;
     m.minv = minv.p;
     m.diag = diag.p;
     m.invDiag = invDiag.p;
@@ -314,6 +328,7 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
         throw InvalidError("invalid mesh descriptor");
     if (d.nPatches <= 0 || !d.patchStart || !d.patchSize || !d.patchType) throw InvalidError("mesh needs patches");
     h.C = C; h.F = F; h.B = B; h.nPatches = d.nPatches;
+    h.V.assign(d.V, d.V + C);
     h.owner.assign(d.owner, d.owner + NF);
     h.neighbour.assign(d.neighbour, d.neighbour + F);
     h.patchStart.assign(d.patchStart, d.patchStart + d.nPatches);
@@ -545,6 +560,8 @@ void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCel
 void uploadMaterial(s4f_mesh* M) {
     HostMesh& h = M->h;
     hipStream_t s = M->ctx->stream;
+    h.diagBase = h.diag;  // the steady matrix
+    M->inerKey = 0.0;
     M->eA.upload(h.eA, s);
     M->eC.upload(h.eC, s);
     M->diag.upload(h.diag, s);
@@ -699,6 +716,7 @@ void checkOpts(const s4f_solve_opts& o) {
     if (o.eagerLaunch != 0 && o.eagerLaunch != 1) throw InvalidError("eagerLaunch must be 0 or 1");
     if (o.relaxationMethod != S4F_RELAX_FIXED && o.relaxationMethod != S4F_RELAX_AITKEN)
         throw InvalidError("relaxationMethod must be fixed (0) or Aitken (1)");
+    if (o.d2dt2 != S4F_D2DT2_STEADY && o.d2dt2 != S4F_D2DT2_EULER) throw InvalidError("d2dt2 must be steady (0) or Euler (1)");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -1150,6 +1168,83 @@ void launchPcgChunkMg(const MgRun& R, const DevMesh& dm, const BlockMap& bm, Pcg
     HIP_CHECK(hipGetLastError());
 }
 
+// device copies of the (effective) diagonal and its inverse, FP64 and FP32
+void uploadDiagonal(s4f_mesh* M) {
+    const HostMesh& h = M->h;
+    hipStream_t s = M->ctx->stream;
+    std::vector<double> inv(h.C);
+    std::vector<float> d32(h.C), i32(h.C);
+    for (int c = 0; c < h.C; ++c) {
+        inv[c] = 1.0 / h.diag[c];
+        d32[c] = (float)h.diag[c];
+        i32[c] = 1.0f / d32[c];
+    }
+    M->diag.upload(h.diag, s);
+    M->invDiag.upload(inv, s);
+    M->diag32.upload(d32, s);
+    M->invDiag32.upload(i32, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// rho*fvm::d2dt2(D) and rho*g of linGeomTotalDispSolid.C:141-149 for this
+// solve.  [OF] EulerD2dt2Scheme::fvmD2dt2: diagonal coefft rDeltaT2 rho V,
+// source rDeltaT2 rho V (coefft0 D_old - coefft00 D_oldOld), coefft =
+// (dt + dt0)/(2 dt), coefft00 = (dt + dt0)/(2 dt0), coefft0 = coefft +
+// coefft00, rDeltaT2 = 4/(dt + dt0)^2.  In defect form the matrix carries the
+// diagonal (uploaded, and the multigrid hierarchy rebuilt, when it changes)
+// and the residual adds fext - iner D with fext = rho V g + source.
+void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
+    HostMesh& h = M->h;
+    const bool transient = o.d2dt2 == S4F_D2DT2_EULER;
+    const bool grav = h.g[0] != 0.0 || h.g[1] != 0.0 || h.g[2] != 0.0;
+    M->srcOn = M->inerOn = false;
+    double key = 0.0, ct = 0.0, ct0 = 0.0, ct00 = 0.0, r2 = 0.0;
+    if (transient || grav) {
+        if (h.instCell0.size() != 1) throw InvalidError("transient and body-force solves take a single mesh");
+        if (h.rho.empty()) throw InvalidError("density not set (s4f_density_set)");
+    }
+    if (transient) {
+        if (!(o.deltaT > 0.0) || !(o.deltaT0 >= 0.0)) throw InvalidError("deltaT must be > 0 (and deltaT0 >= 0)");
+        if (!h.hasOld) throw InvalidError("a transient solve needs the old-time fields (s4f_old_time_set)");
+        const double dt = o.deltaT, dt0 = o.deltaT0 > 0.0 ? o.deltaT0 : o.deltaT;
+        ct = (dt + dt0) / (2.0 * dt);
+        ct00 = (dt + dt0) / (2.0 * dt0);
+        ct0 = ct + ct00;
+        r2 = 4.0 / ((dt + dt0) * (dt + dt0));
+        key = ct * r2;
+    }
+    const int C = h.C;
+    auto rhoV = [&](int c) { return (h.rho.size() == 1 ? h.rho[0] : h.rho[c]) * h.V[c]; };
+    if (key != M->inerKey || (key != 0.0 && M->inerRhoGen != M->rhoGen)) {
+        h.diag = h.diagBase;
+        std::vector<double> iner(C, 0.0);
+        if (key != 0.0)
+            for (int c = 0; c < C; ++c) {
+                iner[c] = ct * r2 * rhoV(c);
+                h.diag[c] += iner[c];
+            }
+        uploadDiagonal(M);
+        M->iner.upload(iner, M->ctx->stream);
+        M->inerKey = key;
+        M->inerRhoGen = M->rhoGen;
+        M->generation++;  // the multigrid hierarchy follows the matrix
+    }
+    if (!(transient || grav)) return;
+    std::vector<double> f(3 * (size_t)C);
+    for (int c = 0; c < C; ++c) {
+        const double rv = rhoV(c);
+        for (int q = 0; q < 3; ++q) {
+            double v = rv * h.g[q];
+            if (transient) v += r2 * rv * (ct0 * h.Do[(size_t)q * C + c] - ct00 * h.Doo[(size_t)q * C + c]);
+            f[(size_t)q * C + c] = v;
+        }
+    }
+    M->fext.upload(f, M->ctx->stream);
+    HIP_CHECK(hipStreamSynchronize(M->ctx->stream));
+    M->srcOn = true;
+    M->inerOn = transient;
+}
+
 // The fields the gradient step reads and writes.
 struct GradBufs {
     double *D0, *D1, *G, *Db, *g, *snb, *Gb, *sigb;
@@ -1234,6 +1329,12 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     SolveState& st = M->st;
     PcgVecs<T> v = pcgVecs<T>(st, C);
     applyStabilisationScale(M, o.rhieChowScale, o.interfaceScale);
+    configureSources(M, o);
+    struct SourceOff {  // the sources belong to this solve only
+        s4f_mesh* m;
+        ~SourceOff() { m->srcOn = m->inerOn = false; }
+    } sourceOff{M};
+    const bool transient = o.d2dt2 == S4F_D2DT2_EULER;
     const bool mg = o.preconditioner == S4F_PRECOND_GAMG;
     if (mg) {
         buildAmg(M);
@@ -1257,13 +1358,19 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         for (int c = 0; c < n; ++c)
             for (int q = 0; q < 3; ++q) D0[(size_t)q * C + c0 + c] = io[i].D_inout[3 * (size_t)c + q];
     }
+    if (transient && !warm) D0 = h.Do;  // a time step starts from the old-time field
     HIP_CHECK(hipMemcpyAsync(st.D0.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    if (warm) HIP_CHECK(hipMemcpyAsync(st.Dold.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (transient) {  // the convergence test's old-time field
+        HIP_CHECK(hipMemcpyAsync(st.Dold.p, h.Do.data(), h.Do.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        warm = true;
+    } else if (warm) {
+        HIP_CHECK(hipMemcpyAsync(st.Dold.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    }
     HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
     HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
     std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
     for (int i = 0; i < nInst; ++i) {
-        c0[i].warm = (o.warmStart && io[i].D_inout) ? 1 : 0;
+        c0[i].warm = ((o.warmStart && io[i].D_inout) || transient) ? 1 : 0;  // a time step continues
         c0[i].sigma0 = c0[i].warm ? 0 : 1;  // a cold start's stress field is zero until the first update
         c0[i].pcMode = mg ? 1 : 0;
         c0[i].solveMask = M->h.instMask.empty() ? 7 : M->h.instMask[i];
@@ -1453,6 +1560,9 @@ void solveDispatch(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>
 void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
     P = HostMesh();
     const int n = (int)ms.size();
+    for (const HostMesh* m : ms)
+        if (m->g[0] != 0.0 || m->g[1] != 0.0 || m->g[2] != 0.0)
+            throw InvalidError("body-force solves take a single mesh");
     long long C = 0, B = 0, E = 0, S = 0;
     for (const HostMesh* m : ms) {
         P.instCell0.push_back((int)C);
@@ -1713,6 +1823,50 @@ s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int3
         M->rcScale = 0.1;
         M->ifScale = 0.01;
         M->materialSet = true;
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_density_set(s4f_mesh* M, const double* rho, int32_t perCell) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !rho) throw InvalidError("null argument");
+        const int n = perCell ? M->h.C : 1;
+        for (int i = 0; i < n; ++i)
+            if (!(rho[i] > 0.0) || !std::isfinite(rho[i])) throw InvalidError("density must be > 0");
+        M->h.rho.assign(rho, rho + n);
+        M->rhoGen++;
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_gravity_set(s4f_mesh* M, const double* g) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !g) throw InvalidError("null argument");
+        for (int q = 0; q < 3; ++q) {
+            if (!std::isfinite(g[q])) throw InvalidError("non-finite gravity");
+            M->h.g[q] = g[q];
+        }
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_old_time_set(s4f_mesh* M, const double* Dold, const double* DoldOld) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !Dold) throw InvalidError("null argument");
+        HostMesh& h = M->h;
+        if (h.instCell0.size() != 1) throw InvalidError("old-time fields belong to single meshes");
+        const int C = h.C;
+        h.Do.resize(3 * (size_t)C);
+        h.Doo.resize(3 * (size_t)C);
+        const double* oo = DoldOld ? DoldOld : Dold;
+        for (int c = 0; c < C; ++c)
+            for (int q = 0; q < 3; ++q) {
+                h.Do[(size_t)q * C + c] = Dold[3 * (size_t)c + q];
+                h.Doo[(size_t)q * C + c] = oo[3 * (size_t)c + q];
+                if (!std::isfinite(h.Do[(size_t)q * C + c]) || !std::isfinite(h.Doo[(size_t)q * C + c]))
+                    throw InvalidError("non-finite old-time field");
+            }
+        h.hasOld = true;
         return S4F_OK;
     });
 }
@@ -2515,12 +2669,12 @@ void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
     };
     putCell(S->cC.p, h.cC, 3, C, 0.0);
     putCell(S->minv.p, h.minv, 6, C, 0.0);
-    putCell(S->diag.p, h.diag, 1, C, 1.0);
+    putCell(S->diag.p, h.diagBase, 1, C, 1.0);
     std::vector<double> inv(C), muv(C), Kv(C), ikv(C);
     std::vector<float> d32(Cp, 1.0f), i32(Cp, 1.0f);
     for (int c = 0; c < C; ++c) {
-        inv[c] = 1.0 / h.diag[c];
-        d32[c] = (float)h.diag[c];
+        inv[c] = 1.0 / h.diagBase[c];
+        d32[c] = (float)h.diagBase[c];
         i32[c] = 1.0f / d32[c];
         const int mc = h.uniformMaterial ? 0 : c;
         muv[c] = h.mu[mc]; Kv[c] = h.K[mc]; ikv[c] = h.impK[mc];
@@ -3053,6 +3207,7 @@ s4f_status s4f_server_create(s4f_ctx* ctx, int64_t budgetBytes, int32_t maxSlots
         if (budgetBytes < (1 << 20)) throw InvalidError("budget must be at least 1 MiB");
         if (maxSlots < 1 || maxSlots > 4096) throw InvalidError("maxSlots must be in 1..4096");
         checkOpts(*opts);
+        if (opts->d2dt2 != S4F_D2DT2_STEADY) throw InvalidError("the server runs steady solves");
         HIP_CHECK(hipSetDevice(ctx->device));
         auto S = std::make_unique<s4f_server>();
         S->ctx = ctx;
@@ -3077,6 +3232,9 @@ s4f_status s4f_server_submit(s4f_server* S, const s4f_request* rq, int64_t* id) 
         if (!m->materialSet) throw InvalidError("material not set");
         if (m->ctx != S->ctx) throw InvalidError("mesh belongs to another context");
         if (m->h.instCell0.size() != 1) throw InvalidError("requests take plain meshes");
+        if (m->inerKey != 0.0 || m->h.g[0] != 0.0 || m->h.g[1] != 0.0 || m->h.g[2] != 0.0)
+            throw InvalidError("requests take static meshes without body force (the last solve was transient or "
+                               "gravity is set)");
         const HostMesh& h = m->h;
         if ((long long)h.nSlices * kSlice > S->Ccap || (long long)h.col.size() > S->Ecap || h.B > S->Bcap)
             return fail(S4F_ERR_ADMISSION, "request footprint exceeds the server's HBM budget");

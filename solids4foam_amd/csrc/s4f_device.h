@@ -33,6 +33,10 @@ struct DevMesh {
     const double* eC;      // Rhie-Chow coefficient gamma_f |S| deltaCoeffs
     const float* eA32;
     const double* cC;      // 3*nCells cell centres
+    // explicit sources of the momentum equation per cell (nullptr: none):
+    // fext = rho V g + inertia source (3*nCells), iner = inertia diagonal (nCells)
+    const double* fext;
+    const double* iner;
     const double* minv;    // 6*nCells inverse least-squares matrices (xx xy xz yy yz zz)
     const double* diag;
     const double* invDiag;

@@ -255,9 +255,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
         const double dg = m.diag[P];
         const double rowsum = dg - asum;
         const double xb[3] = {xb0, xb1, xb2};
+        // body force and the inertia terms (rho g V + source - diag D)
+        double src[3] = {0.0, 0.0, 0.0};
+        if (m.fext) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) src[j] = m.fext[(size_t)j * C + P] - (m.iner ? m.iner[P] * DP[j] : 0.0);
+        }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double R = div[j] + rc[j];
+            const double R = (div[j] + rc[j]) + src[j];
             const double MD = dg * DP[j] - off[j];
             const double t = rowsum * xb[j];
             acc[j] += fabs(R);

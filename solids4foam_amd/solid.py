@@ -153,6 +153,22 @@ class SolverMesh:
             nu = f64(np.broadcast_to(nu, (self.nCells,)))
         check(lib().s4f_material_set(self.h, ptr(E), ptr(nu), int(per_cell)))
 
+    def set_density(self, rho):
+        """mechanicalProperties rho: one value or one per cell"""
+        r = np.atleast_1d(f64(rho))
+        check(lib().s4f_density_set(self.h, ptr(r), int(r.size > 1)))
+
+    def set_gravity(self, g):
+        """constant/g: the body force rho g of the momentum equation"""
+        check(lib().s4f_gravity_set(self.h, ptr(f64(g, (3,)))))
+
+    def set_old_time(self, D_old, D_old_old=None):
+        """D.oldTime() and D.oldTime().oldTime() for a transient (Euler
+        d2dt2) step; the step starts from D_old"""
+        a = f64(D_old, (self.nCells, 3))
+        b = None if D_old_old is None else f64(D_old_old, (self.nCells, 3))
+        check(lib().s4f_old_time_set(self.h, ptr(a), ptr(b)))
+
     def set_bc(self, patch, value, pressure=None):
         sl = self.poly.patch_faces(patch)
         n = sl.stop - sl.start
@@ -268,7 +284,11 @@ class LinGeomTotalDispSolid:
 
     typeName = "linearGeometryTotalDisplacement"
 
-    def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False):
+    def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False, rho=None,
+                 g=None, d2dt2="steadyState", deltaT=None):
+        # rho: mechanicalProperties density (needed with g or a transient
+        # d2dt2); g: constant/g; d2dt2: d2dt2Schemes default (steadyState |
+        # Euler); deltaT: the time step of evolve() when transient
         # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
         # "solidSymmetry"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
@@ -289,15 +309,39 @@ class LinGeomTotalDispSolid:
                 self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))
             elif p["type"] == "solidTraction":
                 self.mesh.set_bc(i, p.get("traction", (0.0, 0.0, 0.0)), p.get("pressure"))
+        if d2dt2 not in ("steadyState", "Euler"):
+            raise ValueError(f"d2dt2 scheme {d2dt2!r} is not on the device path (steadyState, Euler)")
+        if rho is not None:
+            self.mesh.set_density(rho)
+        if g is not None:
+            self.mesh.set_gravity(g)
+        self.transient = d2dt2 == "Euler"
+        if self.transient and not (deltaT and deltaT > 0):
+            raise ValueError("a transient model needs deltaT > 0")
+        self.deltaT = deltaT
+        self.deltaT0 = deltaT
+        self.time = 0.0
         self.opts = dict(coeffs or {})
         self.opts.update(solver or {})
         self.D = np.zeros((polymesh.nCells, 3))
+        self.D_old_old = self.D.copy()
         self.sigma = None
         self.gradD = None
         self.report = None
 
     def evolve(self, warm_start=False):
-        out = self.mesh.solve(D0=self.D if warm_start else None, want_sigma=True, want_grad=True, **self.opts)
+        """one steady solve, or one time step of deltaT when transient (the
+        converged field becomes D.oldTime() of the next step)"""
+        opts = dict(self.opts)
+        if self.transient:
+            self.mesh.set_old_time(self.D, self.D_old_old)
+            opts.update(d2dt2=_abi.D2DT2_EULER, deltaT=self.deltaT, deltaT0=self.deltaT0)
+            out = self.mesh.solve(want_sigma=True, want_grad=True, **opts)
+            self.D_old_old = self.D
+            self.deltaT0 = self.deltaT
+            self.time += self.deltaT
+        else:
+            out = self.mesh.solve(D0=self.D if warm_start else None, want_sigma=True, want_grad=True, **opts)
         self.D, self.sigma, self.gradD = out["D"], out["sigma"], out["gradD"]
         self.report = {k: v for k, v in out.items() if k not in ("D", "sigma", "gradD", "Db")}
         return True

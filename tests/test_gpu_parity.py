@@ -413,6 +413,79 @@ def test_zero_shear_patches(ctx, oracle_mod):
                 assert np.abs(out["D"] - exact).max() < 1e-8 * delta
 
 
+def test_transient_and_body_force(ctx, oracle_mod):
+    # rho*d2dt2(D) (Euler) and rho*g: (i) a free body falls with
+    # D^n = n (n + 1)/2 g dt^2 exactly; (ii) a loaded cantilever under gravity,
+    # three time steps (the last with a longer step), matches the oracle step
+    # by step; (iii) the steady self-weight field matches the oracle
+    from solids4foam_amd import PolyMesh, SolverMesh
+    pm = PolyMesh.box(5, 4, 3, L=(1.0, 0.8, 0.6), jitter=0.1, seed=3)
+    g, dt = np.array([0.0, -9.81, 2.0]), 1e-3
+    # FP64 only: a free body's rigid mode is held by the inertia diagonal
+    # alone (3e-4 of the elastic diagonal here), which FP32 sweeps resolve too
+    # coarsely for the 1e-12 outer tolerance
+    for precision in (0,):
+        m = SolverMesh(ctx, pm, [TRACTION] * 6)
+        m.set_material(200e9, 0.3)
+        m.set_density(7800.0)
+        m.set_gravity(g)
+        for p in range(6):
+            m.set_bc(p, np.zeros((pm.patch_faces(p).stop - pm.patch_faces(p).start, 3)))
+        Dm, Dmm = np.zeros((pm.nCells, 3)), np.zeros((pm.nCells, 3))
+        for n in range(1, 5):
+            m.set_old_time(Dm, Dmm)
+            out = m.solve(precision=precision, d2dt2=1, deltaT=dt, deltaT0=dt, nCorrectors=200,
+                          solutionTolerance=1e-12, alternativeTolerance=1e-12, pcgTolerance=1e-14, pcgRelTol=0.0)
+            assert out["converged"]
+            exact = n * (n + 1) / 2 * g * dt * dt
+            err = np.abs(out["D"] - exact).max() / np.abs(exact).max()
+            assert err < 1e-11, (n, err)
+            Dmm, Dm = Dm, out["D"]
+    # (ii) cantilever: x-min fixed, x-max sudden traction, gravity
+    n3 = (8, 4, 4)
+    types = [FIXED] + [TRACTION] * 5
+    om = oracle_mod.Mesh.box(*n3, (1.0, 0.25, 0.25), 0.1, 5)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(1)] = (0.0, -2e6, 1e6)
+    oc = oracle_mod.Case(om, 200e9, 0.3, types, bv)
+    oc.set("rho", np.full(om.nCells, 7800.0))
+    pm = PolyMesh.box(*n3, L=(1.0, 0.25, 0.25), jitter=0.1, seed=5)
+    m = SolverMesh(ctx, pm, types)
+    m.set_material(200e9, 0.3)
+    m.set_density(7800.0)
+    grav = (0.0, 0.0, -9.81)
+    m.set_gravity(grav)
+    for p in range(6):
+        m.set_bc(p, bv[pm.patch_faces(p)])
+    kw = dict(nCorr=100000, solTol=1e-12, altTol=1e-12, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+    Dm, Dmm = np.zeros((pm.nCells, 3)), np.zeros((pm.nCells, 3))
+    steps = [(2e-5, 2e-5), (2e-5, 2e-5), (3e-5, 2e-5)]
+    for i, (dti, dt0) in enumerate(steps):
+        if i:
+            oc.advance_time()
+        ro = oc.solve(d2dt2=1, deltaT=dti, deltaT0=dt0, g=grav, **kw)
+        m.set_old_time(Dm, Dmm)
+        out = m.solve(d2dt2=1, deltaT=dti, deltaT0=dt0, nCorrectors=100000, solutionTolerance=1e-12,
+                      alternativeTolerance=1e-12, pcgTolerance=1e-15)
+        Do = oc.get("D").reshape(-1, 3)
+        err = rel_l2(out["D"], Do)
+        print(f"  transient step {i}: nOuter={out['nOuter']} (oracle {ro['nOuter']}) err={err:.2e}", flush=True)
+        assert ro["converged"] and out["converged"] and err < 1e-8
+        Dmm, Dm = Dm, out["D"]
+    # (iii) steady self weight
+    oc = oracle_mod.Case(om, 200e9, 0.3, types, np.zeros_like(bv))
+    oc.set("rho", np.full(om.nCells, 7800.0))
+    oc.solve(g=grav, **kw)
+    for p in range(6):
+        m.set_bc(p, np.zeros((pm.patch_faces(p).stop - pm.patch_faces(p).start, 3)))
+    for precision in (0, 1):
+        out = m.solve(precision=precision, nCorrectors=100000, solutionTolerance=1e-12, alternativeTolerance=1e-12,
+                      pcgTolerance=1e-15)
+        err = rel_l2(out["D"], oc.get("D").reshape(-1, 3))
+        print(f"  self weight precision={precision}: nOuter={out['nOuter']} err={err:.2e}", flush=True)
+        assert out["converged"] and err < (1e-8 if precision == 0 else 1e-6)
+
+
 def test_solve_is_bitwise_deterministic(ctx):
     pm, m, _, _ = _product_mesh(ctx, (12, 10, 9), jitter=0.15, seed=8)
     a = m.solve(solutionTolerance=1e-9, alternativeTolerance=1e-9)

@@ -527,3 +527,58 @@ def test_aitken_reaches_the_fixed_point(oracle_mod, driven):
         c1.solve(nCorr=1, pcgTol=1e-13, precond=oracle_mod.PRECOND_DIC, relaxAitken=1)
         ab1 = c1.get("aitkenAlphab")
         assert np.all(ab1[m.patch_faces(0)] == 0.0) and np.all(ab1[m.patch_faces(1)] == 1.0)
+
+
+def test_free_fall_follows_euler_d2dt2(oracle_mod):
+    # a free body (all faces traction free) under gravity: every cell moves
+    # together and the Euler d2dt2 of a start from rest gives exactly
+    # D^n = n (n + 1) / 2 g dt^2 (the discrete second difference of it is g);
+    # no strain, no stress
+    m = oracle_mod.Mesh.box(5, 4, 3, (1.0, 0.8, 0.6), 0.1, 3)
+    c = oracle_mod.Case(m, 200e9, 0.3, [TRACTION] * 6, np.zeros((m.nBoundaryFaces, 3)))
+    c.set("rho", np.full(m.nCells, 7800.0))
+    g, dt = np.array([0.0, -9.81, 2.0]), 1e-3
+    for n in range(1, 5):
+        if n > 1:
+            c.advance_time()
+        r = c.solve(nCorr=200, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, pcgRelTol=0.0,
+                    precond=oracle_mod.PRECOND_DIC, d2dt2=1, deltaT=dt, deltaT0=dt, g=tuple(g))
+        assert r["converged"]
+        exact = n * (n + 1) / 2 * g * dt * dt
+        assert np.abs(c.get("D").reshape(-1, 3) - exact).max() < 1e-12 * np.abs(exact).max()
+        assert np.abs(c.get("sigma")).max() < 1e-6 * 7800.0 * 9.81
+    # a changed time step follows the variable-step coefficients: with
+    # dt0 = dt the constant-step result, and D^{n+1} for dt != dt0 satisfies
+    # coefft (D^{n+1} - D^n)/dt ... i.e. 2/(dt + dt0) ((D^{n+1} - D^n)/dt - (D^n - D^{n-1})/dt0) = g
+    D4, D3 = c.get("D").reshape(-1, 3).copy(), c.get("Dold").reshape(-1, 3).copy()
+    c.advance_time()
+    dt1 = 2e-3
+    c.solve(nCorr=200, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, pcgRelTol=0.0,
+            precond=oracle_mod.PRECOND_DIC, d2dt2=1, deltaT=dt1, deltaT0=dt, g=tuple(g))
+    D5 = c.get("D").reshape(-1, 3)
+    acc = 2.0 / (dt1 + dt) * ((D5 - D4) / dt1 - (D4 - D3) / dt)
+    assert np.abs(acc - g).max() < 1e-9 * np.abs(g).max()
+
+
+def test_self_weight_reaction_equals_weight(oracle_mod):
+    # a body fixed at z-min under gravity (steady): at convergence the
+    # boundary stresses carry the weight, sum_b S_b . sigma_b = -rho g V
+    # (the discrete residual is conservative: interior fluxes cancel,
+    # gamma_b = 0), and the traction-free faces carry none
+    m = oracle_mod.Mesh.box(6, 5, 4, (0.5, 0.4, 1.0), 0.1, 3)
+    c = oracle_mod.Case(m, 1e9, 0.3, [TRACTION] * 4 + [FIXED, TRACTION], np.zeros((m.nBoundaryFaces, 3)))
+    rho, g = 2000.0, np.array([1.0, 0.0, -9.81])
+    c.set("rho", np.full(m.nCells, rho))
+    r = c.solve(nCorr=50000, solTol=1e-12, altTol=1e-12, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC,
+                g=tuple(g))
+    assert r["converged"]
+    sb = c.get("sigmab").reshape(-1, 6)
+    S = m.Sf[m.nInternalFaces:]
+    T = np.array([[sb[:, 0], sb[:, 1], sb[:, 2]], [sb[:, 1], sb[:, 3], sb[:, 4]], [sb[:, 2], sb[:, 4], sb[:, 5]]])
+    f = np.einsum("bi,ijb->bj", S, T)          # S . sigma per boundary face
+    weight = rho * m.V.sum() * g
+    bottom = m.patch_faces(4)
+    assert np.allclose(f[bottom].sum(axis=0), -weight, rtol=1e-7, atol=1e-9 * np.abs(weight).max())
+    bidx = np.arange(m.nBoundaryFaces)[bottom]
+    others = np.setdiff1d(np.arange(m.nBoundaryFaces), bidx)
+    assert np.abs(f[others]).max() < 1e-6 * np.abs(weight).max() / len(bidx)
