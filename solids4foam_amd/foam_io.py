@@ -7,8 +7,11 @@ Reads what a solids4foam case gives the reference and writes what it writes:
   * ASCII polyMesh (points, faces, owner, neighbour, boundary), optionally
     gzip-compressed (``*.gz``);
   * the ``0/D`` boundary conditions the device path supports
-    (fixedDisplacement, solidTraction) and the material of
-    ``constant/mechanicalProperties`` (one linearElastic law);
+    (fixedDisplacement, linearSpatialDisplacement, fixedDisplacementZeroShear,
+    solidTraction, solidSymmetry, empty), the material of
+    ``constant/mechanicalProperties`` (one linearElastic law, its density),
+    ``constant/g``, the d2dt2 scheme of ``system/fvSchemes`` and the time
+    controls of ``system/controlDict``;
   * ASCII volVectorField / volSymmTensorField / volScalarField output.
 
 Reference: solidModel field I/O (src/solids4FoamModels/solidModels/solidModel/solidModel.C:1017-1124,
@@ -382,6 +385,47 @@ def read_linear_elastic(case_dir):
     return 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
 
 
+def read_density(case_dir):
+    """rho of the single mechanical law (None when absent)"""
+    d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
+    laws = d.get("mechanical")
+    law = laws[0][1] if isinstance(laws, list) and laws else laws
+    return dimensioned(law["rho"]) if isinstance(law, dict) and "rho" in law else None
+
+
+def read_gravity(case_dir):
+    """constant/g (uniformDimensionedVectorField) or None"""
+    path = os.path.join(case_dir, "constant", "g")
+    if not (os.path.exists(path) or os.path.exists(path + ".gz")):
+        return None
+    v = read_dict(path).get("value")
+    return np.asarray(v, np.float64).reshape(3)
+
+
+def read_d2dt2_scheme(case_dir, field="D"):
+    """d2dt2Schemes entry for d2dt2(D), default steadyState"""
+    try:
+        fvs = read_dict(os.path.join(case_dir, "system", "fvSchemes"))
+    except FileNotFoundError:
+        return "steadyState"
+    sch = fvs.get("d2dt2Schemes", {}) or {}
+    v = lookup(sch, f"d2dt2({field})", None) or sch.get("default", "steadyState")
+    return str(v[0] if isinstance(v, list) else v)
+
+
+def read_time_controls(case_dir):
+    """controlDict startTime, endTime, deltaT, writeControl, writeInterval"""
+    d = read_dict(os.path.join(case_dir, "system", "controlDict"))
+    return dict(startTime=float(d.get("startTime", 0)), endTime=float(d.get("endTime", 1)),
+                deltaT=float(d.get("deltaT", 1)), writeControl=str(d.get("writeControl", "timeStep")),
+                writeInterval=float(d.get("writeInterval", 1)))
+
+
+def _time_name(t):
+    """OpenFOAM's default time directory name (general format, 6 digits)"""
+    return f"{t:.6g}"
+
+
 def read_solid_properties(case_dir):
     """solidModel type + coefficients with the reference defaults (solidModel.C:1147-1177)"""
     d = read_dict(os.path.join(case_dir, "constant", "solidProperties"))
@@ -453,9 +497,13 @@ def write_vol_field(path, name, cls, values, boundary=None, dims="[0 1 0 0 0 0 0
 
 def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     """Reads a solids4foam case (polyMesh, 0/D, mechanicalProperties,
-    solidProperties, fvSolution), runs linearGeometryTotalDisplacement on
-    the device and writes D, sigma, sigmaEq, epsilon, epsilonEq and pointD
-    to `write_time`."""
+    solidProperties, fvSolution, fvSchemes, constant/g), runs
+    linearGeometryTotalDisplacement on the device and writes D, sigma,
+    sigmaEq, epsilon, epsilonEq and pointD.  Steady (d2dt2 steadyState): one
+    solve written to `write_time`.  Transient (Euler): controlDict's time
+    loop from startTime to endTime by deltaT, written at every writeInterval
+    (time steps, or simulated time with writeControl runTime/adjustableRunTime)
+    to directories named by the time."""
     from . import solid
     model, opts = read_solid_properties(case_dir)
     if model not in ("linearGeometryTotalDisplacement", "linearGeometryTotalDisplacementHIP"):
@@ -478,10 +526,39 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
         grad_scheme = str(lookup(fvs.get("gradSchemes", {}), "grad(D)", "") or "")
     except FileNotFoundError:
         pass
+    scheme = read_d2dt2_scheme(case_dir)
+    if scheme not in ("steadyState", "Euler"):
+        raise ValueError(f"d2dt2 scheme {scheme!r} is not on the device path (steadyState, Euler)")
+    rho, g = read_density(case_dir), read_gravity(case_dir)
+    transient = scheme == "Euler"
+    tc = read_time_controls(case_dir) if transient else None
+    if (transient or g is not None) and rho is None:
+        raise ValueError("mechanicalProperties gives no rho (needed for d2dt2 and g)")
     m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
-                                    ls_full_delta="extendedLeastSquares" in grad_scheme)
-    m.evolve()
-    out = os.path.join(case_dir, write_time)
+                                    ls_full_delta="extendedLeastSquares" in grad_scheme, rho=rho, g=g,
+                                    d2dt2=scheme, deltaT=tc["deltaT"] if tc else None)
+    if not transient:
+        m.evolve()
+        _write_state(m, os.path.join(case_dir, write_time))
+        return m
+    t, step = tc["startTime"], 0
+    m.time = t
+    eps = 1e-9 * tc["deltaT"]
+    while t < tc["endTime"] - eps:
+        m.evolve()
+        t, step = m.time, step + 1
+        if tc["writeControl"] == "timeStep":
+            write = step % max(1, int(round(tc["writeInterval"]))) == 0
+        else:
+            k = round(t / tc["writeInterval"])
+            write = abs(t - k * tc["writeInterval"]) < eps
+        if write or t >= tc["endTime"] - eps:
+            _write_state(m, os.path.join(case_dir, _time_name(t)))
+    return m
+
+
+def _write_state(m, out):
+    """D, sigma, sigmaEq, epsilon, epsilonEq and pointD of the last evolve()"""
     write_vol_field(os.path.join(out, "D"), "D", "volVectorField", m.D)
     write_vol_field(os.path.join(out, "sigma"), "sigma", "volSymmTensorField", m.sigma, dims="[1 -1 -2 0 0 0 0]")
     f = m.write_fields()
@@ -491,7 +568,6 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     write_vol_field(os.path.join(out, "epsilonEq"), "epsilonEq", "volScalarField", f["epsilonEq"],
                     dims="[0 0 0 0 0 0 0]")
     write_point_field(os.path.join(out, "pointD"), "pointD", f["pointD"])
-    return m
 
 
 def write_point_field(path, name, values, dims="[0 1 0 0 0 0 0]"):

@@ -5,6 +5,7 @@
 #include "linGeomTotalDispSolidHIP.H"
 #include "addToRunTimeSelectionTable.H"
 #include "fvc.H"
+#include "fvm.H"
 #include "linearElastic.H"
 #include "solidTractionFvPatchVectorField.H"
 #include "fixedDisplacementFvPatchVectorField.H"
@@ -146,6 +147,14 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
     const double E = law.E().value();
     const double nu = law.nu().value();
     check(s4f_material_set(gpuMesh_, &E, &nu, 0), "createDeviceMesh()");
+
+    // rho (inertia and body force) and g (solidModel::g(), constant/g)
+    std::vector<double> rho(nCells);
+    forAll(rho, cellI) { rho[cellI] = this->rho()[cellI]; }
+    check(s4f_density_set(gpuMesh_, rho.data(), 1), "createDeviceMesh()");
+    const vector gv = g().value();
+    const double gg[3] = {gv.x(), gv.y(), gv.z()};
+    check(s4f_gravity_set(gpuMesh_, gg), "createDeviceMesh()");
 }
 
 
@@ -234,6 +243,20 @@ s4f_solve_opts linGeomTotalDispSolidHIP::solveOptions() const
     {
         o.relaxD = mesh().fieldRelaxationFactor(D().name());
     }
+    // rho*fvm::d2dt2(D): steadyState or Euler (linGeomTotalDispSolid.C:143)
+    const word d2dt2Scheme(mesh().d2dt2Scheme("d2dt2(" + D().name() + ')'));
+    if (d2dt2Scheme == "Euler")
+    {
+        o.d2dt2 = S4F_D2DT2_EULER;
+        o.deltaT = runTime().deltaTValue();
+        o.deltaT0 = runTime().deltaT0Value();
+    }
+    else if (d2dt2Scheme != "steadyState")
+    {
+        FatalErrorIn(type() + "::solveOptions()")
+            << "d2dt2 scheme " << d2dt2Scheme << " is not on the device path "
+            << "(steadyState, Euler)" << abort(FatalError);
+    }
     o.warmStart = 1;
     return o;
 }
@@ -253,6 +276,8 @@ linGeomTotalDispSolidHIP::linGeomTotalDispSolidHIP
     patchType_()
 {
     DisRequired();
+    // old-time fields of D exist from the start (linGeomTotalDispSolid.C:75-76)
+    fvm::d2dt2(D());
     if (solidModelDict().lookupOrDefault<Switch>("predictor", false))
     {
         FatalErrorIn(type() + "::" + type())
@@ -285,6 +310,22 @@ bool linGeomTotalDispSolidHIP::evolve()
     forAll(D(), cellI)
     {
         for (direction d = 0; d < 3; ++d) { Dv[3*cellI + d] = D()[cellI][d]; }
+    }
+    if (o.d2dt2 == S4F_D2DT2_EULER)
+    {
+        // D.oldTime() and D.oldTime().oldTime() of this time step
+        std::vector<double> Do(3*nCells), Doo(3*nCells);
+        const volVectorField& D0 = D().oldTime();
+        const volVectorField& D00 = D().oldTime().oldTime();
+        forAll(D0, cellI)
+        {
+            for (direction d = 0; d < 3; ++d)
+            {
+                Do[3*cellI + d] = D0[cellI][d];
+                Doo[3*cellI + d] = D00[cellI][d];
+            }
+        }
+        check(s4f_old_time_set(gpuMesh_, Do.data(), Doo.data()), "evolve()");
     }
     s4f_solve_report rep;
     const s4f_status st =

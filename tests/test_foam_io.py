@@ -172,3 +172,59 @@ def test_run_case_end_to_end(tmp_path):
     ref = mesh.solve(nCorrectors=20000, alternativeTolerance=1e-10, solutionTolerance=1e-10, rhieChowScale=0.2,
                      pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
     assert np.linalg.norm(ref["D"] - m.D) / np.linalg.norm(ref["D"]) < 1e-9
+
+
+G_FILE = """
+FoamFile { version 2.0; format ascii; class uniformDimensionedVectorField; object g; }
+dimensions [0 1 -2 0 0 0 0];
+value (0 -9.81 2);
+"""
+FV_SCHEMES_EULER = """
+FoamFile { version 2.0; format ascii; class dictionary; object fvSchemes; }
+d2dt2Schemes { default Euler; }
+ddtSchemes { default Euler; }
+gradSchemes { default leastSquares; }
+"""
+CONTROL = """
+FoamFile { version 2.0; format ascii; class dictionary; object controlDict; }
+application solids4Foam; startFrom startTime; startTime 0; stopAt endTime; endTime 0.003;
+deltaT 0.001; writeControl timeStep; writeInterval 1;
+"""
+
+
+def test_dynamic_dictionaries(tmp_path):
+    d = str(tmp_path)
+    _write(d, "constant/mechanicalProperties", MECH)
+    assert foam_io.read_density(d) == 7800.0
+    assert foam_io.read_gravity(d) is None and foam_io.read_d2dt2_scheme(d) == "steadyState"
+    _write(d, "constant/g", G_FILE)
+    _write(d, "system/fvSchemes", FV_SCHEMES_EULER)
+    _write(d, "system/controlDict", CONTROL)
+    assert np.array_equal(foam_io.read_gravity(d), [0.0, -9.81, 2.0])
+    assert foam_io.read_d2dt2_scheme(d) == "Euler"
+    tc = foam_io.read_time_controls(d)
+    assert tc["deltaT"] == 0.001 and tc["endTime"] == 0.003 and tc["writeControl"] == "timeStep"
+    assert foam_io._time_name(0.003) == "0.003" and foam_io._time_name(1.0) == "1"
+
+
+@pytest.mark.gpu
+def test_run_case_transient_free_fall(tmp_path):
+    # all faces traction free, gravity, Euler d2dt2: the controlDict loop
+    # writes every step and the body falls as D^n = n(n+1)/2 g dt^2
+    from solids4foam_amd import Context
+    d = str(tmp_path)
+    pm, a = _box_case(d, n=(4, 3, 3), jitter=0.1)
+    _write(d, "0/D", _d_file([(p, "free") for p, _ in PATCHES]))
+    _write(d, "constant/g", G_FILE)
+    _write(d, "system/fvSchemes", FV_SCHEMES_EULER)
+    _write(d, "system/controlDict", CONTROL)
+    ctx = Context(0)
+    m = foam_io.run_case(d, ctx=ctx, solver_overrides=dict(nCorrectors=200, solutionTolerance=1e-12,
+                                                           alternativeTolerance=1e-12, pcgTolerance=1e-14,
+                                                           pcgRelTol=0.0))
+    assert m.report["converged"] and abs(m.time - 0.003) < 1e-12
+    g = np.array([0.0, -9.81, 2.0])
+    for n, name in ((1, "0.001"), (2, "0.002"), (3, "0.003")):
+        D = foam_io._field_value(foam_io.read_dict(os.path.join(d, name, "D"))["internalField"], pm.nCells, 3)
+        exact = n * (n + 1) / 2 * g * 1e-6
+        assert np.abs(D - exact).max() < 1e-9 * np.abs(exact).max()
