@@ -438,8 +438,7 @@ def read_solid_properties(case_dir):
     method = str(coeffs.get("relaxationMethod", "fixed"))
     if method not in ("fixed", "Aitken"):
         raise ValueError(f"relaxationMethod {method!r} is not on the device path (fixed, Aitken)")
-    if str(coeffs.get("predictor", "no")) in ("yes", "on", "true"):
-        raise ValueError("the predictor is not on the device path")
+    predictor = str(coeffs.get("predictor", "no")) in ("yes", "on", "true")
     opts = dict(nCorrectors=int(coeffs.get("nCorrectors", 10000)),
                 solutionTolerance=float(coeffs.get("solutionTolerance", 1e-6)),
                 alternativeTolerance=float(coeffs.get("alternativeTolerance", 1e-7)),
@@ -448,6 +447,7 @@ def read_solid_properties(case_dir):
     prec = coeffs.get("precision", "fp64")
     opts["precision"] = 1 if prec == "mixed" else 0
     opts["relaxationMethod"] = 1 if method == "Aitken" else 0
+    opts["predictor"] = predictor
     return model, opts
 
 
@@ -510,6 +510,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
         raise ValueError(f"solidModel {model!r} is not on the device path")
     opts.update(read_solver_controls(case_dir))
     opts.update(solver_overrides or {})
+    predictor = opts.pop("predictor", False)
     mesh = read_polymesh(case_dir)
     bnd = mesh["boundary"]
     pm = solid.PolyMesh.from_arrays(mesh["points"], mesh["fOff"], mesh["fVerts"], mesh["owner"], mesh["neighbour"],
@@ -536,7 +537,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
         raise ValueError("mechanicalProperties gives no rho (needed for d2dt2 and g)")
     m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
                                     ls_full_delta="extendedLeastSquares" in grad_scheme, rho=rho, g=g,
-                                    d2dt2=scheme, deltaT=tc["deltaT"] if tc else None)
+                                    d2dt2=scheme, deltaT=tc["deltaT"] if tc else None, predictor=predictor)
     if not transient:
         m.evolve()
         _write_state(m, os.path.join(case_dir, write_time))
@@ -568,6 +569,8 @@ def _write_state(m, out):
     write_vol_field(os.path.join(out, "epsilonEq"), "epsilonEq", "volScalarField", f["epsilonEq"],
                     dims="[0 0 0 0 0 0 0]")
     write_point_field(os.path.join(out, "pointD"), "pointD", f["pointD"])
+    if getattr(m, "transient", False):
+        write_vol_field(os.path.join(out, "U"), "U", "volVectorField", m.U, dims="[0 1 -1 0 0 0 0]")
 
 
 def write_point_field(path, name, values, dims="[0 1 0 0 0 0 0]"):

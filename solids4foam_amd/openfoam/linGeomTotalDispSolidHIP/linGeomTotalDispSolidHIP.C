@@ -273,16 +273,13 @@ linGeomTotalDispSolidHIP::linGeomTotalDispSolidHIP
     rImpK_(1.0/impK_),
     ctx_(nullptr),
     gpuMesh_(nullptr),
-    patchType_()
+    patchType_(),
+    predictor_(false)
 {
     DisRequired();
     // old-time fields of D exist from the start (linGeomTotalDispSolid.C:75-76)
     fvm::d2dt2(D());
-    if (solidModelDict().lookupOrDefault<Switch>("predictor", false))
-    {
-        FatalErrorIn(type() + "::" + type())
-            << "predictor is not available on the device path" << abort(FatalError);
-    }
+    predictor_ = solidModelDict().lookupOrDefault<Switch>("predictor", false);
     check(s4f_ctx_create(0, &ctx_), "linGeomTotalDispSolidHIP()");
     createDeviceMesh();
     D().correctBoundaryConditions();
@@ -304,6 +301,18 @@ bool linGeomTotalDispSolidHIP::evolve()
 
     uploadBoundaryValues();
     const s4f_solve_opts o = solveOptions();
+    if (predictor_)
+    {
+        if (o.d2dt2 != S4F_D2DT2_EULER)
+        {
+            FatalErrorIn(type() + "::evolve()")
+                << "the predictor needs a transient d2dt2 scheme" << abort(FatalError);
+        }
+        // linGeomTotalDispSolid::predict (linGeomTotalDispSolid.C:47-59):
+        // the step starts from D.oldTime() + U deltaT (the device recomputes
+        // gradD and sigma from it)
+        D() = D().oldTime() + U()*runTime().deltaT();
+    }
 
     const label nCells = mesh().nCells();
     std::vector<double> Dv(3*nCells), sig(6*nCells), G(9*nCells);

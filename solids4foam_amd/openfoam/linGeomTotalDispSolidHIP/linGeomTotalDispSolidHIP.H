@@ -16,9 +16,12 @@
     optionally `precision fp64|mixed;`.  The inner PCG reads the D (or "D|DD")
     entry of system/fvSolution (tolerance, relTol, maxIter).
 
-    Supported on the device path: one linearElastic law, fixedDisplacement
-    and solidTraction patches, steady state, no predictor.  Anything else
-    aborts with FatalError, as solidModel::New does for unknown types.
+    Supported on the device path: one linearElastic law (with its rho and
+    constant/g), fixedDisplacement, linearSpatialDisplacement, solidTraction,
+    solidSymmetry, fixedDisplacementZeroShear and empty patches, steady state
+    or the Euler d2dt2 scheme (with the predictor), fixed or Aitken field
+    relaxation.  Anything else aborts with FatalError, as solidModel::New
+    does for unknown types.
 
     After evolve() the OpenFOAM fields D, gradD and sigma are populated, so
     writeFields, end() and the function objects work unchanged.
@@ -52,6 +55,9 @@ class linGeomTotalDispSolidHIP
 
         //- Patch types handed to the device (S4F_PATCH_*)
         labelList patchType_;
+
+        //- linGeomTotalDispSolid's linear predictor (transient only)
+        Switch predictor_;
 
     // Private Member Functions
 

@@ -285,10 +285,11 @@ class LinGeomTotalDispSolid:
     typeName = "linearGeometryTotalDisplacement"
 
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False, rho=None,
-                 g=None, d2dt2="steadyState", deltaT=None):
+                 g=None, d2dt2="steadyState", deltaT=None, predictor=False):
         # rho: mechanicalProperties density (needed with g or a transient
         # d2dt2); g: constant/g; d2dt2: d2dt2Schemes default (steadyState |
-        # Euler); deltaT: the time step of evolve() when transient
+        # Euler); deltaT: the time step of evolve() when transient;
+        # predictor: linGeomTotalDispSolid::predict (:47-59), transient only
         # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
         # "solidSymmetry"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
@@ -318,6 +319,9 @@ class LinGeomTotalDispSolid:
         self.transient = d2dt2 == "Euler"
         if self.transient and not (deltaT and deltaT > 0):
             raise ValueError("a transient model needs deltaT > 0")
+        if predictor and not self.transient:
+            raise ValueError("the predictor needs a transient d2dt2 scheme")
+        self.predictor = predictor
         self.deltaT = deltaT
         self.deltaT0 = deltaT
         self.time = 0.0
@@ -325,6 +329,7 @@ class LinGeomTotalDispSolid:
         self.opts.update(solver or {})
         self.D = np.zeros((polymesh.nCells, 3))
         self.D_old_old = self.D.copy()
+        self.U = np.zeros((polymesh.nCells, 3))  # fvc::ddt(D) after each step (Euler)
         self.sigma = None
         self.gradD = None
         self.report = None
@@ -336,10 +341,13 @@ class LinGeomTotalDispSolid:
         if self.transient:
             self.mesh.set_old_time(self.D, self.D_old_old)
             opts.update(d2dt2=_abi.D2DT2_EULER, deltaT=self.deltaT, deltaT0=self.deltaT0)
-            out = self.mesh.solve(want_sigma=True, want_grad=True, **opts)
+            # predict(): D = D.oldTime() + U deltaT (gradD and sigma follow D)
+            D0 = self.D + self.U * self.deltaT if self.predictor else None
+            out = self.mesh.solve(D0=D0, want_sigma=True, want_grad=True, **opts)
             self.D_old_old = self.D
             self.deltaT0 = self.deltaT
             self.time += self.deltaT
+            self.U = (out["D"] - self.D_old_old) / self.deltaT  # U = fvc::ddt(D), Euler
         else:
             out = self.mesh.solve(D0=self.D if warm_start else None, want_sigma=True, want_grad=True, **opts)
         self.D, self.sigma, self.gradD = out["D"], out["sigma"], out["gradD"]

@@ -472,6 +472,23 @@ def test_transient_and_body_force(ctx, oracle_mod):
         print(f"  transient step {i}: nOuter={out['nOuter']} (oracle {ro['nOuter']}) err={err:.2e}", flush=True)
         assert ro["converged"] and out["converged"] and err < 1e-8
         Dmm, Dm = Dm, out["D"]
+    # the linear predictor (D = D_old + U dt) starts each step elsewhere and
+    # reaches the same fields
+    from solids4foam_amd import LinGeomTotalDispSolid
+    patches = [dict(type="fixedDisplacement")] + [dict(type="solidTraction", traction=bv[pm.patch_faces(p)])
+                                                   for p in range(1, 6)]
+    runs = {}
+    for pred in (False, True):
+        sm = LinGeomTotalDispSolid(ctx, pm, patches, 200e9, 0.3, rho=7800.0, g=grav, d2dt2="Euler", deltaT=2e-5,
+                                   predictor=pred, solver=dict(nCorrectors=100000, solutionTolerance=1e-12,
+                                                               alternativeTolerance=1e-12, pcgTolerance=1e-15))
+        outer = []
+        for _ in range(3):
+            sm.evolve()
+            outer.append(sm.report["nOuter"])
+        runs[pred] = (sm.D.copy(), outer)
+    print(f"  predictor outer iterations {runs[True][1]} (without {runs[False][1]})", flush=True)
+    assert rel_l2(runs[True][0], runs[False][0]) < 1e-9
     # (iii) steady self weight
     oc = oracle_mod.Case(om, 200e9, 0.3, types, np.zeros_like(bv))
     oc.set("rho", np.full(om.nCells, 7800.0))
