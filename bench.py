@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
                     help="summarise rocprofv3 --pmc SQLite outputs (no GPU work) and exit")
     ap.add_argument("--pmc-json", help="with --pmc-summary: also write the per-kernel figures as JSON")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="print a markdown summary of a rocprofv3 --kernel-trace CSV (per-kernel totals and the "
+                         "averages of back-to-back replay runs) and exit")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "pmc_cube200_mixed.json"),
                     help="committed rocprof counter summary that fills roofline.traffic")
     ap.add_argument("--config", choices=["cube", "batch", "serve", "tets"], default="cube",
@@ -121,6 +124,9 @@ def main():
                     help="batch/serve: fixed under-relaxation of D (the sweep reaches nu = 0.45, where the "
                          "unrelaxed segregated loop oscillates)")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.calib_bytes, args.pmc_json,
                     dict(workload="cube 200^3 hex jitter 0.15 seed 2", precision=args.only))
@@ -287,8 +293,9 @@ def main():
 
 
 # bench kernel name -> kernel tag in the rocprof summaries
-PMC_TAGS = {"residual": "k_residual", "spmv": "k_spmvp", "pcg_update": "k_update", "grad": "k_grad",
-            "mg0_down": "k_mg0_down", "mg0_up": "k_mg0_up"}
+PMC_TAGS = {"residual": ("k_residual",), "spmv": ("k_spmvp",), "pcg_update": ("k_update",),
+            "grad": ("k_grad_bnd_pre", "k_grad", "k_grad_bnd_post"),  # the gradient step's three launches
+            "mg0_down": ("k_mg0_down",), "mg0_up": ("k_mg0_up",)}
 
 
 def load_traffic(path, kernel, precision):
@@ -300,15 +307,57 @@ def load_traffic(path, kernel, precision):
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    tag = PMC_TAGS.get(kernel)
-    if not tag:
+    tags = PMC_TAGS.get(kernel)
+    if not tags:
         return None
     suffix = "<f>" if precision == "mixed" else "<d>"
-    for name in (tag + suffix, tag):
-        e = d.get("kernels", {}).get(name)
-        if e and e.get("FETCH_SIZE") is not None and e.get("WRITE_SIZE") is not None:
-            return int(round((e["FETCH_SIZE"] + e["WRITE_SIZE"]) * 1e6))
-    return None
+    total = 0.0
+    for tag in tags:
+        for name in (tag + suffix, tag):
+            e = d.get("kernels", {}).get(name)
+            if e and e.get("FETCH_SIZE") is not None and e.get("WRITE_SIZE") is not None:
+                total += e["FETCH_SIZE"] + e["WRITE_SIZE"]
+                break
+        else:
+            return None
+    return int(round(total * 1e6))
+
+
+def trace_summary(path, min_run=5):
+    """Markdown tables from a rocprofv3 kernel-trace CSV: per kernel the
+    calls, average and total duration; and each run of >= min_run
+    consecutive dispatches of one kernel (the replays of --kernels-only),
+    whose average is what the HIP-event replay timing measures."""
+    import collections
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    name = lambda r: r["Kernel_Name"].split("(")[0].replace("void ", "")
+    dur = lambda r: (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
+    tot = collections.defaultdict(lambda: [0, 0.0])
+    for r in rows:
+        t = tot[name(r)]
+        t[0] += 1
+        t[1] += dur(r)
+    allt = sum(v[1] for v in tot.values())
+    print("| kernel | calls | avg µs | total ms | % |\n|---|---:|---:|---:|---:|")
+    for k, (n, d) in sorted(tot.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {n} | {d / n:.1f} | {d / 1000:.2f} | {100 * d / allt:.1f} |")
+    runs, cur = [], None
+    for r in rows:
+        if cur and cur[0] == name(r):
+            cur[1].append(dur(r))
+        else:
+            if cur:
+                runs.append(cur)
+            cur = [name(r), [dur(r)]]
+    if cur:
+        runs.append(cur)
+    runs = [(k, d) for k, d in runs if len(d) >= min_run and not k.startswith("__amd")]
+    if runs:
+        print("\n| replay run | launches | avg µs |\n|---|---:|---:|")
+        for k, d in runs:
+            print(f"| `{k}` | {len(d)} | {sum(d) / len(d):.1f} |")
 
 
 def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
@@ -339,8 +388,9 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
                 a[2] += dur
             for (k, c), (sv, n, d) in acc.items():
                 short = k.split("(")[0]
-                for tag in ("k_copy", "k_spmvp", "k_update", "k_residual", "k_grad", "k_mg0_down", "k_mg0_up",
-                            "k_mg_down", "k_mg_up", "k_mg_restrict", "k_mg_dense", "k_fin"):
+                for tag in ("k_copy", "k_spmvp", "k_update", "k_residual", "k_grad_bnd_pre", "k_grad_bnd_post",
+                            "k_aitken", "k_grad", "k_mg0_down", "k_mg0_up", "k_mg_down", "k_mg_up", "k_mg_restrict",
+                            "k_mg_dense", "k_fin"):
                     if tag in short:
                         short = tag + ("<f>" if "IfE" in k else "<d>" if "IdE" in k else "")
                         break
