@@ -385,6 +385,63 @@ def read_linear_elastic(case_dir):
     return 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
 
 
+def read_cell_zones(case_dir):
+    """constant/polyMesh/cellZones -> {zone name: cell labels} ({} when absent)"""
+    path = os.path.join(case_dir, "constant", "polyMesh", "cellZones")
+    if not (os.path.exists(path) or os.path.exists(path + ".gz")):
+        return {}
+    n, inner = _read_list_block(read_text(path))
+    out = {}
+    for name, d in _Parser("(" + inner + ")").parse_list():
+        labels = d.get("cellLabels", [])
+        if labels and isinstance(labels[0], tuple):  # List<label> N ( ... )
+            labels = labels[1]
+        out[name] = np.asarray(labels, np.int64)
+    if len(out) != n:
+        raise ValueError(f"cellZones: expected {n} zones, read {len(out)}")
+    return out
+
+
+def _law_constants(law):
+    if law.get("type") != "linearElastic":
+        raise ValueError(f"mechanical law {law.get('type')!r} is not supported (linearElastic)")
+    if "E" in law and "nu" in law:
+        E, nu = dimensioned(law["E"]), dimensioned(law["nu"])
+    else:
+        mu, K = dimensioned(law["mu"]), dimensioned(law["K"])
+        E, nu = 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
+    return E, nu, (dimensioned(law["rho"]) if "rho" in law else None)
+
+
+def read_materials(case_dir, nCells):
+    """(E, nu, rho) of mechanicalProperties: scalars for one law, per-cell
+    arrays when there are several laws, each acting on the cellZone of its
+    name (the reference's solidSubMeshes assignment, mechanicalModel.C:195-245;
+    the device path takes them as a per-cell linearElastic material)."""
+    d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
+    if str(d.get("planeStress", "no")) in ("yes", "on", "true"):
+        raise ValueError("planeStress is not supported on the device path")
+    laws = d.get("mechanical")
+    if not isinstance(laws, list):
+        return _law_constants(laws)
+    if len(laws) == 1:
+        return _law_constants(laws[0][1])
+    zones = read_cell_zones(case_dir)
+    E, nu, rho = np.full(nCells, np.nan), np.full(nCells, np.nan), np.full(nCells, np.nan)
+    for name, law in laws:
+        if name not in zones:
+            raise ValueError(f"mechanical law {name!r} has no cellZone of that name")
+        e, v, r = _law_constants(law)
+        cells = zones[name]
+        if np.any(~np.isnan(E[cells])):
+            raise ValueError(f"cellZone {name!r} overlaps another material's zone")
+        E[cells], nu[cells] = e, v
+        rho[cells] = np.nan if r is None else r
+    if np.any(np.isnan(E)):
+        raise ValueError("some cells belong to no material's cellZone")
+    return E, nu, (None if np.all(np.isnan(rho)) else rho)
+
+
 def read_density(case_dir):
     """rho of the single mechanical law (None when absent)"""
     d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
@@ -517,7 +574,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
                                     mesh["nCells"], [p["startFace"] for p in bnd], [p["nFaces"] for p in bnd])
     Cf = pm.geometry()["Cf"][pm.nInternalFaces:]
     bcs = read_D_boundary(case_dir, bnd, face_centres=Cf)
-    E, nu = read_linear_elastic(case_dir)
+    E, nu, rho = read_materials(case_dir, pm.nCells)
     patches = [dict(type=b["type"], value=b.get("value"), traction=b.get("traction"), pressure=b.get("pressure"))
                for b in bcs]
     ctx = ctx or solid.Context(0)
@@ -530,11 +587,13 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     scheme = read_d2dt2_scheme(case_dir)
     if scheme not in ("steadyState", "Euler"):
         raise ValueError(f"d2dt2 scheme {scheme!r} is not on the device path (steadyState, Euler)")
-    rho, g = read_density(case_dir), read_gravity(case_dir)
+    g = read_gravity(case_dir)
     transient = scheme == "Euler"
     tc = read_time_controls(case_dir) if transient else None
-    if (transient or g is not None) and rho is None:
+    if (transient or g is not None) and (rho is None or np.any(np.isnan(rho))):
         raise ValueError("mechanicalProperties gives no rho (needed for d2dt2 and g)")
+    if rho is not None and np.any(np.isnan(rho)):
+        rho = None
     m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
                                     ls_full_delta="extendedLeastSquares" in grad_scheme, rho=rho, g=g,
                                     d2dt2=scheme, deltaT=tc["deltaT"] if tc else None, predictor=predictor)

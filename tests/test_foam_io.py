@@ -228,3 +228,67 @@ def test_run_case_transient_free_fall(tmp_path):
         D = foam_io._field_value(foam_io.read_dict(os.path.join(d, name, "D"))["internalField"], pm.nCells, 3)
         exact = n * (n + 1) / 2 * g * 1e-6
         assert np.abs(D - exact).max() < 1e-9 * np.abs(exact).max()
+
+
+MECH_TWO = """
+FoamFile { version 2.0; format ascii; class dictionary; object mechanicalProperties; }
+planeStress no;
+mechanical
+(
+    steel { type linearElastic; rho rho [1 -3 0 0 0 0 0] 7800; E E [1 -1 -2 0 0 0 0] 200e+09; nu nu [0 0 0 0 0 0 0] 0.3; }
+    rubber { type linearElastic; rho rho [1 -3 0 0 0 0 0] 1200; E E [1 -1 -2 0 0 0 0] 1e+09; nu nu [0 0 0 0 0 0 0] 0.45; }
+);
+"""
+
+
+def _zones_file(zones):
+    body = "\n".join(f"{name}\n{{\n    type cellZone;\n    cellLabels List<label> {len(c)}({' '.join(map(str, c))});\n}}"
+                     for name, c in zones)
+    return ("FoamFile { version 2.0; format ascii; class regIOobject; object cellZones; }\n"
+            f"{len(zones)}\n(\n{body}\n)\n")
+
+
+def test_multi_material_cell_zones(tmp_path):
+    # several laws: each acts on the cellZone of its name, per cell
+    d = str(tmp_path)
+    pm, a = _box_case(d)
+    C = pm.nCells
+    soft = np.arange(C)[np.arange(C) % 3 == 0]
+    hard = np.setdiff1d(np.arange(C), soft)
+    _write(d, "constant/mechanicalProperties", MECH_TWO)
+    _write(d, "constant/polyMesh/cellZones", _zones_file([("steel", hard), ("rubber", soft)]))
+    assert set(foam_io.read_cell_zones(d)) == {"steel", "rubber"}
+    E, nu, rho = foam_io.read_materials(d, C)
+    assert np.all(E[hard] == 200e9) and np.all(E[soft] == 1e9)
+    assert np.all(nu[soft] == 0.45) and np.all(rho[hard] == 7800.0)
+    _write(d, "constant/polyMesh/cellZones", _zones_file([("steel", hard[1:]), ("rubber", soft)]))
+    with pytest.raises(ValueError, match="no material"):
+        foam_io.read_materials(d, C)
+
+
+@pytest.mark.gpu
+def test_run_case_multi_material(tmp_path):
+    # two laws on two cellZones: the case runs as a per-cell material and
+    # matches the oracle with the same per-cell E and nu
+    import oracle
+    from solids4foam_amd import Context
+    d = str(tmp_path)
+    n = (6, 5, 4)
+    pm, a = _box_case(d, n=n, jitter=0.1)
+    C = pm.nCells
+    soft = np.arange(C)[np.arange(C) % 3 == 0]
+    hard = np.setdiff1d(np.arange(C), soft)
+    _write(d, "constant/mechanicalProperties", MECH_TWO)
+    _write(d, "constant/polyMesh/cellZones", _zones_file([("steel", hard), ("rubber", soft)]))
+    tight = dict(nCorrectors=50000, solutionTolerance=1e-12, alternativeTolerance=1e-12, pcgTolerance=1e-14)
+    m = foam_io.run_case(d, ctx=Context(0), solver_overrides=tight)
+    assert m.report["converged"]
+    E = np.where(np.isin(np.arange(C), soft), 1e9, 200e9)
+    nu = np.where(np.isin(np.arange(C), soft), 0.45, 0.3)
+    om = oracle.Mesh.box(*n, (1.0, 1.0, 1.0), 0.1, 3)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(5)] = (0.0, 0.0, 1e7)
+    oc = oracle.Case(om, E, nu, [1, 1, 1, 1, 0, 1], bv, rc_scale=0.2)
+    oc.solve(nCorr=50000, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, rcScale=0.2, precond=oracle.PRECOND_DIC)
+    Do = oc.get("D").reshape(-1, 3)
+    assert np.linalg.norm(m.D - Do) / np.linalg.norm(Do) < 1e-8
