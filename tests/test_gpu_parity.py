@@ -577,6 +577,50 @@ def test_divergence_and_maxiter_status(ctx):
     assert out["status"] == _abi.S4F_WARN_MAXITER and out["nOuter"] == 3 and not out["converged"]
 
 
+def test_invalid_options_are_refused(ctx):
+    # option and state errors come back as S4F_ERR_INVALID with a message,
+    # before any kernel runs
+    from solids4foam_amd import _abi
+    from solids4foam_amd.batch import Batch
+    from solids4foam_amd.serve import Server
+    pm, m, _, _ = _product_mesh(ctx, (4, 4, 4), jitter=0.1, seed=1)
+    for bad in (dict(relaxationMethod=2), dict(d2dt2=3), dict(eagerLaunch=2), dict(relaxD=1.5),
+                dict(mgCoarseScale=3.0), dict(d2dt2=1, deltaT=1e-3)):  # last: no density / old-time fields
+        with pytest.raises(_abi.S4FError) as e:
+            m.solve(**bad)
+        assert e.value.status == _abi.S4F_ERR_INVALID, bad
+    m.set_density(7800.0)
+    with pytest.raises(_abi.S4FError, match="old-time"):
+        m.solve(d2dt2=1, deltaT=1e-3)
+    with pytest.raises(_abi.S4FError, match="deltaT"):
+        m.set_old_time(np.zeros((pm.nCells, 3)))
+        m.solve(d2dt2=1, deltaT=0.0)
+    with pytest.raises(_abi.S4FError):
+        Server(ctx, 64 << 20, max_slots=2, d2dt2=1, deltaT=1e-3)
+    # body force: single meshes only; a mesh with gravity set is refused by a batch
+    m.set_gravity((0.0, 0.0, -9.81))
+    _, m2, _, _ = _product_mesh(ctx, (4, 4, 4), jitter=0.1, seed=2)
+    b = Batch(ctx)
+    with pytest.raises(_abi.S4FError):
+        b.add(m)
+        b.add(m2)
+        b.solve()
+    b.close()
+    # a mesh whose last solve was transient is refused by the server; the
+    # steady solve after it is the plain steady solve
+    m.set_gravity((0.0, 0.0, 0.0))
+    out = m.solve(d2dt2=1, deltaT=1e-3, solutionTolerance=1e-10, alternativeTolerance=1e-10)
+    assert out["converged"]
+    srv = Server(ctx, 64 << 20, max_slots=2)
+    with pytest.raises(_abi.S4FError, match="static"):
+        srv.submit(m, np.zeros((pm.nBoundaryFaces, 3)))
+    srv.close()
+    a = m.solve(solutionTolerance=1e-10, alternativeTolerance=1e-10)
+    _, m3, _, _ = _product_mesh(ctx, (4, 4, 4), jitter=0.1, seed=1)
+    ref = m3.solve(solutionTolerance=1e-10, alternativeTolerance=1e-10)
+    assert a["nOuter"] == ref["nOuter"] and np.array_equal(a["D"], ref["D"])
+
+
 def test_ldu_solver_matches_oracle_pcg(ctx, oracle_mod):
     # lduMatrix::solver seam: one component of the oracle's M D = M D0 + R,
     # PCG with the diagonal preconditioner from the warm start D0, relTol 0.1:

@@ -94,6 +94,30 @@ def test_server_multigrid_equals_alone(ctx, precision):
         assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
 
 
+def test_server_aitken_equals_alone(ctx):
+    # Aitken relaxation state lives per cell and boundary face of each
+    # request's arena range: a served request gets the bits it gets alone
+    from solids4foam_amd import _abi
+    from solids4foam_amd.serve import Server
+    meshes = [_mesh(ctx, n, **kw) for n, kw in _specs()[:4]]
+    opts = dict(OPTS, relaxationMethod=_abi.RELAX_AITKEN)
+    alone = [m.solve(**opts) for m, _ in meshes]
+    srv = Server(ctx, 256 << 20, max_slots=2, **opts)
+    ids = [srv.submit(m, bv) for m, bv in meshes]
+    got = {}
+    t0 = time.time()
+    while len(got) < len(ids) and time.time() - t0 < 120:
+        rid = srv.poll(100)
+        if rid is not None:
+            got[rid] = srv.result(rid)
+    srv.close()
+    assert len(got) == len(ids)
+    for rid, a in zip(ids, alone):
+        r = got[rid]
+        assert r["converged"] and r["nOuter"] == a["nOuter"]
+        assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
+
+
 def test_server_admission_by_footprint(ctx):
     from solids4foam_amd import S4FError, _abi
     from solids4foam_amd.serve import Server
