@@ -193,6 +193,29 @@ def _nu_box(ctx, oracle_mod, n, nu):
     return oc, m
 
 
+def test_first_iterations_follow_oracle(ctx, oracle_mod):
+    # with the oracle on the same (diagonal) preconditioner the device loop
+    # takes the oracle's path: per outer iteration the same |initRes|,
+    # residualvf and PCG iteration counts, for a loaded cube, a moved box (its
+    # first residual sees the zero initial stress) and a tetrahedral mesh
+    print("[history]", flush=True)
+    cases = [("cube", _oracle_case(oracle_mod, (7, 6, 5), jitter=0.15, seed=3)[1],
+              _product_mesh(ctx, (7, 6, 5), jitter=0.15, seed=3)[1]),
+             ("tets", _oracle_case(oracle_mod, (3, 3, 3), jitter=0.1, seed=5, tets=True, ls_full_delta=True)[1],
+              _product_mesh(ctx, (3, 3, 3), jitter=0.1, seed=5, tets=True, ls_full_delta=True)[1])]
+    oc, m = _moved_box(ctx, oracle_mod, 5, 0.3)
+    cases.append(("moved", oc, m))
+    for name, oc, m in cases:
+        ro = oc.solve(history=12, nCorr=12, precond=oracle_mod.PRECOND_DIAGONAL)
+        m.solve(nCorrectors=12)
+        h = m.history()
+        n = min(len(h["initRes"]), len(ro["history"]["initRes"]))
+        assert n >= 10, name
+        assert np.allclose(h["initRes"][:n], ro["history"]["initRes"][:n], rtol=1e-8, atol=1e-14), name
+        assert np.allclose(h["rvf"][:n], ro["history"]["rvf"][:n], rtol=1e-8), name
+        assert np.array_equal(h["inner"][:n], ro["history"]["inner"][:n]), name
+
+
 def test_field_relaxation_follows_oracle(ctx, oracle_mod):
     # fixed under-relaxation of D (relaxField): the first outer iterations
     # reproduce the oracle's history, and at nu = 0.45 (where the unrelaxed
