@@ -2531,6 +2531,10 @@ struct s4f_server {
     DArr<int> nbmInst, nbmS0, nbmS1, nfinList;  // init map of newly admitted instances
     std::vector<int> hiBlk0, hiNBlk, hiCell0, hiNCell, hiB0, hiNB;
     int nTiles = 0, nActive = 0, chunk = 4;
+    // launch sequences of a step replayed from HIP graphs while the packed
+    // map they were captured on stands (mapGen counts rebuilds)
+    GraphExec gStart, gPair, gEnd;
+    unsigned mapGen = 0, capGen = ~0u, ranGen = ~0u;
     Ctl* hctl = nullptr;
     // bookkeeping (server thread only)
     std::vector<Placement> places;
@@ -3048,6 +3052,7 @@ MgRun serverRun(s4f_server* S) {
 }
 
 void serverRebuildMap(s4f_server* S) {
+    S->mapGen++;  // tiles, multigrid packing and slot set change: captured graphs are stale
     std::vector<int> slots;
     for (int i = 0; i < S->maxSlots; ++i)
         if (S->slotReq[i] >= 0) slots.push_back(i);
@@ -3075,14 +3080,48 @@ void serverStep(s4f_server* S) {
     std::vector<int> slots;
     for (int i = 0; i < S->maxSlots; ++i)
         if (S->slotReq[i] >= 0) slots.push_back(i);
-    hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const double*)S->D0.p,
-                       (const double*)S->D1.p, S->G.p, S->sigb.p, v.r, S->ctl.p, S->part.p);
-    hipLaunchKernelGGL(k_fin_residual, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p);
-    if (S->mgOn) launchVcycle<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, s);  // z0 = V(r0), rho0
+    // one outer iteration of every active request: explicit terms and the
+    // first V-cycle, PCG iterations (in pairs, so the search-direction
+    // buffers return to their places), the gradient step
+    auto launchStart = [&](hipStream_t q) {
+        hipLaunchKernelGGL((k_residual<T>), dim3(bm.nBlocks), dim3(kBlock), 0, q, dm, bm, (const double*)S->D0.p,
+                           (const double*)S->D1.p, S->G.p, S->sigb.p, v.r, S->ctl.p, S->part.p);
+        hipLaunchKernelGGL(k_fin_residual, dim3(nA), dim3(kFinBlock), 0, q, bm, S->ctl.p, (const double*)S->part.p);
+        if (S->mgOn) launchVcycle<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, q);  // z0 = V(r0), rho0
+    };
+    auto launchPcg = [&](int n, hipStream_t q) {
+        if (S->mgOn) launchPcgChunkMg<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, n, q);
+        else launchPcgChunk<T>(dm, bm, nA, v, S->ctl.p, S->part.p, n, q);
+    };
+    auto launchEnd = [&](hipStream_t q) {
+        launchGradStep<T>(dm, bm, nA, serverMaxNB(S), serverGradBufs(S), (const T*)v.x, (const double*)S->Dold.p,
+                          S->ctl.p, S->part.p, 0, q);
+        hipLaunchKernelGGL(k_fin_grad, dim3(nA), dim3(kFinBlock), 0, q, bm, S->ctl.p, (const double*)S->part.p,
+                           (double*)nullptr, 0);
+    };
+    // the first step on a new map runs eagerly (it allocates the multigrid
+    // vectors of the packing); the next one captures
+    bool graphs = false;
+    if (S->opts.eagerLaunch == 0 && S->ranGen == S->mapGen) {
+        if (S->capGen != S->mapGen) {
+            S->gStart.capture(S->ctx->capture, launchStart);
+            S->gPair.capture(S->ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
+            S->gEnd.capture(S->ctx->capture, launchEnd);
+            S->capGen = S->mapGen;
+        }
+        graphs = true;
+    }
+    if (graphs) S->gStart.launch(s);
+    else launchStart(s);
     int total = 0, chunk = S->chunk;
     for (;;) {
-        if (S->mgOn) launchPcgChunkMg<T>(serverRun(S), dm, bm, v, S->ctl.p, S->part.p, chunk, s);
-        else launchPcgChunk<T>(dm, bm, nA, v, S->ctl.p, S->part.p, chunk, s);
+        if (graphs) {
+            const int pairs = (chunk + 1) / 2;
+            for (int k = 0; k < pairs; ++k) S->gPair.launch(s);
+            chunk = 2 * pairs;
+        } else {
+            launchPcg(chunk, s);
+        }
         total += chunk;
         chunk = std::max(2, chunk / 4);
         HIP_CHECK(hipMemcpyAsync(S->hctl, S->ctl.p, S->maxSlots * sizeof(Ctl), hipMemcpyDeviceToHost, s));
@@ -3096,12 +3135,10 @@ void serverStep(s4f_server* S) {
     for (int sl : slots)
         if (!S->hctl[sl].outerDone) used = std::max({used, S->hctl[sl].nIter[0], S->hctl[sl].nIter[1], S->hctl[sl].nIter[2]});
     S->chunk = std::max(1, used - 1);
-    launchGradStep<T>(dm, bm, nA, serverMaxNB(S),
-                      serverGradBufs(S),
-                      (const T*)v.x, (const double*)S->Dold.p, S->ctl.p, S->part.p, 0, s);
-    hipLaunchKernelGGL(k_fin_grad, dim3(nA), dim3(kFinBlock), 0, s, bm, S->ctl.p, (const double*)S->part.p,
-                       (double*)nullptr, 0);
+    if (graphs) S->gEnd.launch(s);
+    else launchEnd(s);
     HIP_CHECK(hipGetLastError());
+    S->ranGen = S->mapGen;
     HIP_CHECK(hipMemcpyAsync(S->hctl, S->ctl.p, S->maxSlots * sizeof(Ctl), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     // retire
