@@ -1436,7 +1436,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         if (mg) launchPcgChunkMg<T>(mgRun, dm, bm, v, st.ctl.p, st.part.p, n, q);
         else launchPcgChunk<T>(dm, bm, nFin, v, st.ctl.p, st.part.p, n, q);
     };
-    const bool graphs = o.eagerLaunch == 0;
+    bool graphs = o.eagerLaunch == 0;
     GraphExec gOuter, gPair;
     bool captured = false;
     int chunk = 4;
@@ -1485,9 +1485,16 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             }
         }
         if (graphs && !captured) {
-            gOuter.capture(ctx->capture, [&](hipStream_t q) { launchGradResidual(q); launchResidual(q); });
-            gPair.capture(ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
-            captured = true;
+            try {
+                gOuter.capture(ctx->capture, [&](hipStream_t q) { launchGradResidual(q); launchResidual(q); });
+                gPair.capture(ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
+                captured = true;
+            } catch (const HipError&) {  // a runtime without capture: the same launches, one by one
+                gOuter.reset();
+                gPair.reset();
+                (void)hipGetLastError();
+                graphs = false;
+            }
         }
         if (captured) {
             gOuter.launch(s);
@@ -3103,13 +3110,22 @@ void serverStep(s4f_server* S) {
     // vectors of the packing); the next one captures
     bool graphs = false;
     if (S->opts.eagerLaunch == 0 && S->ranGen == S->mapGen) {
-        if (S->capGen != S->mapGen) {
-            S->gStart.capture(S->ctx->capture, launchStart);
-            S->gPair.capture(S->ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
-            S->gEnd.capture(S->ctx->capture, launchEnd);
-            S->capGen = S->mapGen;
-        }
         graphs = true;
+        if (S->capGen != S->mapGen) {
+            try {
+                S->gStart.capture(S->ctx->capture, launchStart);
+                S->gPair.capture(S->ctx->capture, [&](hipStream_t q) { launchPcg(2, q); });
+                S->gEnd.capture(S->ctx->capture, launchEnd);
+                S->capGen = S->mapGen;
+            } catch (const HipError&) {  // a runtime without capture: the same launches, one by one
+                S->gStart.reset();
+                S->gPair.reset();
+                S->gEnd.reset();
+                (void)hipGetLastError();
+                S->opts.eagerLaunch = 1;
+                graphs = false;
+            }
+        }
     }
     if (graphs) S->gStart.launch(s);
     else launchStart(s);
