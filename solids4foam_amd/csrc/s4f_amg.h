@@ -11,15 +11,18 @@
 // instance's rows contiguous, and each row looks up its instance's control
 // block (converged instances and components are skipped).
 //
-// Launch shape per level (vectors hold 3 components, SoA, stride = the
-// level's row count; converged components are skipped):
-//   down: x = w D^-1 b and r = b - A x in one pass (the neighbours' x is
-//         recomputed from their b and 1/diag), then the restriction
-//         b_c[I] = sum_{i in I} r[i] (members gathered, no atomics)
+// Launch shape per level (level 0: 3 components, SoA, stride = the row
+// count, converged components skipped; coarser levels: rows of four):
+//   restriction: b_c[I] = sum_{i in I} r[i] (members gathered, no atomics)
+//         and the pre-smoothed value x_c = w D_c^-1 b_c in the same pass
+//   down: r = b - A x (on level 0 the neighbours' x = w D^-1 b is recomputed
+//         from their b and 1/diag)
 //   up:   prolongation x += e[agg] fused into the post-smoothing sweep (the
 //         neighbours' corrected x recomputed on the fly)
 // Level 0 is the solver mesh's own sliced-ELL matrix; coarser levels have
-// their own sliced-ELL arrays.
+// their own sliced-ELL arrays.  Every row's arithmetic runs in a fixed order
+// that depends only on its own instance, so results do not depend on which
+// other instances share a launch.
 #pragma once
 
 #include "s4f_kernels.h"
@@ -72,6 +75,21 @@ template <typename T> __device__ __forceinline__ T mulRn(T a, T b);
 template <> __device__ __forceinline__ float mulRn<float>(float a, float b) { return __fmul_rn(a, b); }
 template <> __device__ __forceinline__ double mulRn<double>(double a, double b) { return __dmul_rn(a, b); }
 
+// Coarse-level vectors are rows of four (three components and a zero pad):
+// one 16-byte (FP32) or 32-byte (FP64) load per neighbour instead of three
+// component gathers.  The fine level keeps the PCG's component-major layout.
+template <typename T> struct Row4Sel;
+template <> struct Row4Sel<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Row4Sel<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <typename T> using Row4 = typename Row4Sel<T>::type;
+
+template <typename T>
+__device__ __forceinline__ Row4<T> row4(T a, T b, T c) {
+    Row4<T> v;
+    v[0] = a; v[1] = b; v[2] = c; v[3] = (T)0;
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // level 0 (the mesh's own matrix, BlockMap iteration)
 // ---------------------------------------------------------------------------
@@ -115,8 +133,8 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
 
 template <typename T, int W, bool COARSE>
 __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
-                                            const int* __restrict__ agg, int nCoarse,
-                                            const T* __restrict__ e, T* __restrict__ z, T w, T w1, T al,
+                                            const int* __restrict__ agg,
+                                            const Row4<T>* __restrict__ e, T* __restrict__ z, T w, T w1, T al,
                                             double* acc) {
     const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
@@ -129,11 +147,12 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int
         a[k] = A[en];
     }
     // pre-smoothed value from zero, recomputed exactly as k_mg0_down forms it
-    auto xf = [&](int j, T wi, T bv, int I) -> T {
+    auto xf = [&](int j, T wi, T bv, const Row4<T>& ev) -> T {
         const T v = mulRn(wi, bv);
-        return COARSE ? v + al * e[(size_t)j * nCoarse + I] : v;
+        return COARSE ? v + al * ev[j] : v;
     };
-    const int IP = COARSE ? agg[P] : 0;
+    Row4<T> eP = {};
+    if (COARSE) eP = e[agg[P]];
     const T iD = invD[P];
     const T wP = w * iD;
     T bP[3], sum[3];
@@ -147,14 +166,15 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int
         const bool v = nb[k] >= 0;
         const int c = v ? nb[k] : P;
         const T av = v ? a[k] : (T)0;
-        const int I = COARSE ? agg[c] : 0;
+        Row4<T> ec = {};
+        if (COARSE) ec = e[agg[c]];
         const T wc = w * invD[c];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, wc, b[(size_t)j * C + c], I);
+        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, wc, b[(size_t)j * C + c], ec);
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T zv = w1 * xf(j, wP, bP[j], IP) + w * iD * sum[j];
+        const T zv = w1 * xf(j, wP, bP[j], eP) + w * iD * sum[j];
         z[(size_t)j * C + P] = zv;
         acc[j] += (double)zv * (double)bP[j];
     }
@@ -226,8 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
 // (b is the PCG residual: z.b = rho of the preconditioned residual)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
-                                                   const int* __restrict__ agg, int nCoarse,
-                                                   const T* __restrict__ e, T* __restrict__ z, double omega,
+                                                   const int* __restrict__ agg,
+                                                   const Row4<T>* __restrict__ e, T* __restrict__ z, double omega,
                                                    double alpha, const Ctl* __restrict__ ctl,
                                                    double* __restrict__ part) {
     const int lb = logicalBlock(bm.nBlocks);
@@ -242,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
     auto xf = [&](int j, int cell) -> T {
         const T v = mulRn(w * invD[cell], b[(size_t)j * C + cell]);
-        return e ? v + al * e[(size_t)j * nCoarse + agg[cell]] : v;
+        return e ? v + al * e[agg[cell]][j] : v;
     };
     double acc[3] = {0, 0, 0};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -254,13 +274,13 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (all && width == 6) {
-            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
+            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
             continue;
         }
         if (all && width == 4) {
-            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, agg, nCoarse, e, z, w, w1, al, acc);
+            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
+            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
             continue;
         }
         T sum[3];
@@ -293,15 +313,20 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
 // coarse levels (grid-stride over the slices of one level)
 // ---------------------------------------------------------------------------
 
-// restriction: bc[I] = sum over members i of I of r[i]   (r of the finer level)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const T* __restrict__ r, int nFine,
-                                                        T* __restrict__ bc, const Ctl* __restrict__ ctl) {
+// restriction: bc[I] = sum over members i of I of r[i] (r of the finer
+// level: component-major on level 0 when FINE, rows of four otherwise), and
+// the coarse pre-smoothed value xc[I] = w D^-1 bc[I] (the coarse level's
+// smoothing from zero, formed once here instead of at every use)
+template <typename T, bool FINE>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const T* __restrict__ r0,
+                                                        const Row4<T>* __restrict__ r4, int nFine,
+                                                        Row4<T>* __restrict__ bc, Row4<T>* __restrict__ xc,
+                                                        double omega, const Ctl* __restrict__ ctl) {
+    const T* __restrict__ iDg = MgSel<T>::iD(Lc);
+    const T w = (T)omega;
     for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
         if (Lc.rowInst[I] < 0) continue;  // padding row
-        const Ctl* c = ctl + Lc.rowInst[I];
-        if (mgSkip(c)) continue;
-        const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+        if (mgSkip(ctl + Lc.rowInst[I])) continue;
         T s[3] = {(T)0, (T)0, (T)0};
         const int k1 = Lc.memOff[I + 1];
         // members 8 at a time, loads unconditional (past the end: the last
@@ -312,35 +337,43 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
             for (int u = 0; u < 8; ++u) mi[u] = Lc.mem[k0 + u < k1 ? k0 + u : k1 - 1];
             T rv[8][3];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < 8; ++u) {
+                if (FINE) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) rv[u][j] = r[(size_t)j * nFine + mi[u]];
+                    for (int j = 0; j < 3; ++j) rv[u][j] = r0[(size_t)j * nFine + mi[u]];
+                } else {
+                    const Row4<T> v = r4[mi[u]];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) rv[u][j] = v[j];
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
                     if (k0 + u < k1) s[j] += rv[u][j];
         }
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (act[j]) bc[(size_t)j * Lc.n + I] = s[j];
+        // all three components stored: a converged component's coarse
+        // values are never read for an active one
+        const T wi = w * iDg[I];
+        bc[I] = row4(s[0], s[1], s[2]);
+        xc[I] = row4(wi * s[0], wi * s[1], wi * s[2]);
     }
 }
 
 // Coarse-level rows are wider (Galerkin stencils) and of varying width:
 // slots are processed 8 at a time with every load unconditional (slots past
 // the row's width or empty read an in-bounds slot and the row's own value
-// with a zero coefficient), all three components computed, active ones stored.
+// with a zero coefficient).
 constexpr int kMgChunk = 8;
 
-// down on a coarse level: x = w D^-1 b, r = b - A x
+// down on a coarse level: r = b - A x with x = w D^-1 b from the restriction
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __restrict__ b, T* __restrict__ x,
-                                                    T* __restrict__ r, double omega, const Ctl* __restrict__ ctl) {
+__global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const Row4<T>* __restrict__ b,
+                                                    const Row4<T>* __restrict__ x, Row4<T>* __restrict__ r,
+                                                    const Ctl* __restrict__ ctl) {
     const T* __restrict__ A = MgSel<T>::a(L);
-    const T* __restrict__ iDg = MgSel<T>::iD(L);
     const T* __restrict__ dg = MgSel<T>::d(L);
-    const T w = (T)omega;
     const int n = L.n;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wavesTotal = gridDim.x * (kBlock / 64);
@@ -349,25 +382,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
         if (P >= n) continue;
         const int inst = L.rowInst[P];
         if (inst < 0) continue;  // padding row
-        const Ctl* cP = ctl + inst;
-        if (mgSkip(cP)) continue;
-        const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
+        if (mgSkip(ctl + inst)) continue;
+        const Row4<T> bP = b[P];
         if (L.pass[inst]) {  // already coarsest for this instance: hand b down unchanged
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                if (act[j]) r[(size_t)j * n + P] = b[(size_t)j * n + P];
+            r[P] = bP;
             continue;
         }
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
-        const T iD = iDg[P];
-        T bP[3], xP[3], y[3];
+        const Row4<T> xP = x[P];
+        const T d = dg[P];
+        T y[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            bP[j] = b[(size_t)j * n + P];
-            xP[j] = w * iD * bP[j];
-            y[j] = dg[P] * xP[j];
-        }
+        for (int j = 0; j < 3; ++j) y[j] = d * xP[j];
         for (int k0 = 0; k0 < width; k0 += kMgChunk) {
             int nb[kMgChunk];
             T a[kMgChunk];
@@ -381,28 +408,22 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const T* __
 #pragma unroll
             for (int u = 0; u < kMgChunk; ++u) {
                 const bool v = nb[u] >= 0;
-                const int c = v ? nb[u] : P;
+                const Row4<T> xc = x[v ? nb[u] : P];
                 const T av = v ? a[u] : (T)0;
-                const T wi = w * iDg[c];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) y[j] -= av * (wi * b[(size_t)j * n + c]);
+                for (int j = 0; j < 3; ++j) y[j] -= av * xc[j];
             }
         }
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            if (!act[j]) continue;
-            const size_t i = (size_t)j * n + P;
-            x[i] = xP[j];
-            r[i] = bP[j] - y[j];
-        }
+        r[P] = row4(bP[0] - y[0], bP[1] - y[1], bP[2] - y[2]);
     }
 }
 
 // up on a coarse level: xf = x + e[agg], out = (1 - w) xf + w D^-1 (b + sum a xf_N)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
-                                                  int nCoarse, const T* __restrict__ e, T* __restrict__ out,
-                                                  double omega, double alpha, const Ctl* __restrict__ ctl) {
+__global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>* __restrict__ b,
+                                                  const Row4<T>* __restrict__ x, const Row4<T>* __restrict__ e,
+                                                  Row4<T>* __restrict__ out, double omega, double alpha,
+                                                  const Ctl* __restrict__ ctl) {
     const T* __restrict__ A = MgSel<T>::a(L);
     const T* __restrict__ iDg = MgSel<T>::iD(L);
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
@@ -415,23 +436,20 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
         if (P >= n) continue;
         const int inst = L.rowInst[P];
         if (inst < 0) continue;  // padding row
-        const Ctl* cP = ctl + inst;
-        if (mgSkip(cP)) continue;
-        const bool act[3] = {cP->active[0] != 0, cP->active[1] != 0, cP->active[2] != 0};
-        const int IP = agg[P];
+        if (mgSkip(ctl + inst)) continue;
+        const Row4<T> eP = e[agg[P]];
         if (L.pass[inst]) {  // already coarsest for this instance: the coarser solution unchanged
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                if (act[j]) out[(size_t)j * n + P] = e[(size_t)j * nCoarse + IP];
+            out[P] = eP;
             continue;
         }
         const int e0 = L.sliceStart[s];
         const int width = L.sliceWidth[s];
+        const Row4<T> bP = b[P], xv = x[P];
         T sum[3], xP[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            sum[j] = b[(size_t)j * n + P];
-            xP[j] = x[(size_t)j * n + P] + al * e[(size_t)j * nCoarse + IP];
+            sum[j] = bP[j];
+            xP[j] = xv[j] + al * eP[j];
         }
         for (int k0 = 0; k0 < width; k0 += kMgChunk) {
             int nb[kMgChunk];
@@ -448,48 +466,42 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const T* __re
                 const bool v = nb[u] >= 0;
                 const int c = v ? nb[u] : P;
                 const T av = v ? a[u] : (T)0;
-                const int I = agg[c];
+                const Row4<T> xc = x[c], ec = e[agg[c]];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) sum[j] += av * (x[(size_t)j * n + c] + al * e[(size_t)j * nCoarse + I]);
+                for (int j = 0; j < 3; ++j) sum[j] += av * (xc[j] + al * ec[j]);
             }
         }
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (act[j]) out[(size_t)j * n + P] = w1 * xP[j] + w * iDg[P] * sum[j];
+        const T wi = w * iDg[P];
+        out[P] = row4(w1 * xP[0] + wi * sum[0], w1 * xP[1] + wi * sum[1], w1 * xP[2] + wi * sum[2]);
     }
 }
 
 // coarsest level: x = A^-1 b with the explicit (symmetric) inverse of the
-// row's instance block, one wavefront per row
+// row's instance block, one wavefront per row (each lane's terms in column
+// order, then the wavefront reduction)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_dense(const MgLevel L, const T* __restrict__ Ainv,
-                                                     const T* __restrict__ b, T* __restrict__ x,
+                                                     const Row4<T>* __restrict__ b, Row4<T>* __restrict__ x,
                                                      const Ctl* __restrict__ ctl) {
     const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= L.n) return;
     const int inst = L.rowInst[row];
     if (inst < 0) return;  // padding row
-    const Ctl* c = ctl + inst;
-    if (mgSkip(c)) return;
-    const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    if (mgSkip(ctl + inst)) return;
     const int r0 = L.blkRow0[inst], nb = L.blkN[inst];
-    const int n = L.n;
     double acc[3] = {0, 0, 0};
     const T* __restrict__ a = Ainv + (size_t)L.invOff[inst] + (size_t)(row - r0) * nb;
+#pragma unroll 4
     for (int k = lane; k < nb; k += 64) {
         const double v = (double)a[k];
+        const Row4<T> bk = b[r0 + k];
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (act[j]) acc[j] += v * (double)b[(size_t)j * n + r0 + k];
+        for (int j = 0; j < 3; ++j) acc[j] += v * (double)bk[j];
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[j] = waveReduce<false>(acc[j]);
-    if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (act[j]) x[(size_t)j * n + row] = (T)acc[j];
-    }
+    if (lane == 0) x[row] = row4((T)acc[0], (T)acc[1], (T)acc[2]);
 }
 
 // rho' = z.r finalize for a V-cycle-preconditioned PCG: beta = rho'/rho

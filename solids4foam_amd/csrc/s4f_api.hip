@@ -1056,23 +1056,35 @@ void buildAmg(s4f_mesh* M) {
     A.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// V-cycle work vectors: level 0 holds only the residual (component-major,
+// like the PCG vectors); coarser levels hold rows of four
 template <typename T>
 struct MgVec;
 template <>
 struct MgVec<double> {
-    static double* x(AmgDevLevel& L) { L.x64.alloc(3 * (size_t)L.n); return L.x64.p; }
-    static double* b(AmgDevLevel& L) { L.b64.alloc(3 * (size_t)L.n); return L.b64.p; }
-    static double* r(AmgDevLevel& L) { L.r64.alloc(3 * (size_t)L.n); return L.r64.p; }
-    static double* xs(AmgDevLevel& L) { L.xs64.alloc(3 * (size_t)L.n); return L.xs64.p; }
+    static double* r0(AmgDevLevel& L) { L.r64.alloc(3 * (size_t)L.n); return L.r64.p; }
+    static Row4<double>* x(AmgDevLevel& L) { return v4(L.x64, L.n); }
+    static Row4<double>* b(AmgDevLevel& L) { return v4(L.b64, L.n); }
+    static Row4<double>* r(AmgDevLevel& L) { return v4(L.r64, L.n); }
+    static Row4<double>* xs(AmgDevLevel& L) { return v4(L.xs64, L.n); }
     static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
+    static Row4<double>* v4(DArr<double>& a, int n) {
+        a.alloc(4 * (size_t)n);
+        return reinterpret_cast<Row4<double>*>(a.p);
+    }
 };
 template <>
 struct MgVec<float> {
-    static float* x(AmgDevLevel& L) { L.x32.alloc(3 * (size_t)L.n); return L.x32.p; }
-    static float* b(AmgDevLevel& L) { L.b32.alloc(3 * (size_t)L.n); return L.b32.p; }
-    static float* r(AmgDevLevel& L) { L.r32.alloc(3 * (size_t)L.n); return L.r32.p; }
-    static float* xs(AmgDevLevel& L) { L.xs32.alloc(3 * (size_t)L.n); return L.xs32.p; }
+    static float* r0(AmgDevLevel& L) { L.r32.alloc(3 * (size_t)L.n); return L.r32.p; }
+    static Row4<float>* x(AmgDevLevel& L) { return v4(L.x32, L.n); }
+    static Row4<float>* b(AmgDevLevel& L) { return v4(L.b32, L.n); }
+    static Row4<float>* r(AmgDevLevel& L) { return v4(L.r32, L.n); }
+    static Row4<float>* xs(AmgDevLevel& L) { return v4(L.xs32, L.n); }
     static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
+    static Row4<float>* v4(DArr<float>& a, int n) {
+        a.alloc(4 * (size_t)n);
+        return reinterpret_cast<Row4<float>*>(a.p);
+    }
 };
 
 inline int gridRows(int n) { return std::max(1, std::min(8192, (n + kBlock - 1) / kBlock)); }
@@ -1102,49 +1114,50 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
     const int nl = (int)A.d.size();
     const double w = R.omega;
     AmgDevLevel& D0 = *A.d[0];
-    T* r0 = MgVec<T>::r(D0);
+    T* r0 = MgVec<T>::r0(D0);
     hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, r0, w,
                        (const Ctl*)ctl);
     if (nl > 1) {
-        // down: restrict the finer level's residual, smooth + residual
+        // down: restrict the finer level's residual (with the coarse
+        // pre-smoothing), then the coarse residual
         for (int l = 1; l < nl; ++l) {
             AmgDevLevel& L = *A.d[l];
             AmgDevLevel& Lf = *A.d[l - 1];
-            const T* rf = l == 1 ? (const T*)r0 : (const T*)MgVec<T>::r(Lf);
-            T* b = MgVec<T>::b(L);
-            hipLaunchKernelGGL((k_mg_restrict<T>), dim3(gridRows(L.n)), dim3(kBlock), 0, s, L.view(), rf, Lf.n, b,
-                               (const Ctl*)ctl);
+            Row4<T>* b = MgVec<T>::b(L);
+            Row4<T>* x = MgVec<T>::x(L);
+            if (l == 1)
+                hipLaunchKernelGGL((k_mg_restrict<T, true>), dim3(gridRows(L.n)), dim3(kBlock), 0, s, L.view(),
+                                   (const T*)r0, (const Row4<T>*)nullptr, Lf.n, b, x, w, (const Ctl*)ctl);
+            else
+                hipLaunchKernelGGL((k_mg_restrict<T, false>), dim3(gridRows(L.n)), dim3(kBlock), 0, s, L.view(),
+                                   (const T*)nullptr, (const Row4<T>*)MgVec<T>::r(Lf), Lf.n, b, x, w,
+                                   (const Ctl*)ctl);
             if (l < nl - 1)
                 hipLaunchKernelGGL((k_mg_down<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
-                                   (const T*)b, MgVec<T>::x(L), MgVec<T>::r(L), w, (const Ctl*)ctl);
+                                   (const Row4<T>*)b, (const Row4<T>*)x, MgVec<T>::r(L), (const Ctl*)ctl);
         }
-        // coarsest: exact solve
+        // coarsest: exact solve into xs
         {
             AmgDevLevel& L = *A.d[nl - 1];
             hipLaunchKernelGGL((k_mg_dense<T>), dim3((L.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, s,
-                               L.view(), MgVec<T>::inv(L), (const T*)MgVec<T>::b(L), MgVec<T>::x(L), (const Ctl*)ctl);
+                               L.view(), MgVec<T>::inv(L), (const Row4<T>*)MgVec<T>::b(L), MgVec<T>::xs(L),
+                               (const Ctl*)ctl);
         }
-        // up: prolongate the coarser correction into the post-smoothing sweep;
-        // a level's result goes to xs (x keeps the pre-smoothed values), the
-        // coarsest level's solution is in its x.  No buffer changes role, so a
-        // captured launch sequence replays with the same pointers.
-        auto corr = [&](int l) -> const T* {
-            AmgDevLevel& Lc = *A.d[l];
-            return l == nl - 1 ? (const T*)MgVec<T>::x(Lc) : (const T*)MgVec<T>::xs(Lc);
-        };
+        // up: prolongate the coarser correction (its xs) into the
+        // post-smoothing sweep; a level's result goes to its xs (x keeps the
+        // pre-smoothed values).  No buffer changes role, so a captured launch
+        // sequence replays with the same pointers.
         for (int l = nl - 2; l >= 1; --l) {
             AmgDevLevel& L = *A.d[l];
             AmgDevLevel& Lc = *A.d[l + 1];
             hipLaunchKernelGGL((k_mg_up<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
-                               (const T*)MgVec<T>::b(L), (const T*)MgVec<T>::x(L), Lc.n, corr(l + 1),
-                               MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
+                               (const Row4<T>*)MgVec<T>::b(L), (const Row4<T>*)MgVec<T>::x(L),
+                               (const Row4<T>*)MgVec<T>::xs(Lc), MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
         }
     }
-    const T* e1 = nullptr;
-    if (nl > 1) e1 = nl - 1 == 1 ? (const T*)MgVec<T>::x(*A.d[1]) : (const T*)MgVec<T>::xs(*A.d[1]);
-    const int n1 = nl > 1 ? A.d[1]->n : 0;
+    const Row4<T>* e1 = nl > 1 ? (const Row4<T>*)MgVec<T>::xs(*A.d[1]) : nullptr;
     hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                       (const int*)D0.agg.p, n1, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
+                       (const int*)D0.agg.p, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
     hipLaunchKernelGGL(k_fin_rho, dim3(R.nFin), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
 }
@@ -2203,16 +2216,16 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
             case S4F_KERNEL_MG0_DOWN: {
                 AmgDevLevel& D0 = *M->amg.d[0];
                 hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   MgVec<T>::r(D0), M->amg.omega, (const Ctl*)rc.p);
+                                   MgVec<T>::r0(D0), M->amg.omega, (const Ctl*)rc.p);
                 break;
             }
             case S4F_KERNEL_MG0_UP: {
                 AmgDevLevel& D0 = *M->amg.d[0];
                 const bool two = M->amg.d.size() > 1;
                 hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   (const int*)D0.agg.p, two ? M->amg.d[1]->n : 0,
-                                   two ? (const T*)MgVec<T>::x(*M->amg.d[1]) : (const T*)nullptr, v.z, M->amg.omega,
-                                   M->amg.alpha, (const Ctl*)rc.p, st.part.p);
+                                   (const int*)D0.agg.p,
+                                   two ? (const Row4<T>*)MgVec<T>::xs(*M->amg.d[1]) : (const Row4<T>*)nullptr, v.z,
+                                   M->amg.omega, M->amg.alpha, (const Ctl*)rc.p, st.part.p);
                 break;
             }
             case S4F_KERNEL_VCYCLE:
@@ -2226,13 +2239,15 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
         buildAmg(M);
         for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
             AmgDevLevel& L = *M->amg.d[l];
-            const size_t n3 = 3 * (size_t)L.n;
-            if (l > 0) {  // the fine level keeps only the residual
-                HIP_CHECK(hipMemsetAsync(MgVec<T>::x(L), 0, n3 * sizeof(T), s));
-                HIP_CHECK(hipMemsetAsync(MgVec<T>::xs(L), 0, n3 * sizeof(T), s));
+            if (l == 0) {  // the fine level keeps only the residual
+                HIP_CHECK(hipMemsetAsync(MgVec<T>::r0(L), 0, 3 * (size_t)L.n * sizeof(T), s));
+                continue;
             }
-            HIP_CHECK(hipMemsetAsync(MgVec<T>::b(L), 0, n3 * sizeof(T), s));
-            HIP_CHECK(hipMemsetAsync(MgVec<T>::r(L), 0, n3 * sizeof(T), s));
+            const size_t n4 = 4 * (size_t)L.n * sizeof(T);
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::x(L), 0, n4, s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::xs(L), 0, n4, s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::b(L), 0, n4, s));
+            HIP_CHECK(hipMemsetAsync(MgVec<T>::r(L), 0, n4, s));
         }
     }
     launch();  // warm
