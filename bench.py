@@ -27,10 +27,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 
-def algorithmic_bytes(kernel, C, F, B, T):
+def algorithmic_bytes(kernel, C, F, B, T, q_entries=None):
     """Compulsory HBM bytes per launch: each array a kernel touches counted
     once; neighbour gathers assumed cache-resident.  C cells, F internal faces,
-    B boundary faces, T = bytes of the PCG scalar (8 FP64, 4 FP32)."""
+    B boundary faces, T = bytes of the PCG scalar (8 FP64, 4 FP32),
+    q_entries = stored entries of the multigrid's level-0 grouped operator
+    (given: the multigrid-preconditioned PCG's kernels)."""
+    if kernel == "spmv" and q_entries is not None:  # z, p old/new, q, diag per cell; a + col per entry
+        return C * 13 * T + F * 2 * (T + 4)
     if kernel == "spmv":            # r, 1/diag, p old/new, q, diag per cell; a + col per ELL entry (2 per face)
         return C * 14 * T + F * 2 * (T + 4)
     if kernel == "pcg_update":      # x r/w, p, r r/w, q, 1/diag
@@ -39,8 +43,8 @@ def algorithmic_bytes(kernel, C, F, B, T):
         return 0
     if kernel == "mg0_down":        # b, 1/diag, diag; r write; a + col per ELL entry
         return C * 8 * T + F * 2 * (T + 4)
-    if kernel == "mg0_up":          # b, 1/diag, agg; z write; a + col per ELL entry (coarse e cache-resident)
-        return C * (7 * T + 4) + F * 2 * (T + 4)
+    if kernel == "mg0_up":          # b, r, 1/diag, agg; z write; q + column per entry (coarse e cache-resident)
+        return C * (10 * T + 4) + q_entries * (T + 4)
     if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
         return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
     if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w
@@ -194,8 +198,9 @@ def main():
         mesh.solve(want_sigma=False, precision=precision, **dict(opts, nCorrectors=1, pcgMaxIter=2))
         prof = replay_all(precision)
         T = 8 if precision == 0 else 4
+        qn = mesh.mg_info()["qEntries"] if args.precond == "gamg" else None
         out = {k: dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"],
-                       gbps=(round(algorithmic_bytes(k, C, F, B, T) / (p["avg_ms"] * 1e-3) / 1e9, 1)
+                       gbps=(round(algorithmic_bytes(k, C, F, B, T, qn) / (p["avg_ms"] * 1e-3) / 1e9, 1)
                              if k not in ("finalize", "vcycle") else None)) for k, p in prof.items()}
         print(json.dumps(dict(mode="kernels-only", cells=C, precision="fp64" if precision == 0 else "mixed",
                               kernels=out)))
@@ -228,11 +233,12 @@ def main():
     mg = args.precond == "gamg"
     per_solve = {"spmv": pcg_it, "pcg_update": pcg_it, "finalize": (3 if mg else 2) * pcg_it, "residual": nOuter,
                  "grad": nOuter, "mg0_down": pcg_it + nOuter, "mg0_up": pcg_it + nOuter, "vcycle": pcg_it + nOuter}
+    mgi = mesh.mg_info() if mg else None
     for k, p in prof.items():
         e = dict(avg_ms=round(p["avg_ms"], 5), replay_launches=p["launches"], launches_per_solve=per_solve[k],
                  ms_per_solve_est=round(p["avg_ms"] * per_solve[k], 1))
         if k not in ("finalize", "vcycle"):
-            b = algorithmic_bytes(k, C, F, B, T)
+            b = algorithmic_bytes(k, C, F, B, T, mgi["qEntries"] if mgi else None)
             e.update(alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1))
         kern[k] = e
     dom = max((k for k in kern if k not in ("finalize", "vcycle")), key=lambda k: kern[k]["ms_per_solve_est"]) \
@@ -288,6 +294,8 @@ def main():
     }
     if amg_setup_s is not None:
         line["multigrid_setup_s"] = amg_setup_s
+    if mgi:
+        line["multigrid_levels"] = dict(rows=mgi["rows"], entries=mgi["entries"], q_entries=mgi["qEntries"])
     line.update(extra)
     print(json.dumps(line))
 

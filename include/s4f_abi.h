@@ -154,6 +154,14 @@ s4f_status s4f_mesh_create_from_polymesh(s4f_ctx *ctx, const s4f_polymesh *pm,
 s4f_status s4f_mesh_destroy(s4f_mesh *mesh);
 /* nCells, nInternalFaces, nBoundaryFaces, ELL entries, ELL slices, device bytes */
 s4f_status s4f_mesh_info(const s4f_mesh *mesh, int64_t out[6]);
+/* Multigrid hierarchy of the mesh's current matrix, built by its first GAMG
+   solve (S4F_ERR_INVALID before): the level count, and for the first
+   maxLevels levels their rows and stored off-diagonal entries; *qEntries
+   (may be NULL): stored entries of the level-0 operator offdiag(A) P grouped
+   per coarse row.  No counterpart in the reference: OpenFOAM's
+   GAMGAgglomeration reports its level sizes only in debug output. */
+s4f_status s4f_mesh_mg_info(const s4f_mesh *mesh, int32_t maxLevels, int32_t *nLevels, int64_t *rows,
+                            int64_t *entries, int64_t *qEntries);
 
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
@@ -295,7 +303,8 @@ enum { S4F_KERNEL_RESIDUAL = 0,   /* div(sigma) + Rhie-Chow face loop */
        S4F_KERNEL_GRAD = 4,       /* boundary update + least-squares gradient */
        S4F_KERNEL_MG0_DOWN = 5,   /* multigrid: fine-level smoothing + residual */
        S4F_KERNEL_MG0_UP = 6,     /* multigrid: fine-level prolongation + smoothing */
-       S4F_KERNEL_VCYCLE = 7 };   /* multigrid: every launch of one V-cycle */
+       S4F_KERNEL_VCYCLE = 7,     /* multigrid: every launch of one V-cycle */
+       S4F_KERNEL_SPMV_MG = 8 };  /* multigrid PCG: direction update from the V-cycle's z + SpMV */
 /* Replays `kernel` back to back `launches` times on the mesh's current state
    (e.g. right after a solve) with all three components active, on the
    context's stream between one pair of hipEvents, and returns the average

@@ -83,7 +83,7 @@ EXPORTED = [
     "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
     "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
-    "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set",
+    "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info",
 ]
 
 
@@ -92,7 +92,7 @@ class Request(ctypes.Structure):
                 ("warmStart", ctypes.c_int32)]
 
 KERNELS = {"residual": 0, "spmv": 1, "pcg_update": 2, "finalize": 3, "grad": 4}
-MG_KERNELS = {"mg0_down": 5, "mg0_up": 6, "vcycle": 7}
+MG_KERNELS = {"spmv": 8, "mg0_down": 5, "mg0_up": 6, "vcycle": 7}  # spmv: the multigrid PCG's variant
 
 
 def lib():
@@ -121,6 +121,7 @@ def lib():
         L.s4f_mesh_create_from_polymesh.argtypes = [vp, vp, ip, ctypes.c_int32, ctypes.POINTER(vp)]
         L.s4f_mesh_destroy.argtypes = [vp]
         L.s4f_mesh_info.argtypes = [vp, vp]
+        L.s4f_mesh_mg_info.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
         L.s4f_material_set.argtypes = [vp, dp, dp, ctypes.c_int32]
         L.s4f_bc_set.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.s4f_density_set.argtypes = [vp, dp, ctypes.c_int32]

@@ -18,7 +18,8 @@
 //   down: r = b - A x (on level 0 the neighbours' x = w D^-1 b is recomputed
 //         from their b and 1/diag)
 //   up:   prolongation x += e[agg] fused into the post-smoothing sweep (the
-//         neighbours' corrected x recomputed on the fly)
+//         neighbours' corrected x recomputed on the fly; on level 0 through
+//         the down sweep's residual and q = offdiag(A) P)
 // Level 0 is the solver mesh's own sliced-ELL matrix; coarser levels have
 // their own sliced-ELL arrays.  Every row's arithmetic runs in a fixed order
 // that depends only on its own instance, so results do not depend on which
@@ -68,12 +69,6 @@ template <> struct MgSel<float> {
 };
 
 __device__ __forceinline__ bool mgSkip(const Ctl* c) { return c->outerDone || c->innerDone; }
-
-// a product rounded on its own (never fused into a following add), so a
-// value recomputed from its inputs has the bits of the one computed before
-template <typename T> __device__ __forceinline__ T mulRn(T a, T b);
-template <> __device__ __forceinline__ float mulRn<float>(float a, float b) { return __fmul_rn(a, b); }
-template <> __device__ __forceinline__ double mulRn<double>(double a, double b) { return __dmul_rn(a, b); }
 
 // Coarse-level vectors are rows of four (three components and a zero pad):
 // one 16-byte (FP32) or 32-byte (FP64) load per neighbour instead of three
@@ -131,57 +126,48 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
     for (int j = 0; j < 3; ++j) r[(size_t)j * C + P] = bP[j] - y[j];
 }
 
-template <typename T, int W, bool COARSE>
-__device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, int P, int e0, int lane, int C, const T* __restrict__ b,
-                                            const int* __restrict__ agg,
+// level-0 prolongation + post-smoothing of one row (all three components,
+// compile-time slot count W, every load unconditional: an empty slot reads
+// the row's own coarse row with a zero weight)
+template <typename T, int W>
+__device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, const int* __restrict__ qCol, const T* __restrict__ q,
+                                            int P, int e0, int lane, int C, const T* __restrict__ b,
+                                            const T* __restrict__ r0, const int* __restrict__ agg,
                                             const Row4<T>* __restrict__ e, T* __restrict__ z, T w, T w1, T al,
                                             double* acc) {
-    const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
-    int nb[W];
-    T a[W];
+    int J[W];
+    T qv[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int en = e0 + k * kSlice + lane;
-        nb[k] = m.col[en];
-        a[k] = A[en];
+        J[k] = qCol[en];
+        qv[k] = q[en];
     }
-    // pre-smoothed value from zero, recomputed exactly as k_mg0_down forms it
-    auto xf = [&](int j, T wi, T bv, const Row4<T>& ev) -> T {
-        const T v = mulRn(wi, bv);
-        return COARSE ? v + al * ev[j] : v;
-    };
-    Row4<T> eP = {};
-    if (COARSE) eP = e[agg[P]];
-    const T iD = invD[P];
-    const T wP = w * iD;
-    T bP[3], sum[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        bP[j] = b[(size_t)j * C + P];
-        sum[j] = bP[j];
-    }
+    const int IP = agg[P];
+    const T wi = w * invD[P];
+    T s[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const bool v = nb[k] >= 0;
-        const int c = v ? nb[k] : P;
-        const T av = v ? a[k] : (T)0;
-        Row4<T> ec = {};
-        if (COARSE) ec = e[agg[c]];
-        const T wc = w * invD[c];
+        const bool v = J[k] >= 0;
+        const Row4<T> ek = e[v ? J[k] : IP];
+        const T c = v ? qv[k] : (T)0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] += av * xf(j, wc, b[(size_t)j * C + c], ec);
+        for (int j = 0; j < 3; ++j) s[j] += c * ek[j];
     }
+    const Row4<T> eP = e[IP];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T zv = w1 * xf(j, wP, bP[j], eP) + w * iD * sum[j];
-        z[(size_t)j * C + P] = zv;
-        acc[j] += (double)zv * (double)bP[j];
+        const size_t i = (size_t)j * C + P;
+        const T bv = b[i];
+        const T zv = wi * (bv + r0[i]) + al * (w1 * eP[j] + wi * s[j]);
+        z[i] = zv;
+        acc[j] += (double)zv * (double)bv;
     }
 }
 
 // pre-smoothing from zero and the residual for the restriction:
-// x = w D^-1 b (not stored: k_mg0_up forms it again from b), r = b - A x
+// x = w D^-1 b (not stored: k_mg0_up works from b and r), r = b - A x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
                                                      T* __restrict__ r, double omega,
@@ -240,13 +226,18 @@ __global__ __launch_bounds__(kBlock) void k_mg0_down(const DevMesh m, const Bloc
     }
 }
 
-// prolongation + post-smoothing: xf = x + e[agg] with x = w D^-1 b the
-// pre-smoothed value (e == nullptr: no coarse correction),
-// z = (1 - w) xf + w D^-1 (b + sum a xf_N), with partial z.b
-// (b is the PCG residual: z.b = rho of the preconditioned residual)
+// Prolongation + post-smoothing on level 0.  With x = w D^-1 b the
+// pre-smoothed value and r0 = b - A x from k_mg0_down, the corrected
+// xf = x + alpha P e and its Jacobi sweep z = xf + w D^-1 (b - A xf) are
+//   z = w D^-1 (b + r0) + alpha ((1 - w) e[agg] + w D^-1 sum_J q_J e_J)
+// with q = offdiag(A) P grouped per coarse row J (host-built in the fine
+// matrix's slots): no fine-matrix pass and no neighbour b, 1/diag or agg
+// gathers.  e == nullptr: no coarse correction.  Partial z.b per block (b is
+// the PCG residual: z.b = rho of the preconditioned residual).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
-                                                   const int* __restrict__ agg,
+                                                   const T* __restrict__ r0, const int* __restrict__ agg,
+                                                   const int* __restrict__ qCol, const T* __restrict__ q,
                                                    const Row4<T>* __restrict__ e, T* __restrict__ z, double omega,
                                                    double alpha, const Ctl* __restrict__ ctl,
                                                    double* __restrict__ part) {
@@ -255,15 +246,10 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     const Ctl* c = ctl + inst;
     if (mgSkip(c)) return;
     const int C = m.nCells;
-    const T* __restrict__ A = MatSel<T>::a(m);
     const T* __restrict__ invD = MatSel<T>::invDiag(m);
     const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
-    const bool all = act[0] && act[1] && act[2];
+    const bool all = act[0] && act[1] && act[2] && e != nullptr;
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
-    auto xf = [&](int j, int cell) -> T {
-        const T v = mulRn(w * invD[cell], b[(size_t)j * C + cell]);
-        return e ? v + al * e[agg[cell]][j] : v;
-    };
     double acc[3] = {0, 0, 0};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s1 = bm.s1[lb];
@@ -274,36 +260,37 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
         const int e0 = m.sliceStart[s];
         const int width = m.sliceWidth[s];
         if (all && width == 6) {
-            if (e) mg0UpRowAll<T, 6, true>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 6, false>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
+            mg0UpRowAll<T, 6>(m, qCol, q, P, e0, lane, C, b, r0, agg, e, z, w, w1, al, acc);
             continue;
         }
         if (all && width == 4) {
-            if (e) mg0UpRowAll<T, 4, true>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
-            else mg0UpRowAll<T, 4, false>(m, P, e0, lane, C, b, agg, e, z, w, w1, al, acc);
+            mg0UpRowAll<T, 4>(m, qCol, q, P, e0, lane, C, b, r0, agg, e, z, w, w1, al, acc);
             continue;
         }
-        T sum[3];
+        const T wi = w * invD[P];
+        T s3[3] = {(T)0, (T)0, (T)0};
+        Row4<T> eP = {};
+        if (e) {
+            eP = e[agg[P]];
+            for (int k = 0; k < width; ++k) {
+                const int en = e0 + k * kSlice + lane;
+                const int J = qCol[en];
+                if (J >= 0) {
+                    const Row4<T> ek = e[J];
+                    const T cq = q[en];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] = act[j] ? b[(size_t)j * C + P] : (T)0;
-        for (int k = 0; k < width; ++k) {
-            const int en = e0 + k * kSlice + lane;
-            const int n = m.col[en];
-            if (n >= 0) {
-                const T a = A[en];
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if (act[j]) sum[j] += a * xf(j, n);
+                    for (int j = 0; j < 3; ++j) s3[j] += cq * ek[j];
+                }
             }
         }
-        const T iD = invD[P];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             if (!act[j]) continue;
             const size_t i = (size_t)j * C + P;
-            const T zv = w1 * xf(j, P) + w * iD * sum[j];
+            const T bv = b[i];
+            const T zv = e ? wi * (bv + r0[i]) + al * (w1 * eP[j] + wi * s3[j]) : wi * (bv + r0[i]);
             z[i] = zv;
-            acc[j] += (double)zv * (double)b[i];
+            acc[j] += (double)zv * (double)bv;
         }
     }
     storeBlockPartials<3>(acc, 0u, part + (size_t)lb * kMaxPartials);

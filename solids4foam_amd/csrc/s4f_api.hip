@@ -178,6 +178,12 @@ struct AmgDevLevel {
     DArr<float> a32, d32, iD32;
     DArr<double> x64, b64, r64, xs64, inv64;  // inv: explicit inverses (coarsest level only)
     DArr<float> x32, b32, r32, xs32, inv32;
+    // level 0 only: q = offdiag(A) P, the off-diagonal magnitudes of each row
+    // summed per coarse row, in the fine matrix's own sliced-ELL slots
+    // (qCol < 0: empty slot)
+    DArr<int> qCol;
+    DArr<float> q32;
+    DArr<double> q64;
     MgLevel view() const {
         MgLevel L{};
         L.n = n; L.nSlices = nSlices;
@@ -195,6 +201,7 @@ struct Amg {
     double omega = 2.0 / 3.0;
     double alpha = 1.0;  // coarse-correction scale (set per solve)
     double buildSeconds = 0.0;
+    long long qEntries = 0;  // stored entries of the level-0 grouped operator q
 };
 
 struct s4f_mesh {
@@ -946,6 +953,32 @@ bool spdInverse(const AmgHostLevel& H, int r0, int n, std::vector<double>& inv) 
     return true;
 }
 
+// q = offdiag(A) P for level 0: per row, the magnitudes of its neighbour
+// entries summed per coarse row of the neighbour, in ascending coarse-row
+// order, in the row's own sliced-ELL slots (the rest empty: column -1)
+void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<double>& q) {
+    qc.assign(F.col.size(), -1);
+    q.assign(F.col.size(), 0.0);
+    std::vector<std::pair<int, double>> row;
+    for (int i = 0; i < F.n; ++i) {
+        row.clear();
+        forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) { row.push_back({F.agg[j], F.a[e]}); });
+        std::stable_sort(row.begin(), row.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+            return x.first < y.first;
+        });
+        const int sI = i / kSlice, l = i % kSlice;
+        int k = -1, last = -1;
+        for (const std::pair<int, double>& t : row) {
+            if (t.first != last) {
+                ++k;
+                last = t.first;
+                qc[(size_t)F.sliceStart[sI] + (size_t)k * kSlice + l] = t.first;
+            }
+            q[(size_t)F.sliceStart[sI] + (size_t)k * kSlice + l] += t.second;
+        }
+    }
+}
+
 void buildAmg(s4f_mesh* M) {
     Amg& A = M->amg;
     if (A.gen == M->generation && !A.d.empty()) return;
@@ -1021,6 +1054,16 @@ void buildAmg(s4f_mesh* M) {
         D->n = H.n;
         D->nSlices = H.nSlices;
         if (!H.agg.empty()) D->agg.upload(H.agg, s);
+        if (l == 0 && A.h.size() > 1) {
+            std::vector<int> qc;
+            std::vector<double> q;
+            groupByAggregate(H, qc, q);
+            A.qEntries = (long long)std::count_if(qc.begin(), qc.end(), [](int c) { return c >= 0; });
+            std::vector<float> q32(q.begin(), q.end());
+            D->qCol.upload(qc, s);
+            D->q64.upload(q, s);
+            D->q32.upload(q32, s);
+        }
         if (l > 0) {
             D->sliceStart.upload(H.sliceStart, s);
             D->sliceWidth.upload(H.sliceWidth, s);
@@ -1068,6 +1111,7 @@ struct MgVec<double> {
     static Row4<double>* r(AmgDevLevel& L) { return v4(L.r64, L.n); }
     static Row4<double>* xs(AmgDevLevel& L) { return v4(L.xs64, L.n); }
     static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
+    static const double* q(AmgDevLevel& L) { return L.q64.p; }
     static Row4<double>* v4(DArr<double>& a, int n) {
         a.alloc(4 * (size_t)n);
         return reinterpret_cast<Row4<double>*>(a.p);
@@ -1081,6 +1125,7 @@ struct MgVec<float> {
     static Row4<float>* r(AmgDevLevel& L) { return v4(L.r32, L.n); }
     static Row4<float>* xs(AmgDevLevel& L) { return v4(L.xs32, L.n); }
     static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
+    static const float* q(AmgDevLevel& L) { return L.q32.p; }
     static Row4<float>* v4(DArr<float>& a, int n) {
         a.alloc(4 * (size_t)n);
         return reinterpret_cast<Row4<float>*>(a.p);
@@ -1156,8 +1201,9 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
         }
     }
     const Row4<T>* e1 = nl > 1 ? (const Row4<T>*)MgVec<T>::xs(*A.d[1]) : nullptr;
-    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                       (const int*)D0.agg.p, e1, v.z, w, A.alpha, (const Ctl*)ctl, part);
+    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)r0,
+                       (const int*)D0.agg.p, (const int*)D0.qCol.p, MgVec<T>::q(D0), e1, v.z, w, A.alpha,
+                       (const Ctl*)ctl, part);
     hipLaunchKernelGGL(k_fin_rho, dim3(R.nFin), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
 }
@@ -1833,6 +1879,23 @@ s4f_status s4f_mesh_info(const s4f_mesh* m, int64_t out[6]) {
     return S4F_OK;
 }
 
+s4f_status s4f_mesh_mg_info(const s4f_mesh* m, int32_t maxLevels, int32_t* nLevels, int64_t* rows, int64_t* entries,
+                            int64_t* qEntries) {
+    if (!m || !nLevels || maxLevels < 0 || (maxLevels > 0 && (!rows || !entries)))
+        return fail(S4F_ERR_INVALID, "null argument");
+    const Amg& A = m->amg;
+    if (A.d.empty() || A.gen != m->generation)
+        return fail(S4F_ERR_INVALID, "no multigrid hierarchy for the current matrix (run a GAMG solve first)");
+    *nLevels = (int32_t)A.h.size();
+    for (int l = 0; l < std::min<int>(maxLevels, (int)A.h.size()); ++l) {
+        const AmgHostLevel& H = A.h[l];
+        rows[l] = H.n;
+        entries[l] = (int64_t)std::count_if(H.col.begin(), H.col.end(), [](int c) { return c >= 0; });
+    }
+    if (qEntries) *qEntries = A.qEntries;
+    return S4F_OK;
+}
+
 s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int32_t perCell) {
     return guarded([&]() -> s4f_status {
         if (!M || !E || !nu) throw InvalidError("null argument");
@@ -2201,6 +2264,11 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                                    (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1, (const T*)nullptr);
                 std::swap(v.p, v.p2);
                 break;
+            case S4F_KERNEL_SPMV_MG:
+                hipLaunchKernelGGL((k_spmvp<T, true>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                                   (const T*)v.p, v.p2, v.q, rc.p, st.part.p, 1, (const T*)v.z);
+                std::swap(v.p, v.p2);
+                break;
             case S4F_KERNEL_PCG_UPDATE:
                 hipLaunchKernelGGL((k_update<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, v.x, (const T*)v.p, v.r,
                                    (const T*)v.q, (const Ctl*)rc.p, st.part.p);
@@ -2223,7 +2291,8 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 AmgDevLevel& D0 = *M->amg.d[0];
                 const bool two = M->amg.d.size() > 1;
                 hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   (const int*)D0.agg.p,
+                                   (const T*)MgVec<T>::r0(D0), (const int*)D0.agg.p, (const int*)D0.qCol.p,
+                                   MgVec<T>::q(D0),
                                    two ? (const Row4<T>*)MgVec<T>::xs(*M->amg.d[1]) : (const Row4<T>*)nullptr, v.z,
                                    M->amg.omega, M->amg.alpha, (const Ctl*)rc.p, st.part.p);
                 break;
@@ -2235,7 +2304,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 throw InvalidError("unknown kernel id");
         }
     };
-    if (kernel >= S4F_KERNEL_MG0_DOWN) {
+    if (kernel >= S4F_KERNEL_MG0_DOWN && kernel != S4F_KERNEL_SPMV_MG) {
         buildAmg(M);
         for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
             AmgDevLevel& L = *M->amg.d[l];
@@ -2590,7 +2659,10 @@ long long footprintBytes(const HostMesh& h) {
 void serverAllocate(s4f_server* S) {
     // split the budget over cells, ELL entries (7 per cell) and boundary faces (1 per cell)
     const bool aitken = S->opts.relaxationMethod == S4F_RELAX_AITKEN;
-    const double perCell = kCellBytes + 7.0 * kEntryBytes + 1.0 * kBndBytes + (aitken ? 64.0 : 0.0);
+    // multigrid: the level-0 grouped operator q (column + value per entry)
+    const double qBytes = S->opts.preconditioner != S4F_PRECOND_GAMG ? 0.0
+                          : S->opts.precision == S4F_PRECISION_MIXED ? 8.0 : 12.0;
+    const double perCell = kCellBytes + 7.0 * (kEntryBytes + qBytes) + 1.0 * kBndBytes + (aitken ? 64.0 : 0.0);
     long long C = (long long)(S->budget / perCell);
     C = std::max<long long>(C / kSlice * kSlice, kSlice);
     if (C > INT32_MAX / 16) C = INT32_MAX / 16 / kSlice * kSlice;
@@ -2619,6 +2691,9 @@ void serverAllocate(s4f_server* S) {
         S->mg.push_back(std::make_unique<AmgDevLevel>());
         S->mg[0]->n = (int)Cc;
         S->mg[0]->agg.alloc(Cc);
+        S->mg[0]->qCol.alloc(Ec);
+        if (S->opts.precision == S4F_PRECISION_MIXED) S->mg[0]->q32.alloc(Ec);
+        else S->mg[0]->q64.alloc(Ec);
     }
     if (S->opts.precision == S4F_PRECISION_MIXED) {
         S->r32.alloc(3 * Cc); S->p32.alloc(3 * Cc); S->pb32.alloc(3 * Cc); S->q32.alloc(3 * Cc); S->x32.alloc(3 * Cc);
@@ -3038,10 +3113,18 @@ void serverPackMg(s4f_server* S, const std::vector<int>& slots) {
             }
         }
     }
-    // level 0: arena cell -> packed level-1 row
+    // level 0: arena cell -> packed level-1 row; q in the arena's entry
+    // slots with its columns moved to the packed level-1 rows
     AmgDevLevel& L0 = *S->mg[0];
-    for (int k = 0; k < nI; ++k)
-        addI(ms[k]->amg.d[0]->agg.p, L0.agg.p + c0[k], ms[k]->amg.h[0].n, R[1][k], 0);
+    for (int k = 0; k < nI; ++k) {
+        const AmgDevLevel& M0 = *ms[k]->amg.d[0];
+        addI(M0.agg.p, L0.agg.p + c0[k], ms[k]->amg.h[0].n, R[1][k], 0);
+        const size_t e0 = (size_t)S->places[S->live[S->slotReq[slots[k]]].place].e0;
+        const int nE = (int)M0.qCol.n;
+        addI(M0.qCol.p, L0.qCol.p + e0, nE, R[1][k], 1);
+        if (f32) addW(M0.q32.p, L0.q32.p + e0, nE);
+        else addW(M0.q64.p, L0.q64.p + e0, 2LL * nE);
+    }
     if (!si.empty()) {
         S->segI.upload(si, s);
         int maxN = 0;

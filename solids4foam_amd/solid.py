@@ -144,6 +144,18 @@ class SolverMesh:
         return dict(nCells=int(out[0]), nInternalFaces=int(out[1]), nBoundaryFaces=int(out[2]),
                     ellEntries=int(out[3]), ellSlices=int(out[4]), deviceBytes=int(out[5]))
 
+    def mg_info(self):
+        """Multigrid hierarchy of the current matrix (after a GAMG solve):
+        rows and stored off-diagonal entries per level, and the stored entries
+        of the level-0 grouped operator offdiag(A) P"""
+        n = ctypes.c_int32(0)
+        rows = np.zeros(32, np.int64)
+        ents = np.zeros(32, np.int64)
+        q = ctypes.c_int64(0)
+        check(lib().s4f_mesh_mg_info(self.h, 32, ctypes.byref(n), ptr(rows), ptr(ents), ctypes.byref(q)))
+        k = min(n.value, 32)
+        return dict(rows=[int(v) for v in rows[:k]], entries=[int(v) for v in ents[:k]], qEntries=int(q.value))
+
     def set_material(self, E, nu):
         E = np.atleast_1d(f64(E))
         nu = np.atleast_1d(f64(nu))

@@ -153,6 +153,25 @@ def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precis
     assert out["pcgIterations"] < jac["pcgIterations"]
 
 
+def test_multigrid_hierarchy_info(ctx):
+    # level 0 is the mesh (two stored entries per internal face), every level
+    # coarsens, the coarsest fits one dense block; q = offdiag(A) P groups a
+    # row's neighbour entries per coarse row, so it never stores more
+    from solids4foam_amd import PRECOND_GAMG, S4FError
+    pm, m, _, _ = _product_mesh(ctx, (40, 40, 40), jitter=0.15, seed=2)
+    with pytest.raises(S4FError):
+        m.mg_info()
+    out = m.solve(preconditioner=PRECOND_GAMG, nCorrectors=2)
+    info, mi = m.mg_info(), m.info()
+    print(f"[mg info] {info}", flush=True)
+    rows, ents = info["rows"], info["entries"]
+    assert len(rows) >= 3 and rows[0] == mi["nCells"] and ents[0] == 2 * mi["nInternalFaces"]
+    assert all(a > b for a, b in zip(rows, rows[1:])) and rows[-1] <= 1024
+    assert all(e > 0 for e in ents)
+    assert mi["nCells"] < info["qEntries"] < ents[0]
+    assert out["nOuter"] >= 1
+
+
 def test_multigrid_batch_equals_alone(ctx):
     # a batch is one block-diagonal matrix: each instance's hierarchy, cycle
     # and reductions are the ones it has alone (an instance already at its
