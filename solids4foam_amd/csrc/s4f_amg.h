@@ -54,15 +54,21 @@ struct MgLevel {
     const int* blkN;     //   rows of its block
     const int* invOff;   //   offset of its explicit inverse (blkN^2 entries, row-major)
     const int* pass;     // per instance: passes through this level unchanged (no smoothing, no scaling)
+    const int* qCol;     // q = offdiag(A) P in the level's slots (< 0: empty; nullptr on the coarsest)
+    const float* q32;
+    const double* q64;
+    const int* qWidth;   // per slice: filled q slots
 };
 
 template <typename T> struct MgSel;
 template <> struct MgSel<double> {
+    __device__ static const double* q(const MgLevel& L) { return L.q64; }
     __device__ static const double* a(const MgLevel& L) { return L.a64; }
     __device__ static const double* d(const MgLevel& L) { return L.diag64; }
     __device__ static const double* iD(const MgLevel& L) { return L.invD64; }
 };
 template <> struct MgSel<float> {
+    __device__ static const float* q(const MgLevel& L) { return L.q32; }
     __device__ static const float* a(const MgLevel& L) { return L.a32; }
     __device__ static const float* d(const MgLevel& L) { return L.diag32; }
     __device__ static const float* iD(const MgLevel& L) { return L.invD32; }
@@ -405,13 +411,16 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const Row4<
     }
 }
 
-// up on a coarse level: xf = x + e[agg], out = (1 - w) xf + w D^-1 (b + sum a xf_N)
+// up on a coarse level: the corrected xf = x + alpha e[agg] and its Jacobi
+// sweep out = xf + w D^-1 (b - A xf), formed as on level 0 from the down
+// sweep's residual r = b - A x and q = offdiag(A) P:
+//   out = x + w D^-1 r + alpha ((1 - w) e[agg] + w D^-1 sum_J q_J e_J)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>* __restrict__ b,
-                                                  const Row4<T>* __restrict__ x, const Row4<T>* __restrict__ e,
+__global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>* __restrict__ x,
+                                                  const Row4<T>* __restrict__ r, const Row4<T>* __restrict__ e,
                                                   Row4<T>* __restrict__ out, double omega, double alpha,
                                                   const Ctl* __restrict__ ctl) {
-    const T* __restrict__ A = MgSel<T>::a(L);
+    const T* __restrict__ q = MgSel<T>::q(L);
     const T* __restrict__ iDg = MgSel<T>::iD(L);
     const T w = (T)omega, w1 = (T)(1.0 - omega), al = (T)alpha;
     const int n = L.n;
@@ -424,42 +433,39 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>
         const int inst = L.rowInst[P];
         if (inst < 0) continue;  // padding row
         if (mgSkip(ctl + inst)) continue;
-        const Row4<T> eP = e[agg[P]];
+        const int IP = agg[P];
+        const Row4<T> eP = e[IP];
         if (L.pass[inst]) {  // already coarsest for this instance: the coarser solution unchanged
             out[P] = eP;
             continue;
         }
         const int e0 = L.sliceStart[s];
-        const int width = L.sliceWidth[s];
-        const Row4<T> bP = b[P], xv = x[P];
-        T sum[3], xP[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            sum[j] = bP[j];
-            xP[j] = xv[j] + al * eP[j];
-        }
+        const int width = L.qWidth[s];
+        T sum[3] = {(T)0, (T)0, (T)0};
         for (int k0 = 0; k0 < width; k0 += kMgChunk) {
-            int nb[kMgChunk];
-            T a[kMgChunk];
+            int J[kMgChunk];
+            T c[kMgChunk];
 #pragma unroll
             for (int u = 0; u < kMgChunk; ++u) {
                 const int k = k0 + u < width ? k0 + u : width - 1;
                 const int en = e0 + k * kSlice + lane;
-                nb[u] = k0 + u < width ? L.col[en] : -1;
-                a[u] = A[en];
+                J[u] = k0 + u < width ? L.qCol[en] : -1;
+                c[u] = q[en];
             }
 #pragma unroll
             for (int u = 0; u < kMgChunk; ++u) {
-                const bool v = nb[u] >= 0;
-                const int c = v ? nb[u] : P;
-                const T av = v ? a[u] : (T)0;
-                const Row4<T> xc = x[c], ec = e[agg[c]];
+                const bool v = J[u] >= 0;
+                const Row4<T> ev = e[v ? J[u] : IP];
+                const T cv = v ? c[u] : (T)0;
 #pragma unroll
-                for (int j = 0; j < 3; ++j) sum[j] += av * (xc[j] + al * ec[j]);
+                for (int j = 0; j < 3; ++j) sum[j] += cv * ev[j];
             }
         }
+        const Row4<T> xv = x[P], rv = r[P];
         const T wi = w * iDg[P];
-        out[P] = row4(w1 * xP[0] + wi * sum[0], w1 * xP[1] + wi * sum[1], w1 * xP[2] + wi * sum[2]);
+        out[P] = row4(xv[0] + wi * rv[0] + al * (w1 * eP[0] + wi * sum[0]),
+                      xv[1] + wi * rv[1] + al * (w1 * eP[1] + wi * sum[1]),
+                      xv[2] + wi * rv[2] + al * (w1 * eP[2] + wi * sum[2]));
     }
 }
 

@@ -181,7 +181,7 @@ struct AmgDevLevel {
     // level 0 only: q = offdiag(A) P, the off-diagonal magnitudes of each row
     // summed per coarse row, in the fine matrix's own sliced-ELL slots
     // (qCol < 0: empty slot)
-    DArr<int> qCol;
+    DArr<int> qCol, qWidth;  // qWidth: per slice, the filled q slots (coarse levels)
     DArr<float> q32;
     DArr<double> q64;
     MgLevel view() const {
@@ -191,6 +191,7 @@ struct AmgDevLevel {
         L.a32 = a32.p; L.a64 = a64.p; L.diag32 = d32.p; L.diag64 = d64.p; L.invD32 = iD32.p; L.invD64 = iD64.p;
         L.agg = agg.p; L.memOff = memOff.p; L.mem = mem.p;
         L.rowInst = rowInst.p; L.blkRow0 = blkRow0.p; L.blkN = blkN.p; L.invOff = invOff.p; L.pass = instPass.p;
+        L.qCol = qCol.p; L.q32 = q32.p; L.q64 = q64.p; L.qWidth = qWidth.p;
         return L;
     }
 };
@@ -953,12 +954,14 @@ bool spdInverse(const AmgHostLevel& H, int r0, int n, std::vector<double>& inv) 
     return true;
 }
 
-// q = offdiag(A) P for level 0: per row, the magnitudes of its neighbour
+// q = offdiag(A) P of a level: per row, the magnitudes of its neighbour
 // entries summed per coarse row of the neighbour, in ascending coarse-row
-// order, in the row's own sliced-ELL slots (the rest empty: column -1)
-void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<double>& q) {
+// order, in the row's own sliced-ELL slots (the rest empty: column -1);
+// qw = per slice, the largest filled slot count
+void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<double>& q, std::vector<int>& qw) {
     qc.assign(F.col.size(), -1);
     q.assign(F.col.size(), 0.0);
+    qw.assign(F.nSlices, 0);
     std::vector<std::pair<int, double>> row;
     for (int i = 0; i < F.n; ++i) {
         row.clear();
@@ -976,6 +979,7 @@ void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<d
             }
             q[(size_t)F.sliceStart[sI] + (size_t)k * kSlice + l] += t.second;
         }
+        qw[sI] = std::max(qw[sI], k + 1);
     }
 }
 
@@ -1054,15 +1058,16 @@ void buildAmg(s4f_mesh* M) {
         D->n = H.n;
         D->nSlices = H.nSlices;
         if (!H.agg.empty()) D->agg.upload(H.agg, s);
-        if (l == 0 && A.h.size() > 1) {
-            std::vector<int> qc;
+        if (!H.agg.empty()) {  // every level with a coarser one: q = offdiag(A) P
+            std::vector<int> qc, qw;
             std::vector<double> q;
-            groupByAggregate(H, qc, q);
-            A.qEntries = (long long)std::count_if(qc.begin(), qc.end(), [](int c) { return c >= 0; });
+            groupByAggregate(H, qc, q, qw);
+            if (l == 0) A.qEntries = (long long)std::count_if(qc.begin(), qc.end(), [](int c) { return c >= 0; });
             std::vector<float> q32(q.begin(), q.end());
             D->qCol.upload(qc, s);
             D->q64.upload(q, s);
             D->q32.upload(q32, s);
+            D->qWidth.upload(qw, s);
         }
         if (l > 0) {
             D->sliceStart.upload(H.sliceStart, s);
@@ -1196,7 +1201,7 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
             AmgDevLevel& L = *A.d[l];
             AmgDevLevel& Lc = *A.d[l + 1];
             hipLaunchKernelGGL((k_mg_up<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
-                               (const Row4<T>*)MgVec<T>::b(L), (const Row4<T>*)MgVec<T>::x(L),
+                               (const Row4<T>*)MgVec<T>::x(L), (const Row4<T>*)MgVec<T>::r(L),
                                (const Row4<T>*)MgVec<T>::xs(Lc), MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
         }
     }
@@ -3055,7 +3060,13 @@ void serverPackMg(s4f_server* S, const std::vector<int>& slots) {
         ensureCap(L.memOff, (size_t)Rt[l] + 1);
         ensureCap(L.mem, Mt[l]);
         ensureCap(L.rowInst, Rt[l]);
-        if (l < D - 1) ensureCap(L.agg, Rt[l]);
+        if (l < D - 1) {
+            ensureCap(L.agg, Rt[l]);
+            ensureCap(L.qCol, Et[l]);
+            ensureCap(L.qWidth, St[l]);
+            if (f32) ensureCap(L.q32, Et[l]);
+            else ensureCap(L.q64, Et[l]);
+        }
         std::vector<int> pass(S->maxSlots, 0);
         for (int k = 0; k < nI; ++k) {
             const bool real = l < dk[k];
@@ -3081,15 +3092,26 @@ void serverPackMg(s4f_server* S, const std::vector<int>& slots) {
                 addI(Dl.memOff.p, L.memOff.p + R[l][k], rows, Mo[l][k], 0);
                 addI(Dl.mem.p, L.mem.p + Mo[l][k], (int)H.mem.size(), finerOff, 0);
                 if (l < D - 1) {
-                    if (l < dk[k] - 1) addI(Dl.agg.p, L.agg.p + R[l][k], rows, R[l + 1][k], 0);
-                    else addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+                    if (l < dk[k] - 1) {
+                        addI(Dl.agg.p, L.agg.p + R[l][k], rows, R[l + 1][k], 0);
+                        addI(Dl.qCol.p, L.qCol.p + E[l][k], ents, R[l + 1][k], 1);
+                        if (f32) addW(Dl.q32.p, L.q32.p + E[l][k], ents);
+                        else addW(Dl.q64.p, L.q64.p + E[l][k], 2LL * ents);
+                        addI(Dl.qWidth.p, L.qWidth.p + Sl[l][k], slices, 0, 0);
+                    } else {
+                        addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+                        addI(nullptr, L.qWidth.p + Sl[l][k], slices, 0, 3);
+                    }
                 }
             } else {  // pass-through rows: no matrix, identity members and aggregation
                 addI(nullptr, L.sliceStart.p + Sl[l][k], slices, E[l][k], 3);
                 addI(nullptr, L.sliceWidth.p + Sl[l][k], slices, 0, 3);
                 addI(nullptr, L.memOff.p + R[l][k], rows, Mo[l][k], 2);
                 addI(nullptr, L.mem.p + Mo[l][k], rows, finerOff, 2);
-                if (l < D - 1) addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+                if (l < D - 1) {
+                    addI(nullptr, L.agg.p + R[l][k], rows, R[l + 1][k], 2);
+                    addI(nullptr, L.qWidth.p + Sl[l][k], slices, 0, 3);
+                }
             }
             const int memN = real ? (int)ms[k]->amg.h[l].mem.size() : rows;
             addI(nullptr, L.rowInst.p + R[l][k], rows, slots[k], 3);
