@@ -372,8 +372,9 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
     """Per-kernel averages of rocprofv3 --pmc counters (FETCH_SIZE / WRITE_SIZE,
     KiB per dispatch) from its SQLite output, each scaled by what the same
     counter reads for the plain copy kernel k_copy of known size (copy_bytes
-    read + copy_bytes written).  Prints a text table; reads only the given
-    files."""
+    read + copy_bytes written).  Dispatches shorter than a tenth of the
+    kernel's longest are left out: they are launches past convergence that
+    exit at entry.  Prints a text table; reads only the given files."""
     import collections
     import glob
     import sqlite3
@@ -386,14 +387,16 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
             ncol = "kernel_name" if "kernel_name" in cols else "display_name"
             names = {r[0]: r[1] for r in cur.execute(f"select id, {ncol} from rocpd_info_kernel_symbol")}
             pmc = {r[0]: r[1] for r in cur.execute("select id, name from rocpd_info_pmc")}
-            acc = collections.defaultdict(lambda: [0.0, 0, 0.0])
+            ev = collections.defaultdict(list)
             q = ("select d.kernel_id, p.pmc_id, p.value, d.end - d.start from rocpd_pmc_event p "
                  "join rocpd_kernel_dispatch d on d.event_id = p.event_id")
             for kid, pid, v, dur in cur.execute(q):
-                a = acc[(names.get(kid, str(kid)), pmc.get(pid, str(pid)))]
-                a[0] += v
-                a[1] += 1
-                a[2] += dur
+                ev[(names.get(kid, str(kid)), pmc.get(pid, str(pid)))].append((v, dur))
+            acc = {}
+            for key, lst in ev.items():
+                top = max(dur for _, dur in lst)
+                kept = [(v, dur) for v, dur in lst if dur >= 0.1 * top]
+                acc[key] = (sum(v for v, _ in kept), len(kept), sum(dur for _, dur in kept))
             for (k, c), (sv, n, d) in acc.items():
                 short = k.split("(")[0]
                 for tag in ("k_copy", "k_spmvp", "k_update", "k_residual", "k_grad_bnd_pre", "k_grad_bnd_post",

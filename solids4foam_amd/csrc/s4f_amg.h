@@ -108,8 +108,8 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
-        nb[k] = m.col[e];
-        a[k] = A[e];
+        nb[k] = ldS(m.col + e);
+        a[k] = ldS(A + e);
     }
     const T iD = invD[P], dP = MatSel<T>::diag(m)[P];
     T bP[3], xP[3], y[3];
@@ -129,7 +129,7 @@ __device__ __forceinline__ void mg0DownRowAll(const DevMesh& m, int P, int e0, i
         for (int j = 0; j < 3; ++j) y[j] -= av * (wi * b[(size_t)j * C + c]);
     }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) r[(size_t)j * C + P] = bP[j] - y[j];
+    for (int j = 0; j < 3; ++j) stS(r + (size_t)j * C + P, bP[j] - y[j]);
 }
 
 // level-0 prolongation + post-smoothing of one row (all three components,
@@ -147,8 +147,8 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, const int* __restr
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int en = e0 + k * kSlice + lane;
-        J[k] = qCol[en];
-        qv[k] = q[en];
+        J[k] = ldS(qCol + en);
+        qv[k] = ldS(q + en);
     }
     const int IP = agg[P];
     const T wi = w * invD[P];
@@ -167,7 +167,7 @@ __device__ __forceinline__ void mg0UpRowAll(const DevMesh& m, const int* __restr
         const size_t i = (size_t)j * C + P;
         const T bv = b[i];
         const T zv = wi * (bv + r0[i]) + al * (w1 * eP[j] + wi * s[j]);
-        z[i] = zv;
+        stS(z + i, zv);
         acc[j] += (double)zv * (double)bv;
     }
 }
@@ -395,8 +395,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const Row4<
             for (int u = 0; u < kMgChunk; ++u) {
                 const int k = k0 + u < width ? k0 + u : width - 1;
                 const int e = e0 + k * kSlice + lane;
-                nb[u] = k0 + u < width ? L.col[e] : -1;
-                a[u] = A[e];
+                nb[u] = k0 + u < width ? ldS(L.col + e) : -1;
+                a[u] = ldS(A + e);
             }
 #pragma unroll
             for (int u = 0; u < kMgChunk; ++u) {
@@ -449,8 +449,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>
             for (int u = 0; u < kMgChunk; ++u) {
                 const int k = k0 + u < width ? k0 + u : width - 1;
                 const int en = e0 + k * kSlice + lane;
-                J[u] = k0 + u < width ? L.qCol[en] : -1;
-                c[u] = q[en];
+                J[u] = k0 + u < width ? ldS(L.qCol + en) : -1;
+                c[u] = ldS(q + en);
             }
 #pragma unroll
             for (int u = 0; u < kMgChunk; ++u) {

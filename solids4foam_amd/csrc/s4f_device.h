@@ -13,6 +13,12 @@ constexpr int kMaxPartials = 16;     // reduction values per block
 constexpr double kSmall = 1e-15;     // OpenFOAM SMALL (double)
 constexpr double kVSmall = 1e-300;   // OpenFOAM VSMALL (double)
 
+// Streamed data (matrix and face entries, read once per launch; outputs that
+// the same launch does not read again) goes through non-temporal accesses,
+// so the L2 keeps the neighbour rows that the gathers reuse.
+template <typename T> __device__ __forceinline__ T ldS(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void stS(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Device view of one solver mesh (or of a packed batch of meshes).
 // Cell-indexed arrays are structure-of-arrays: component k of cell c at
 // [k*nCells + c].  ELL entries of slice s, slot k, lane l sit at

@@ -162,12 +162,12 @@ __device__ __forceinline__ void residualInternal(const DevMesh& m, int P, int e0
 #pragma unroll 1
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
-        const int n = m.col[e];
+        const int n = ldS(m.col + e);
         const bool v = n >= 0;
         hasB |= (!v && n != kPad);
         const int c = v ? n : P;
-        const double S0r = m.eS[e], S1r = m.eS[E + e], S2r = m.eS[2 * E + e];
-        const double w = m.eW[e], cfr = m.eC[e], ar = m.eA[e];
+        const double S0r = ldS(m.eS + e), S1r = ldS(m.eS + E + e), S2r = ldS(m.eS + 2 * E + e);
+        const double w = ldS(m.eW + e), cfr = ldS(m.eC + e), ar = ldS(m.eA + e);
         const double S0 = v ? S0r : 0.0, S1 = v ? S1r : 0.0, S2 = v ? S2r : 0.0;
         const double cf = v ? cfr : 0.0, a = v ? ar : 0.0;
         double GN[9], sN[6], DN[3], d[3];
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
             acc[3 + j] += fabs(MD - t);
             acc[6 + j] += fabs((MD + R) - t);
             const T rT = (T)R;
-            r[(size_t)j * C + P] = rT;
+            stS(r + (size_t)j * C + P, rT);
             const T z = invD[P] * rT;
             acc[9 + j] += (double)z * (double)rT;
         }
@@ -294,8 +294,8 @@ __device__ __forceinline__ void spmvpRowAll(const DevMesh& m, int P, int e0, int
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
-        nb[k] = m.col[e];
-        a[k] = A[e];
+        nb[k] = ldS(m.col + e);
+        a[k] = ldS(A + e);
     }
     auto dir = [&](int j, int c, T iDc) -> T {
         const size_t i = (size_t)j * C + c;
@@ -323,8 +323,8 @@ __device__ __forceinline__ void spmvpRowAll(const DevMesh& m, int P, int e0, int
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const size_t i = (size_t)j * C + P;
-        pNew[i] = pP[j];
-        q[i] = y[j];
+        stS(pNew + i, pP[j]);
+        stS(q + i, y[j]);
         acc[j] += (double)pP[j] * (double)y[j];
     }
 }
@@ -462,7 +462,7 @@ __device__ __forceinline__ void gradInternalW(const DevMesh& m, int P, int e0, i
 #pragma unroll 2
     for (int k = 0; k < W; ++k) {
         const int e = e0 + k * kSlice + lane;
-        const int n = m.col[e];
+        const int n = ldS(m.col + e);
         const bool v = n >= 0;
         hasB |= (!v && n != kPad);
         const int c = v ? n : P;
@@ -767,13 +767,13 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         const double M3 = m.minv[3 * (size_t)C + P], M4 = m.minv[4 * (size_t)C + P], M5 = m.minv[5 * (size_t)C + P];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            G[(size_t)j * C + P] = M0 * A[j] + M1 * A[3 + j] + M2 * A[6 + j];
-            G[(size_t)(3 + j) * C + P] = M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j];
-            G[(size_t)(6 + j) * C + P] = M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j];
+            stS(G + (size_t)j * C + P, M0 * A[j] + M1 * A[3 + j] + M2 * A[6 + j]);
+            stS(G + (size_t)(3 + j) * C + P, M1 * A[j] + M3 * A[3 + j] + M4 * A[6 + j]);
+            stS(G + (size_t)(6 + j) * C + P, M2 * A[j] + M4 * A[3 + j] + M5 * A[6 + j]);
         }
         if (Dn && !aitken) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) Dn[(size_t)q * C + P] = DnP[q];
+            for (int q = 0; q < 3; ++q) stS(Dn + (size_t)q * C + P, DnP[q]);
         }
         if (mode != 2) {
             double dx2 = 0.0, dold2 = 0.0, dn2 = 0.0;
