@@ -3,7 +3,7 @@
 //
 // Hierarchy: plain (piecewise-constant) aggregation of strongly connected
 // cells, Galerkin coarse operators A_c = P^T A P, built once per matrix on the
-// host.  Cycle: V(1,1) with damped Jacobi smoothing (omega = 2/3) and an
+// host.  Cycle: V(1,1) with damped Jacobi smoothing (omega = kMgOmega) and an
 // exact solve on the coarsest level (explicit inverse of each instance's
 // block, <= kCoarseBlock rows), so the cycle is a fixed symmetric positive
 // definite operator and CG stays CG.  A batch of meshes is one block-diagonal
@@ -32,6 +32,11 @@ namespace s4f {
 
 constexpr int kCoarseBlock = 1024;  // rows per instance on the coarsest level (explicit inverse)
 constexpr double kMgCoarseScaleDefault = 1.8;  // coarse-correction scale when the caller gives 0
+// Jacobi damping of the smoothing sweeps: near the best smoothing factor of
+// damped Jacobi for 7-point-like operators in 3-D (6/7) and below 1, where
+// the sweep converges on every level's diagonally dominant M-matrix, so the
+// cycle stays symmetric positive definite
+constexpr double kMgOmega = 0.9;
 
 // device view of one coarse level
 struct MgLevel {

@@ -199,7 +199,7 @@ struct Amg {
     unsigned gen = ~0u;  // material generation the hierarchy was built for
     std::vector<AmgHostLevel> h;
     std::vector<std::unique_ptr<AmgDevLevel>> d;
-    double omega = 2.0 / 3.0;
+    double omega = kMgOmega;
     double alpha = 1.0;  // coarse-correction scale (set per solve)
     double buildSeconds = 0.0;
     long long qEntries = 0;  // stored entries of the level-0 grouped operator q
@@ -3174,7 +3174,7 @@ void serverPackMg(s4f_server* S, const std::vector<int>& slots) {
 }
 
 MgRun serverRun(s4f_server* S) {
-    return MgRun{&S->mg, 2.0 / 3.0, S->opts.mgCoarseScale > 0.0 ? S->opts.mgCoarseScale : kMgCoarseScaleDefault,
+    return MgRun{&S->mg, kMgOmega, S->opts.mgCoarseScale > 0.0 ? S->opts.mgCoarseScale : kMgCoarseScaleDefault,
                  S->nActive};
 }
 
