@@ -17,9 +17,9 @@
 //         and the pre-smoothed value x_c = w D_c^-1 b_c in the same pass
 //   down: r = b - A x (on level 0 the neighbours' x = w D^-1 b is recomputed
 //         from their b and 1/diag)
-//   up:   prolongation x += e[agg] fused into the post-smoothing sweep (the
-//         neighbours' corrected x recomputed on the fly; on level 0 through
-//         the down sweep's residual and q = offdiag(A) P)
+//   up:   prolongation x += e[agg] fused into the post-smoothing sweep,
+//         formed from the down sweep's residual and q = offdiag(A) P (no
+//         matrix pass; only the coarser correction is gathered)
 // Level 0 is the solver mesh's own sliced-ELL matrix; coarser levels have
 // their own sliced-ELL arrays.  Every row's arithmetic runs in a fixed order
 // that depends only on its own instance, so results do not depend on which
