@@ -3,7 +3,8 @@
 Reads what a solids4foam case gives the reference and writes what it writes:
   * dictionaries (FoamFile subset: keywords, sub-dictionaries, lists,
     ``uniform``/``nonuniform List<...>`` fields, dimensioned scalars, quoted
-    regex keys, // and /* */ comments; ``#include`` is not supported);
+    regex keys, // and /* */ comments, ``#include`` / ``#includeIfPresent``
+    of files; other ``#`` directives are refused);
   * ASCII polyMesh (points, faces, owner, neighbour, boundary), optionally
     gzip-compressed (``*.gz``);
   * the ``0/D`` boundary conditions the device path supports
@@ -146,8 +147,35 @@ def parse_dict(text):
     return _Parser(text).parse_dict_body(end=None)
 
 
+_INCLUDE = re.compile(r'#(includeIfPresent|sinclude|include)\s+"([^"]+)"')
+
+
+def _expand_includes(text, path, depth=0):
+    """Replaces ``#include "file"`` (and ``#includeIfPresent`` / ``#sinclude``,
+    which skip a missing file) by the file's text.  A relative name is looked
+    up next to the including file, then in the case directory (the parent of
+    system/, constant/ or the time directory); $FOAM_CASE and <case> name the
+    case directory."""
+    if depth > 8:
+        raise ValueError(f"{path}: #include nested too deeply")
+    here = os.path.dirname(os.path.abspath(path))
+    case = os.path.dirname(here)
+
+    def repl(m):
+        kind, name = m.group(1), m.group(2)
+        name = name.replace("$FOAM_CASE", case).replace("<case>", case)
+        for c in ([name] if os.path.isabs(name) else [os.path.join(here, name), os.path.join(case, name)]):
+            if os.path.exists(c) or os.path.exists(c + ".gz"):
+                return _expand_includes(_strip_comments(read_text(c)), c, depth + 1)
+        if kind == "include":
+            raise FileNotFoundError(f"{path}: #include {name!r} not found")
+        return ""
+
+    return _INCLUDE.sub(repl, text)
+
+
 def read_dict(path):
-    return parse_dict(read_text(path))
+    return parse_dict(_expand_includes(_strip_comments(read_text(path)), path))
 
 
 def lookup(d, key, default=None):

@@ -112,6 +112,25 @@ def test_parse_dictionaries(tmp_path):
     assert sc == dict(pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
 
 
+def test_include_directives(tmp_path):
+    # next to the including file, then in the case directory; IfPresent
+    # skips a missing file; other directives are still refused
+    _write(str(tmp_path), "system/controlDict", 'application solids4Foam;\n#include "times"\n'
+           'functions { #include "./system/functions" }\n#includeIfPresent "missing"\nendTime 3;\n')
+    _write(str(tmp_path), "system/times", "deltaT 0.5; // a comment\nwriteInterval 2;\n")
+    _write(str(tmp_path), "system/functions", 'probe { type probes; fields (D); }\n')
+    d = foam_io.read_dict(os.path.join(str(tmp_path), "system/controlDict"))
+    assert d["deltaT"] == 0.5 and d["writeInterval"] == 2 and d["endTime"] == 3
+    assert d["functions"]["probe"]["type"] == "probes"
+    assert foam_io.read_time_controls(str(tmp_path))["deltaT"] == 0.5
+    _write(str(tmp_path), "system/bad", '#include "nowhere"\n')
+    with pytest.raises(FileNotFoundError):
+        foam_io.read_dict(os.path.join(str(tmp_path), "system/bad"))
+    _write(str(tmp_path), "system/etc", '#includeEtc "caseDicts/setConstraintTypes"\n')
+    with pytest.raises(ValueError, match="not supported"):
+        foam_io.read_dict(os.path.join(str(tmp_path), "system/etc"))
+
+
 def test_polymesh_and_bc_roundtrip(tmp_path):
     pm, a = _box_case(str(tmp_path))
     m = foam_io.read_polymesh(str(tmp_path))
@@ -151,6 +170,9 @@ def test_parse_reference_tutorial_dictionaries():
     assert model == "linearGeometryTotalDisplacement" and opts["nCorrectors"] == 100000
     assert foam_io.read_solver_controls(case) == dict(pcgTolerance=1e-8, pcgRelTol=0.1, pcgMaxIter=50)
     assert foam_io.read_linear_elastic(case) == (200e9, 0.3)
+    # a controlDict that includes its function objects from system/functions
+    tc = foam_io.read_time_controls("/root/reference/tutorials/solids/linearElasticity/contactPatchTest")
+    assert tc["endTime"] == 1.0 and tc["deltaT"] == 1.0
 
 
 @pytest.mark.gpu
