@@ -166,6 +166,11 @@ s4f_status s4f_mesh_mg_info(const s4f_mesh *mesh, int32_t maxLevels, int32_t *nL
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
 s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);
+/* mechanicalProperties planeStress (mechanicalLaw.C:485-497): nonzero selects
+   linearElastic's plane-stress moduli (lambda = nu E/((1+nu)(1-nu)),
+   K = E/(3(1-nu)), linearElastic.C:109-113) for 2-D cases.  Takes effect at
+   the next s4f_material_set. */
+s4f_status s4f_plane_stress_set(s4f_mesh *mesh, int32_t planeStress);
 
 /* Density (mechanicalProperties rho, linearElastic.C:62-104): perCell ? nCells
    values : one.  Needed for the inertia term and the body force. */

@@ -24,8 +24,11 @@ PRECOND_DIC, PRECOND_DIAGONAL, PRECOND_NONE = 0, 1, 2
 
 
 def build(force=False):
-    """Compile the oracle with its Makefile (gcc)."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile the oracle with its Makefile (gcc): when forced, missing, or
+    older than its source."""
+    src = os.path.join(_HERE, "s4f_oracle.c")
+    stale = os.path.exists(_LIB_PATH) and os.path.getmtime(src) > os.path.getmtime(_LIB_PATH)
+    if force or stale or not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
 
 
@@ -46,7 +49,8 @@ def lib():
         L.or_mesh_get.argtypes = [vp, dp, ip, ip, ip, ip, ip, ip]
         L.or_mesh_geometry.argtypes = [vp, dp, dp, dp, dp]
         L.or_case_create.restype = vp
-        L.or_case_create.argtypes = [vp, dp, dp, ip, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.or_case_create.argtypes = [vp, dp, dp, ip, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int]
         L.or_case_free.argtypes = [vp]
         L.or_point_interpolate.argtypes = [vp, dp, dp, dp]
         L.or_field_size.restype = ctypes.c_long
@@ -160,7 +164,7 @@ class Case:
     """A solid case: mesh + linearElastic material + per-patch BCs + state."""
 
     def __init__(self, mesh, E, nu, patch_types, b_value, b_pressure=None, ls_full_delta=False,
-                 rc_scale=0.1, interface_scale=0.01):
+                 rc_scale=0.1, interface_scale=0.01, plane_stress=False):
         self.mesh = mesh
         C, B = mesh.nCells, mesh.nBoundaryFaces
         self.E = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float64), (C,)))
@@ -169,7 +173,8 @@ class Case:
         bv = np.ascontiguousarray(np.asarray(b_value, np.float64).reshape(B, 3))
         bp = np.zeros(B) if b_pressure is None else np.ascontiguousarray(b_pressure, np.float64)
         self.h = lib().or_case_create(mesh.h, _dp(self.E), _dp(self.nu), _ip(pt), _dp(bv), _dp(bp),
-                                      int(ls_full_delta), float(rc_scale), float(interface_scale))
+                                      int(ls_full_delta), float(rc_scale), float(interface_scale),
+                                      int(bool(plane_stress)))
         self.init()
 
     def __del__(self):

@@ -535,7 +535,7 @@ static void patch_delta(const or_case *c, int b, int full, double *d)
 
 or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, const int *patchType,
                         const double *bValue, const double *bPressure, int lsFullDelta,
-                        double rcScale, double interfaceScale)
+                        double rcScale, double interfaceScale, int planeStress)
 {
     or_case *c = xcalloc(1, sizeof(or_case));
     c->m = m;
@@ -560,14 +560,15 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
     c->bValue = xcalloc((size_t)B*3, sizeof(double)); memcpy(c->bValue, bValue, sizeof(double)*3*B);
     c->bPressure = xcalloc((size_t)B, sizeof(double)); if (bPressure) memcpy(c->bPressure, bPressure, sizeof(double)*B);
 
-    /* linearElastic.C:62-133 (E,nu branch, 3-D), impK linearElastic.C:204-245 */
+    /* linearElastic.C:62-133 (E,nu branch; planeStress :109-113 changes
+       lambda and K), impK linearElastic.C:204-245 */
     c->mu = xcalloc(C, sizeof(double)); c->lambda = xcalloc(C, sizeof(double));
     c->K = xcalloc(C, sizeof(double)); c->impK = xcalloc(C, sizeof(double));
     for (int i = 0; i < C; ++i) {
         double e = E[i], v = nu[i];
         c->mu[i] = e/(2.0*(1.0 + v));
-        c->lambda[i] = v*e/((1.0 + v)*(1.0 - 2.0*v));
-        c->K[i] = e/(3.0*(1.0 - 2.0*v));
+        c->lambda[i] = planeStress ? v*e/((1.0 + v)*(1.0 - v)) : v*e/((1.0 + v)*(1.0 - 2.0*v));
+        c->K[i] = planeStress ? e/(3.0*(1.0 - v)) : e/(3.0*(1.0 - 2.0*v));
         c->impK[i] = (v == 0.5) ? 2.0*c->mu[i] : 2.0*c->mu[i] + c->lambda[i];
     }
 

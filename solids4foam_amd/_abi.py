@@ -83,7 +83,7 @@ EXPORTED = [
     "s4f_server_create", "s4f_server_submit", "s4f_server_poll", "s4f_server_result", "s4f_server_stats",
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
     "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
-    "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info",
+    "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info", "s4f_plane_stress_set",
 ]
 
 
@@ -125,6 +125,7 @@ def lib():
         L.s4f_material_set.argtypes = [vp, dp, dp, ctypes.c_int32]
         L.s4f_bc_set.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.s4f_density_set.argtypes = [vp, dp, ctypes.c_int32]
+        L.s4f_plane_stress_set.argtypes = [vp, ctypes.c_int32]
         L.s4f_gravity_set.argtypes = [vp, dp]
         L.s4f_old_time_set.argtypes = [vp, dp, dp]
         L.s4f_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]

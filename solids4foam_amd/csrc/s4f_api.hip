@@ -129,6 +129,7 @@ struct HostMesh {
     std::vector<double> bN, bKv, bLs;       // SoA 3B
     // material / boundary values
     bool uniformMaterial = true;
+    bool planeStress = false;  // mechanicalProperties planeStress: the 2-D moduli of linearElastic.C:109-113
     std::vector<double> mu, K, impK;        // C (or 1)
     std::vector<double> eA, eC;             // E
     std::vector<double> diag;               // C
@@ -531,9 +532,10 @@ void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCel
         if (!(e > 0.0) || !std::isfinite(e) || !(v > -1.0 && v <= 0.5) || !std::isfinite(v))
             throw InvalidError("linearElastic needs E > 0 and -1 < nu <= 0.5");
         const double mu = e / (2.0 * (1.0 + v));
-        const double lam = v * e / ((1.0 + v) * (1.0 - 2.0 * v));
+        // linearElastic.C:109-128: plane stress changes lambda and K, not mu
+        const double lam = h.planeStress ? v * e / ((1.0 + v) * (1.0 - v)) : v * e / ((1.0 + v) * (1.0 - 2.0 * v));
         h.mu[i] = mu;
-        h.K[i] = e / (3.0 * (1.0 - 2.0 * v));
+        h.K[i] = h.planeStress ? e / (3.0 * (1.0 - v)) : e / (3.0 * (1.0 - 2.0 * v));
         h.impK[i] = (v == 0.5) ? 2.0 * mu : 2.0 * mu + lam;
     }
     if (h.impK.size() == 1 && !std::isfinite(h.impK[0])) throw InvalidError("nu = 0.5 needs the pressure equation");
@@ -1918,6 +1920,12 @@ s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int3
         M->materialSet = true;
         return S4F_OK;
     });
+}
+
+s4f_status s4f_plane_stress_set(s4f_mesh* M, int32_t planeStress) {
+    if (!M) return fail(S4F_ERR_INVALID, "null argument");
+    M->h.planeStress = planeStress != 0;
+    return S4F_OK;
 }
 
 s4f_status s4f_density_set(s4f_mesh* M, const double* rho, int32_t perCell) {

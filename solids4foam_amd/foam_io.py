@@ -405,12 +405,16 @@ def read_linear_elastic(case_dir):
         law = laws
     if law.get("type") != "linearElastic":
         raise ValueError(f"mechanical law {law.get('type')!r} is not supported (linearElastic)")
-    if str(d.get("planeStress", "no")) in ("yes", "on", "true"):
-        raise ValueError("planeStress is not supported on the device path")
     if "E" in law and "nu" in law:
         return dimensioned(law["E"]), dimensioned(law["nu"])
     mu, K = dimensioned(law["mu"]), dimensioned(law["K"])
     return 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
+
+
+def read_plane_stress(case_dir):
+    """mechanicalProperties planeStress switch (mechanicalLaw.C:485-497)"""
+    d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
+    return str(d.get("planeStress", "no")).lower() in ("yes", "on", "true", "1")
 
 
 def read_cell_zones(case_dir):
@@ -447,8 +451,6 @@ def read_materials(case_dir, nCells):
     name (the reference's solidSubMeshes assignment, mechanicalModel.C:195-245;
     the device path takes them as a per-cell linearElastic material)."""
     d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
-    if str(d.get("planeStress", "no")) in ("yes", "on", "true"):
-        raise ValueError("planeStress is not supported on the device path")
     laws = d.get("mechanical")
     if not isinstance(laws, list):
         return _law_constants(laws)
@@ -623,6 +625,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     if rho is not None and np.any(np.isnan(rho)):
         rho = None
     m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
+                                    plane_stress=read_plane_stress(case_dir),
                                     ls_full_delta="extendedLeastSquares" in grad_scheme, rho=rho, g=g,
                                     d2dt2=scheme, deltaT=tc["deltaT"] if tc else None, predictor=predictor)
     if not transient:

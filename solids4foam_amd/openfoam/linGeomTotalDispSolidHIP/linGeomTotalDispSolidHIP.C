@@ -146,6 +146,13 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
     const linearElastic& law = refCast<const linearElastic>(laws[0]);
     const double E = law.E().value();
     const double nu = law.nu().value();
+    // planeStress (mechanicalLaw.C:485-497): linearElastic's 2-D lambda and K
+    const Switch planeStress
+    (
+        m.lookupObject<IOdictionary>("mechanicalProperties")
+            .lookupOrDefault<Switch>("planeStress", false)
+    );
+    check(s4f_plane_stress_set(gpuMesh_, planeStress ? 1 : 0), "createDeviceMesh()");
     check(s4f_material_set(gpuMesh_, &E, &nu, 0), "createDeviceMesh()");
 
     // rho (inertia and body force) and g (solidModel::g(), constant/g)

@@ -165,6 +165,11 @@ class SolverMesh:
             nu = f64(np.broadcast_to(nu, (self.nCells,)))
         check(lib().s4f_material_set(self.h, ptr(E), ptr(nu), int(per_cell)))
 
+    def set_plane_stress(self, on=True):
+        """mechanicalProperties planeStress: linearElastic's 2-D moduli, used
+        by the next set_material"""
+        check(lib().s4f_plane_stress_set(self.h, int(bool(on))))
+
     def set_density(self, rho):
         """mechanicalProperties rho: one value or one per cell"""
         r = np.atleast_1d(f64(rho))
@@ -297,11 +302,12 @@ class LinGeomTotalDispSolid:
     typeName = "linearGeometryTotalDisplacement"
 
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False, rho=None,
-                 g=None, d2dt2="steadyState", deltaT=None, predictor=False):
+                 g=None, d2dt2="steadyState", deltaT=None, predictor=False, plane_stress=False):
         # rho: mechanicalProperties density (needed with g or a transient
         # d2dt2); g: constant/g; d2dt2: d2dt2Schemes default (steadyState |
         # Euler); deltaT: the time step of evolve() when transient;
-        # predictor: linGeomTotalDispSolid::predict (:47-59), transient only
+        # predictor: linGeomTotalDispSolid::predict (:47-59), transient only;
+        # plane_stress: mechanicalProperties planeStress (2-D moduli)
         # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
         # "solidSymmetry"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
@@ -316,6 +322,8 @@ class LinGeomTotalDispSolid:
                                  "solidSymmetry, empty)")
             types.append(kinds[t])
         self.mesh = SolverMesh(ctx, polymesh, types, ls_full_delta)
+        if plane_stress:
+            self.mesh.set_plane_stress(True)
         self.mesh.set_material(E, nu)
         for i, p in enumerate(patches):
             if p["type"] in ("fixedDisplacement", "fixedDisplacementZeroShear"):

@@ -153,6 +153,42 @@ def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precis
     assert out["pcgIterations"] < jac["pcgIterations"]
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_plane_stress_matches_oracle(ctx, oracle_mod, precision):
+    # 2-D cantilever (empty z faces) with mechanicalProperties planeStress:
+    # the device path's converged field equals the oracle's, and differs
+    # from the plane-strain answer (the switch reaches the moduli)
+    from solids4foam_amd import PolyMesh, SolverMesh
+    print(f"[plane stress] precision={precision}", flush=True)
+    E, nu, t = 200e9, 0.3, 1e6
+    n, L = (16, 4, 1), (1.0, 0.2, 0.05)
+    pt = [FIXED, TRACTION, TRACTION, TRACTION, oracle_mod.EMPTY, oracle_mod.EMPTY]
+    om = oracle_mod.Mesh.box(*n, L=L)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    bv[om.patch_faces(1)] = (0.0, t, 0.0)
+    oc = oracle_mod.Case(om, E, nu, pt, bv, plane_stress=True)
+    r = oc.solve(nCorr=200000, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, precond=oracle_mod.PRECOND_DIC)
+    assert r["converged"]
+    Do = oc.get("D").reshape(-1, 3)
+    pm = PolyMesh.box(*n, L=L)
+    sl = [pm.patch_faces(p) for p in range(6)]
+    fields = {}
+    for ps in (True, False):
+        m = SolverMesh(ctx, pm, pt)
+        m.set_plane_stress(ps)
+        m.set_material(E, nu)
+        for p in range(4):
+            m.set_bc(p, bv[sl[p]])
+        out = m.solve(precision=precision, **TIGHT)
+        assert out["converged"]
+        fields[ps] = out["D"]
+    tol = 1e-8 if precision == 0 else 1e-6
+    err = rel_l2(fields[True], Do)
+    print(f"  err vs oracle {err:.2e}, plane strain differs by {rel_l2(fields[False], Do):.2e}", flush=True)
+    assert err < tol
+    assert rel_l2(fields[False], Do) > 1e-3
+
+
 def test_multigrid_hierarchy_info(ctx):
     # level 0 is the mesh (two stored entries per internal face), every level
     # coarsens, the coarsest fits one dense block; q = offdiag(A) P groups a

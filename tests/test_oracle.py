@@ -414,6 +414,37 @@ def test_symmetry_planes_uniaxial_exact(oracle_mod):
     assert np.abs(Db[m.patch_faces(0), 0]).max() < 1e-20 and np.abs(Db[m.patch_faces(4), 2]).max() < 1e-20
 
 
+@pytest.mark.parametrize("plane_stress", [True, False])
+def test_plane_stress_2d_uniaxial_exact(oracle_mod, plane_stress):
+    # 2-D plate (empty z faces), solidSymmetry rollers on x-min and y-min,
+    # traction t on x-max: the discrete solution is the uniform strain with
+    # sigma_xx = t, sigma_yy = 0 under linearElastic's Hooke law
+    # sigma = 2 mu dev(eps) + K tr(eps) I (eps_zz = 0), whose K is
+    # E/(3(1-nu)) with planeStress and E/(3(1-2nu)) without
+    # (linearElastic.C:62-133, :336-338).  Solving those two equations for
+    # (eps_xx, eps_yy) fixes the strain; a wrong plane-stress K or the 3-D
+    # one fails the other case.
+    E, nu, t = 200e9, 0.3, 1e7
+    m = oracle_mod.Mesh.box(6, 5, 1, (1.0, 0.8, 0.1))
+    SYM, EMPTY = 3, oracle_mod.EMPTY
+    bv = np.zeros((m.nBoundaryFaces, 3))
+    bv[m.patch_faces(1)] = (t, 0.0, 0.0)
+    c = oracle_mod.Case(m, E, nu, [SYM, TRACTION, SYM, TRACTION, EMPTY, EMPTY], bv, plane_stress=plane_stress)
+    r = c.solve(nCorr=50000, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+    assert r["converged"]
+    mu = E / (2 * (1 + nu))
+    K = E / (3 * (1 - nu)) if plane_stress else E / (3 * (1 - 2 * nu))
+    le = K - 2 * mu / 3  # sigma = 2 mu eps + le tr(eps) I
+    A = np.array([[2 * mu + le, le], [le, 2 * mu + le]])
+    exx, eyy = np.linalg.solve(A, [t, 0.0])
+    D = c.get("D").reshape(-1, 3)
+    X = m.C
+    exact = np.stack([exx * X[:, 0], eyy * X[:, 1], np.zeros(m.nCells)], axis=1)
+    assert np.abs(D - exact).max() < 1e-8 * np.abs(exact).max()
+    s = c.get("sigma").reshape(-1, 6)
+    assert np.allclose(s[:, 0], t, rtol=1e-7) and np.abs(s[:, 3]).max() < 1e-6 * t
+
+
 def _zero_shear_box(oracle_mod, jitter, top, relax=None):
     # fixedDisplacementZeroShear rollers (zero normal displacement) on x-min,
     # y-min, z-min; x-max / y-max traction free; z-max either a normal
