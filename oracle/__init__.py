@@ -175,6 +175,8 @@ class Case:
         self.h = lib().or_case_create(mesh.h, _dp(self.E), _dp(self.nu), _ip(pt), _dp(bv), _dp(bp),
                                       int(ls_full_delta), float(rc_scale), float(interface_scale),
                                       int(bool(plane_stress)))
+        if not self.h:
+            raise ValueError("linearElastic needs E > 0 and -1 < nu < 0.5 (nu = 0.5 needs the pressure equation)")
         self.init()
 
     def __del__(self):

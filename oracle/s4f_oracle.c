@@ -537,6 +537,11 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
                         const double *bValue, const double *bPressure, int lsFullDelta,
                         double rcScale, double interfaceScale, int planeStress)
 {
+    /* linearElastic.C:101-147: -1 <= nu <= 0.5; nu = 0.5 sets lambda = K =
+       GREAT for the pressure-equation formulation, which is not restated:
+       refused (NULL) like an unphysical nu or E <= 0 */
+    for (int i = 0; i < m->nCells; ++i)
+        if (!(E[i] > 0.0) || !(nu[i] > -1.0 && nu[i] < 0.5)) return NULL;
     or_case *c = xcalloc(1, sizeof(or_case));
     c->m = m;
     const int C = m->nCells, F = m->nInternalFaces, B = m->nFaces - m->nInternalFaces;
@@ -569,7 +574,7 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         c->mu[i] = e/(2.0*(1.0 + v));
         c->lambda[i] = planeStress ? v*e/((1.0 + v)*(1.0 - v)) : v*e/((1.0 + v)*(1.0 - 2.0*v));
         c->K[i] = planeStress ? e/(3.0*(1.0 - v)) : e/(3.0*(1.0 - 2.0*v));
-        c->impK[i] = (v == 0.5) ? 2.0*c->mu[i] : 2.0*c->mu[i] + c->lambda[i];
+        c->impK[i] = 2.0*c->mu[i] + c->lambda[i];
     }
 
     /* [OF surfaceInterpolation] weights, nonOrthDeltaCoeffs, nonOrthCorrectionVectors */

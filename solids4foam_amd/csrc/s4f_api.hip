@@ -130,6 +130,7 @@ struct HostMesh {
     // material / boundary values
     bool uniformMaterial = true;
     bool planeStress = false;  // mechanicalProperties planeStress: the 2-D moduli of linearElastic.C:109-113
+    std::vector<double> matE, matNu;        // the E and nu the moduli were computed from (C or 1)
     std::vector<double> mu, K, impK;        // C (or 1)
     std::vector<double> eA, eC;             // E
     std::vector<double> diag;               // C
@@ -522,45 +523,71 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
     h.instMask = {emptyDir < 0 ? 7 : (7 & ~(1 << emptyDir))};
 }
 
+// Rhie-Chow coefficient gamma_f |S_f| deltaCoeffs of every ELL entry
+// (momentumStabilisation.C:112-150): gamma_f = rcScale * interpolate(impK),
+// on material-interface faces interfaceScale * mean(impK)
+std::vector<double> stabilisationCoeffs(const HostMesh& h, double rcScale, double ifScale) {
+    std::vector<double> eC(h.col.size(), 0.0);
+    auto ik = [&](int c) { return h.uniformMaterial ? h.impK[0] : h.impK[c]; };
+    for (int f = 0; f < h.F; ++f) {
+        const int o = h.owner[f], nn = h.neighbour[f];
+        const double wf = h.w[f];
+        const double go = rcScale * ik(o), gn = rcScale * ik(nn);
+        double gam = wf * (go - gn) + gn;
+        if (std::fabs(ik(o) - ik(nn)) > kSmall) gam = ifScale * 0.5 * (ik(o) + ik(nn));
+        const double c = gam * h.magSf[f] * h.nonOrthDelta[f];
+        eC[h.entOwn[f]] = c;
+        eC[h.entNei[f]] = c;
+    }
+    return eC;
+}
+
+constexpr double kRhieChowScaleDefault = 0.1, kInterfaceScaleDefault = 0.01;
+
 void computeMaterial(HostMesh& h, const double* E, const double* nu, bool perCell) {
     const int C = h.C, F = h.F, B = h.B;
     const int n = perCell ? C : 1;
+    // linearElastic.C:101-133: nu = 0.5 gives lambda = K = GREAT, a case the
+    // reference solves with its pressure equation (solvePressureEqn); the
+    // device path does not build that formulation
+    for (int i = 0; i < n; ++i) {
+        const double e = E[i], v = nu[i];
+        if (!(e > 0.0) || !std::isfinite(e) || !std::isfinite(v) || !(v > -1.0 && v <= 0.5))
+            throw InvalidError("linearElastic needs E > 0 and -1 < nu <= 0.5");
+        if (v == 0.5)
+            throw InvalidError("nu = 0.5 (incompressible) needs linearElastic's pressure equation (solvePressureEqn), "
+                               "which the device path does not build");
+    }
     h.uniformMaterial = !perCell;
+    h.matE.assign(E, E + n);
+    h.matNu.assign(nu, nu + n);
     h.mu.resize(n); h.K.resize(n); h.impK.resize(n);
     for (int i = 0; i < n; ++i) {
         const double e = E[i], v = nu[i];
-        if (!(e > 0.0) || !std::isfinite(e) || !(v > -1.0 && v <= 0.5) || !std::isfinite(v))
-            throw InvalidError("linearElastic needs E > 0 and -1 < nu <= 0.5");
         const double mu = e / (2.0 * (1.0 + v));
         // linearElastic.C:109-128: plane stress changes lambda and K, not mu
         const double lam = h.planeStress ? v * e / ((1.0 + v) * (1.0 - v)) : v * e / ((1.0 + v) * (1.0 - 2.0 * v));
         h.mu[i] = mu;
         h.K[i] = h.planeStress ? e / (3.0 * (1.0 - v)) : e / (3.0 * (1.0 - 2.0 * v));
-        h.impK[i] = (v == 0.5) ? 2.0 * mu : 2.0 * mu + lam;
+        h.impK[i] = 2.0 * mu + lam;
+        if (!std::isfinite(h.K[i]) || !std::isfinite(h.impK[i])) throw InvalidError("non-finite elastic moduli");
     }
-    if (h.impK.size() == 1 && !std::isfinite(h.impK[0])) throw InvalidError("nu = 0.5 needs the pressure equation");
     auto ik = [&](int c) { return perCell ? h.impK[c] : h.impK[0]; };
     const size_t E_ = h.col.size();
     h.eA.assign(E_, 0.0);
-    h.eC.assign(E_, 0.0);
     h.diag.assign(C, 0.0);
-    const double rcScale = 0.1, ifScale = 0.01;  // defaults; scaled at solve time if different
     for (int f = 0; f < F; ++f) {
         const int o = h.owner[f], nn = h.neighbour[f];
         const double wf = h.w[f];
         const double impKf = wf * (ik(o) - ik(nn)) + ik(nn);
         const double a = h.nonOrthDelta[f] * (impKf * h.magSf[f]);
-        const double go = rcScale * ik(o), gn = rcScale * ik(nn);
-        double gam = wf * (go - gn) + gn;
-        if (std::fabs(ik(o) - ik(nn)) > kSmall) gam = ifScale * 0.5 * (ik(o) + ik(nn));
-        const double c = gam * h.magSf[f] * h.nonOrthDelta[f];
         h.eA[h.entOwn[f]] = a;
         h.eA[h.entNei[f]] = a;
-        h.eC[h.entOwn[f]] = c;
-        h.eC[h.entNei[f]] = c;
         h.diag[o] += a;
         h.diag[nn] += a;
     }
+    // the defaults; a solve with other scales rebuilds the device copy
+    h.eC = stabilisationCoeffs(h, kRhieChowScaleDefault, kInterfaceScaleDefault);
     for (int b = 0; b < B; ++b)
         // symmetry and zero shear: the full fixed-value coefficient (their
         // implicit parts are per component; one matrix serves the three
@@ -604,20 +631,9 @@ void uploadMaterial(s4f_mesh* M) {
 void applyStabilisationScale(s4f_mesh* M, double rcScale, double ifScale) {
     if (rcScale == M->rcScale && ifScale == M->ifScale) return;
     HostMesh& h = M->h;
-    if (h.instCell0.size() != 1) throw InvalidError("batched solves use the stabilisation scales the meshes were set up with");
-    std::vector<double> eC(h.eC.size(), 0.0);
-    auto ik = [&](int c) { return h.uniformMaterial ? h.impK[0] : h.impK[c]; };
-    for (int f = 0; f < h.F; ++f) {
-        const int o = h.owner[f], nn = h.neighbour[f];
-        const double wf = h.w[f];
-        const double go = rcScale * ik(o), gn = rcScale * ik(nn);
-        double gam = wf * (go - gn) + gn;
-        if (std::fabs(ik(o) - ik(nn)) > kSmall) gam = ifScale * 0.5 * (ik(o) + ik(nn));
-        const double c = gam * h.magSf[f] * h.nonOrthDelta[f];
-        eC[h.entOwn[f]] = c;
-        eC[h.entNei[f]] = c;
-    }
-    M->eC.upload(eC, M->ctx->stream);
+    // a packed batch is packed with its solve's scales (s4f_batch_solve repacks when they change)
+    if (h.instCell0.size() != 1) throw InvalidError("packed batch with other stabilisation scales than its solve");
+    M->eC.upload(stabilisationCoeffs(h, rcScale, ifScale), M->ctx->stream);
     M->rcScale = rcScale;
     M->ifScale = ifScale;
 }
@@ -1634,13 +1650,16 @@ void solveDispatch(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>
 
 // Concatenates meshes (their material and boundary values included) into one
 // packed structure: each instance's cells start on a slice boundary; column,
-// boundary-face and cell indices are offset; material data is per cell.
-void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
+// boundary-face and cell indices are offset; material data is per cell.  The
+// steady matrix (diagBase: no inertia term of an earlier transient solve) and
+// the Rhie-Chow coefficients of the given stabilisation scales are packed.
+void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P, double rcScale, double ifScale) {
     P = HostMesh();
     const int n = (int)ms.size();
     for (const HostMesh* m : ms)
         if (m->g[0] != 0.0 || m->g[1] != 0.0 || m->g[2] != 0.0)
             throw InvalidError("body-force solves take a single mesh");
+    const bool defaultScales = rcScale == kRhieChowScaleDefault && ifScale == kInterfaceScaleDefault;
     long long C = 0, B = 0, E = 0, S = 0;
     for (const HostMesh* m : ms) {
         P.instCell0.push_back((int)C);
@@ -1672,6 +1691,8 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
         const HostMesh& m = *ms[i];
         const int c0 = P.instCell0[i], b0 = P.instB0[i];
         const size_t Em = m.col.size();
+        const std::vector<double> eCs = defaultScales ? std::vector<double>() : stabilisationCoeffs(m, rcScale, ifScale);
+        const std::vector<double>& eC = defaultScales ? m.eC : eCs;
         for (int s = 0; s < m.nSlices; ++s) {
             P.sliceStart[s0 + s] = (int)(e0 + m.sliceStart[s]);
             P.sliceWidth[s0 + s] = m.sliceWidth[s];
@@ -1682,12 +1703,12 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P) {
             for (int q = 0; q < 3; ++q) P.eS[(size_t)q * E + e0 + e] = m.eS[(size_t)q * Em + e];
             P.eW[e0 + e] = m.eW[e];
             P.eA[e0 + e] = m.eA[e];
-            P.eC[e0 + e] = m.eC[e];
+            P.eC[e0 + e] = eC[e];
         }
         for (int c = 0; c < m.C; ++c) {
             for (int q = 0; q < 3; ++q) P.cC[(size_t)q * C + c0 + c] = m.cC[(size_t)q * m.C + c];
             for (int q = 0; q < 6; ++q) P.minv[(size_t)q * C + c0 + c] = m.minv[(size_t)q * m.C + c];
-            P.diag[c0 + c] = m.diag[c];
+            P.diag[c0 + c] = m.diagBase.empty() ? m.diag[c] : m.diagBase[c];
             const int mc = m.uniformMaterial ? 0 : c;
             P.mu[c0 + c] = m.mu[mc];
             P.K[c0 + c] = m.K[mc];
@@ -1915,17 +1936,30 @@ s4f_status s4f_material_set(s4f_mesh* M, const double* E, const double* nu, int3
         computeMaterial(M->h, E, nu, perCell != 0);
         uploadMaterial(M);
         M->generation++;
-        M->rcScale = 0.1;
-        M->ifScale = 0.01;
+        M->rcScale = kRhieChowScaleDefault;
+        M->ifScale = kInterfaceScaleDefault;
         M->materialSet = true;
         return S4F_OK;
     });
 }
 
 s4f_status s4f_plane_stress_set(s4f_mesh* M, int32_t planeStress) {
-    if (!M) return fail(S4F_ERR_INVALID, "null argument");
-    M->h.planeStress = planeStress != 0;
-    return S4F_OK;
+    return guarded([&]() -> s4f_status {
+        if (!M) throw InvalidError("null argument");
+        const bool on = planeStress != 0;
+        if (M->h.planeStress == on) return S4F_OK;
+        M->h.planeStress = on;
+        if (M->materialSet) {  // the moduli change now, from the E and nu last set
+            HIP_CHECK(hipSetDevice(M->ctx->device));
+            const std::vector<double> E = M->h.matE, nu = M->h.matNu;
+            computeMaterial(M->h, E.data(), nu.data(), !M->h.uniformMaterial);
+            uploadMaterial(M);
+            M->generation++;
+            M->rcScale = kRhieChowScaleDefault;
+            M->ifScale = kInterfaceScaleDefault;
+        }
+        return S4F_OK;
+    });
 }
 
 s4f_status s4f_density_set(s4f_mesh* M, const double* rho, int32_t perCell) {
@@ -2449,7 +2483,9 @@ s4f_status s4f_polymesh_point_cells(const s4f_polymesh* p, int32_t* offsets, int
 struct s4f_batch {
     s4f_ctx* ctx = nullptr;
     std::vector<s4f_mesh*> members;   // borrowed; must outlive the solve
-    std::unique_ptr<s4f_mesh> packed; // rebuilt when membership changes
+    std::unique_ptr<s4f_mesh> packed; // rebuilt when membership, a member's material or the scales change
+    std::vector<unsigned> packedGen;  // member generations the packing was built from
+    double packedRc = 0.0, packedIf = 0.0;
     bool dirty = true;
 };
 
@@ -2471,6 +2507,8 @@ s4f_status s4f_batch_add(s4f_batch* b, s4f_mesh* m, int32_t* index) {
         if (!m->materialSet) throw InvalidError("material not set");
         if (m->ctx != b->ctx) throw InvalidError("mesh belongs to another context");
         if (m->h.instCell0.size() != 1) throw InvalidError("cannot nest batches");
+        if (m->h.g[0] != 0.0 || m->h.g[1] != 0.0 || m->h.g[2] != 0.0)
+            throw InvalidError("batched solves are steady solves without body force (gravity is set on this mesh)");
         b->members.push_back(m);
         b->dirty = true;
         if (index) *index = (int32_t)b->members.size() - 1;
@@ -2484,17 +2522,30 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
         if (!b || !opts) throw InvalidError("null argument");
         if (b->members.empty()) throw InvalidError("empty batch");
         checkOpts(*opts);
+        if (opts->d2dt2 != S4F_D2DT2_STEADY) throw InvalidError("batched solves are steady solves");
         HIP_CHECK(hipSetDevice(b->ctx->device));
-        if (b->dirty || !b->packed) {
+        bool stale = b->dirty || !b->packed || opts->rhieChowScale != b->packedRc || opts->interfaceScale != b->packedIf;
+        for (size_t i = 0; !stale && i < b->members.size(); ++i) stale = b->members[i]->generation != b->packedGen[i];
+        if (stale) {
             std::vector<const HostMesh*> hs;
-            for (s4f_mesh* m : b->members) hs.push_back(&m->h);
+            for (s4f_mesh* m : b->members) {
+                if (m->h.g[0] != 0.0 || m->h.g[1] != 0.0 || m->h.g[2] != 0.0)
+                    throw InvalidError("batched solves are steady solves without body force (gravity is set on a member)");
+                hs.push_back(&m->h);
+            }
             auto P = std::make_unique<s4f_mesh>();
             P->ctx = b->ctx;
-            packMeshes(hs, P->h);
+            packMeshes(hs, P->h, opts->rhieChowScale, opts->interfaceScale);
             uploadGeometry(P.get());
             uploadMaterial(P.get());
+            P->rcScale = opts->rhieChowScale;
+            P->ifScale = opts->interfaceScale;
             P->materialSet = true;
             b->packed = std::move(P);
+            b->packedGen.clear();
+            for (s4f_mesh* m : b->members) b->packedGen.push_back(m->generation);
+            b->packedRc = opts->rhieChowScale;
+            b->packedIf = opts->interfaceScale;
             b->dirty = false;
         } else {
             // refresh boundary values (loads may change between solves)
@@ -2773,7 +2824,13 @@ void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
     for (int q = 0; q < 3; ++q) hcopy(S->eS.p + (size_t)q * Ec + e0, h.eS.data() + (size_t)q * E, E);
     hcopy(S->eW.p + e0, h.eW.data(), E);
     hcopy(S->eA.p + e0, h.eA.data(), E);
-    hcopy(S->eC.p + e0, h.eC.data(), E);
+    // Rhie-Chow coefficients of the server's stabilisation scales
+    if (S->opts.rhieChowScale == kRhieChowScaleDefault && S->opts.interfaceScale == kInterfaceScaleDefault) {
+        hcopy(S->eC.p + e0, h.eC.data(), E);
+    } else {
+        const std::vector<double> eC = stabilisationCoeffs(h, S->opts.rhieChowScale, S->opts.interfaceScale);
+        hcopy(S->eC.p + e0, eC.data(), E);
+    }
     std::vector<float> a32(E);
     for (size_t e = 0; e < E; ++e) a32[e] = (float)h.eA[e];
     hcopy(S->eA32.p + e0, a32.data(), E);

@@ -403,12 +403,8 @@ def read_linear_elastic(case_dir):
         _, law = laws[0]
     else:
         law = laws
-    if law.get("type") != "linearElastic":
-        raise ValueError(f"mechanical law {law.get('type')!r} is not supported (linearElastic)")
-    if "E" in law and "nu" in law:
-        return dimensioned(law["E"]), dimensioned(law["nu"])
-    mu, K = dimensioned(law["mu"]), dimensioned(law["K"])
-    return 9 * K * mu / (3 * K + mu), (3 * K - 2 * mu) / (2 * (3 * K + mu))
+    E, nu, _ = _law_constants(law)
+    return E, nu
 
 
 def read_plane_stress(case_dir):
@@ -434,9 +430,20 @@ def read_cell_zones(case_dir):
     return out
 
 
+def _check_law_options(law):
+    """linearElastic options the device path does not build raise: the
+    pressure-equation formulation (mechanicalLaw.C:1374, 1525-1527) and an
+    initial stress sigma0 (linearElastic.C:158-166) change the stress"""
+    if str(law.get("solvePressureEqn", "no")).lower() in ("yes", "on", "true", "1"):
+        raise ValueError("linearElastic solvePressureEqn is not supported on the device path")
+    if "sigma0" in law:
+        raise ValueError("linearElastic sigma0 (initial stress) is not supported on the device path")
+
+
 def _law_constants(law):
     if law.get("type") != "linearElastic":
         raise ValueError(f"mechanical law {law.get('type')!r} is not supported (linearElastic)")
+    _check_law_options(law)
     if "E" in law and "nu" in law:
         E, nu = dimensioned(law["E"]), dimensioned(law["nu"])
     else:
@@ -605,6 +612,8 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     Cf = pm.geometry()["Cf"][pm.nInternalFaces:]
     bcs = read_D_boundary(case_dir, bnd, face_centres=Cf)
     E, nu, rho = read_materials(case_dir, pm.nCells)
+    if os.path.exists(os.path.join(case_dir, "0", "sigma0")):  # the initial-stress field (mechanicalLaw.C:157-230)
+        raise ValueError("0/sigma0 (initial stress field) is not supported on the device path")
     patches = [dict(type=b["type"], value=b.get("value"), traction=b.get("traction"), pressure=b.get("pressure"))
                for b in bcs]
     ctx = ctx or solid.Context(0)

@@ -144,6 +144,21 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
             << abort(FatalError);
     }
     const linearElastic& law = refCast<const linearElastic>(laws[0]);
+    // the pressure-equation formulation (mechanicalLaw.C:1374) and an initial
+    // stress (linearElastic.C:158-166) change the reference's stress; the
+    // device path builds neither
+    if (law.solvePressureEqn())
+    {
+        FatalErrorIn(type() + "::createDeviceMesh()")
+            << "solvePressureEqn is not supported by the device path"
+            << abort(FatalError);
+    }
+    if (gMax(mag(law.sigma0())()) > SMALL)
+    {
+        FatalErrorIn(type() + "::createDeviceMesh()")
+            << "a nonzero sigma0 is not supported by the device path"
+            << abort(FatalError);
+    }
     const double E = law.E().value();
     const double nu = law.nu().value();
     // planeStress (mechanicalLaw.C:485-497): linearElastic's 2-D lambda and K

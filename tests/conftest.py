@@ -12,14 +12,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
 
 
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def oracle_mod():
     import oracle

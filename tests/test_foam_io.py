@@ -314,3 +314,26 @@ def test_run_case_multi_material(tmp_path):
     oc.solve(nCorr=50000, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, rcScale=0.2, precond=oracle.PRECOND_DIC)
     Do = oc.get("D").reshape(-1, 3)
     assert np.linalg.norm(m.D - Do) / np.linalg.norm(Do) < 1e-8
+
+
+def test_unsupported_law_options_are_refused(tmp_path):
+    # solvePressureEqn (mechanicalLaw.C:1374) and sigma0 (linearElastic.C:158-166)
+    # change the reference's stress: the case reader refuses them
+    d = str(tmp_path)
+    _box_case(d)
+    assert foam_io.read_materials(d, 60)[0] == 200e9
+    _write(d, "constant/mechanicalProperties", MECH.replace("nu    nu", "solvePressureEqn yes;\n        nu    nu"))
+    with pytest.raises(ValueError, match="solvePressureEqn"):
+        foam_io.read_materials(d, 60)
+    with pytest.raises(ValueError, match="solvePressureEqn"):
+        foam_io.read_linear_elastic(d)
+    _write(d, "constant/mechanicalProperties", MECH.replace("nu    nu", "solvePressureEqn no;\n        nu    nu"))
+    assert foam_io.read_materials(d, 60)[1] == 0.3
+    _write(d, "constant/mechanicalProperties",
+           MECH.replace("nu    nu", "sigma0 sigma0 [1 -1 -2 0 0 0 0] (1e6 0 0 0 0 0);\n        nu    nu"))
+    with pytest.raises(ValueError, match="sigma0"):
+        foam_io.read_materials(d, 60)
+    _write(d, "constant/mechanicalProperties", MECH)
+    _write(d, "0/sigma0", "FoamFile { version 2.0; format ascii; class volSymmTensorField; object sigma0; }\n")
+    with pytest.raises(ValueError, match="sigma0"):
+        foam_io.run_case(d)

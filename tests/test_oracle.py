@@ -613,3 +613,14 @@ def test_self_weight_reaction_equals_weight(oracle_mod):
     bidx = np.arange(m.nBoundaryFaces)[bottom]
     others = np.setdiff1d(np.arange(m.nBoundaryFaces), bidx)
     assert np.abs(f[others]).max() < 1e-6 * np.abs(weight).max() / len(bidx)
+
+
+def test_incompressible_nu_is_refused(oracle_mod):
+    # linearElastic.C:101-133: nu = 0.5 needs the pressure equation; the
+    # oracle (like the device path) refuses it rather than dividing by zero
+    om = oracle_mod.Mesh.box(2, 2, 2)
+    bv = np.zeros((om.nBoundaryFaces, 3))
+    oracle_mod.Case(om, 200e9, 0.49, [0, 1, 1, 1, 1, 1], bv)
+    for nu in (0.5, 0.6, -1.0):
+        with pytest.raises(ValueError, match="nu"):
+            oracle_mod.Case(om, 200e9, nu, [0, 1, 1, 1, 1, 1], bv)
