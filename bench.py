@@ -56,10 +56,21 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def cpu_baseline(n, jitter, seed, nOuter, iters, reltol, tol):
-    """Oracle (single-threaded C restatement, PCG + DIC relTol 0.1) on the same
-    mesh: seconds for `iters` outer iterations after the first, scaled by the
-    GPU FP64 outer-iteration count to solves/s."""
+def cpu_threads():
+    """host threads for the CPU baseline: OMP_NUM_THREADS when set (the
+    process's CPU share), else every core"""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_baseline(n, jitter, seed, nOuter, iters, reltol, tol, threads):
+    """The oracle's OpenMP outer iteration (the reference's work per
+    iteration: four least-squares gradients, the literal fvm/fvc face fluxes,
+    PCG with DIC per thread block, relTol 0.1 as the reference's fvSolution)
+    on `threads` host threads, same mesh: seconds per outer iteration over
+    `iters` iterations after two warm-up ones, scaled by the GPU FP64
+    outer-iteration count to solves/s (an estimate: the CPU loop is not run
+    to convergence)."""
     import oracle
     t0 = time.time()
     om = oracle.Mesh.box(n, n, n, (1.0, 1.0, 1.0), jitter, seed)
@@ -67,16 +78,20 @@ def cpu_baseline(n, jitter, seed, nOuter, iters, reltol, tol):
     bv[om.patch_faces(5)] = (0.0, 0.0, 1e7)
     oc = oracle.Case(om, 200e9, 0.3, [1, 1, 1, 1, 0, 1], bv)
     setup = time.time() - t0
-    kw = dict(pcgRelTol=reltol, pcgTol=1e-9, solTol=tol, altTol=tol)
-    oc.time_iterations(1, **kw)  # iteration 0 (zero residual) and warm-up
-    sec = oc.time_iterations(iters, **kw)
+    kw = dict(pcgRelTol=0.1, pcgTol=1e-9, solTol=tol, altTol=tol)
+    oc.time_iterations_omp(2, threads, **kw)  # iteration 0 (zero stress, zero residual) and one more
+    sec = oc.time_iterations_omp(iters, threads, **kw)
+    if sec < 0:
+        raise RuntimeError("the oracle's OpenMP path refused the case")
     per_iter = sec / iters
     cell_iter_s = om.nCells / per_iter
-    return dict(value=1.0 / (per_iter * nOuter), unit="solves/s", cores=1, kind="port",
-                cell_iter_per_s=cell_iter_s,
-                sample=f"oracle (C restatement, 1 thread, PCG+DIC relTol {reltol}) on the same {n}^3 mesh: "
-                       f"{iters} outer iteration(s) timed ({sec:.1f} s, setup {setup:.1f} s untimed); "
-                       f"solves/s = 1/(s_per_outer_iter x {nOuter} outer iterations of the GPU FP64 solve)")
+    return dict(value=1.0 / (per_iter * nOuter), unit="solves/s", cores=threads, kind="port",
+                cell_iter_per_s=round(cell_iter_s, 1), seconds_per_outer_iteration=round(per_iter, 3),
+                sample=f"oracle C restatement on {threads} OpenMP threads (host nproc {os.cpu_count()}): the "
+                       f"reference's per-iteration work (4 LS gradients, literal fvm/fvc face fluxes, PCG + DIC per "
+                       f"thread block, relTol 0.1) on the same {n}^3 mesh, {iters} outer iterations timed "
+                       f"({sec:.1f} s) after 2 warm-up ones, setup {setup:.1f} s untimed; solves/s = "
+                       f"1/(s_per_outer_iter x {nOuter} outer iterations of the GPU FP64 solve), an estimate")
 
 
 def main():
@@ -94,7 +109,15 @@ def main():
     ap.add_argument("--precond", choices=["diagonal", "gamg"], default="gamg",
                     help="PCG preconditioner of the device solves")
     ap.add_argument("--replay", type=int, default=40, help="back-to-back launches per kernel for the roofline")
-    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--cpu-iters", type=int, default=5, help="CPU baseline: timed outer iterations")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: OMP_NUM_THREADS or nproc)")
+    ap.add_argument("--ref-tol", type=float, default=1e-10,
+                    help="outer tolerance of the untimed FP64 reference solve the errors are measured against")
+    ap.add_argument("--no-batch", action="store_true", help="default run: skip the batched-sweep sub-object")
+    ap.add_argument("--no-serve", action="store_true", help="default run: skip the serving sub-object")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="default run: requests of the serving sub-object (a bounded sample of the stream)")
+    ap.add_argument("--serve-nmax", type=int, default=128, help="default run: serving boxes' largest n")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch every kernel individually (no HIP graphs)")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
@@ -124,6 +147,8 @@ def main():
     ap.add_argument("--slots", type=int, default=64, help="serve: in-flight solves")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serve: HBM arena")
     ap.add_argument("--sweep-tol", type=float, default=1e-9, help="batch/serve: outer tolerance")
+    ap.add_argument("--relax-method", choices=["fixed", "aitken"], default="fixed",
+                    help="batch/serve: relaxationMethod of the field relaxation (aitken: --relax is its first factor)")
     ap.add_argument("--relax", type=float, default=0.6,
                     help="batch/serve: fixed under-relaxation of D (the sweep reaches nu = 0.45, where the "
                          "unrelaxed segregated loop oscillates)")
@@ -268,11 +293,36 @@ def main():
                      err_vonMises_vs_fp64=rel_l2(s4f.sigma_eq(outs[-1]["sigma"]), s4f.sigma_eq(outs64[-1]["sigma"])),
                      fp64_nOuter=outs64[-1]["nOuter"], fp64_ms_per_step=round(1e3 * sum(lat64) / args.steps, 1),
                      fp64_wall_ms_per_step_with_pcie=round(1e3 * sum(wall64) / args.steps, 1))
+    if "fp64" in res:
+        # error reference: FP64 converged to a tight outer tolerance (untimed)
+        t = time.perf_counter()
+        ref = mesh.solve(want_sigma=True, precision=0, **dict(opts, solutionTolerance=args.ref_tol,
+                                                                alternativeTolerance=args.ref_tol))
+        Dr, qr = ref["D"], s4f.sigma_eq(ref["sigma"])
+        print(f"[bench] FP64 reference to {args.ref_tol:g}: nOuter={ref['nOuter']} converged={ref['converged']} "
+              f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+        errs = {}
+        for name, (_, _, o_, _) in res.items():
+            errs[name] = dict(err_D=rel_l2(o_[-1]["D"], Dr), err_vonMises=rel_l2(s4f.sigma_eq(o_[-1]["sigma"]), qr))
+        extra["error_reference"] = dict(precision="fp64", outer_tolerance=args.ref_tol, nOuter=ref["nOuter"],
+                                        converged=ref["converged"], vs_reference=errs)
+        del ref
+    sub = {}
+    if args.config == "cube" and args.n == 200 and not args.kernels_only:
+        mesh.close()
+        pm.close()
+        if not args.no_batch:
+            print("[bench] batched sweep: 256 boxes ...", file=sys.stderr, flush=True)
+            sub["batch"] = batch_run(args, ctx, 256, 64)
+        if not args.no_serve:
+            print(f"[bench] serving: {args.serve_requests} requests ...", file=sys.stderr, flush=True)
+            sub["serve"] = serve_run(args, ctx, args.serve_requests, args.serve_nmax)
     cpu = None
     if not args.no_cpu_baseline:
         n64 = extra.get("fp64_nOuter", nOuter)
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.n, args.jitter, args.seed, n64, args.cpu_iters, args.reltol, args.tol)
+        cpu = cpu_baseline(args.n, args.jitter, args.seed, n64, args.cpu_iters, args.reltol, args.tol,
+                           args.cpu_threads or cpu_threads())
     line = {
         "metric": "converged solves/s & cell-iter/s at 1e-6 err vs FP64; p50/p99 solve latency",
         "value": round(value, 6),
@@ -292,8 +342,9 @@ def main():
         "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
         "nOuter": nOuter,
         "pcg_iterations": pcg_it,
-        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
-        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "p50_latency_s": sub["serve"]["p50_latency_s"] if "serve" in sub else round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": sub["serve"]["p99_latency_s"] if "serve" in sub else round(float(np.percentile(lat, 99)), 4),
+        "latency_source": "serving stream (see serve)" if "serve" in sub else "back-to-back solves of this mesh",
         "timing": "device stream, inputs resident in HBM (solve report: events after the input upload and "
                   "before the result copies)",
         "wall_ms_per_step_with_pcie": round(1e3 * sum(wall) / args.steps, 2),
@@ -306,6 +357,7 @@ def main():
     if mgi:
         line["multigrid_levels"] = dict(rows=mgi["rows"], entries=mgi["entries"], q_entries=mgi["qEntries"])
     line.update(extra)
+    line.update(sub)
     print(json.dumps(line))
 
 
@@ -452,17 +504,26 @@ def _sequential(ctx, cases_, specs, opts, label):
     return secs, Ds
 
 
+def sweep_opts(args):
+    """outer/inner settings of the batched sweep and the serving stream"""
+    return dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
+                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax,
+                relaxationMethod=1 if args.relax_method == "aitken" else 0, eagerLaunch=int(args.eager))
+
+
 def run_batch(args, ctx):
+    print(json.dumps(batch_run(args, ctx, args.count, args.nmax)))
+
+
+def batch_run(args, ctx, count, nmax):
     """Batched sweep: `count` uneven boxes packed into one structure and solved
     together (mixed precision, PCG with the block-diagonal multigrid V-cycle,
     each instance stopping on its own convergence), against plain FP64 solves
     of a sample of the same boxes one at a time."""
     from solids4foam_amd import cases
     from solids4foam_amd.batch import Batch
-    specs = cases.box_sweep_specs(args.count, args.seed, 8, args.nmax)
-    opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax,
-                eagerLaunch=int(args.eager))
+    specs = cases.box_sweep_specs(count, args.seed, 8, nmax)
+    opts = sweep_opts(args)
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
     b = Batch(ctx)
@@ -491,54 +552,49 @@ def run_batch(args, ctx):
         "metric": "batched converged solves/s (sweep of uneven boxes) vs sequential plain FP64 solves",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{len(specs)} boxes n in 8..{args.nmax} per direction, jitter 0.1, E log-normal "
+        "config": {"workload": f"{len(specs)} boxes n in 8..{nmax} per direction, jitter 0.1, E log-normal "
                                f"(median 100 GPa, sigma 0.5), nu U(0.2,0.45), x-max traction U(1e5,1e7) Pa random "
                                f"direction, seed {args.seed}", "cells": int(cells), "outer_tolerance": args.sweep_tol,
                    "pcg": f"{args.precond}, relTol {args.reltol} (batched); gamg (sequential)",
-                   "relaxD": args.relax},
+                   "relaxD": args.relax, "relaxation": args.relax_method},
         "batch_seconds": round(T, 3), "converged": int(nconv), "outer_median": float(np.median(outer)),
-        "outer_max": int(outer.max()), "cell_iter_per_s": round(float(sum(m.nCells * r["nOuter"] for (_, m, _), r in
+        "outer_max": int(outer.max()),
+        "outer_percentiles": {q: float(np.percentile(outer, q)) for q in (10, 50, 90, 99, 100)},
+        "slowest": [dict(n=specs[i]["n"], nu=round(specs[i]["nu"], 4), nOuter=int(outer[i]))
+                    for i in np.argsort(outer)[-3:][::-1]], "cell_iter_per_s": round(float(sum(m.nCells * r["nOuter"] for (_, m, _), r in
                                                                          zip(meshes, res)) / T), 1),
         "sequential_fp64_solves_per_s": round(seq_rate, 4), "sequential_sample": len(secs),
         "sequential_sample_seconds": [round(x, 3) for x in secs],
         "speedup_vs_sequential_fp64": round(value / seq_rate, 2),
         "max_err_D_vs_fp64": max(errs) if errs else None,
     }
-    print(json.dumps(line))
+    return line
 
 
 def run_serve(args, ctx):
+    print(json.dumps(serve_run(args, ctx, args.requests, args.nmax)))
+
+
+def serve_run(args, ctx, requests, nmax):
     """Serving: a Poisson stream of solve requests (boxes from the sweep
     generator, n up to nmax, a fraction repeating an earlier mesh with a new
     load and warm start) through the continuous-batching server."""
     from solids4foam_amd import cases
     from solids4foam_amd.serve import Server
-    rng = np.random.default_rng(args.seed + 1000)
-    nUnique = max(1, int(round(args.requests * (1.0 - args.repeat))))
-    specs = cases.box_sweep_specs(nUnique, args.seed, 8, args.nmax)
+    specs, stream, arrivals = cases.serve_requests(requests, args.seed, 8, nmax, args.repeat, args.rate)
+    nUnique = len(specs)
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
     print(f"[bench] serve: {nUnique} meshes, {sum(m.nCells for _, m, _ in meshes)} cells, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    reqs, seen = [], []
-    u = 0
-    for i in range(args.requests):
-        if seen and (u >= nUnique or rng.random() < args.repeat):
-            j = int(rng.choice(seen))
-            d = rng.normal(size=3)
-            t = rng.uniform(1e5, 1e7) * d / np.linalg.norm(d)
-            pm, m, _ = meshes[j]
+    reqs = []
+    for j, t, warm in stream:
+        pm, m, bv = meshes[j]
+        if t is not None:
             bv = np.zeros((pm.nBoundaryFaces, 3))
             bv[pm.patch_faces(1)] = t
-            reqs.append((j, bv, True))
-        else:
-            reqs.append((u, meshes[u][2], False))
-            seen.append(u)
-            u += 1
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
-    opts = dict(solutionTolerance=args.sweep_tol, alternativeTolerance=args.sweep_tol, pcgTolerance=1e-12,
-                pcgRelTol=args.reltol, nCorrectors=200000, relaxD=args.relax,
-                eagerLaunch=int(args.eager))
+        reqs.append((j, bv, warm))
+    opts = sweep_opts(args)
     from solids4foam_amd import _abi
     srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1,
                  preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL, **opts)
@@ -578,11 +634,11 @@ def run_serve(args, ctx):
         "metric": "served converged solves/s and p50/p99 latency (Poisson arrivals) vs sequential plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, boxes n in 8..{args.nmax}, "
+        "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, boxes n in 8..{nmax}, "
                                f"{args.repeat:.0%} repeat an earlier mesh with a new load (warm start), seed "
                                f"{args.seed}", "slots": args.slots, "budget_GiB": args.budget_gib,
                    "outer_tolerance": args.sweep_tol, "pcg": f"{args.precond}, relTol {args.reltol}",
-                   "relaxD": args.relax},
+                   "relaxD": args.relax, "relaxation": args.relax_method},
         "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
         "converged": int(conv), "warm_started": int(warm),
@@ -590,7 +646,7 @@ def run_serve(args, ctx):
         "speedup_vs_sequential_fp64": round(value / seq_rate, 2),
         "max_err_D_vs_fp64": max(errs) if errs else None,
     }
-    print(json.dumps(line))
+    return line
 
 
 def run_tets(args, ctx):
@@ -599,7 +655,7 @@ def run_tets(args, ctx):
     import solids4foam_amd as s4f
     from solids4foam_amd import _abi, cases
     t0 = time.time()
-    pm, mesh, frac, nInc = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
+    pm, mesh, frac, nInc, _ = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
     C = mesh.nCells
     print(f"[bench] tets {args.n}^3 x 6: C={C}, {nInc} inclusions, {frac:.3f} of cells soft, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)

@@ -67,6 +67,14 @@ def lib():
         L.or_time_iterations.restype = ctypes.c_double
         L.or_time_iterations.argtypes = [vp, vp, ctypes.c_int]
         L.or_advance_time.argtypes = [vp]
+        L.or_omp_create.restype = vp
+        L.or_omp_create.argtypes = [vp, ctypes.c_int]
+        L.or_omp_free.argtypes = [vp]
+        L.or_bump_icorr.argtypes = [vp]
+        L.or_iterate_omp.restype = ctypes.c_int
+        L.or_iterate_omp.argtypes = [vp, vp, vp, dp, dp, ip]
+        L.or_time_iterations_omp.restype = ctypes.c_double
+        L.or_time_iterations_omp.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
         L.or_reference_source.argtypes = [vp, ctypes.c_int, dp]
         L.or_amul.argtypes = [vp, dp, dp]
         L.or_sigma_eq.argtypes = [vp, dp]
@@ -247,6 +255,39 @@ class Case:
     def time_iterations(self, n, **kw):
         o = self._opts(default_opts(**kw))
         return lib().or_time_iterations(self.h, ctypes.byref(o), int(n))
+
+    def solve_omp(self, threads, **kw):
+        """The outer loop on `threads` OpenMP threads (or_iterate_omp: block
+        DIC per thread, the reference's four gradient evaluations per
+        iteration) until solidModel::converged or nCorr"""
+        o = self._opts(default_opts(**kw))
+        p = lib().or_omp_create(self.h, int(threads))
+        if not p:
+            raise ValueError("the OpenMP path covers fixedDisplacement, solidTraction and empty patches")
+        ir, rv, inn = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        n, inner, conv = 0, 0, 0
+        try:
+            while True:
+                conv = lib().or_iterate_omp(self.h, p, ctypes.byref(o), ctypes.byref(ir), ctypes.byref(rv),
+                                            ctypes.byref(inn))
+                if conv < 0:
+                    raise ValueError("options not covered by the OpenMP path")
+                n += 1
+                inner += inn.value
+                if conv or n >= o.nCorr or not np.isfinite(ir.value):
+                    break
+                self._bump()
+        finally:
+            lib().or_omp_free(p)
+        return dict(nOuter=n, converged=bool(conv), innerTotal=inner, initRes=ir.value, rvf=rv.value)
+
+    def _bump(self):
+        lib().or_bump_icorr(self.h)
+
+    def time_iterations_omp(self, n, threads, **kw):
+        """wall seconds of n outer iterations on `threads` OpenMP threads"""
+        o = self._opts(default_opts(**kw))
+        return lib().or_time_iterations_omp(self.h, ctypes.byref(o), int(n), int(threads))
 
     def point_interpolate(self, D, G):
         """pointD from cell D (C, 3) and gradD (C, 9)"""

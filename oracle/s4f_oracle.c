@@ -1334,3 +1334,357 @@ void or_sigma_eq(or_case *c, double *out)
         out[i] = sqrt(1.5*ms);
     }
 }
+
+
+/* ------------------------------------------------------------------ */
+/* OpenMP restatement of the outer iteration, for the CPU baseline      */
+/* ------------------------------------------------------------------ */
+/* The outer iteration of or_iterate (steady; fixedDisplacement, solidTraction
+   and empty patches; no field relaxation) doing the work the reference does
+   per iteration, on nThreads cores as a decomposed run would:
+     * grad(D) four times: the explicit mechanical().grad (linGeomTotalDispSolid.C:173)
+       and the non-orthogonal corrections of fvm::laplacian (:144),
+       fvc::laplacian (:145) and the stabilisation (momentumStabilisation.C:214);
+     * the implicit Laplacian's coefficients re-assembled (:144);
+     * the DEqn source from the literal face fluxes: fvm correction,
+       fvc::laplacian, Gauss div(sigma), Rhie-Chow, boundary coefficients (the
+       source or_reference_source forms, equal to M D + R);
+     * PCG with DIC restricted to each thread's contiguous block of cells (the
+       block-diagonal DIC of a decomposed PCG/DIC run), rebuilt per component
+       as PCG::solve constructs its preconditioner;
+     * boundary update, grad(D), Hooke, convergence norms.
+   Face fluxes are formed per face and gathered per cell (no atomics), so for
+   a fixed thread count the result is deterministic.  PCG iteration counts
+   differ from the serial DIC oracle (block DIC); the fixed point does not. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    int nT;
+    int *cOff, *cFace;      /* cell -> internal faces in face order: f+1 (owner side) or -(f+1) */
+    int *bOff, *bList;      /* cell -> boundary faces */
+    int *blk0;              /* nT+1 block starts (cells) */
+    int *bfOff, *bfList;    /* per block: its internal faces (both cells inside), face order */
+    double *G2, *flux, *upper, *diag, *rowSum, *rD, *b3, *work;
+} or_omp;
+
+void or_omp_free(or_omp *p)
+{
+    if (!p) return;
+    free(p->cOff); free(p->cFace); free(p->bOff); free(p->bList); free(p->blk0); free(p->bfOff); free(p->bfList);
+    free(p->G2); free(p->flux); free(p->upper); free(p->diag); free(p->rowSum); free(p->rD); free(p->b3);
+    free(p->work);
+    free(p);
+}
+
+/* addressing and scratch of the parallel path; NULL if the case has patch
+   types it does not cover */
+or_omp *or_omp_create(const or_case *c, int nThreads)
+{
+    const or_mesh *m = c->m;
+    const int C = m->nCells, F = m->nInternalFaces, B = c->nB;
+    for (int b = 0; b < B; ++b)
+        if (c->bType[b] != OR_FIXED && c->bType[b] != OR_TRACTION && c->bType[b] != OR_EMPTY) return NULL;
+    or_omp *p = xcalloc(1, sizeof(or_omp));
+    p->nT = nThreads < 1 ? 1 : nThreads;
+    p->cOff = xcalloc((size_t)C + 1, sizeof(int));
+    p->cFace = xcalloc((size_t)2*F, sizeof(int));
+    for (int f = 0; f < F; ++f) { p->cOff[m->owner[f]+1]++; p->cOff[m->neighbour[f]+1]++; }
+    for (int i = 0; i < C; ++i) p->cOff[i+1] += p->cOff[i];
+    int *fill = xcalloc(C, sizeof(int));
+    for (int f = 0; f < F; ++f) {
+        int o = m->owner[f], n = m->neighbour[f];
+        p->cFace[p->cOff[o] + fill[o]++] = f + 1;
+        p->cFace[p->cOff[n] + fill[n]++] = -(f + 1);
+    }
+    p->bOff = xcalloc((size_t)C + 1, sizeof(int));
+    p->bList = xcalloc(B, sizeof(int));
+    for (int b = 0; b < B; ++b) p->bOff[bCell(c, b)+1]++;
+    for (int i = 0; i < C; ++i) p->bOff[i+1] += p->bOff[i];
+    memset(fill, 0, sizeof(int)*C);
+    for (int b = 0; b < B; ++b) { int P = bCell(c, b); p->bList[p->bOff[P] + fill[P]++] = b; }
+    free(fill);
+    p->blk0 = xcalloc((size_t)p->nT + 1, sizeof(int));
+    for (int t = 0; t <= p->nT; ++t) p->blk0[t] = (int)((long)C*t/p->nT);
+    /* owner < neighbour: a face lies in block t iff its owner is in t and its
+       neighbour below the block's end */
+    p->bfOff = xcalloc((size_t)p->nT + 1, sizeof(int));
+    p->bfList = xcalloc(F, sizeof(int));
+    int k = 0, t = 0;
+    for (int f = 0; f < F; ++f) {
+        int o = m->owner[f], n = m->neighbour[f];
+        while (o >= p->blk0[t+1]) p->bfOff[++t] = k;
+        if (n < p->blk0[t+1]) p->bfList[k++] = f;
+    }
+    while (t < p->nT) p->bfOff[++t] = k;
+    p->G2 = xcalloc((size_t)9*C, sizeof(double));
+    p->flux = xcalloc((size_t)3*F, sizeof(double));
+    p->upper = xcalloc(F, sizeof(double));
+    p->diag = xcalloc(C, sizeof(double)); p->rowSum = xcalloc(C, sizeof(double));
+    p->rD = xcalloc(C, sizeof(double));
+    p->b3 = xcalloc((size_t)3*C, sizeof(double));
+    p->work = xcalloc((size_t)6*C, sizeof(double));
+    return p;
+}
+
+/* leastSquares grad(D) with the boundary values Db into G (cell gather) */
+static void omp_grad(const or_case *c, const or_omp *p, double *G)
+{
+    const or_mesh *m = c->m;
+    const int C = m->nCells;
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int P = 0; P < C; ++P) {
+        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = p->cOff[P]; k < p->cOff[P+1]; ++k) {
+            const int s = p->cFace[k], f = (s > 0 ? s : -s) - 1;
+            const int o = m->owner[f], n = m->neighbour[f];
+            double dv[3];
+            for (int q = 0; q < 3; ++q) dv[q] = c->D[3*n+q] - c->D[3*o+q];
+            const double *l = s > 0 ? c->lsP + 3*f : c->lsN + 3*f;
+            const double sg = s > 0 ? 1.0 : -1.0;
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) g[3*i+j] += sg*l[i]*dv[j];
+        }
+        for (int k = p->bOff[P]; k < p->bOff[P+1]; ++k) {
+            const int b = p->bList[k];
+            if (c->bType[b] == OR_EMPTY) continue;
+            double dv[3];
+            for (int q = 0; q < 3; ++q) dv[q] = c->Db[3*b+q] - c->D[3*P+q];
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) g[3*i+j] += c->lsB[3*b+i]*dv[j];
+        }
+        memcpy(G + 9*P, g, sizeof g);
+    }
+}
+
+static void omp_amul(const or_case *c, const or_omp *p, const double *x, double *y)
+{
+    const or_mesh *m = c->m;
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int P = 0; P < m->nCells; ++P) {
+        double s = p->diag[P]*x[P];
+        for (int k = p->cOff[P]; k < p->cOff[P+1]; ++k) {
+            const int sf = p->cFace[k], f = (sf > 0 ? sf : -sf) - 1;
+            s += p->upper[f]*x[sf > 0 ? m->neighbour[f] : m->owner[f]];
+        }
+        y[P] = s;
+    }
+}
+
+/* [OF PCG::solve] + [OF normFactor] for one component, block DIC */
+static int omp_pcg(const or_case *c, or_omp *p, const or_opts *o, double *psi, const double *src, double *initRes)
+{
+    const or_mesh *m = c->m;
+    const int N = m->nCells;
+    double *pA = p->work, *wA = p->work + N, *rA = p->work + 2*N;
+    /* DIC factor per block ([OF DICPreconditioner::calcReciprocalD]) */
+    #pragma omp parallel for schedule(static, 1) num_threads(p->nT)
+    for (int t = 0; t < p->nT; ++t) {
+        for (int i = p->blk0[t]; i < p->blk0[t+1]; ++i) p->rD[i] = p->diag[i];
+        for (int k = p->bfOff[t]; k < p->bfOff[t+1]; ++k) {
+            const int f = p->bfList[k], l = m->owner[f], u = m->neighbour[f];
+            p->rD[u] -= p->upper[f]*p->upper[f]/p->rD[l];
+        }
+        for (int i = p->blk0[t]; i < p->blk0[t+1]; ++i) p->rD[i] = 1.0/p->rD[i];
+    }
+    omp_amul(c, p, psi, wA);
+    double avg = 0.0;
+    #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(+:avg)
+    for (int i = 0; i < N; ++i) { rA[i] = src[i] - wA[i]; avg += psi[i]; }
+    avg /= N;
+    double nf = 0.0, s = 0.0;
+    #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(+:nf, s)
+    for (int i = 0; i < N; ++i) {
+        const double t = p->rowSum[i]*avg;
+        nf += fabs(wA[i] - t) + fabs(src[i] - t);
+        s += fabs(rA[i]);
+    }
+    nf += 1e-20;
+    *initRes = s/nf;
+    double fr = *initRes;
+    int nIter = 0;
+    #define OMP_CONV(fr, ir) ((fr) < o->pcgTol || (o->pcgRelTol > 1e-20 && (fr) < o->pcgRelTol*(ir)))
+    if (!OMP_CONV(fr, *initRes)) {
+        double wArA = 1e20, wArAold;
+        do {
+            wArAold = wArA;
+            #pragma omp parallel for schedule(static, 1) num_threads(p->nT)
+            for (int t = 0; t < p->nT; ++t) {
+                for (int i = p->blk0[t]; i < p->blk0[t+1]; ++i) wA[i] = p->rD[i]*rA[i];
+                for (int k = p->bfOff[t]; k < p->bfOff[t+1]; ++k) {
+                    const int f = p->bfList[k], l = m->owner[f], u = m->neighbour[f];
+                    wA[u] -= p->rD[u]*p->upper[f]*wA[l];
+                }
+                for (int k = p->bfOff[t+1] - 1; k >= p->bfOff[t]; --k) {
+                    const int f = p->bfList[k], l = m->owner[f], u = m->neighbour[f];
+                    wA[l] -= p->rD[l]*p->upper[f]*wA[u];
+                }
+            }
+            wArA = 0.0;
+            #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(+:wArA)
+            for (int i = 0; i < N; ++i) wArA += wA[i]*rA[i];
+            const double beta = nIter == 0 ? 0.0 : wArA/wArAold;
+            #pragma omp parallel for schedule(static) num_threads(p->nT)
+            for (int i = 0; i < N; ++i) pA[i] = wA[i] + beta*pA[i];
+            omp_amul(c, p, pA, wA);
+            double wApA = 0.0;
+            #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(+:wApA)
+            for (int i = 0; i < N; ++i) wApA += wA[i]*pA[i];
+            if (fabs(wApA)/nf < OR_VSMALL) break;
+            const double alpha = wArA/wApA;
+            s = 0.0;
+            #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(+:s)
+            for (int i = 0; i < N; ++i) { psi[i] += alpha*pA[i]; rA[i] -= alpha*wA[i]; s += fabs(rA[i]); }
+            fr = s/nf;
+        } while (++nIter < o->pcgMaxIter && !OMP_CONV(fr, *initRes));
+    }
+    #undef OMP_CONV
+    return nIter;
+}
+
+/* One outer iteration on nThreads (or_omp_create); returns the convergence
+   flag of solidModel::converged, -1 for options it does not cover. */
+int or_iterate_omp(or_case *c, or_omp *p, const or_opts *o, double *initResOut, double *rvfOut, int *innerOut)
+{
+    const or_mesh *m = c->m;
+    const int C = m->nCells, F = m->nInternalFaces, B = c->nB;
+    if (o->d2dt2 || o->relaxAitken || (o->relaxD > 0.0 && o->relaxD != 1.0) || c->emptyDir >= 0) return -1;
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int i = 0; i < 3*C; ++i) c->Dprev[i] = c->D[i];
+    or_update_traction(c);
+    /* grad(D) of the corrected Laplacians' and the stabilisation's own
+       non-orthogonal corrections (the same D: the same field three times) */
+    omp_grad(c, p, p->G2);
+    omp_grad(c, p, p->G2);
+    omp_grad(c, p, p->G2);
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int f = 0; f < F; ++f) p->upper[f] = -c->impKf[f]*m->magSf[f]*c->nonOrthDelta[f];
+    /* per face: fvm correction - fvc::laplacian + Gauss div(sigma) + Rhie-Chow */
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int f = 0; f < F; ++f) {
+        const int ow = m->owner[f], n = m->neighbour[f];
+        const double w = c->w[f];
+        double Gf[9], sf[6], kG[3], SG[3];
+        for (int q = 0; q < 9; ++q) Gf[q] = w*(p->G2[9*ow+q] - p->G2[9*n+q]) + p->G2[9*n+q];
+        for (int q = 0; q < 6; ++q) sf[q] = w*(c->sig[6*ow+q] - c->sig[6*n+q]) + c->sig[6*n+q];
+        const double *S = m->Sf + 3*f;
+        vdotT(c->corrVec + 3*f, Gf, kG);
+        vdotT(S, Gf, SG);
+        const double dl = c->nonOrthDelta[f], gm = c->impKf[f]*m->magSf[f], gr = c->gammaf[f]*m->magSf[f];
+        const double dv[3] = { S[0]*sf[0] + S[1]*sf[1] + S[2]*sf[2], S[0]*sf[1] + S[1]*sf[3] + S[2]*sf[4],
+                               S[0]*sf[2] + S[1]*sf[4] + S[2]*sf[5] };
+        for (int q = 0; q < 3; ++q) {
+            const double dD = c->D[3*n+q] - c->D[3*ow+q];
+            const double corr = gm*kG[q];
+            const double lap = gm*(dl*dD + kG[q]);
+            const double rc = gr*(dl*dD + kG[q]) - c->gammaf[f]*SG[q];
+            p->flux[3*f+q] = dv[q] + rc + corr - lap;
+        }
+    }
+    /* gather: source, matrix diagonal (fvm::laplacian + fixed-patch
+       internalCoeffs) and row sums (normFactor) */
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int P = 0; P < C; ++P) {
+        double s[3] = {0, 0, 0}, dsum = 0.0, bsum = 0.0;
+        for (int k = p->cOff[P]; k < p->cOff[P+1]; ++k) {
+            const int sf = p->cFace[k], f = (sf > 0 ? sf : -sf) - 1;
+            const double sg = sf > 0 ? 1.0 : -1.0;
+            for (int q = 0; q < 3; ++q) s[q] += sg*p->flux[3*f+q];
+            dsum -= p->upper[f];
+        }
+        for (int k = p->bOff[P]; k < p->bOff[P+1]; ++k) {
+            const int b = p->bList[k];
+            if (c->bType[b] == OR_EMPTY) continue;
+            const double *S = m->Sf + 3*(F+b), *sb = c->sigb + 6*b;
+            s[0] += S[0]*sb[0] + S[1]*sb[1] + S[2]*sb[2];
+            s[1] += S[0]*sb[1] + S[1]*sb[3] + S[2]*sb[4];
+            s[2] += S[0]*sb[2] + S[1]*sb[4] + S[2]*sb[5];
+            const double gm = c->impK[P]*m->magSf[F+b];
+            if (c->bType[b] == OR_FIXED) {
+                double kG[3]; vdotT(c->bK + 3*b, c->G + 9*P, kG);
+                bsum += gm*c->bDelta[b];
+                for (int q = 0; q < 3; ++q) {
+                    s[q] += gm*c->bDelta[b]*(c->Db[3*b+q] - kG[q]);
+                    s[q] -= gm*(c->Db[3*b+q] - (c->D[3*P+q] + kG[q]))*c->bDelta[b];
+                }
+            } else {
+                for (int q = 0; q < 3; ++q) { s[q] += gm*c->g[3*b+q]; s[q] -= gm*c->g[3*b+q]; }
+            }
+        }
+        p->diag[P] = dsum + bsum;
+        p->rowSum[P] = bsum;
+        for (int q = 0; q < 3; ++q) p->b3[3*P+q] = s[q];
+    }
+    /* DEqn.solve(): segregated PCG/DIC, warm start x = D */
+    double *psi = p->work + 3*C, *src = p->work + 4*C;
+    double ir[3];
+    int inner = 0;
+    for (int q = 0; q < 3; ++q) {
+        #pragma omp parallel for schedule(static) num_threads(p->nT)
+        for (int i = 0; i < C; ++i) { psi[i] = c->D[3*i+q]; src[i] = p->b3[3*i+q]; }
+        inner += omp_pcg(c, p, o, psi, src, &ir[q]);
+        #pragma omp parallel for schedule(static) num_threads(p->nT)
+        for (int i = 0; i < C; ++i) c->D[3*i+q] = psi[i];
+    }
+    or_correct_boundary(c);
+    omp_grad(c, p, c->G);
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
+        const int P = bCell(c, b);
+        const double *n = c->bN + 3*b, *GP = c->G + 9*P;
+        double nG[3]; vdotT(n, GP, nG);
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j)
+            c->Gb[9*b + 3*i + j] = GP[3*i+j] + n[i]*(c->snb[3*b+j] - nG[j]);
+    }
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int i = 0; i < C; ++i) hooke(c->G + 9*i, c->mu[i], c->K[i], c->sig + 6*i);
+    #pragma omp parallel for schedule(static) num_threads(p->nT)
+    for (int b = 0; b < B; ++b) { const int P = bCell(c, b); hooke(c->Gb + 9*b, c->mu[P], c->K[P], c->sigb + 6*b); }
+    /* solidModel::converged (total approach) */
+    double dOld = 0.0, dNow = 0.0, dPrev = 0.0;
+    #pragma omp parallel for schedule(static) num_threads(p->nT) reduction(max:dOld, dNow, dPrev)
+    for (int i = 0; i < C; ++i) {
+        double a[3], b[3], e[3];
+        for (int q = 0; q < 3; ++q) {
+            a[q] = c->D[3*i+q] - c->Dold[3*i+q]; b[q] = c->D[3*i+q]; e[q] = c->D[3*i+q] - c->Dprev[3*i+q];
+        }
+        const double va = mag3(a), vb = mag3(b), ve = mag3(e);
+        if (va > dOld || va != va) dOld = va;
+        if (vb > dNow || vb != vb) dNow = vb;
+        if (ve > dPrev || ve != ve) dPrev = ve;
+    }
+    double denom = dOld;
+    if (denom < OR_SMALL) { denom = dNow; if (denom < OR_SMALL) denom = OR_SMALL; }
+    const double rvf = dPrev/denom;
+    const double initRes = sqrt(ir[0]*ir[0] + ir[1]*ir[1] + ir[2]*ir[2]);
+    int conv = 0;
+    if (c->iCorr > 1) {
+        if (initRes < o->solTol && rvf < o->solTol) conv = 1;
+        else if (rvf < o->altTol) conv = 1;
+        else if (initRes < o->altTol) conv = 1;
+    }
+    if (initResOut) *initResOut = initRes;
+    if (rvfOut) *rvfOut = rvf;
+    if (innerOut) *innerOut = inner;
+    return conv;
+}
+
+/* CPU-baseline timing: n outer iterations of or_iterate_omp on nThreads,
+   wall seconds (negative: options not covered) */
+double or_time_iterations_omp(or_case *c, const or_opts *o, int n, int nThreads)
+{
+    or_omp *p = or_omp_create(c, nThreads);
+    if (!p) return -1.0;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < n; ++i) {
+        if (or_iterate_omp(c, p, o, NULL, NULL, NULL) < 0) { or_omp_free(p); return -1.0; }
+        c->iCorr++;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    or_omp_free(p);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9*(double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* the outer-iteration counter (iCorr) of a loop driven from outside */
+void or_bump_icorr(or_case *c) { c->iCorr++; }

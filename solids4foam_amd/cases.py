@@ -115,17 +115,18 @@ def inside_spheres_box(C, n, centres, radii, per_hex=1):
 
 
 def tet_inclusions(ctx, n=200, jitter=0.15, seed=2, inc_seed=3, E_matrix=200e9, E_inc=20e9, nu=0.3,
-                   traction=(0.0, 5e6, 0.0)):
+                   traction=(0.0, 5e6, 0.0), rmin=0.02, rmax=0.06):
     """The jittered n^3 cube split into 6 tetrahedra per hex (48M cells at
     n = 200), per-cell E: E_inc for cells whose centroid lies in a seeded
     spherical inclusion (10 % volume fraction), E_matrix elsewhere; x-min
     fixed, x-max shear traction, other faces traction-free.  Tetrahedra use
     the full-delta least-squares gradient (extendedLeastSquares 0): with the
-    normal-delta leastSquares the segregated loop diverges on tets."""
+    normal-delta leastSquares the segregated loop diverges on tets.  Returns
+    (PolyMesh, SolverMesh, inclusion cell fraction, inclusion count, E per cell)."""
     pm = PolyMesh.box(n, n, n, (1.0, 1.0, 1.0), jitter, seed, True)
     types = [FIX, TRAC, TRAC, TRAC, TRAC, TRAC]
     m = SolverMesh(ctx, pm, types, ls_full_delta=True)
-    cen, rad = inclusions(inc_seed)
+    cen, rad = inclusions(inc_seed, rmin=rmin, rmax=rmax)
     inc = inside_spheres_box(pm.cell_centres(), n, cen, rad, per_hex=6)
     E = np.where(inc, E_inc, E_matrix)
     m.set_material(E, nu)
@@ -133,4 +134,31 @@ def tet_inclusions(ctx, n=200, jitter=0.15, seed=2, inc_seed=3, E_matrix=200e9, 
     m.set_bc(1, traction)
     for p in range(2, 6):
         m.set_bc(p, (0.0, 0.0, 0.0))
-    return pm, m, float(inc.mean()), len(rad)
+    return pm, m, float(inc.mean()), len(rad), E
+
+
+def serve_requests(count, seed, nmin=8, nmax=128, repeat=0.2, rate=100.0):
+    """The serving request stream: `count` requests whose boxes come from
+    box_sweep_specs (n up to nmax); after the first, each request repeats an
+    earlier mesh with probability `repeat` (a new x-max traction of magnitude
+    uniform 1e5-1e7 Pa in a random direction, warm started from the mesh's
+    resident field), else brings the next new mesh with its own load; Poisson
+    arrival times at `rate` per second.  Returns (specs of the unique meshes,
+    [(mesh index, traction or None for the mesh's own, warm start)], arrival
+    times in seconds)."""
+    rng = np.random.default_rng(seed + 1000)
+    n_unique = max(1, int(round(count * (1.0 - repeat))))
+    specs = box_sweep_specs(n_unique, seed, nmin, nmax)
+    reqs, seen = [], []
+    u = 0
+    for _ in range(count):
+        if seen and (u >= n_unique or rng.random() < repeat):
+            j = int(rng.choice(seen))
+            d = rng.normal(size=3)
+            reqs.append((j, tuple(float(v) for v in rng.uniform(1e5, 1e7) * d / np.linalg.norm(d)), True))
+        else:
+            reqs.append((u, None, False))
+            seen.append(u)
+            u += 1
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    return specs, reqs, arrivals
