@@ -293,6 +293,18 @@ s4f_status s4f_spmv3(s4f_mesh *mesh, const double *x, double *y, int32_t precisi
    (OpenFOAM lduMatrix::solver::normFactor); iters (3) may be NULL. */
 s4f_status s4f_pcg3(s4f_mesh *mesh, const s4f_solve_opts *opts, const double *b,
                     const double normFactor[3], double *x, int32_t iters[3]);
+/* s4f_pcg3 for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+   counterpart in the reference.  b and x are the caller's DEVICE arrays in
+   component-major layout (component j of cell c at j*nCells + c, FP64); the
+   control block, reduction partials and PCG vectors live in a fixed device
+   array of the library (about 26 MiB per GPU, held until the process ends),
+   so the call allocates nothing.  Limit: single meshes of at most 262144
+   cells (larger meshes return S4F_ERR_INVALID); diagonal preconditioner.
+   Every call on one GPU shares that array: calls that overlap on one GPU
+   (several contexts or threads) race.  Same kernels as s4f_pcg3, same result. */
+s4f_status s4f_pcg3_noalloc(s4f_mesh *mesh, const s4f_solve_opts *opts, const double *b_dev,
+                            const double normFactor[3], double *x_dev, int32_t iters[3]);
 /* M's diagonal (C) and per-face coefficient a_f (F, upper = -a_f) */
 s4f_status s4f_matrix(s4f_mesh *mesh, double *diag, double *faceCoeff);
 

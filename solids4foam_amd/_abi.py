@@ -84,6 +84,7 @@ EXPORTED = [
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
     "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
     "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info", "s4f_plane_stress_set",
+    "s4f_pcg3_noalloc",
 ]
 
 
@@ -137,6 +138,7 @@ def lib():
         L.s4f_residual.argtypes = [vp, dp, dp, dp, dp]
         L.s4f_spmv3.argtypes = [vp, dp, dp, ctypes.c_int32]
         L.s4f_pcg3.argtypes = [vp, ctypes.POINTER(SolveOpts), dp, dp, dp, ip]
+        L.s4f_pcg3_noalloc.argtypes = [vp, ctypes.POINTER(SolveOpts), vp, dp, vp, ip]
         L.s4f_matrix.argtypes = [vp, dp, dp]
         L.s4f_batch_create.argtypes = [vp, ctypes.POINTER(vp)]
         L.s4f_batch_add.argtypes = [vp, vp, ip]

@@ -95,6 +95,48 @@ struct s4f_ctx {
     bool destroyPending = false;
 };
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free PCG entry point (s4f_pcg3_noalloc): one
+// file-scope device array per GPU, holding the control block, the per-
+// workgroup reduction partials and the PCG vectors of a mesh of at most
+// kNoAllocMaxCells cells.  In host code the array's name is only a stand-in
+// for the symbol: its device address is taken once per device with
+// hipGetSymbolAddress (at context creation on that device) and only that
+// address is ever passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxCells = 262144;  // 64^3, the largest batched-box instance
+constexpr size_t kNoAllocCtlBytes = 1024;
+constexpr size_t kNoAllocPartBytes = (size_t)8192 * kMaxPartials * sizeof(double);
+// r, p, p2, q as FP64 (FP32 sweeps need r, p, p2, q, x: 60 of these 96 bytes per cell)
+constexpr size_t kNoAllocVecBytes = (size_t)4 * 3 * kNoAllocMaxCells * sizeof(double);
+constexpr size_t kNoAllocScratchBytes = kNoAllocCtlBytes + kNoAllocPartBytes + kNoAllocVecBytes;
+static_assert(sizeof(Ctl) <= kNoAllocCtlBytes, "control block exceeds its scratch slot");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;
+void* gScratchPtr[kMaxDevices] = {};
+
+namespace {
+// with `device` current: the device address of this device's scratch array
+void scratchInit(int device) {
+    if (device >= kMaxDevices) throw InvalidError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    if (gScratchPtr[device]) return;
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array");
+    gScratchPtr[device] = p;
+}
+unsigned char* scratchPtr(int device) {
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    void* p = (device >= 0 && device < kMaxDevices) ? gScratchPtr[device] : nullptr;
+    if (!p) throw HipError("no scratch address for this device (context not created on it)");
+    return (unsigned char*)p;
+}
+}  // namespace
+
 namespace {
 void ctxRelease(s4f_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
@@ -1749,6 +1791,7 @@ s4f_status s4f_ctx_create(int device, s4f_ctx** out) {
         HIP_CHECK(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) throw InvalidError("no HIP device " + std::to_string(device));
         HIP_CHECK(hipSetDevice(device));
+        scratchInit(device);
         auto c = std::make_unique<s4f_ctx>();
         c->device = device;
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -2247,6 +2290,75 @@ s4f_status s4f_pcg3(s4f_mesh* M, const s4f_solve_opts* o, const double* b, const
         };
         if (o->precision == S4F_PRECISION_MIXED) run(pcgVecs<float>(st, C), 0.0f);
         else run(pcgVecs<double>(st, C), 0.0);
+        if (iters)
+            for (int j = 0; j < 3; ++j) iters[j] = hc.nIter[j];
+        return S4F_OK;
+    });
+}
+
+// s4f_pcg3 without any allocation: b and x are the caller's device arrays
+// (component-major, j*nCells + c), the control block, reduction partials and
+// PCG vectors live in this device's scratch array.  Same kernels and
+// workgroup tiles as s4f_pcg3, so the same bits.
+s4f_status s4f_pcg3_noalloc(s4f_mesh* M, const s4f_solve_opts* o, const double* b, const double normFactor[3],
+                            double* x, int32_t iters[3]) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !o || !b || !normFactor || !x) throw InvalidError("null argument");
+        if (!M->materialSet) throw InvalidError("material not set");
+        checkOpts(*o);
+        singleMeshOnly(*o, "s4f_pcg3_noalloc");
+        const int C = M->h.C;
+        if (C > kNoAllocMaxCells)
+            throw InvalidError("s4f_pcg3_noalloc: " + std::to_string(C) + " cells exceed its scratch limit of " +
+                               std::to_string(kNoAllocMaxCells) + " cells (use s4f_pcg3)");
+        if ((size_t)M->nBlocks * kMaxPartials * sizeof(double) > kNoAllocPartBytes)
+            throw InvalidError("s4f_pcg3_noalloc: too many workgroup tiles for its scratch");
+        if (M->h.instCell0.size() != 1) throw InvalidError("s4f_pcg3_noalloc: single meshes only");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        unsigned char* base = scratchPtr(M->ctx->device);
+        Ctl* ctl = (Ctl*)base;
+        double* part = (double*)(base + kNoAllocCtlBytes);
+        unsigned char* vec = base + kNoAllocCtlBytes + kNoAllocPartBytes;
+        const DevMesh dm = M->dm();
+        const BlockMap bm = M->bmap();
+        Ctl c0 = makeCtl(*o, 0);
+        for (int j = 0; j < 3; ++j) c0.normFactor[j] = normFactor[j];
+        hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, s, ctl, c0);
+        const size_t n3 = 3 * (size_t)C;
+        Ctl hc;
+        auto run = [&](auto tag, auto* xv) {
+            using T = decltype(tag);
+            PcgVecs<T> v;
+            T* r = (T*)vec;
+            v.r = r;
+            v.p = r + n3;
+            v.p2 = r + 2 * n3;
+            v.q = r + 3 * n3;
+            v.x = xv;
+            hipLaunchKernelGGL((k_pcg_start<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, b, v.r, v.p, ctl, part);
+            hipLaunchKernelGGL(k_fin_pcg_start, dim3(1), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
+            // x = 0 when no iteration runs
+            hipLaunchKernelGGL((k_fill<T>), dim3(gridRows((int)n3)), dim3(kBlock), 0, s, v.x, n3, (T)0);
+            HIP_CHECK(hipGetLastError());
+            int total = 0;
+            for (;;) {
+                launchPcgChunk<T>(dm, bm, 1, v, ctl, part, 8, s);
+                total += 8;
+                HIP_CHECK(hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                if (hc.innerDone || total > o->pcgMaxIter + 8) break;
+            }
+        };
+        if (o->precision == S4F_PRECISION_MIXED) {
+            float* x32 = (float*)vec + 4 * n3;
+            run(0.0f, x32);
+            hipLaunchKernelGGL(k_widen, dim3(gridRows((int)n3)), dim3(kBlock), 0, s, (const float*)x32, x, n3);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            run(0.0, x);
+        }
         if (iters)
             for (int j = 0; j < 3; ++j) iters[j] = hc.nIter[j];
         return S4F_OK;

@@ -1203,6 +1203,23 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_grad(const BlockMap bm, Ctl* 
     }
 }
 
+// control block set from a host value passed by argument (no host->device copy)
+__global__ void k_ctl_set(Ctl* __restrict__ c, const Ctl v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *c = v;
+}
+
+// p[0..n) = v (a kernel rather than a memset node, so callers may capture it)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, size_t n, T v) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// b = (double)a, exact
+__global__ __launch_bounds__(kBlock) void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        b[i] = (double)a[i];
+}
+
 // plain streaming copy (traffic-counter calibration: known bytes moved)
 __global__ __launch_bounds__(kBlock) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) b[i] = a[i];
