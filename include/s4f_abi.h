@@ -65,7 +65,8 @@ enum {
 enum { S4F_RELAX_FIXED = 0, S4F_RELAX_AITKEN = 1 };
 
 /* d2dt2Schemes of rho*fvm::d2dt2(D) (linGeomTotalDispSolid.C:143) */
-enum { S4F_D2DT2_STEADY = 0, S4F_D2DT2_EULER = 1 };
+enum { S4F_D2DT2_STEADY = 0, S4F_D2DT2_EULER = 1,
+       S4F_D2DT2_BACKWARD = 2 /* backwardD2dt2Scheme.C:227-300, constant time step only */ };
 
 /* precision of the inner linear solve */
 enum {
@@ -185,6 +186,13 @@ s4f_status s4f_gravity_set(s4f_mesh *mesh, const double *g);
    The step starts from Dold unless the solve is given a warm-start field, and
    the convergence test's old-time field is Dold (solidModelTemplates.C:26-188). */
 s4f_status s4f_old_time_set(s4f_mesh *mesh, const double *Dold, const double *DoldOld);
+/* Old-time levels D.oldTime(), D.oldTime().oldTime(), ... (nLevels = 1..4
+   fields of nCells*3, AoS, one after the other; levels not given repeat the
+   deepest one, as OpenFOAM creates a missing old-time level as a copy).
+   S4F_D2DT2_BACKWARD reads four levels: backwardD2dt2Scheme::fvmD2dt2
+   (backwardD2dt2Scheme.C:227-300) applies backward ddt to D.oldTime() and
+   D.oldTime().oldTime(), which read their own two old levels. */
+s4f_status s4f_old_times_set(s4f_mesh *mesh, int32_t nLevels, const double *levels);
 /* Per-face boundary data of patch `patch`: fixedDisplacement and
    fixedDisplacementZeroShear -> displacement (size*3; only its normal
    component binds the zero-shear patch); solidTraction -> traction (size*3)
@@ -220,10 +228,16 @@ typedef struct {
        S4F_RELAX_AITKEN (solidModel.C:842-897: per cell and boundary face,
        D -= alpha (D_prev - D), alpha updated by Aitken's formula, clipped to [0, 2]) */
     int32_t relaxationMethod;
-    int32_t reserved1;             /* zero */
+    /* mixed precision's fallback to FP64 sweeps: when an iterating instance's
+       |initRes| finds no new minimum 1 % below the previous one within this
+       many outer iterations, the solve continues with FP64 sweeps from where it
+       stands (report: mixedFallback); 0 = 200, negative = never */
+    int32_t mixedStallWindow;
     /* transient: S4F_D2DT2_EULER ([OF] EulerD2dt2Scheme: matrix diagonal
        coefft rDeltaT2 rho V, source rDeltaT2 rho V (coefft0 D_old - coefft00 D_oldOld))
-       with the time step deltaT and the previous one deltaT0 (0: = deltaT) */
+       with the time step deltaT and the previous one deltaT0 (0: = deltaT);
+       S4F_D2DT2_BACKWARD (backwardD2dt2Scheme.C:227-300: diagonal coefft^2 rho V/dt^2,
+       four old-time levels, deltaT0 must be 0 or deltaT) */
     int32_t d2dt2;
     int32_t reserved2;             /* zero */
     double deltaT;
@@ -237,9 +251,11 @@ typedef struct {
     double initialResidual;  /* |initRes| of the last outer iteration */
     double residualvf;       /* relative displacement change of the last outer iteration */
     double seconds;          /* wall time of the solve on the device stream */
-    int32_t mixedFallback;   /* mixed precision fell back to FP64 inner sweeps */
+    int32_t mixedFallback;   /* mixed precision fell back to FP64 inner sweeps (mixedStallWindow;
+                                solves and batches: the server keeps FP32 sweeps) */
     int32_t warmStarted;     /* the solve started from a resident / supplied field */
-    int64_t pcgIterations;   /* PCG iterations (3-RHS sweeps) summed over outer iterations */
+    int64_t pcgIterations;   /* PCG iterations (3-RHS sweeps) summed over outer iterations
+                                (solves and batches; 0 from the server, which reports innerTotal) */
 } s4f_solve_report;
 
 /* Fills opts with the reference defaults. */

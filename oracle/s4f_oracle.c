@@ -75,7 +75,7 @@ typedef struct {
     double *histInitRes, *histRvf; int *histInner;
     double relaxD;          /* fvSolution relaxationFactors { fields { D } }: 0 or 1 = none */
     int relaxAitken;        /* solidProperties relaxationMethod: 0 fixed, 1 Aitken */
-    int d2dt2;              /* d2dt2Schemes: 0 steadyState, 1 Euler */
+    int d2dt2;              /* d2dt2Schemes: 0 steadyState, 1 Euler, 2 backward (constant step) */
     double deltaT, deltaT0; /* time step and the previous one (Euler d2dt2) */
     double g[3];            /* gravitational acceleration (constant/g), body force rho g */
 } or_opts;
@@ -118,6 +118,7 @@ typedef struct {
     double *aAlpha, *aRes;           /* Aitken: aitkenAlpha (C), aitkenResidual (C*3) */
     double *rho;                     /* density per cell (mechanicalProperties rho) */
     double *Doo;                     /* D.oldTime().oldTime(), C*3 */
+    double *Dooo, *Doooo;            /* the third and fourth old-time levels (backward d2dt2), C*3 */
     double *diag0;                   /* the steady matrix diagonal (the inertia term is added per solve) */
     double *aAlphaB, *aResB;         /* their boundary values (nB, nB*3) */
     int iCorr;
@@ -695,6 +696,7 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
     c->aAlphaB = xcalloc((size_t)B, sizeof(double)); c->aResB = xcalloc((size_t)B*3, sizeof(double));
     c->R = xcalloc((size_t)C*3, sizeof(double));
     c->rho = xcalloc((size_t)C, sizeof(double)); c->Doo = xcalloc((size_t)C*3, sizeof(double));
+    c->Dooo = xcalloc((size_t)C*3, sizeof(double)); c->Doooo = xcalloc((size_t)C*3, sizeof(double));
     c->diag0 = xcalloc((size_t)C, sizeof(double)); memcpy(c->diag0, c->diag, sizeof(double)*C);
     return c;
 }
@@ -703,7 +705,7 @@ void or_case_free(or_case *c)
 {
     if (!c) return;
     free(c->patchType); free(c->bType); free(c->bValue); free(c->bPressure);
-    free(c->rho); free(c->Doo); free(c->diag0);
+    free(c->rho); free(c->Doo); free(c->Dooo); free(c->Doooo); free(c->diag0);
     free(c->mu); free(c->lambda); free(c->K); free(c->impK);
     free(c->w); free(c->nonOrthDelta); free(c->corrVec); free(c->bDelta); free(c->bK); free(c->bN);
     free(c->lsP); free(c->lsN); free(c->lsB); free(c->impKf); free(c->gammaf);
@@ -1125,23 +1127,42 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
        coefft00 = (dt + dt0)/(2 dt0), coefft0 = coefft + coefft00,
        rDeltaT2 = 4/(dt + dt0)^2.  In defect form the matrix gains the diagonal
        and the explicit residual gains source - diag D + rho g V. */
+    /* backwardD2dt2Scheme::fvmD2dt2 (backwardD2dt2Scheme.C:227-300; constant
+       step only, :238-243): coefft rDeltaT fvmDdt(D) [OF backwardDdtScheme]
+       + rDeltaT V (coefft0 fvcDdt(D.oldTime()) - coefft00 fvcDdt(D.oldTime().oldTime())),
+       coefft = 1 + dt/(dt + dt0), coefft00 = dt^2/(dt0 (dt + dt0)),
+       coefft0 = coefft + coefft00 (:112-117); [OF] backward fvmDdt(X): diag
+       coefft V/dt, source V/dt (coefft0 X0 - coefft00 X00); fvcDdt(X) =
+       (coefft X - coefft0 X0 + coefft00 X00)/dt.  Diagonal coefft^2 V rho/dt^2. */
     {
         const double *V = m->V;
         double ct = 0.0, ct0 = 0.0, ct00 = 0.0, r2 = 0.0;
-        if (o->d2dt2) {
+        double bt = 0.0, bt0 = 0.0, bt00 = 0.0, rdt = 0.0;
+        if (o->d2dt2 == 1) {
             const double dt = o->deltaT, dt0 = o->deltaT0 > 0.0 ? o->deltaT0 : o->deltaT;
             ct = (dt + dt0)/(2.0*dt); ct00 = (dt + dt0)/(2.0*dt0); ct0 = ct + ct00;
             r2 = 4.0/((dt + dt0)*(dt + dt0));
+        } else if (o->d2dt2 == 2) {
+            const double dt = o->deltaT, dt0 = dt;
+            bt = 1.0 + dt/(dt + dt0); bt00 = dt*dt/(dt0*(dt + dt0)); bt0 = bt + bt00;
+            rdt = 1.0/dt;
         }
         for (int i = 0; i < C; ++i) {
             const double rv = c->rho[i]*V[i];
-            const double iner = ct*r2*rv;
+            const double iner = o->d2dt2 == 2 ? bt*rdt*bt*rdt*rv : ct*r2*rv;
             c->diag[i] = c->diag0[i] + iner;
             for (int q = 0; q < 3; ++q) {
+                const int k = 3*i+q;
                 double add = rv*o->g[q];
-                if (o->d2dt2)
-                    add += r2*rv*(ct0*c->Dold[3*i+q] - ct00*c->Doo[3*i+q]) - iner*c->D[3*i+q];
-                c->R[3*i+q] += add;
+                if (o->d2dt2 == 1)
+                    add += r2*rv*(ct0*c->Dold[k] - ct00*c->Doo[k]) - iner*c->D[k];
+                else if (o->d2dt2 == 2) {
+                    const double fvmSrc = bt*rdt*rdt*rv*(bt0*c->Dold[k] - bt00*c->Doo[k]);
+                    const double ddt0 = rdt*(bt*c->Dold[k] - bt0*c->Doo[k] + bt00*c->Dooo[k]);
+                    const double ddt00 = rdt*(bt*c->Doo[k] - bt0*c->Dooo[k] + bt00*c->Doooo[k]);
+                    add += fvmSrc + rdt*rv*(bt0*ddt0 - bt00*ddt00) - iner*c->D[k];
+                }
+                c->R[k] += add;
             }
         }
     }
@@ -1277,6 +1298,8 @@ int or_solve(or_case *c, const or_opts *o, or_report *rep)
 void or_advance_time(or_case *c)
 {
     const int C = c->m->nCells;
+    memcpy(c->Doooo, c->Dooo, sizeof(double)*3*C);
+    memcpy(c->Dooo, c->Doo, sizeof(double)*3*C);
     memcpy(c->Doo, c->Dold, sizeof(double)*3*C);
     memcpy(c->Dold, c->D, sizeof(double)*3*C);
     c->iCorr = 0;
@@ -1308,7 +1331,7 @@ static double *field_ptr(or_case *c, const char *name, size_t *n)
     F_("impK", c->impK, C) F_("impKf", c->impKf, F) F_("gammaf", c->gammaf, F)
     F_("mu", c->mu, C) F_("lambda", c->lambda, C) F_("K", c->K, C)
     F_("aitkenAlpha", c->aAlpha, C) F_("aitkenResidual", c->aRes, 3*C)
-    F_("rho", c->rho, C) F_("Doo", c->Doo, 3*C)
+    F_("rho", c->rho, C) F_("Doo", c->Doo, 3*C) F_("Dooo", c->Dooo, 3*C) F_("Doooo", c->Doooo, 3*C)
     F_("aitkenAlphab", c->aAlphaB, B) F_("aitkenResidualb", c->aResB, 3*B)
     #undef F_
     *n = 0; return NULL;

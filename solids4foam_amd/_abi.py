@@ -26,7 +26,7 @@ PATCH_EMPTY = 2
 PATCH_SYMMETRY = 3
 PATCH_ZERO_SHEAR = 4
 RELAX_FIXED, RELAX_AITKEN = 0, 1
-D2DT2_STEADY, D2DT2_EULER = 0, 1
+D2DT2_STEADY, D2DT2_EULER, D2DT2_BACKWARD = 0, 1, 2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECOND_DIAGONAL = 0
@@ -49,7 +49,7 @@ class SolveOpts(ctypes.Structure):
                 ("pcgRelTol", ctypes.c_double), ("pcgMaxIter", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
                 ("eagerLaunch", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
                 ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double),
-                ("relaxationMethod", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+                ("relaxationMethod", ctypes.c_int32), ("mixedStallWindow", ctypes.c_int32),
                 ("d2dt2", ctypes.c_int32), ("reserved2", ctypes.c_int32), ("deltaT", ctypes.c_double),
                 ("deltaT0", ctypes.c_double)]
 
@@ -84,7 +84,7 @@ EXPORTED = [
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
     "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
     "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info", "s4f_plane_stress_set",
-    "s4f_pcg3_noalloc",
+    "s4f_pcg3_noalloc", "s4f_old_times_set",
 ]
 
 
@@ -129,6 +129,7 @@ def lib():
         L.s4f_plane_stress_set.argtypes = [vp, ctypes.c_int32]
         L.s4f_gravity_set.argtypes = [vp, dp]
         L.s4f_old_time_set.argtypes = [vp, dp, dp]
+        L.s4f_old_times_set.argtypes = [vp, ctypes.c_int32, dp]
         L.s4f_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
         L.s4f_solve_opts_default.restype = None
         L.s4f_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), dp, dp, dp, dp, ctypes.POINTER(SolveReport)]

@@ -42,7 +42,8 @@ class Batch:
             r = reps[i]
             out.append(dict(D=Ds[i], sigma=sig[i] if sig else None, nOuter=r.nOuter, converged=bool(r.converged),
                             innerTotal=int(r.innerTotal), initialResidual=r.initialResidual,
-                            residualvf=r.residualvf, seconds=r.seconds))
+                            residualvf=r.residualvf, seconds=r.seconds, pcgIterations=int(r.pcgIterations),
+                            mixedFallback=int(r.mixedFallback)))
         return out
 
     def close(self):

@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <cmath>
 #include <cstdio>
@@ -187,6 +188,7 @@ struct HostMesh {
     double g[3] = {0.0, 0.0, 0.0};
     std::vector<double> diagBase; // the steady diagonal (diag may carry the inertia term)
     std::vector<double> Do, Doo;  // old-time fields (SoA 3C) of a transient step
+    std::vector<double> Dooo, Doooo;  // third and fourth old-time levels (backward d2dt2)
     bool hasOld = false;
 };
 
@@ -789,7 +791,10 @@ void checkOpts(const s4f_solve_opts& o) {
     if (o.eagerLaunch != 0 && o.eagerLaunch != 1) throw InvalidError("eagerLaunch must be 0 or 1");
     if (o.relaxationMethod != S4F_RELAX_FIXED && o.relaxationMethod != S4F_RELAX_AITKEN)
         throw InvalidError("relaxationMethod must be fixed (0) or Aitken (1)");
-    if (o.d2dt2 != S4F_D2DT2_STEADY && o.d2dt2 != S4F_D2DT2_EULER) throw InvalidError("d2dt2 must be steady (0) or Euler (1)");
+    if (o.d2dt2 != S4F_D2DT2_STEADY && o.d2dt2 != S4F_D2DT2_EULER && o.d2dt2 != S4F_D2DT2_BACKWARD)
+        throw InvalidError("d2dt2 must be steady (0), Euler (1) or backward (2)");
+    if (o.d2dt2 == S4F_D2DT2_BACKWARD && o.deltaT0 > 0.0 && o.deltaT0 != o.deltaT)
+        throw InvalidError("backward d2dt2 needs a constant time step (backwardD2dt2Scheme.C:238-243)");
     if (!(o.pcgTolerance >= 0) || !(o.pcgRelTol >= 0) || !(o.solutionTolerance >= 0) || !(o.alternativeTolerance >= 0))
         throw InvalidError("tolerances must be >= 0");
 }
@@ -1322,12 +1327,30 @@ void uploadDiagonal(s4f_mesh* M) {
 // coefft00, rDeltaT2 = 4/(dt + dt0)^2.  In defect form the matrix carries the
 // diagonal (uploaded, and the multigrid hierarchy rebuilt, when it changes)
 // and the residual adds fext - iner D with fext = rho V g + source.
+// D.oldTime() .. its fourth level from AoS host fields (levels[l], nCells*3)
+void setOldTimes(HostMesh& h, const double* const* levels) {
+    const int C = h.C;
+    std::vector<double>* dst[4] = {&h.Do, &h.Doo, &h.Dooo, &h.Doooo};
+    for (int l = 0; l < 4; ++l) {
+        std::vector<double>& d = *dst[l];
+        d.resize(3 * (size_t)C);
+        for (int c = 0; c < C; ++c)
+            for (int q = 0; q < 3; ++q) {
+                const double v = levels[l][3 * (size_t)c + q];
+                if (!std::isfinite(v)) throw InvalidError("non-finite old-time field");
+                d[(size_t)q * C + c] = v;
+            }
+    }
+    h.hasOld = true;
+}
+
 void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
     HostMesh& h = M->h;
-    const bool transient = o.d2dt2 == S4F_D2DT2_EULER;
+    const bool transient = o.d2dt2 != S4F_D2DT2_STEADY;
+    const bool backward = o.d2dt2 == S4F_D2DT2_BACKWARD;
     const bool grav = h.g[0] != 0.0 || h.g[1] != 0.0 || h.g[2] != 0.0;
     M->srcOn = M->inerOn = false;
-    double key = 0.0, ct = 0.0, ct0 = 0.0, ct00 = 0.0, r2 = 0.0;
+    double key = 0.0, ct = 0.0, ct0 = 0.0, ct00 = 0.0, r2 = 0.0, rdt = 0.0;
     if (transient || grav) {
         if (h.instCell0.size() != 1) throw InvalidError("transient and body-force solves take a single mesh");
         if (h.rho.empty()) throw InvalidError("density not set (s4f_density_set)");
@@ -1336,11 +1359,21 @@ void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
         if (!(o.deltaT > 0.0) || !(o.deltaT0 >= 0.0)) throw InvalidError("deltaT must be > 0 (and deltaT0 >= 0)");
         if (!h.hasOld) throw InvalidError("a transient solve needs the old-time fields (s4f_old_time_set)");
         const double dt = o.deltaT, dt0 = o.deltaT0 > 0.0 ? o.deltaT0 : o.deltaT;
-        ct = (dt + dt0) / (2.0 * dt);
-        ct00 = (dt + dt0) / (2.0 * dt0);
-        ct0 = ct + ct00;
-        r2 = 4.0 / ((dt + dt0) * (dt + dt0));
-        key = ct * r2;
+        if (backward) {
+            // backwardD2dt2Scheme.C:112-117 (constant step: dt0 = dt)
+            ct = 1.0 + dt / (dt + dt0);
+            ct00 = dt * dt / (dt0 * (dt + dt0));
+            ct0 = ct + ct00;
+            rdt = 1.0 / dt;
+            key = ct * rdt * ct * rdt;
+        } else {
+            // [OF] EulerD2dt2Scheme
+            ct = (dt + dt0) / (2.0 * dt);
+            ct00 = (dt + dt0) / (2.0 * dt0);
+            ct0 = ct + ct00;
+            r2 = 4.0 / ((dt + dt0) * (dt + dt0));
+            key = ct * r2;
+        }
     }
     const int C = h.C;
     auto rhoV = [&](int c) { return (h.rho.size() == 1 ? h.rho[0] : h.rho[c]) * h.V[c]; };
@@ -1349,7 +1382,7 @@ void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
         std::vector<double> iner(C, 0.0);
         if (key != 0.0)
             for (int c = 0; c < C; ++c) {
-                iner[c] = ct * r2 * rhoV(c);
+                iner[c] = key * rhoV(c);
                 h.diag[c] += iner[c];
             }
         uploadDiagonal(M);
@@ -1363,9 +1396,18 @@ void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
     for (int c = 0; c < C; ++c) {
         const double rv = rhoV(c);
         for (int q = 0; q < 3; ++q) {
+            const size_t k = (size_t)q * C + c;
             double v = rv * h.g[q];
-            if (transient) v += r2 * rv * (ct0 * h.Do[(size_t)q * C + c] - ct00 * h.Doo[(size_t)q * C + c]);
-            f[(size_t)q * C + c] = v;
+            if (transient && !backward) v += r2 * rv * (ct0 * h.Do[k] - ct00 * h.Doo[k]);
+            if (backward) {
+                // coefft rDeltaT fvmDdt(D) source + rDeltaT V (coefft0 fvcDdt(D.oldTime())
+                // - coefft00 fvcDdt(D.oldTime().oldTime())), backward ddt of each level
+                const double fvmSrc = ct * rdt * rdt * rv * (ct0 * h.Do[k] - ct00 * h.Doo[k]);
+                const double ddt0 = rdt * (ct * h.Do[k] - ct0 * h.Doo[k] + ct00 * h.Dooo[k]);
+                const double ddt00 = rdt * (ct * h.Doo[k] - ct0 * h.Dooo[k] + ct00 * h.Doooo[k]);
+                v += fvmSrc + rdt * rv * (ct0 * ddt0 - ct00 * ddt00);
+            }
+            f[k] = v;
         }
     }
     M->fext.upload(f, M->ctx->stream);
@@ -1445,10 +1487,28 @@ struct InstanceIO {
     s4f_status status = S4F_OK;
 };
 
+// Mixed precision's fallback to FP64 sweeps: the FP32 run stops at an
+// outer-iteration boundary (the last PCG finished, the gradient step not yet
+// launched) and an FP64 run continues from the device state as it stands.
+struct Fallback {
+    bool stalled = false;            // the FP32 run stopped for the fallback
+    std::vector<char> live;          // instances still iterating at the switch
+    std::vector<long long> instPcg;  // 3-RHS PCG sweeps so far, per instance
+};
+
+// Default window of mixed precision's stall test: outer iterations within
+// which every iterating instance's |initRes| must reach a new minimum at
+// least 1 % below its previous one.
+constexpr int kMixedStallWindow = 200;
+
 // The outer momentum loop of linGeomTotalDispSolid::evolve on every instance
-// packed in M, with shared launches; instances finish independently.
+// packed in M, with shared launches; instances finish independently.  FP32
+// sweeps (T = float) given `fb` stop for the FP64 fallback when an iterating
+// instance's |initRes| stalls for the window (fb->stalled); `resume` runs the
+// FP64 continuation of such a run.
 template <typename T>
-void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
+void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, Fallback* fb = nullptr,
+            bool resume = false) {
     s4f_ctx* ctx = M->ctx;
     hipStream_t s = ctx->stream;
     HostMesh& h = M->h;
@@ -1463,7 +1523,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         s4f_mesh* m;
         ~SourceOff() { m->srcOn = m->inerOn = false; }
     } sourceOff{M};
-    const bool transient = o.d2dt2 == S4F_D2DT2_EULER;
+    const bool transient = o.d2dt2 != S4F_D2DT2_STEADY;
     const bool mg = o.preconditioner == S4F_PRECOND_GAMG;
     if (mg) {
         buildAmg(M);
@@ -1477,40 +1537,44 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     BlockMap bm = bmFull;  // narrowed to the iterating instances as a batch's instances finish
     int nFin = nInst;
     MgRun mgRun = meshRun(M);
-    // initial and old-time fields
-    std::vector<double> D0(3 * (size_t)C, 0.0);
+    // initial and old-time fields (a resumed run keeps the device state)
+    std::vector<double> D0(resume ? 0 : 3 * (size_t)C, 0.0);
     bool warm = false;
     for (int i = 0; i < nInst; ++i) {
         if (!(o.warmStart && io[i].D_inout)) continue;
         warm = true;
+        if (resume) continue;
         const int c0 = h.instCell0[i], n = h.instNCell[i];
         for (int c = 0; c < n; ++c)
             for (int q = 0; q < 3; ++q) D0[(size_t)q * C + c0 + c] = io[i].D_inout[3 * (size_t)c + q];
     }
-    if (transient && !warm) D0 = h.Do;  // a time step starts from the old-time field
-    HIP_CHECK(hipMemcpyAsync(st.D0.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    if (transient) {  // the convergence test's old-time field
-        HIP_CHECK(hipMemcpyAsync(st.Dold.p, h.Do.data(), h.Do.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        warm = true;
-    } else if (warm) {
-        HIP_CHECK(hipMemcpyAsync(st.Dold.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const bool warmIn = warm;
+    if (transient) warm = true;
+    if (!resume) {
+        if (transient && !warmIn) D0 = h.Do;  // a time step starts from the old-time field
+        HIP_CHECK(hipMemcpyAsync(st.D0.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        if (transient) {  // the convergence test's old-time field
+            HIP_CHECK(hipMemcpyAsync(st.Dold.p, h.Do.data(), h.Do.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        } else if (warm) {
+            HIP_CHECK(hipMemcpyAsync(st.Dold.p, D0.data(), D0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        }
+        HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
+        HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
+        std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
+        for (int i = 0; i < nInst; ++i) {
+            c0[i].warm = ((o.warmStart && io[i].D_inout) || transient) ? 1 : 0;  // a time step continues
+            c0[i].sigma0 = c0[i].warm ? 0 : 1;  // a cold start's stress field is zero until the first update
+            c0[i].pcMode = mg ? 1 : 0;
+            c0[i].solveMask = M->h.instMask.empty() ? 7 : M->h.instMask[i];
+        }
+        HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
     }
-    HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
-    HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
-    std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
-    for (int i = 0; i < nInst; ++i) {
-        c0[i].warm = ((o.warmStart && io[i].D_inout) || transient) ? 1 : 0;  // a time step continues
-        c0[i].sigma0 = c0[i].warm ? 0 : 1;  // a cold start's stress field is zero until the first update
-        c0[i].pcMode = mg ? 1 : 0;
-        c0[i].solveMask = M->h.instMask.empty() ? 7 : M->h.instMask[i];
-    }
-    HIP_CHECK(hipMemcpyAsync(st.ctl.p, c0.data(), nInst * sizeof(Ctl), hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemsetAsync(st.hist.p, 0, st.hist.bytes(), s));
     Ctl* hctl = nullptr;
     HIP_CHECK(hipHostMalloc((void**)&hctl, nInst * sizeof(Ctl), hipHostMallocDefault));
     std::unique_ptr<Ctl, void (*)(Ctl*)> hctlGuard(hctl, [](Ctl* p) { (void)hipHostFree(p); });
     const double* Dold = warm ? st.Dold.p : nullptr;
-    HIP_CHECK(hipEventRecord(ctx->ev0, s));
+    if (!resume) HIP_CHECK(hipEventRecord(ctx->ev0, s));  // a resumed run's time counts from the FP32 start
     // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
     GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
     if (o.relaxationMethod == S4F_RELAX_AITKEN) {
@@ -1518,10 +1582,12 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         st.aAlphaB.alloc(std::max(1, h.B)); st.aResB.alloc(3 * (size_t)std::max(1, h.B));
         gbuf.aAlpha = st.aAlpha.p; gbuf.aRes = st.aRes.p; gbuf.aAlphaB = st.aAlphaB.p; gbuf.aResB = st.aResB.p;
     }
-    launchGradStep<T>(dm, bm, nInst, M->maxInstNB, gbuf, nullptr, Dold, st.ctl.p, st.part.p, 1, s);
-    hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
-                       st.hist.p, 1);
-    HIP_CHECK(hipGetLastError());
+    if (!resume) {
+        launchGradStep<T>(dm, bm, nInst, M->maxInstNB, gbuf, nullptr, Dold, st.ctl.p, st.part.p, 1, s);
+        hipLaunchKernelGGL(k_fin_grad, dim3(nInst), dim3(kFinBlock), 0, s, bm, st.ctl.p, (const double*)st.part.p,
+                           st.hist.p, 1);
+        HIP_CHECK(hipGetLastError());
+    }
     // Batches: when instances finish, launch only the tiles of those still
     // iterating.  Each instance keeps its own tiles (and the order of its
     // partials), so its result does not change.
@@ -1569,9 +1635,19 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
     GraphExec gOuter, gPair;
     bool captured = false;
     int chunk = 4;
-    long long pcgIters = 0;
+    std::vector<long long> instPcg(nInst, 0);
+    if (resume) {
+        // the FP32 run's last PCG solution in FP64, then the outer iteration it stopped before
+        instPcg = fb->instPcg;
+        hipLaunchKernelGGL(k_widen, dim3(gridRows(3 * C)), dim3(kBlock), 0, s, (const float*)st.x32.p, (double*)v.x,
+                           3 * (size_t)C);
+        launchGradResidual(s);
+    }
     launchResidual(s);
     HIP_CHECK(hipGetLastError());
+    const int stallWindow = o.mixedStallWindow == 0 ? kMixedStallWindow : o.mixedStallWindow;
+    std::vector<double> bestRes(nInst, HUGE_VAL);
+    std::vector<int> bestIt(nInst, 0);
     for (int it = 0;; ++it) {
         bool allDone = false;
         int total = 0;
@@ -1600,7 +1676,31 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
         int used = 1;
         for (int i = 0; i < nInst; ++i)
             if (!hctl[i].outerDone) used = std::max({used, hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});
-        pcgIters += std::max({hctl[0].nIter[0], hctl[0].nIter[1], hctl[0].nIter[2]});
+        for (int i = 0; i < nInst; ++i)
+            if (!hctl[i].outerDone) instPcg[i] += std::max({hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});
+        // mixed precision's stall test: an iterating instance whose |initRes|
+        // found no new minimum 1 % below the last one within the window
+        if (std::is_same<T, float>::value && fb && stallWindow > 0) {
+            bool stall = false;
+            for (int i = 0; i < nInst; ++i) {
+                if (hctl[i].outerDone || hctl[i].iCorr < 2) continue;
+                const double r = hctl[i].initResMag;
+                if (r < 0.99 * bestRes[i]) {
+                    bestRes[i] = r;
+                    bestIt[i] = it;
+                } else if (it - bestIt[i] >= stallWindow) {
+                    stall = true;
+                }
+            }
+            if (stall) {
+                HIP_CHECK(hipStreamSynchronize(s));
+                fb->stalled = true;
+                fb->live.assign(nInst, 0);
+                for (int i = 0; i < nInst; ++i) fb->live[i] = hctl[i].outerDone ? 0 : 1;
+                fb->instPcg = instPcg;
+                return;
+            }
+        }
         // next outer iteration: launch just under the last count, then top up
         chunk = std::max(1, used - 1);
         if (nInst > 1) {  // instances that finished at the previous outer iteration leave the launch
@@ -1677,17 +1777,22 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
             rep->initialResidual = cf.initResMag;
             rep->residualvf = cf.rvf;
             rep->seconds = ms * 1e-3;
-            rep->mixedFallback = 0;
+            rep->mixedFallback = (resume && fb->live[i]) ? 1 : 0;
             rep->warmStarted = cf.warm;
-            rep->pcgIterations = nInst == 1 ? pcgIters : 0;
+            rep->pcgIterations = instPcg[i];
         }
         io[i].status = cf.diverged ? S4F_ERR_DIVERGED : (cf.converged ? S4F_OK : S4F_WARN_MAXITER);
     }
 }
 
 void solveDispatch(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io) {
-    if (o.precision == S4F_PRECISION_MIXED) solveT<float>(M, o, io);
-    else solveT<double>(M, o, io);
+    if (o.precision == S4F_PRECISION_MIXED) {
+        Fallback fb;
+        solveT<float>(M, o, io, &fb);
+        if (fb.stalled) solveT<double>(M, o, io, &fb, true);
+    } else {
+        solveT<double>(M, o, io);
+    }
 }
 
 // Concatenates meshes (their material and boundary values included) into one
@@ -2033,18 +2138,22 @@ s4f_status s4f_old_time_set(s4f_mesh* M, const double* Dold, const double* DoldO
         if (!M || !Dold) throw InvalidError("null argument");
         HostMesh& h = M->h;
         if (h.instCell0.size() != 1) throw InvalidError("old-time fields belong to single meshes");
-        const int C = h.C;
-        h.Do.resize(3 * (size_t)C);
-        h.Doo.resize(3 * (size_t)C);
-        const double* oo = DoldOld ? DoldOld : Dold;
-        for (int c = 0; c < C; ++c)
-            for (int q = 0; q < 3; ++q) {
-                h.Do[(size_t)q * C + c] = Dold[3 * (size_t)c + q];
-                h.Doo[(size_t)q * C + c] = oo[3 * (size_t)c + q];
-                if (!std::isfinite(h.Do[(size_t)q * C + c]) || !std::isfinite(h.Doo[(size_t)q * C + c]))
-                    throw InvalidError("non-finite old-time field");
-            }
-        h.hasOld = true;
+        const double* lv[4] = {Dold, DoldOld ? DoldOld : Dold, nullptr, nullptr};
+        lv[2] = lv[3] = lv[1];
+        setOldTimes(h, lv);
+        return S4F_OK;
+    });
+}
+
+s4f_status s4f_old_times_set(s4f_mesh* M, int32_t nLevels, const double* levels) {
+    return guarded([&]() -> s4f_status {
+        if (!M || !levels) throw InvalidError("null argument");
+        if (nLevels < 1 || nLevels > 4) throw InvalidError("nLevels must be 1..4");
+        HostMesh& h = M->h;
+        if (h.instCell0.size() != 1) throw InvalidError("old-time fields belong to single meshes");
+        const double* lv[4];
+        for (int l = 0; l < 4; ++l) lv[l] = levels + (size_t)std::min(l, nLevels - 1) * 3 * (size_t)h.C;
+        setOldTimes(h, lv);
         return S4F_OK;
     });
 }
@@ -2379,14 +2488,15 @@ s4f_status s4f_calibrate_copy(s4f_ctx* ctx, int64_t bytes, double* ms) {
         if (!ctx || bytes < 32) throw InvalidError("bad argument");
         HIP_CHECK(hipSetDevice(ctx->device));
         const size_t n = (size_t)bytes / 32;
+        if ((n + kBlock - 1) / kBlock > (size_t)INT32_MAX) throw InvalidError("copy too large");
         DArr<double> a, b;
         a.alloc(4 * n);
         b.alloc(4 * n);
         HIP_CHECK(hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));
-        const int grid = (int)std::min<size_t>(8192, (n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, ctx->stream, (const double4*)a.p, (double4*)b.p, n);
+        const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_copy, grid, dim3(kBlock), 0, ctx->stream, (const dbl4*)a.p, (dbl4*)b.p, n);
         HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, ctx->stream, (const double4*)a.p, (double4*)b.p, n);
+        hipLaunchKernelGGL(k_copy, grid, dim3(kBlock), 0, ctx->stream, (const dbl4*)a.p, (dbl4*)b.p, n);
         HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipEventSynchronize(ctx->ev1));

@@ -1220,9 +1220,16 @@ __global__ __launch_bounds__(kBlock) void k_widen(const float* __restrict__ a, d
         b[i] = (double)a[i];
 }
 
-// plain streaming copy (traffic-counter calibration: known bytes moved)
-__global__ __launch_bounds__(kBlock) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) b[i] = a[i];
+// plain streaming copy: the measured streaming ceiling and the known-size
+// reference of the traffic counters.  One 32-byte load and store per thread
+// over a grid covering the whole array (no grid-stride loop): of the copy
+// forms measured on MI355X (grid-stride with one or four loads in flight per
+// thread, 4-32 workgroups per CU, plain or non-temporal) this one streams
+// fastest.  n is in 32-byte units.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void k_copy(const dbl4* __restrict__ a, dbl4* __restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
 }
 
 }  // namespace s4f

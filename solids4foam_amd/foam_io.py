@@ -594,7 +594,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     solidProperties, fvSolution, fvSchemes, constant/g), runs
     linearGeometryTotalDisplacement on the device and writes D, sigma,
     sigmaEq, epsilon, epsilonEq and pointD.  Steady (d2dt2 steadyState): one
-    solve written to `write_time`.  Transient (Euler): controlDict's time
+    solve written to `write_time`.  Transient (Euler or backward): controlDict's time
     loop from startTime to endTime by deltaT, written at every writeInterval
     (time steps, or simulated time with writeControl runTime/adjustableRunTime)
     to directories named by the time."""
@@ -624,10 +624,10 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     except FileNotFoundError:
         pass
     scheme = read_d2dt2_scheme(case_dir)
-    if scheme not in ("steadyState", "Euler"):
-        raise ValueError(f"d2dt2 scheme {scheme!r} is not on the device path (steadyState, Euler)")
+    if scheme not in ("steadyState", "Euler", "backward"):
+        raise ValueError(f"d2dt2 scheme {scheme!r} is not on the device path (steadyState, Euler, backward)")
     g = read_gravity(case_dir)
-    transient = scheme == "Euler"
+    transient = scheme in ("Euler", "backward")
     tc = read_time_controls(case_dir) if transient else None
     if (transient or g is not None) and (rho is None or np.any(np.isnan(rho))):
         raise ValueError("mechanicalProperties gives no rho (needed for d2dt2 and g)")

@@ -265,11 +265,11 @@ s4f_solve_opts linGeomTotalDispSolidHIP::solveOptions() const
     {
         o.relaxD = mesh().fieldRelaxationFactor(D().name());
     }
-    // rho*fvm::d2dt2(D): steadyState or Euler (linGeomTotalDispSolid.C:143)
+    // rho*fvm::d2dt2(D): steadyState, Euler or backward (linGeomTotalDispSolid.C:143)
     const word d2dt2Scheme(mesh().d2dt2Scheme("d2dt2(" + D().name() + ')'));
-    if (d2dt2Scheme == "Euler")
+    if (d2dt2Scheme == "Euler" || d2dt2Scheme == "backward")
     {
-        o.d2dt2 = S4F_D2DT2_EULER;
+        o.d2dt2 = d2dt2Scheme == "Euler" ? S4F_D2DT2_EULER : S4F_D2DT2_BACKWARD;
         o.deltaT = runTime().deltaTValue();
         o.deltaT0 = runTime().deltaT0Value();
     }
@@ -277,7 +277,7 @@ s4f_solve_opts linGeomTotalDispSolidHIP::solveOptions() const
     {
         FatalErrorIn(type() + "::solveOptions()")
             << "d2dt2 scheme " << d2dt2Scheme << " is not on the device path "
-            << "(steadyState, Euler)" << abort(FatalError);
+            << "(steadyState, Euler, backward)" << abort(FatalError);
     }
     o.warmStart = 1;
     return o;
@@ -325,7 +325,7 @@ bool linGeomTotalDispSolidHIP::evolve()
     const s4f_solve_opts o = solveOptions();
     if (predictor_)
     {
-        if (o.d2dt2 != S4F_D2DT2_EULER)
+        if (o.d2dt2 == S4F_D2DT2_STEADY)
         {
             FatalErrorIn(type() + "::evolve()")
                 << "the predictor needs a transient d2dt2 scheme" << abort(FatalError);
@@ -357,6 +357,22 @@ bool linGeomTotalDispSolidHIP::evolve()
             }
         }
         check(s4f_old_time_set(gpuMesh_, Do.data(), Doo.data()), "evolve()");
+    }
+    else if (o.d2dt2 == S4F_D2DT2_BACKWARD)
+    {
+        // backwardD2dt2Scheme::fvmD2dt2 reads D.oldTime() down to its fourth
+        // level (backward ddt of D.oldTime() and D.oldTime().oldTime())
+        std::vector<double> lv(4*3*nCells);
+        const volVectorField* Dl = &D().oldTime();
+        for (label l = 0; l < 4; ++l)
+        {
+            forAll(*Dl, cellI)
+            {
+                for (direction d = 0; d < 3; ++d) { lv[(l*nCells + cellI)*3 + d] = (*Dl)[cellI][d]; }
+            }
+            if (l < 3) { Dl = &Dl->oldTime(); }
+        }
+        check(s4f_old_times_set(gpuMesh_, 4, lv.data()), "evolve()");
     }
     s4f_solve_report rep;
     const s4f_status st =

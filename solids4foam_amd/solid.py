@@ -186,6 +186,11 @@ class SolverMesh:
         b = None if D_old_old is None else f64(D_old_old, (self.nCells, 3))
         check(lib().s4f_old_time_set(self.h, ptr(a), ptr(b)))
 
+    def set_old_times(self, levels):
+        """D.oldTime(), D.oldTime().oldTime(), ... (1 to 4 fields of (nCells, 3))"""
+        a = f64(np.stack([np.asarray(l, np.float64).reshape(self.nCells, 3) for l in levels]))
+        check(lib().s4f_old_times_set(self.h, len(levels), ptr(a)))
+
     def set_bc(self, patch, value, pressure=None):
         sl = self.poly.patch_faces(patch)
         n = sl.stop - sl.start
@@ -206,7 +211,8 @@ class SolverMesh:
         check(st, allow=(_abi.S4F_OK, _abi.S4F_WARN_MAXITER))
         out = dict(D=D, sigma=sig, gradD=G, Db=Db, nOuter=rep.nOuter, converged=bool(rep.converged),
                    innerTotal=int(rep.innerTotal), initialResidual=rep.initialResidual, residualvf=rep.residualvf,
-                   seconds=rep.seconds, pcgIterations=int(rep.pcgIterations), status=st)
+                   seconds=rep.seconds, pcgIterations=int(rep.pcgIterations), mixedFallback=int(rep.mixedFallback),
+                   status=st)
         return out
 
     def strain_fields(self):
@@ -305,7 +311,7 @@ class LinGeomTotalDispSolid:
                  g=None, d2dt2="steadyState", deltaT=None, predictor=False, plane_stress=False):
         # rho: mechanicalProperties density (needed with g or a transient
         # d2dt2); g: constant/g; d2dt2: d2dt2Schemes default (steadyState |
-        # Euler); deltaT: the time step of evolve() when transient;
+        # Euler | backward); deltaT: the time step of evolve() when transient;
         # predictor: linGeomTotalDispSolid::predict (:47-59), transient only;
         # plane_stress: mechanicalProperties planeStress (2-D moduli)
         # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
@@ -330,13 +336,14 @@ class LinGeomTotalDispSolid:
                 self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))
             elif p["type"] == "solidTraction":
                 self.mesh.set_bc(i, p.get("traction", (0.0, 0.0, 0.0)), p.get("pressure"))
-        if d2dt2 not in ("steadyState", "Euler"):
-            raise ValueError(f"d2dt2 scheme {d2dt2!r} is not on the device path (steadyState, Euler)")
+        if d2dt2 not in ("steadyState", "Euler", "backward"):
+            raise ValueError(f"d2dt2 scheme {d2dt2!r} is not on the device path (steadyState, Euler, backward)")
         if rho is not None:
             self.mesh.set_density(rho)
         if g is not None:
             self.mesh.set_gravity(g)
-        self.transient = d2dt2 == "Euler"
+        self.transient = d2dt2 in ("Euler", "backward")
+        self.scheme = d2dt2
         if self.transient and not (deltaT and deltaT > 0):
             raise ValueError("a transient model needs deltaT > 0")
         if predictor and not self.transient:
@@ -349,6 +356,8 @@ class LinGeomTotalDispSolid:
         self.opts.update(solver or {})
         self.D = np.zeros((polymesh.nCells, 3))
         self.D_old_old = self.D.copy()
+        # backward d2dt2: D.oldTime().oldTime() and the two levels below it
+        self.D_olds = [self.D.copy(), self.D.copy(), self.D.copy()]
         self.U = np.zeros((polymesh.nCells, 3))  # fvc::ddt(D) after each step (Euler)
         self.sigma = None
         self.gradD = None
@@ -359,11 +368,16 @@ class LinGeomTotalDispSolid:
         converged field becomes D.oldTime() of the next step)"""
         opts = dict(self.opts)
         if self.transient:
-            self.mesh.set_old_time(self.D, self.D_old_old)
-            opts.update(d2dt2=_abi.D2DT2_EULER, deltaT=self.deltaT, deltaT0=self.deltaT0)
+            if self.scheme == "backward":
+                self.mesh.set_old_times([self.D] + self.D_olds)
+                opts.update(d2dt2=_abi.D2DT2_BACKWARD, deltaT=self.deltaT, deltaT0=self.deltaT)
+            else:
+                self.mesh.set_old_time(self.D, self.D_old_old)
+                opts.update(d2dt2=_abi.D2DT2_EULER, deltaT=self.deltaT, deltaT0=self.deltaT0)
             # predict(): D = D.oldTime() + U deltaT (gradD and sigma follow D)
             D0 = self.D + self.U * self.deltaT if self.predictor else None
             out = self.mesh.solve(D0=D0, want_sigma=True, want_grad=True, **opts)
+            self.D_olds = [self.D] + self.D_olds[:2]
             self.D_old_old = self.D
             self.deltaT0 = self.deltaT
             self.time += self.deltaT

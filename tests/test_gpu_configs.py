@@ -13,6 +13,9 @@ fields, plain FP64 to 1e-8 and mixed precision to 1e-6 relative L2):
                              and repeated meshes with a new load, warm started)
                              through the continuous-batching server, each
                              result against the oracle solve of its mesh + load
+  Cook's membrane            the reference tutorial's skewed 2-D mesh, field
+                             against the oracle and the top-right displacement
+                             against its README value
 """
 import time
 
@@ -142,3 +145,35 @@ def test_serving_stream_matches_oracle(ctx, oracle_mod):
               f"sigma {es:.2e}", flush=True)
         assert r["converged"]
         assert eD < 1e-6 and es < 1e-6
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cooks_membrane_matches_oracle_and_reference_value(ctx, oracle_mod, precision):
+    # tutorials/solids/linearElasticity/cooksMembrane on its own skewed 20x20
+    # mesh (extendedLeastSquares 0, 2-D): converged field against the oracle
+    # and the top-right vertical displacement against the README's 32.12 mm
+    from solids4foam_amd import PATCH_EMPTY, PolyMesh, SolverMesh
+    from tests import _blockmesh as bm
+    from tests.test_oracle_pins import cooks_membrane, top_right_uy
+    a = bm.single_block(bm.COOK_CORNERS, (20, 20, 1), bm.COOK_PATCHES)
+    pm = PolyMesh.from_arrays(a["points"], a["fOff"], a["fVerts"], a["owner"], a["neighbour"], a["nCells"],
+                              a["patchStart"], a["patchSize"])
+    # the zero-size defaultFaces patch (hex_blocks appends it) carries no values: typed empty
+    m = SolverMesh(ctx, pm, [TRACTION, FIXED, TRACTION, PATCH_EMPTY, PATCH_EMPTY], ls_full_delta=True)
+    m.set_material(bm.COOK_E, bm.COOK_NU)
+    m.set_bc(0, (0.0, 0.0, 0.0))
+    m.set_bc(1, (0.0, 0.0, 0.0))
+    m.set_bc(2, bm.COOK_TRACTION)
+    out = m.solve(precision=precision, **TIGHT)
+    om, oc = cooks_membrane(oracle_mod, 20)
+    ro = oc.solve(**ORACLE_TIGHT)
+    eD = rel_l2(out["D"], oc.get("D").reshape(-1, 3))
+    pD = m.point_displacement()
+    p = int(np.argmin(np.linalg.norm(om.points - np.array(bm.COOK_TOP_RIGHT), axis=1)))
+    uy, uyo = float(pD[p, 1]), top_right_uy(oracle_mod, om, oc)
+    print(f"[cook] precision {precision}: nOuter {out['nOuter']} (oracle {ro['nOuter']}) err D {eD:.2e} "
+          f"uy {uy * 1e3:.4f} mm (oracle {uyo * 1e3:.4f}, README {bm.COOK_UY_README * 1e3:.2f})", flush=True)
+    assert out["converged"] and ro["converged"]
+    assert eD < _bar(precision)
+    assert abs(uy / uyo - 1.0) < _bar(precision)
+    assert abs(uy / bm.COOK_UY_README - 1.0) < 0.01

@@ -624,3 +624,35 @@ def test_incompressible_nu_is_refused(oracle_mod):
     for nu in (0.5, 0.6, -1.0):
         with pytest.raises(ValueError, match="nu"):
             oracle_mod.Case(om, 200e9, nu, [0, 1, 1, 1, 1, 1], bv)
+
+
+def test_backward_d2dt2_is_exact_for_cubic_motion(oracle_mod):
+    # backwardD2dt2Scheme (backwardD2dt2Scheme.C:227-300) applies backward
+    # ddt to D and its two old levels: a five-level second difference exact
+    # for cubics.  A free body under the body force rho (g0 + 6 j t) whose four
+    # old-time levels lie on D(t) = a + v t + g0 t^2 / 2 + j t^3 moves on
+    # exactly that curve, with no strain and no stress; Euler's three-level
+    # difference (exact only for quadratics) leaves it
+    m = oracle_mod.Mesh.box(4, 3, 3, (1.0, 0.8, 0.6), 0.1, 3)
+    c = oracle_mod.Case(m, 200e9, 0.3, [TRACTION] * 6, np.zeros((m.nBoundaryFaces, 3)))
+    c.set("rho", np.full(m.nCells, 7800.0))
+    g0, a, v = np.array([0.0, -9.81, 2.0]), np.array([1e-3, 0.0, -2e-3]), np.array([0.5, 0.2, -0.1])
+    j, dt = np.array([300.0, -150.0, 80.0]), 1e-3
+    curve = lambda t: np.broadcast_to(a + v * t + 0.5 * g0 * t * t + j * t ** 3, (m.nCells, 3))  # noqa: E731
+    for name, k in (("Dold", 0), ("Doo", 1), ("Dooo", 2), ("Doooo", 3)):
+        c.set(name, curve(-k * dt))
+    c.init(curve(0.0))
+    opts = dict(nCorr=200, solTol=1e-13, altTol=1e-13, pcgTol=1e-15, pcgRelTol=0.0, precond=oracle_mod.PRECOND_DIC,
+                deltaT=dt, deltaT0=dt)
+    for n in range(1, 5):
+        if n > 1:
+            c.advance_time()
+        r = c.solve(d2dt2=2, g=tuple(g0 + 6.0 * j * n * dt), **opts)
+        assert r["converged"]
+        exact = curve(n * dt)
+        assert np.abs(c.get("D").reshape(-1, 3) - exact).max() < 1e-9 * np.abs(exact).max(), n
+        assert np.abs(c.get("sigma")).max() < 1e-6 * 7800.0 * 9.81
+    c.advance_time()
+    c.solve(d2dt2=1, g=tuple(g0 + 6.0 * j * 5 * dt), **opts)
+    exact = curve(5 * dt)
+    assert np.abs(c.get("D").reshape(-1, 3) - exact).max() > 1e-7 * np.abs(exact).max()
