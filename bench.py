@@ -113,6 +113,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: OMP_NUM_THREADS or nproc)")
     ap.add_argument("--ref-tol", type=float, default=1e-10,
                     help="outer tolerance of the untimed FP64 reference solve the errors are measured against")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default workgroup tiles instead of the autotuned table "
+                         "(solids4foam_amd/tuning/mi355x_tiles.json)")
     ap.add_argument("--no-batch", action="store_true", help="default run: skip the batched-sweep sub-object")
     ap.add_argument("--no-serve", action="store_true", help="default run: skip the serving sub-object")
     ap.add_argument("--serve-requests", type=int, default=64,
@@ -177,6 +180,8 @@ def main():
         return run_tets(args, ctx)
     t0 = time.time()
     pm, mesh = cases.cube_tension(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
+    from solids4foam_amd import autotune
+    tiles = None if args.no_tune else autotune.apply(mesh)
     setup_s = time.time() - t0
     info = mesh.info()
     C, B = info["nCells"], info["nBoundaryFaces"]
@@ -338,7 +343,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cube {args.n}^3 hex, jitter {args.jitter} seed {args.seed}, z-min fixed, "
                                f"z-max traction 1e7 Pa, linearElastic E=200GPa nu=0.3",
-                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}, maxIter {args.pcg_maxiter}"},
+                   "cells": C, "precision": head, "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}, maxIter {args.pcg_maxiter}",
+                   "tiles": list(tiles) if tiles else "library default (8192 tiles of >= 4 slices)"},
         "cell_iter_per_s": round(C * nOuter * args.steps / total, 1),
         "nOuter": nOuter,
         "pcg_iterations": pcg_it,
@@ -487,13 +493,15 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
                       indent=1, sort_keys=True)
 
 
-def _sequential(ctx, cases_, specs, opts, label):
+def _sequential(ctx, cases_, specs, opts, label, tune=True):
     """Plain FP64 solves of `specs`, one at a time (GAMG preconditioner, the
     fastest single-mesh path): per-solve seconds and fields."""
-    from solids4foam_amd import _abi
+    from solids4foam_amd import _abi, autotune
+    table = autotune.load_table() if tune else {}
     secs, Ds = [], []
     for sp in specs:
         _, m, _ = cases_.box_case(ctx, sp)
+        autotune.apply(m, table)
         m.solve(want_sigma=False, precision=0, preconditioner=_abi.PRECOND_GAMG, **dict(opts, nCorrectors=1))
         t = time.perf_counter()
         out = m.solve(want_sigma=False, precision=0, preconditioner=_abi.PRECOND_GAMG, **opts)
@@ -526,6 +534,11 @@ def batch_run(args, ctx, count, nmax):
     opts = sweep_opts(args)
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
+    if not args.no_tune:
+        from solids4foam_amd import autotune
+        table = autotune.load_table()
+        for _, m, _ in meshes:
+            autotune.apply(m, table)
     b = Batch(ctx)
     for _, m, _ in meshes:
         b.add(m)
@@ -544,7 +557,7 @@ def batch_run(args, ctx, count, nmax):
     print(f"[bench] batched mixed: {T:.2f}s, converged {nconv}/{len(res)}, outer median {np.median(outer):.0f} "
           f"max {outer.max()}", file=sys.stderr, flush=True)
     idx = list(range(min(args.sample, len(specs))))
-    secs, Ds = _sequential(ctx, cases, [specs[i] for i in idx], opts, "sequential fp64")
+    secs, Ds = _sequential(ctx, cases, [specs[i] for i in idx], opts, "sequential fp64", not args.no_tune)
     errs = [rel_l2(res[i]["D"], D) for i, D in zip(idx, Ds)]
     seq_rate = len(secs) / sum(secs)
     value = len(res) / T
@@ -585,6 +598,11 @@ def serve_run(args, ctx, requests, nmax):
     nUnique = len(specs)
     t0 = time.time()
     meshes = [cases.box_case(ctx, sp) for sp in specs]
+    if not args.no_tune:
+        from solids4foam_amd import autotune
+        table = autotune.load_table()
+        for _, m, _ in meshes:
+            autotune.apply(m, table)
     print(f"[bench] serve: {nUnique} meshes, {sum(m.nCells for _, m, _ in meshes)} cells, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     reqs = []
@@ -627,7 +645,7 @@ def serve_run(args, ctx, requests, nmax):
     print(f"[bench] served {len(reqs)} in {wall:.1f}s: {value:.2f} solves/s p50 {np.percentile(lat, 50):.2f}s "
           f"p99 {np.percentile(lat, 99):.2f}s converged {conv} warm {warm}", file=sys.stderr, flush=True)
     idx = [i for i in range(len(reqs)) if not reqs[i][2]][:args.sample]
-    secs, Ds = _sequential(ctx, cases, [specs[reqs[i][0]] for i in idx], opts, "sequential fp64")
+    secs, Ds = _sequential(ctx, cases, [specs[reqs[i][0]] for i in idx], opts, "sequential fp64", not args.no_tune)
     errs = [rel_l2(got[i]["D"], D) for i, D in zip(idx, Ds)]
     seq_rate = len(secs) / sum(secs)
     line = {

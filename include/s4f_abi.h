@@ -164,6 +164,14 @@ s4f_status s4f_mesh_info(const s4f_mesh *mesh, int64_t out[6]);
 s4f_status s4f_mesh_mg_info(const s4f_mesh *mesh, int32_t maxLevels, int32_t *nLevels, int64_t *rows,
                             int64_t *entries, int64_t *qEntries);
 
+/* Workgroup tiles of a mesh: at most maxTiles tiles of at least minSlices
+   64-cell slices each (defaults 8192 and 4; one slice per wave of a
+   256-thread workgroup).  Every kernel over the mesh's cells runs one
+   workgroup per tile, and its reductions sum per tile, so the tiles decide
+   occupancy, cache locality and the finalize kernels' work; an instance keeps
+   its own rule in batches and the server.  Set by the autotuner
+   (solids4foam_amd/autotune.py).  No counterpart in the reference. */
+s4f_status s4f_mesh_tiles_set(s4f_mesh *mesh, int32_t maxTiles, int32_t minSlices);
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
 s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);

@@ -182,6 +182,9 @@ struct HostMesh {
     // on a 64-row slice boundary, padding rows carry no faces
     std::vector<int> instCell0, instNCell, instB0, instNB;
     std::vector<int> instMask;  // per instance: solved components (bit j: component j; 2-D leaves one out)
+    // per instance: workgroup-tile rule (at most tileCap tiles of at least
+    // tileMin slices; s4f_mesh_tiles_set, the autotuner's knob)
+    std::vector<int> instTileCap, instTileMin;
     // dynamics and body force (single meshes)
     std::vector<double> V;        // cell volumes
     std::vector<double> rho;      // density, C values or 1 (empty: not set)
@@ -266,6 +269,7 @@ struct s4f_mesh {
     DArr<int> aInst, aS0, aS1, aBlk0, aNBlk, aFin;   // tiles of the instances still iterating (batches)
     bool materialSet = false;
     unsigned generation = 0;  // bumped when material data changes (serving cache key)
+    unsigned tileGen = 0;     // bumped when the workgroup-tile rule changes
     double rcScale = 0.1, ifScale = 0.01;  // stabilisation scales baked into eC
     std::vector<char> bcSet;
     SolveState st;
@@ -375,6 +379,15 @@ inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1]
 // tiles, for about 10 us more per finalize (its one workgroup reads 4x the
 // partial rows).
 int gMaxBlocks = 8192;
+// default smallest tile: one slice per wave of a workgroup
+constexpr int kMinTileSlices = kBlock / 64;
+
+// tile rule of instance i of h
+inline int tileSlices(const HostMesh& h, int i, int nS) {
+    const int cap = h.instTileCap.empty() ? gMaxBlocks : h.instTileCap[i];
+    const int mn = h.instTileMin.empty() ? kMinTileSlices : h.instTileMin[i];
+    return std::max(mn, (nS + cap - 1) / cap);
+}
 
 
 // Host-side precomputation shared by every entry point: OpenFOAM
@@ -565,6 +578,8 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
     h.instB0 = {0};
     h.instNB = {B};
     h.instMask = {emptyDir < 0 ? 7 : (7 & ~(1 << emptyDir))};
+    h.instTileCap = {gMaxBlocks};
+    h.instTileMin = {kMinTileSlices};
 }
 
 // Rhie-Chow coefficient gamma_f |S_f| deltaCoeffs of every ELL entry
@@ -682,6 +697,8 @@ void applyStabilisationScale(s4f_mesh* M, double rcScale, double ifScale) {
     M->ifScale = ifScale;
 }
 
+void uploadTiles(s4f_mesh* M);
+
 void uploadGeometry(s4f_mesh* M) {
     HostMesh& h = M->h;
     hipStream_t s = M->ctx->stream;
@@ -700,10 +717,16 @@ void uploadGeometry(s4f_mesh* M) {
     M->bLs.upload(h.bLs, s);
     M->bVal.upload(h.bVal, s);
     M->bPres.upload(h.bPres, s);
-    // workgroups: each instance is cut into workgroup tiles by a rule that
-    // depends on that instance alone (>= 4 slices = one per wave, at most
-    // gMaxBlocks tiles), so its reductions -- and therefore its result -- are
-    // bitwise the same whether it is solved alone or packed with others
+    uploadTiles(M);
+}
+
+// Workgroups: each instance is cut into workgroup tiles by a rule that
+// depends on that instance alone (tileSlices: at least one slice per wave,
+// at most its tile cap), so its reductions -- and therefore its result -- are
+// bitwise the same whether it is solved alone or packed with others.
+void uploadTiles(s4f_mesh* M) {
+    HostMesh& h = M->h;
+    hipStream_t s = M->ctx->stream;
     const int nInst = (int)h.instCell0.size();
     std::vector<int> inst, s0, s1, blk0(nInst), nblk(nInst);
     int spb = 0;
@@ -711,7 +734,7 @@ void uploadGeometry(s4f_mesh* M) {
         const int a = h.instCell0[i] / kSlice;
         const int nS = (h.instNCell[i] + kSlice - 1) / kSlice;
         const int e = a + nS;
-        const int spbI = std::max(4, (nS + gMaxBlocks - 1) / gMaxBlocks);
+        const int spbI = tileSlices(h, i, nS);
         spb = std::max(spb, spbI);
         blk0[i] = (int)inst.size();
         for (int x = a; x < e; x += spbI) {
@@ -1814,6 +1837,8 @@ void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P, double rcSc
         P.instB0.push_back((int)B);
         P.instNB.push_back(m->B);
         P.instMask.push_back(m->instMask.empty() ? 7 : m->instMask[0]);
+        P.instTileCap.push_back(m->instTileCap.empty() ? gMaxBlocks : m->instTileCap[0]);
+        P.instTileMin.push_back(m->instTileMin.empty() ? kMinTileSlices : m->instTileMin[0]);
         C += (long long)m->nSlices * kSlice;
         B += m->B;
         E += (long long)m->col.size();
@@ -2047,6 +2072,21 @@ s4f_status s4f_mesh_destroy(s4f_mesh* mesh) {
     delete mesh;
     ctxChildGone(ctx);
     return S4F_OK;
+}
+
+s4f_status s4f_mesh_tiles_set(s4f_mesh* M, int32_t maxTiles, int32_t minSlices) {
+    return guarded([&]() -> s4f_status {
+        if (!M) throw InvalidError("null argument");
+        if (M->h.instCell0.size() != 1) throw InvalidError("tile rules belong to single meshes");
+        if (maxTiles < 1 || minSlices < 1 || minSlices > 1024) throw InvalidError("maxTiles >= 1, 1 <= minSlices <= 1024");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        HIP_CHECK(hipStreamSynchronize(M->ctx->stream));
+        M->h.instTileCap = {maxTiles};
+        M->h.instTileMin = {minSlices};
+        uploadTiles(M);
+        M->tileGen++;  // a batch packing this mesh re-packs
+        return S4F_OK;
+    });
 }
 
 s4f_status s4f_mesh_info(const s4f_mesh* m, int64_t out[6]) {
@@ -2706,7 +2746,7 @@ struct s4f_batch {
     s4f_ctx* ctx = nullptr;
     std::vector<s4f_mesh*> members;   // borrowed; must outlive the solve
     std::unique_ptr<s4f_mesh> packed; // rebuilt when membership, a member's material or the scales change
-    std::vector<unsigned> packedGen;  // member generations the packing was built from
+    std::vector<unsigned> packedGen, packedTileGen;  // member generations the packing was built from
     double packedRc = 0.0, packedIf = 0.0;
     bool dirty = true;
 };
@@ -2747,7 +2787,8 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
         if (opts->d2dt2 != S4F_D2DT2_STEADY) throw InvalidError("batched solves are steady solves");
         HIP_CHECK(hipSetDevice(b->ctx->device));
         bool stale = b->dirty || !b->packed || opts->rhieChowScale != b->packedRc || opts->interfaceScale != b->packedIf;
-        for (size_t i = 0; !stale && i < b->members.size(); ++i) stale = b->members[i]->generation != b->packedGen[i];
+        for (size_t i = 0; !stale && i < b->members.size(); ++i)
+            stale = b->members[i]->generation != b->packedGen[i] || b->members[i]->tileGen != b->packedTileGen[i];
         if (stale) {
             std::vector<const HostMesh*> hs;
             for (s4f_mesh* m : b->members) {
@@ -2765,7 +2806,9 @@ s4f_status s4f_batch_solve(s4f_batch* b, const s4f_solve_opts* opts, double** D_
             P->materialSet = true;
             b->packed = std::move(P);
             b->packedGen.clear();
+            b->packedTileGen.clear();
             for (s4f_mesh* m : b->members) b->packedGen.push_back(m->generation);
+            for (s4f_mesh* m : b->members) b->packedTileGen.push_back(m->tileGen);
             b->packedRc = opts->rhieChowScale;
             b->packedIf = opts->interfaceScale;
             b->dirty = false;
@@ -3170,7 +3213,7 @@ void buildTiles(const s4f_server* S, const std::vector<int>& slots, std::vector<
         const Request& rq = S->live[S->slotReq[slot]];
         const Placement& pl = S->places[rq.place];
         const int a = pl.slice0, nS = pl.nSlices, e = a + nS;
-        const int spbI = std::max(4, (nS + gMaxBlocks - 1) / gMaxBlocks);
+        const int spbI = tileSlices(rq.mesh->h, 0, nS);
         blk0[slot] = (int)inst.size();
         for (int x = a; x < e; x += spbI) {
             inst.push_back(slot);

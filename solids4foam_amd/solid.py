@@ -186,6 +186,10 @@ class SolverMesh:
         b = None if D_old_old is None else f64(D_old_old, (self.nCells, 3))
         check(lib().s4f_old_time_set(self.h, ptr(a), ptr(b)))
 
+    def set_tiles(self, max_tiles, min_slices):
+        """workgroup tiles: at most max_tiles tiles of at least min_slices 64-cell slices"""
+        check(lib().s4f_mesh_tiles_set(self.h, int(max_tiles), int(min_slices)))
+
     def set_old_times(self, levels):
         """D.oldTime(), D.oldTime().oldTime(), ... (1 to 4 fields of (nCells, 3))"""
         a = f64(np.stack([np.asarray(l, np.float64).reshape(self.nCells, 3) for l in levels]))
