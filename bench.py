@@ -288,6 +288,8 @@ def main():
                         traffic=traffic, kernel=dom, alg_bytes_per_launch=kern[dom]["alg_bytes"])
         if traffic:
             roofline["traffic_source"] = os.path.relpath(args.traffic_json, ROOT)
+            roofline["traffic_kind"] = ("L2-egress bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated): "
+                                        "the memory side includes the Infinity Cache, an upper bound of HBM bytes")
     extra = {}
     if "mixed" in res and "fp64" in res:
         lat64, wall64, outs64, _ = res["fp64"]
@@ -374,9 +376,10 @@ PMC_TAGS = {"residual": ("k_residual",), "spmv": ("k_spmvp",), "pcg_update": ("k
 
 
 def load_traffic(path, kernel, precision):
-    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled) of
-    `kernel` from a committed rocprof summary (JSON written by --pmc-json),
-    or None."""
+    """Bytes per launch between L2 and memory (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel scaled; the memory side includes the Infinity Cache, so an
+    upper bound of the HBM bytes) of `kernel` from a committed rocprof
+    summary (JSON written by --pmc-json), or None."""
     try:
         with open(path) as f:
             d = json.load(f)
@@ -533,12 +536,10 @@ def batch_run(args, ctx, count, nmax):
     specs = cases.box_sweep_specs(count, args.seed, 8, nmax)
     opts = sweep_opts(args)
     t0 = time.time()
+    # batch members keep the library's tile rule: the tuned table is for a
+    # mesh solved alone, where small meshes under-fill the chip; packed
+    # together they fill it, and smaller tiles would only idle waves
     meshes = [cases.box_case(ctx, sp) for sp in specs]
-    if not args.no_tune:
-        from solids4foam_amd import autotune
-        table = autotune.load_table()
-        for _, m, _ in meshes:
-            autotune.apply(m, table)
     b = Batch(ctx)
     for _, m, _ in meshes:
         b.add(m)
@@ -597,12 +598,7 @@ def serve_run(args, ctx, requests, nmax):
     specs, stream, arrivals = cases.serve_requests(requests, args.seed, 8, nmax, args.repeat, args.rate)
     nUnique = len(specs)
     t0 = time.time()
-    meshes = [cases.box_case(ctx, sp) for sp in specs]
-    if not args.no_tune:
-        from solids4foam_amd import autotune
-        table = autotune.load_table()
-        for _, m, _ in meshes:
-            autotune.apply(m, table)
+    meshes = [cases.box_case(ctx, sp) for sp in specs]  # served together: the library's tile rule
     print(f"[bench] serve: {nUnique} meshes, {sum(m.nCells for _, m, _ in meshes)} cells, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     reqs = []

@@ -8,8 +8,10 @@ solver's kernels on a live mesh for each candidate rule (back-to-back
 replays, s4f_kernel_time), weights them by their launches per outer
 iteration of a multigrid-preconditioned PCG solve and keeps the cheapest;
 `main` tunes a ladder of cube sizes and writes the table that `apply` reads
-(keyed by cell count, nearest power of two).  No counterpart in the
-reference, whose loops have no launch geometry.
+(keyed by cell count, nearest power of two).  The table is for a mesh
+solved alone: packed into a batch or served with others, small meshes fill
+the chip together and keep the library's default rule.  No counterpart in
+the reference, whose loops have no launch geometry.
 
     python -m solids4foam_amd.autotune --sizes 16 32 64 100 128 200 --out solids4foam_amd/tuning/mi355x_tiles.json
 """
