@@ -172,6 +172,22 @@ s4f_status s4f_mesh_mg_info(const s4f_mesh *mesh, int32_t maxLevels, int32_t *nL
    its own rule in batches and the server.  Set by the autotuner
    (solids4foam_amd/autotune.py).  No counterpart in the reference. */
 s4f_status s4f_mesh_tiles_set(s4f_mesh *mesh, int32_t maxTiles, int32_t minSlices);
+
+/* Material zones of a multi-law case: mechanicalProperties with one
+   mechanical law per cellZone (mechanicalModel.C:571-615 grad(), :477-498
+   correct(), :790-877 interpolate(); solidSubMeshes.C:47-110,1550-2031).
+   Each zone is solved on its own sub-mesh: an internal face between two
+   zones becomes a boundary face of both sub-meshes with an interface
+   displacement, updated every outer iteration toward traction balance, that
+   each side's least-squares gradient and boundary stress see; pointD is the
+   mean of the sub-meshes' interpolations at points shared by materials.
+   cellZone: the zone of every cell (nCells values; the material itself is
+   still set per cell by s4f_material_set), or NULL for one material.
+   faceCentres: the mesh's face centres, 3 values per face in face order (at
+   least the nInternalFaces internal ones).  The interface state starts at
+   zero; a warm or transient solve continues it.  A mesh with zones is
+   solved alone (batches and the server refuse it). */
+s4f_status s4f_mesh_zones_set(s4f_mesh *mesh, const int32_t *cellZone, const double *faceCentres);
 /* linearElastic (E, nu) per cell (perCell != 0) or uniform (E[0], nu[0]);
    linearElastic.C:62-133 (mu, lambda, K), :204-245 (impK). */
 s4f_status s4f_material_set(s4f_mesh *mesh, const double *E, const double *nu, int32_t perCell);

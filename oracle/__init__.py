@@ -78,6 +78,10 @@ def lib():
         L.or_reference_source.argtypes = [vp, ctypes.c_int, dp]
         L.or_amul.argtypes = [vp, dp, dp]
         L.or_sigma_eq.argtypes = [vp, dp]
+        L.or_iface_sigma.argtypes = [vp]
+        L.or_iface_update.argtypes = [vp]
+        L.or_case_set_zones.restype = ctypes.c_int
+        L.or_case_set_zones.argtypes = [vp, ip]
         _lib = L
     return _lib
 
@@ -205,6 +209,22 @@ class Case:
         a = np.ascontiguousarray(np.asarray(value, np.float64).reshape(-1))
         assert a.size == n, (name, a.size, n)
         lib().or_set_field(self.h, name.encode(), _dp(a))
+
+    def set_zones(self, zone):
+        """Material zone per cell of a multi-law case (None: one material):
+        each zone is solved on its own sub-mesh with interface displacements
+        between them (solidSubMeshes).  Resets the interface state; returns
+        the number of interface faces."""
+        if zone is None:
+            self._zone = None
+            return lib().or_case_set_zones(self.h, None)
+        self._zone = np.ascontiguousarray(np.asarray(zone, np.int32).reshape(self.mesh.nCells))
+        return lib().or_case_set_zones(self.h, _ip(self._zone))
+
+    def iface_step(self, name):
+        """one interface step on the current state: "sigma" (the sub-meshes'
+        interface stress) or "update" (D_I from D.prevIter() to D)"""
+        getattr(lib(), "or_iface_" + name)(self.h)
 
     def init(self, D0=None):
         if D0 is None:

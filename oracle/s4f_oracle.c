@@ -23,6 +23,9 @@
  *   src/solids4FoamModels/solidModels/fvPatchFields/fixedDisplacement/
  *       fixedDisplacementFvPatchVectorField.C:258-356
  *   src/solids4FoamModels/numerics/patchCorrectionVectors/patchCorrectionVectors.C:24-36
+ *   src/solids4FoamModels/materialModels/mechanicalModel/mechanicalModel.C:571-615,790-823
+ *   src/solids4FoamModels/materialModels/mechanicalModel/solidSubMeshes/
+ *       solidSubMeshes.C:1550-1950,1952-2031, solidSubMeshesTemplates.C:30-92,271-336
  *   src/solids4FoamModels/numerics/extendedLeastSquaresGrad/
  *       extendedLeastSquaresVectors.C:123-285, extendedLeastSquaresGrad.C:103-167
  *
@@ -122,6 +125,17 @@ typedef struct {
     double *diag0;                   /* the steady matrix diagonal (the inertia term is added per solve) */
     double *aAlphaB, *aResB;         /* their boundary values (nB, nB*3) */
     int iCorr;
+    /* bi-material interfaces of a multi-law case (solidSubMeshes; set by
+       or_case_set_zones): each material is its own sub-mesh, the internal
+       faces between cells of different zones are boundary faces of both */
+    int *zone;                       /* material zone of each cell, NULL: one material */
+    int nIf;                         /* interface faces */
+    int *ifFace;                     /* nIf: their internal-face indices */
+    int *faceIf;                     /* internal faces: interface index or -1 */
+    double *ifLs;                    /* nIf*6: least-squares boundary vectors, side a (owner) then side b */
+    double *ifDc;                    /* nIf*2: sub-mesh patch deltaCoeffs 1/|n.(Cf - C)|, side a then b */
+    double *ifDI;                    /* nIf*3: interface displacement (the two sub-meshes hold equal values) */
+    double *ifSig;                   /* nIf*12: sub-mesh boundary stress at the face, side a then b */
 } or_case;
 
 /* ------------------------------------------------------------------ */
@@ -534,6 +548,102 @@ static void patch_delta(const or_case *c, int b, int full, double *d)
     for (int q = 0; q < 3; ++q) d[q] = n[q]*nd;
 }
 
+/* cell of side a (owner, side 0) or side b (neighbour, side 1) of interface face i */
+static inline int if_cell(const or_case *c, int i, int side)
+{
+    int f = c->ifFace[i];
+    return side ? c->m->neighbour[f] : c->m->owner[f];
+}
+
+/* the sub-mesh patch delta of interface face i seen from one side: Cf - C
+   (lsFullDelta) or its normal part n (n.(Cf - C)), as patch_delta does for
+   boundary faces; the sign of the sub-mesh normal cancels in n n */
+static void if_delta(const or_case *c, int i, int side, double *d)
+{
+    const or_mesh *m = c->m;
+    int f = c->ifFace[i], P = if_cell(c, i, side);
+    double dl[3]; for (int q = 0; q < 3; ++q) dl[q] = m->Cf[3*f+q] - m->C[3*P+q];
+    if (c->lsFullDelta) { for (int q = 0; q < 3; ++q) d[q] = dl[q]; return; }
+    double n[3]; for (int q = 0; q < 3; ++q) n[q] = m->Sf[3*f+q]/m->magSf[f];
+    double nd = dot3(n, dl);
+    for (int q = 0; q < 3; ++q) d[q] = n[q]*nd;
+}
+
+/* [OF leastSquaresVectors] (the patch-delta switch: lsFullDelta) of the
+   mesh, or of every sub-mesh when zones are set: a bi-material interface
+   face is then a boundary face of the sub-mesh on each side, with that
+   side's patch delta (solidSubMeshes.C:47-110 builds the sub-meshes,
+   mechanicalModel.C:584-593 takes fvc::grad on each) */
+static void ls_vectors(or_case *c)
+{
+    const or_mesh *m = c->m;
+    const int C = m->nCells, F = m->nInternalFaces, B = c->nB;
+    double *dd = xcalloc((size_t)C*6, sizeof(double));
+    for (int f = 0; f < F; ++f) {
+        if (c->faceIf && c->faceIf[f] >= 0) continue;  /* a boundary face of each sub-mesh, below */
+        int o = m->owner[f], n = m->neighbour[f];
+        double d[3]; for (int q = 0; q < 3; ++q) d[q] = m->C[3*n+q] - m->C[3*o+q];
+        double md2 = dot3(d, d);
+        double wdd[6] = { d[0]*d[0]/md2, d[0]*d[1]/md2, d[0]*d[2]/md2, d[1]*d[1]/md2, d[1]*d[2]/md2, d[2]*d[2]/md2 };
+        for (int q = 0; q < 6; ++q) { dd[6*o+q] += wdd[q]; dd[6*n+q] += wdd[q]; }
+    }
+    for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;  /* empty faces carry no values */
+        int P = bCell(c, b);
+        double d[3]; patch_delta(c, b, c->lsFullDelta, d);
+        double md2 = dot3(d, d);
+        double wdd[6] = { d[0]*d[0]/md2, d[0]*d[1]/md2, d[0]*d[2]/md2, d[1]*d[1]/md2, d[1]*d[2]/md2, d[2]*d[2]/md2 };
+        for (int q = 0; q < 6; ++q) dd[6*P+q] += wdd[q];
+    }
+    for (int i = 0; i < c->nIf; ++i)
+        for (int side = 0; side < 2; ++side) {
+            int P = if_cell(c, i, side);
+            double d[3]; if_delta(c, i, side, d);
+            double md2 = dot3(d, d);
+            double wdd[6] = { d[0]*d[0]/md2, d[0]*d[1]/md2, d[0]*d[2]/md2, d[1]*d[1]/md2, d[1]*d[2]/md2, d[2]*d[2]/md2 };
+            for (int q = 0; q < 6; ++q) dd[6*P+q] += wdd[q];
+        }
+    double *invDd = xcalloc((size_t)C*6, sizeof(double));
+    /* 2-D: dd has a zero row and column along the empty direction; invert
+       dd + e e (e that axis) and take e e off the inverse again, so the
+       gradient has no component along it */
+    static const int diagIdx[3] = {0, 3, 5};
+    for (int i = 0; i < C; ++i) {
+        if (c->emptyDir >= 0) dd[6*i + diagIdx[c->emptyDir]] += 1.0;
+        inv_symm(dd + 6*i, invDd + 6*i);
+        if (c->emptyDir >= 0) invDd[6*i + diagIdx[c->emptyDir]] -= 1.0;
+    }
+    for (int f = 0; f < F; ++f) {
+        if (c->faceIf && c->faceIf[f] >= 0) {
+            for (int q = 0; q < 3; ++q) { c->lsP[3*f+q] = 0.0; c->lsN[3*f+q] = 0.0; }
+            continue;
+        }
+        int o = m->owner[f], n = m->neighbour[f];
+        double d[3]; for (int q = 0; q < 3; ++q) d[q] = m->C[3*n+q] - m->C[3*o+q];
+        double r = 1.0/dot3(d, d);
+        double t[3];
+        symm_dot_v(invDd + 6*o, d, t); for (int q = 0; q < 3; ++q) c->lsP[3*f+q] = r*t[q];
+        symm_dot_v(invDd + 6*n, d, t); for (int q = 0; q < 3; ++q) c->lsN[3*f+q] = -r*t[q];
+    }
+    for (int b = 0; b < B; ++b) {
+        if (c->bType[b] == OR_EMPTY) continue;
+        int P = bCell(c, b);
+        double d[3]; patch_delta(c, b, c->lsFullDelta, d);
+        double r = 1.0/dot3(d, d), t[3];
+        symm_dot_v(invDd + 6*P, d, t);
+        for (int q = 0; q < 3; ++q) c->lsB[3*b+q] = r*t[q];
+    }
+    for (int i = 0; i < c->nIf; ++i)
+        for (int side = 0; side < 2; ++side) {
+            int P = if_cell(c, i, side);
+            double d[3]; if_delta(c, i, side, d);
+            double r = 1.0/dot3(d, d), t[3];
+            symm_dot_v(invDd + 6*P, d, t);
+            for (int q = 0; q < 3; ++q) c->ifLs[6*i + 3*side + q] = r*t[q];
+        }
+    free(dd); free(invDd);
+}
+
 or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, const int *patchType,
                         const double *bValue, const double *bPressure, int lsFullDelta,
                         double rcScale, double interfaceScale, int planeStress)
@@ -607,52 +717,9 @@ or_case *or_case_create(const or_mesh *m, const double *E, const double *nu, con
         c->bDelta[b] = 1.0/mag3(dn);
     }
 
-    /* [OF leastSquaresVectors] with the patch-delta switch */
-    double *dd = xcalloc((size_t)C*6, sizeof(double));
-    for (int f = 0; f < F; ++f) {
-        int o = m->owner[f], n = m->neighbour[f];
-        double d[3]; for (int q = 0; q < 3; ++q) d[q] = m->C[3*n+q] - m->C[3*o+q];
-        double md2 = dot3(d, d);
-        double wdd[6] = { d[0]*d[0]/md2, d[0]*d[1]/md2, d[0]*d[2]/md2, d[1]*d[1]/md2, d[1]*d[2]/md2, d[2]*d[2]/md2 };
-        for (int q = 0; q < 6; ++q) { dd[6*o+q] += wdd[q]; dd[6*n+q] += wdd[q]; }
-    }
-    for (int b = 0; b < B; ++b) {
-        if (c->bType[b] == OR_EMPTY) continue;  /* empty faces carry no values */
-        int P = bCell(c, b);
-        double d[3]; patch_delta(c, b, lsFullDelta, d);
-        double md2 = dot3(d, d);
-        double wdd[6] = { d[0]*d[0]/md2, d[0]*d[1]/md2, d[0]*d[2]/md2, d[1]*d[1]/md2, d[1]*d[2]/md2, d[2]*d[2]/md2 };
-        for (int q = 0; q < 6; ++q) dd[6*P+q] += wdd[q];
-    }
-    double *invDd = xcalloc((size_t)C*6, sizeof(double));
-    /* 2-D: dd has a zero row and column along the empty direction; invert
-       dd + e e (e that axis) and take e e off the inverse again, so the
-       gradient has no component along it */
-    static const int diagIdx[3] = {0, 3, 5};
-    for (int i = 0; i < C; ++i) {
-        if (c->emptyDir >= 0) dd[6*i + diagIdx[c->emptyDir]] += 1.0;
-        inv_symm(dd + 6*i, invDd + 6*i);
-        if (c->emptyDir >= 0) invDd[6*i + diagIdx[c->emptyDir]] -= 1.0;
-    }
     c->lsP = xcalloc((size_t)F*3, sizeof(double)); c->lsN = xcalloc((size_t)F*3, sizeof(double));
     c->lsB = xcalloc((size_t)B*3, sizeof(double));
-    for (int f = 0; f < F; ++f) {
-        int o = m->owner[f], n = m->neighbour[f];
-        double d[3]; for (int q = 0; q < 3; ++q) d[q] = m->C[3*n+q] - m->C[3*o+q];
-        double r = 1.0/dot3(d, d);
-        double t[3];
-        symm_dot_v(invDd + 6*o, d, t); for (int q = 0; q < 3; ++q) c->lsP[3*f+q] = r*t[q];
-        symm_dot_v(invDd + 6*n, d, t); for (int q = 0; q < 3; ++q) c->lsN[3*f+q] = -r*t[q];
-    }
-    for (int b = 0; b < B; ++b) {
-        if (c->bType[b] == OR_EMPTY) continue;
-        int P = bCell(c, b);
-        double d[3]; patch_delta(c, b, lsFullDelta, d);
-        double r = 1.0/dot3(d, d), t[3];
-        symm_dot_v(invDd + 6*P, d, t);
-        for (int q = 0; q < 3; ++q) c->lsB[3*b+q] = r*t[q];
-    }
-    free(dd); free(invDd);
+    ls_vectors(c);
 
     /* impKf = interpolate(impK) (mechanicalModel.C:409-415);
        gammaf: momentumStabilisation.C:111-150 */
@@ -713,7 +780,48 @@ void or_case_free(or_case *c)
     free(c->D); free(c->Dprev); free(c->Dold); free(c->Db); free(c->G); free(c->Gb);
     free(c->sig); free(c->sigb); free(c->g); free(c->snb); free(c->R);
     free(c->aAlpha); free(c->aRes); free(c->aAlphaB); free(c->aResB);
+    free(c->zone); free(c->ifFace); free(c->faceIf); free(c->ifLs); free(c->ifDc); free(c->ifDI); free(c->ifSig);
     free(c);
+}
+
+/* Material zones of a multi-law case (mechanicalModel with one law per
+   cellZone): every zone becomes a sub-mesh (solidSubMeshes.C:47-110) and
+   the internal faces between zones become bi-material interface faces,
+   boundary faces of the sub-mesh on either side.  The interface
+   displacement and the sub-mesh boundary stress start at zero, as the
+   sub-mesh fields are constructed (solidSubMeshes.C:274-321).  zone NULL
+   returns to one material.  Returns the number of interface faces. */
+int or_case_set_zones(or_case *c, const int *zone)
+{
+    const or_mesh *m = c->m;
+    const int C = m->nCells, F = m->nInternalFaces;
+    free(c->zone); free(c->ifFace); free(c->faceIf); free(c->ifLs); free(c->ifDc); free(c->ifDI); free(c->ifSig);
+    c->zone = NULL; c->ifFace = NULL; c->faceIf = NULL; c->ifLs = c->ifDc = c->ifDI = c->ifSig = NULL;
+    c->nIf = 0;
+    if (zone) {
+        c->zone = xcalloc(C, sizeof(int)); memcpy(c->zone, zone, sizeof(int)*C);
+        c->faceIf = xcalloc(F, sizeof(int));
+        for (int f = 0; f < F; ++f) c->faceIf[f] = zone[m->owner[f]] != zone[m->neighbour[f]] ? c->nIf++ : -1;
+        c->ifFace = xcalloc(c->nIf, sizeof(int));
+        for (int f = 0; f < F; ++f) if (c->faceIf[f] >= 0) c->ifFace[c->faceIf[f]] = f;
+        c->ifLs = xcalloc((size_t)c->nIf*6, sizeof(double));
+        c->ifDc = xcalloc((size_t)c->nIf*2, sizeof(double));
+        c->ifDI = xcalloc((size_t)c->nIf*3, sizeof(double));
+        c->ifSig = xcalloc((size_t)c->nIf*12, sizeof(double));
+        for (int i = 0; i < c->nIf; ++i) {
+            int f = c->ifFace[i];
+            double n[3]; for (int q = 0; q < 3; ++q) n[q] = m->Sf[3*f+q]/m->magSf[f];
+            for (int side = 0; side < 2; ++side) {
+                int P = if_cell(c, i, side);
+                double dl[3]; for (int q = 0; q < 3; ++q) dl[q] = m->Cf[3*f+q] - m->C[3*P+q];
+                /* [OF] fvPatch deltaCoeffs of a non-coupled patch: 1/|n (n.(Cf - Cn))| */
+                c->ifDc[2*i+side] = 1.0/fabs(dot3(n, dl));
+            }
+        }
+        if (c->nIf == 0) { free(c->faceIf); c->faceIf = NULL; }
+    }
+    ls_vectors(c);
+    return c->nIf;
 }
 
 /* ------------------------------------------------------------------ */
@@ -730,6 +838,7 @@ void or_grad(or_case *c)
     const int C = m->nCells, F = m->nInternalFaces, B = c->nB;
     memset(c->G, 0, sizeof(double)*9*C);
     for (int f = 0; f < F; ++f) {
+        if (c->faceIf && c->faceIf[f] >= 0) continue;
         int o = m->owner[f], n = m->neighbour[f];
         double dv[3]; for (int q = 0; q < 3; ++q) dv[q] = c->D[3*n+q] - c->D[3*o+q];
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
@@ -743,6 +852,15 @@ void or_grad(or_case *c)
         double dv[3]; for (int q = 0; q < 3; ++q) dv[q] = c->Db[3*b+q] - c->D[3*P+q];
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->G[9*P + 3*i + j] += c->lsB[3*b+i]*dv[j];
     }
+    /* sub-meshes: each side of an interface face sees the interface
+       displacement as a boundary value (mechanicalModel.C:584-593) */
+    for (int k = 0; k < c->nIf; ++k)
+        for (int side = 0; side < 2; ++side) {
+            int P = if_cell(c, k, side);
+            double dv[3]; for (int q = 0; q < 3; ++q) dv[q] = c->ifDI[3*k+q] - c->D[3*P+q];
+            const double *l = c->ifLs + 6*k + 3*side;
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->G[9*P + 3*i + j] += l[i]*dv[j];
+        }
     for (int b = 0; b < B; ++b) {
         if (c->bType[b] == OR_EMPTY) continue;
         int P = bCell(c, b);
@@ -774,6 +892,70 @@ void or_sigma(or_case *c)
     for (int i = 0; i < m->nCells; ++i) hooke(c->G + 9*i, c->mu[i], c->K[i], c->sig + 6*i);
     for (int b = 0; b < c->nB; ++b) { int P = bCell(c, b); hooke(c->Gb + 9*b, c->mu[P], c->K[P], c->sigb + 6*b); }
 }
+
+/* solidSubMeshes::interpolateDtoSubMeshD (solidSubMeshes.C:1550-1950), a
+   bi-material interface face with side a the base owner, side b the
+   neighbour, n = Sf/|Sf| (a to b):
+     D_I += wab (n.sigma_b - n.sigma_a) + wa (D_a - Dprev_a) + wb (D_b - Dprev_b)
+   da = |n.(Cf - C_a)|, db = |n.(C_b - Cf)|, Ka, Kb = impK of the two cells,
+   wab = da db/(db Ka + da Kb), wa = db Ka/(db Ka + da Kb), wb = 1 - wa;
+   sigma_a, sigma_b are the sub-meshes' boundary stresses at the face from
+   their last correct() (the shadow stress is the other side's), D the field
+   after the solve and relaxation, Dprev = D.prevIter().  Seen from sub-mesh
+   b the normal, the weights and the sides swap and the update is the same,
+   so one value serves both sub-meshes (mechanicalModel.C:584-587 calls it
+   from grad(), mechanicalModel.C:790-823 once more after the loop). */
+static void iface_update(or_case *c, const double *D, const double *Dprev)
+{
+    const or_mesh *m = c->m;
+    for (int i = 0; i < c->nIf; ++i) {
+        const int f = c->ifFace[i], a = m->owner[f], b = m->neighbour[f];
+        double n[3]; for (int q = 0; q < 3; ++q) n[q] = m->Sf[3*f+q]/m->magSf[f];
+        const double da = 1.0/c->ifDc[2*i], db = 1.0/c->ifDc[2*i+1];
+        const double Ka = c->impK[a], Kb = c->impK[b];
+        const double wab = da*db/(db*Ka + da*Kb);
+        const double wa = db*Ka/(db*Ka + da*Kb), wb = 1.0 - wa;
+        double ta[3], tb[3];
+        symm_dot_v(c->ifSig + 12*i, n, ta);
+        symm_dot_v(c->ifSig + 12*i + 6, n, tb);
+        for (int q = 0; q < 3; ++q)
+            c->ifDI[3*i+q] += wab*(tb[q] - ta[q]) + wa*(D[3*a+q] - Dprev[3*a+q]) + wb*(D[3*b+q] - Dprev[3*b+q]);
+    }
+}
+
+/* The sub-meshes' boundary stress at the interface faces: gradient
+   G_I = G_P + n_s ((D_I - D_P) deltaCoeffs - n_s.G_P) (fvc::grad's
+   boundary value, then solidSubMeshes::correctBoundarySnGrad,
+   solidSubMeshes.C:1952-2031, whose correction vector
+   k = delta - n (n.delta) vanishes for the normal patch delta), n_s the
+   sub-mesh's outward normal (+n on side a, -n on side b), then the law's
+   Hooke's law on that sub-mesh (mechanicalModel.C:477-498). */
+static void iface_sigma(or_case *c)
+{
+    const or_mesh *m = c->m;
+    for (int i = 0; i < c->nIf; ++i) {
+        const int f = c->ifFace[i];
+        for (int side = 0; side < 2; ++side) {
+            const int P = if_cell(c, i, side);
+            const double sg = side ? -1.0 : 1.0;
+            double n[3]; for (int q = 0; q < 3; ++q) n[q] = sg*m->Sf[3*f+q]/m->magSf[f];
+            const double *GP = c->G + 9*P;
+            double nG[3]; vdotT(n, GP, nG);
+            double GI[9];
+            for (int j = 0; j < 3; ++j) {
+                const double sn = (c->ifDI[3*i+j] - c->D[3*P+j])*c->ifDc[2*i+side];
+                for (int k = 0; k < 3; ++k) GI[3*k+j] = GP[3*k+j] + n[k]*(sn - nG[j]);
+            }
+            hooke(GI, c->mu[P], c->K[P], c->ifSig + 12*i + 6*side);
+        }
+    }
+}
+
+/* the two interface steps on their own, for tests: the sub-meshes'
+   interface stress from the current gradient and D_I, and one interface
+   update from D.prevIter() (Dprev) to D */
+void or_iface_sigma(or_case *c) { iface_sigma(c); }
+void or_iface_update(or_case *c) { iface_update(c, c->D, c->Dprev); }
 
 /* solidTractionFvPatchVectorField::updateCoeffs (:321-395) with
    linGeomTotalDispSolid::tractionBoundarySnGrad (:235-271), relaxFac 1.
@@ -1052,12 +1234,19 @@ void or_init(or_case *c, const double *D0)
     memset(c->G, 0, sizeof(double)*9*C);
     memset(c->g, 0, sizeof(double)*3*B);
     or_correct_boundary(c);
+    /* sub-meshes: the interface displacement and boundary stress start at
+       zero; the constructor's grad() leaves D_I unchanged (D == prevIter,
+       zero stress) */
+    if (c->nIf) {
+        memset(c->ifDI, 0, sizeof(double)*3*c->nIf);
+        memset(c->ifSig, 0, sizeof(double)*12*c->nIf);
+    }
     or_grad(c);
     /* the constructor computes no stress: a cold start keeps the zero sigma
        field until the first mechanical().correct(), so the first residual
        and traction update see sigma = 0; a restart (D0 given) reads the
        stress written with D, i.e. Hooke's law of its gradient */
-    if (D0) or_sigma(c);
+    if (D0) { or_sigma(c); iface_sigma(c); }
     else {
         memset(c->sig, 0, sizeof(double)*6*C);
         memset(c->sigb, 0, sizeof(double)*6*B);
@@ -1069,7 +1258,12 @@ void or_init(or_case *c, const double *D0)
    mechanicalModel.C:829-845 -> enhancedVolPointInterpolation::interpolate with
    the gradient, enhancedVolPointInterpolate.C:350-420): for every point p,
    sum over its cells c (ascending) of w (D_c + (x_p - C_c).G_c), w = 1/|x_p - C_c|,
-   divided by the sum of w.  D (C*3) and G (C*9) are given; out (P*3). */
+   divided by the sum of w.  D (C*3) and G (C*9) are given; out (P*3).
+   With zones (mechanicalModel.C:829-845 with more than one law) each
+   sub-mesh interpolates from its own cells, and a point takes the mean of
+   the values of the materials around it (mapSubMeshPointFields,
+   solidSubMeshesTemplates.C:271-336: value/noMat summed over the
+   sub-meshes, here in the order of their first cell in the point's list). */
 void or_point_interpolate(const or_case *c, const double *D, const double *G, double *out)
 {
     const or_mesh *m = c->m;
@@ -1095,18 +1289,34 @@ void or_point_interpolate(const or_case *c, const double *D, const double *G, do
             while (j >= a && lst[j] > v) { lst[j+1] = lst[j]; --j; }
             lst[j+1] = v;
         }
-        double acc[3] = {0, 0, 0}, sw = 0.0;
         const double *x = m->points + 3*(size_t)p;
+        /* the materials around the point, in the order of their first cell */
+        int zl[64], nz = 0;
         for (int i = a; i < b; ++i) {
-            if (i > a && lst[i] == lst[i-1]) continue;
-            int cc = lst[i];
-            double d[3] = {x[0] - m->C[3*cc], x[1] - m->C[3*cc+1], x[2] - m->C[3*cc+2]};
-            double w = 1.0/sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
-            for (int j = 0; j < 3; ++j)
-                acc[j] += w*(D[3*cc+j] + d[0]*G[9*cc+j] + d[1]*G[9*cc+3+j] + d[2]*G[9*cc+6+j]);
-            sw += w;
+            const int z = c->zone ? c->zone[lst[i]] : 0;
+            int seen = 0;
+            for (int k = 0; k < nz; ++k) seen |= zl[k] == z;
+            if (!seen && nz < 64) zl[nz++] = z;
         }
-        for (int j = 0; j < 3; ++j) out[3*(size_t)p+j] = sw > 1e-300 ? acc[j]/sw : acc[j];
+        double res[3] = {0, 0, 0};
+        for (int k = 0; k < nz; ++k) {
+            double acc[3] = {0, 0, 0}, sw = 0.0;
+            for (int i = a; i < b; ++i) {
+                if (i > a && lst[i] == lst[i-1]) continue;
+                int cc = lst[i];
+                if (c->zone && c->zone[cc] != zl[k]) continue;
+                double d[3] = {x[0] - m->C[3*cc], x[1] - m->C[3*cc+1], x[2] - m->C[3*cc+2]};
+                double w = 1.0/sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
+                for (int j = 0; j < 3; ++j)
+                    acc[j] += w*(D[3*cc+j] + d[0]*G[9*cc+j] + d[1]*G[9*cc+3+j] + d[2]*G[9*cc+6+j]);
+                sw += w;
+            }
+            for (int j = 0; j < 3; ++j) {
+                const double v = sw > 1e-300 ? acc[j]/sw : acc[j];
+                res[j] = nz == 1 ? v : res[j] + v/nz;
+            }
+        }
+        for (int j = 0; j < 3; ++j) out[3*(size_t)p+j] = res[j];
     }
     free(cnt); free(lst); free(fill);
 }
@@ -1250,8 +1460,10 @@ int or_iterate(or_case *c, const or_opts *o, double *initResOut, double *rvfOut,
         }
         free(DbPrev);
     }
+    iface_update(c, c->D, c->Dprev);                        /* sub-meshes: interface displacement */
     or_grad(c);                                             /* mechanical().grad(D, gradD) */
     or_sigma(c);                                            /* mechanical().correct(sigma) */
+    iface_sigma(c);                                         /* sub-meshes: their interface stress */
     /* solidModel::converged (solidModelTemplates.C:26-188), total approach */
     double initRes = sqrt(ir[0]*ir[0] + ir[1]*ir[1] + ir[2]*ir[2]);
     int nIter = nit[0] > nit[1] ? nit[0] : nit[1]; if (nit[2] > nIter) nIter = nit[2];
@@ -1289,6 +1501,10 @@ int or_solve(or_case *c, const or_opts *o, or_report *rep)
         if (!isfinite(ir) || !isfinite(rvf)) { rep->converged = 0; return -1; }
         if (!conv && c->iCorr == o->nCorr - 1) rep->maxIterReached = 1;
     } while (!conv && ++c->iCorr < o->nCorr);
+    /* mechanical().interpolate(D, gradD, pointD) after the loop
+       (linGeomTotalDispSolid.C:211-212) updates the interface displacement
+       once more (mechanicalModel.C:851-852) */
+    iface_update(c, c->D, c->Dprev);
     rep->converged = conv;
     return conv;
 }
@@ -1333,6 +1549,8 @@ static double *field_ptr(or_case *c, const char *name, size_t *n)
     F_("aitkenAlpha", c->aAlpha, C) F_("aitkenResidual", c->aRes, 3*C)
     F_("rho", c->rho, C) F_("Doo", c->Doo, 3*C) F_("Dooo", c->Dooo, 3*C) F_("Doooo", c->Doooo, 3*C)
     F_("aitkenAlphab", c->aAlphaB, B) F_("aitkenResidualb", c->aResB, 3*B)
+    F_("ifDI", c->ifDI, 3*c->nIf) F_("ifSigma", c->ifSig, 12*c->nIf) F_("ifLs", c->ifLs, 6*c->nIf)
+    F_("ifDc", c->ifDc, 2*c->nIf)
     #undef F_
     *n = 0; return NULL;
 }
@@ -1570,7 +1788,7 @@ int or_iterate_omp(or_case *c, or_omp *p, const or_opts *o, double *initResOut, 
 {
     const or_mesh *m = c->m;
     const int C = m->nCells, F = m->nInternalFaces, B = c->nB;
-    if (o->d2dt2 || o->relaxAitken || (o->relaxD > 0.0 && o->relaxD != 1.0) || c->emptyDir >= 0) return -1;
+    if (o->d2dt2 || o->relaxAitken || (o->relaxD > 0.0 && o->relaxD != 1.0) || c->emptyDir >= 0 || c->nIf) return -1;
     #pragma omp parallel for schedule(static) num_threads(p->nT)
     for (int i = 0; i < 3*C; ++i) c->Dprev[i] = c->D[i];
     or_update_traction(c);

@@ -84,7 +84,7 @@ EXPORTED = [
     "s4f_server_destroy", "s4f_ldu_create", "s4f_ldu_solve", "s4f_ldu_destroy",
     "s4f_polymesh_point_cells", "s4f_strain_fields", "s4f_point_displacement",
     "s4f_density_set", "s4f_gravity_set", "s4f_old_time_set", "s4f_mesh_mg_info", "s4f_plane_stress_set",
-    "s4f_pcg3_noalloc", "s4f_old_times_set", "s4f_mesh_tiles_set",
+    "s4f_pcg3_noalloc", "s4f_old_times_set", "s4f_mesh_tiles_set", "s4f_mesh_zones_set",
 ]
 
 
@@ -131,6 +131,7 @@ def lib():
         L.s4f_old_time_set.argtypes = [vp, dp, dp]
         L.s4f_old_times_set.argtypes = [vp, ctypes.c_int32, dp]
         L.s4f_mesh_tiles_set.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+        L.s4f_mesh_zones_set.argtypes = [vp, vp, dp]
         L.s4f_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
         L.s4f_solve_opts_default.restype = None
         L.s4f_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), dp, dp, dp, dp, ctypes.POINTER(SolveReport)]

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -173,6 +174,7 @@ struct HostMesh {
     // material / boundary values
     bool uniformMaterial = true;
     bool planeStress = false;  // mechanicalProperties planeStress: the 2-D moduli of linearElastic.C:109-113
+    bool lsFullDelta = false;  // least-squares boundary delta Cf - C (extendedLeastSquares) instead of its normal part
     std::vector<double> matE, matNu;        // the E and nu the moduli were computed from (C or 1)
     std::vector<double> mu, K, impK;        // C (or 1)
     std::vector<double> eA, eC;             // E
@@ -193,10 +195,20 @@ struct HostMesh {
     std::vector<double> Do, Doo;  // old-time fields (SoA 3C) of a transient step
     std::vector<double> Dooo, Doooo;  // third and fourth old-time levels (backward d2dt2)
     bool hasOld = false;
+    // material zones of a multi-law case and their bi-material interface
+    // faces (s4f_mesh_zones_set); zone empty: one material
+    std::vector<int> zone;
+    int nIf = 0;
+    std::vector<int> ifCell;              // 2 nIf: side a (owner), side b (neighbour)
+    std::vector<double> ifN, ifDc, ifLs;  // SoA 3 nIf; 2 nIf; SoA 3 x 2 nIf
+    std::vector<int> eIf;                 // per ELL entry: interface side, -1 (empty: no interfaces)
+    std::vector<unsigned char> sliceIf;   // per slice: holds interface entries
+    std::vector<double> minv0;            // whole-mesh LS inverses, kept once zones replace some
 };
 
 struct SolveState {
     DArr<double> D0, D1, Dold, G, Db, g, snb, Gb, sigb, sig;
+    DArr<double> DI, sigI;  // sub-meshes: interface displacement (3 nIf), boundary stress (6 x 2 nIf)
     DArr<double> aAlpha, aRes, aAlphaB, aResB;  // Aitken relaxation state (cells, boundary faces)
     DArr<double> r64, p64, q64, x64, pb64, z64;
     DArr<float> r32, p32, q32, x32, pb32, z32;
@@ -262,6 +274,9 @@ struct s4f_mesh {
     DArr<int> sliceStart, sliceWidth, col, bCell, bType;
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
     DArr<float> eA32, diag32, invDiag32;
+    DArr<int> ifCell, eIf, zone;           // bi-material interfaces (HostMesh)
+    DArr<double> ifN, ifDc, ifLs;
+    DArr<unsigned char> sliceIf;
     // block map (single instance)
     DArr<int> bmInst, bmS0, bmS1, iBlk0, iNBlk, iCell0, iNCell, iB0, iNB;
     int nBlocks = 0, slicesPerBlock = 0, maxInstNB = 0;
@@ -326,6 +341,16 @@ DevMesh s4f_mesh::dm() const {
     m.bPres = bPres.p;
     m.cellInst = nullptr;
     m.nInst = 1;
+    if (h.nIf > 0) {
+        m.nIf = h.nIf;
+        m.ifCell = ifCell.p;
+        m.ifN = ifN.p;
+        m.ifDc = ifDc.p;
+        m.ifLs = ifLs.p;
+        m.eIf = eIf.p;
+        m.sliceIf = sliceIf.p;
+    }
+    m.zone = h.zone.empty() ? nullptr : zone.p;
     return m;
 }
 
@@ -486,6 +511,7 @@ void setupHostMesh(HostMesh& h, const s4f_mesh_desc& d) {
             dn[q] = n[q] * nd;
         }
         h.bDelta[b] = 1.0 / std::sqrt(dot3(dn, dn));
+        h.lsFullDelta = d.lsFullDelta != 0;
         const double* ld = d.lsFullDelta ? dl : dn;
         for (int q = 0; q < 3; ++q) bdl[3 * (size_t)b + q] = ld[q];
         if (h.bType[b] == S4F_PATCH_EMPTY) {  // empty faces carry no values
@@ -1443,7 +1469,19 @@ void configureSources(s4f_mesh* M, const s4f_solve_opts& o) {
 struct GradBufs {
     double *D0, *D1, *G, *Db, *g, *snb, *Gb, *sigb;
     double *aAlpha = nullptr, *aRes = nullptr, *aAlphaB = nullptr, *aResB = nullptr;  // Aitken state
+    double *DI = nullptr, *sigI = nullptr;  // sub-meshes: interface displacement and boundary stress
 };
+
+// the gradient-step buffers of a single mesh's solve state
+GradBufs meshGradBufs(s4f_mesh* M) {
+    SolveState& st = M->st;
+    GradBufs f{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+    if (M->h.nIf > 0) {
+        f.DI = st.DI.p;
+        f.sigI = st.sigI.p;
+    }
+    return f;
+}
 
 // boundary values, cell gradients, boundary gradients and stress (modes as
 // k_grad); nInst instances (bm.finInst, or 0..nInst-1), at most maxNB
@@ -1459,8 +1497,19 @@ void launchGradStep(const DevMesh& dm, const BlockMap& bm, int nInst, int maxNB,
         hipLaunchKernelGGL((k_grad_bnd_pre<T>), fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.D0, x,
                            (const double*)f.D1, (const double*)f.G, f.Db, (const double*)f.g, f.snb,
                            (const Ctl*)ctl, f.aAlphaB, f.aResB, mode);
+    // sub-meshes: the interface displacement before the gradient, the
+    // sub-meshes' interface stress after it (a mode-2 gradient uses the
+    // interface state as it is)
+    const bool sub = dm.nIf > 0 && mode != 2;
+    const dim3 ig(std::max(1, std::min(1024, (dm.nIf + kBlock - 1) / kBlock)));
+    if (sub)
+        hipLaunchKernelGGL((k_iface_pre<T>), ig, dim3(kBlock), 0, s, dm, (const double*)f.D0, x, (const double*)f.D1,
+                           f.DI, (const double*)f.sigI, (const Ctl*)ctl, mode);
     hipLaunchKernelGGL((k_grad<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, f.D0, x, f.D1, f.G,
-                       (const double*)f.Db, Dold, ctl, part, mode);
+                       (const double*)f.Db, (const double*)f.DI, Dold, ctl, part, mode);
+    if (sub)
+        hipLaunchKernelGGL(k_iface_post, ig, dim3(kBlock), 0, s, dm, (const double*)f.D0, (const double*)f.D1,
+                           (const double*)f.G, (const double*)f.DI, f.sigI, (const Ctl*)ctl, mode);
     hipLaunchKernelGGL(k_grad_bnd_post, fg, dim3(kBlock), 0, s, dm, bm, (const double*)f.G, (const double*)f.snb,
                        f.Gb, f.sigb, f.g, (const Ctl*)ctl, mode);
 }
@@ -1583,6 +1632,10 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
         }
         HIP_CHECK(hipMemsetAsync(st.G.p, 0, st.G.bytes(), s));
         HIP_CHECK(hipMemsetAsync(st.g.p, 0, st.g.bytes(), s));
+        if (h.nIf > 0 && !warm) {  // sub-meshes: a new run's interface state starts at zero
+            HIP_CHECK(hipMemsetAsync(st.DI.p, 0, st.DI.bytes(), s));
+            HIP_CHECK(hipMemsetAsync(st.sigI.p, 0, st.sigI.bytes(), s));
+        }
         std::vector<Ctl> c0(nInst, makeCtl(o, st.histCap));
         for (int i = 0; i < nInst; ++i) {
             c0[i].warm = ((o.warmStart && io[i].D_inout) || transient) ? 1 : 0;  // a time step continues
@@ -1599,7 +1652,7 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
     const double* Dold = warm ? st.Dold.p : nullptr;
     if (!resume) HIP_CHECK(hipEventRecord(ctx->ev0, s));  // a resumed run's time counts from the FP32 start
     // constructor: D.correctBoundaryConditions(); grad(D) (linGeomTotalDispSolid.C:82-84)
-    GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+    GradBufs gbuf = meshGradBufs(M);
     if (o.relaxationMethod == S4F_RELAX_AITKEN) {
         st.aAlpha.alloc(C); st.aRes.alloc(3 * (size_t)C);
         st.aAlphaB.alloc(std::max(1, h.B)); st.aResB.alloc(3 * (size_t)std::max(1, h.B));
@@ -1757,6 +1810,10 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
         HIP_CHECK(hipGetLastError());
         if (it > o.nCorrectors + 2) break;  // the device loop ends at nCorrectors
     }
+    if (h.nIf > 0)  // sub-meshes: the interface update of the point interpolation after the loop
+        hipLaunchKernelGGL((k_iface_pre<T>), dim3(std::max(1, std::min(1024, (h.nIf + kBlock - 1) / kBlock))),
+                           dim3(kBlock), 0, s, dm, (const double*)st.D0.p, (const T*)nullptr, (const double*)st.D1.p,
+                           st.DI.p, (const double*)st.sigI.p, (const Ctl*)st.ctl.p, 3);
     HIP_CHECK(hipEventRecord(ctx->ev1, s));
     HIP_CHECK(hipMemcpyAsync(hctl, st.ctl.p, nInst * sizeof(Ctl), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -1826,6 +1883,8 @@ void solveDispatch(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>
 void packMeshes(const std::vector<const HostMesh*>& ms, HostMesh& P, double rcScale, double ifScale) {
     P = HostMesh();
     const int n = (int)ms.size();
+    for (const HostMesh* m : ms)
+        if (!m->zone.empty()) throw InvalidError("a mesh with material zones (sub-meshes) is solved alone");
     for (const HostMesh* m : ms)
         if (m->g[0] != 0.0 || m->g[1] != 0.0 || m->g[2] != 0.0)
             throw InvalidError("body-force solves take a single mesh");
@@ -2089,13 +2148,154 @@ s4f_status s4f_mesh_tiles_set(s4f_mesh* M, int32_t maxTiles, int32_t minSlices) 
     });
 }
 
+// Least-squares inverse of cell P over its ELL row, the bi-material
+// interface entries taken as boundary faces of the cell's sub-mesh
+// (interface delta dIf(entry)); boundary entries from their vectors
+// bLs = d/|d|^2 (d d^T/|d|^2 = bLs bLs^T/|bLs|^2).  Returns the inverse and,
+// through dd, nothing else: the caller forms the interface vectors.
+static void subMeshLsInverse(const HostMesh& h, int P, int emptyDir,
+                             const std::function<void(int, double*)>& dIf, double inv[6]) {
+    const int s = P / kSlice, lane = P % kSlice, E = (int)h.col.size();
+    (void)E;
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    auto add = [&](const double* v, double md2) {
+        t[0] += v[0] * v[0] / md2; t[1] += v[0] * v[1] / md2; t[2] += v[0] * v[2] / md2;
+        t[3] += v[1] * v[1] / md2; t[4] += v[1] * v[2] / md2; t[5] += v[2] * v[2] / md2;
+    };
+    for (int k = 0; k < h.sliceWidth[s]; ++k) {
+        const int e = h.sliceStart[s] + k * kSlice + lane;
+        const int n = h.col[e];
+        if (n == kPad) continue;
+        double v[3];
+        if (n >= 0 && h.eIf[e] >= 0) {
+            dIf(e, v);
+        } else if (n >= 0) {
+            for (int q = 0; q < 3; ++q) v[q] = h.cC[(size_t)q * h.C + n] - h.cC[(size_t)q * h.C + P];
+        } else {
+            const int b = -1 - n;
+            for (int q = 0; q < 3; ++q) v[q] = h.bLs[(size_t)q * h.B + b];
+            if (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0) continue;
+        }
+        add(v, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    const int emptyDiag = emptyDir < 0 ? -1 : (emptyDir == 0 ? 0 : (emptyDir == 1 ? 3 : 5));
+    if (emptyDiag >= 0) t[emptyDiag] += 1.0;
+    const double xx = t[0], xy = t[1], xz = t[2], yy = t[3], yz = t[4], zz = t[5];
+    const double det = xx * yy * zz + xy * yz * xz + xz * xy * yz - xx * yz * yz - xy * xy * zz - xz * yy * xz;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+        throw InvalidError("singular sub-mesh least-squares matrix in cell " + std::to_string(P));
+    inv[0] = (yy * zz - yz * yz) / det; inv[1] = (xz * yz - xy * zz) / det; inv[2] = (xy * yz - xz * yy) / det;
+    inv[3] = (xx * zz - xz * xz) / det; inv[4] = (xy * xz - xx * yz) / det; inv[5] = (xx * yy - xy * xy) / det;
+    if (emptyDiag >= 0) inv[emptyDiag] -= 1.0;
+}
+
+// Material zones of a multi-law case (mechanicalModel with one law per
+// cellZone, solidSubMeshes.C:47-110): the internal faces between zones
+// become bi-material interface faces, each a boundary face of the sub-mesh
+// on either side (interface data, sub-mesh least-squares inverses of the
+// cells next to them, the ELL entries that carry them).
+s4f_status s4f_mesh_zones_set(s4f_mesh* M, const int32_t* cellZone, const double* faceCentres) {
+    return guarded([&]() -> s4f_status {
+        if (!M) throw InvalidError("null argument");
+        HostMesh& h = M->h;
+        if (h.instCell0.size() != 1) throw InvalidError("material zones belong to single meshes");
+        if (cellZone && !faceCentres) throw InvalidError("material zones need the mesh's face centres");
+        HIP_CHECK(hipSetDevice(M->ctx->device));
+        hipStream_t s = M->ctx->stream;
+        HIP_CHECK(hipStreamSynchronize(s));
+        const int C = h.C, F = h.F;
+        const size_t E = h.col.size();
+        if (!h.minv0.empty()) h.minv = h.minv0;  // the whole-mesh inverses back
+        h.zone.clear();
+        h.nIf = 0;
+        h.ifCell.clear(); h.ifN.clear(); h.ifDc.clear(); h.ifLs.clear(); h.eIf.clear(); h.sliceIf.clear();
+        if (cellZone) {
+            h.zone.assign(cellZone, cellZone + C);
+            std::vector<int> ifFace;
+            for (int f = 0; f < F; ++f)
+                if (h.zone[h.owner[f]] != h.zone[h.neighbour[f]]) ifFace.push_back(f);
+            const int nI = (int)ifFace.size(), nI2 = 2 * nI;
+            h.nIf = nI;
+            if (nI > 0) {
+                if (h.minv0.empty()) h.minv0 = h.minv;
+                int emptyDir = -1;
+                const int mask = h.instMask.empty() ? 7 : h.instMask[0];
+                for (int q = 0; q < 3; ++q)
+                    if (!(mask & (1 << q))) emptyDir = q;
+                h.ifCell.resize(nI2); h.ifN.resize(3 * (size_t)nI); h.ifDc.resize(nI2); h.ifLs.assign(3 * (size_t)nI2, 0.0);
+                h.eIf.assign(E, -1);
+                h.sliceIf.assign(h.nSlices, 0);
+                std::vector<double> dIf(3 * (size_t)nI2);  // each side's patch delta
+                for (int i = 0; i < nI; ++i) {
+                    const int f = ifFace[i];
+                    const int eo = h.entOwn[f], en = h.entNei[f];
+                    double n[3];
+                    for (int q = 0; q < 3; ++q) n[q] = h.eS[(size_t)q * E + eo];
+                    const double ms = std::sqrt(dot3(n, n));
+                    for (int q = 0; q < 3; ++q) {
+                        n[q] /= ms;
+                        h.ifN[(size_t)q * nI + i] = n[q];
+                    }
+                    for (int side = 0; side < 2; ++side) {
+                        const int P = side ? h.neighbour[f] : h.owner[f];
+                        double dl[3];
+                        for (int q = 0; q < 3; ++q) dl[q] = faceCentres[3 * (size_t)f + q] - h.cC[(size_t)q * C + P];
+                        const double nd = dot3(n, dl);
+                        h.ifDc[2 * i + side] = 1.0 / std::fabs(nd);
+                        for (int q = 0; q < 3; ++q) dIf[3 * (size_t)(2 * i + side) + q] = h.lsFullDelta ? dl[q] : n[q] * nd;
+                        h.ifCell[2 * i + side] = P;
+                        const int e = side ? en : eo;
+                        h.eIf[e] = 2 * i + side;
+                        h.sliceIf[P / kSlice] = 1;
+                    }
+                }
+                auto dOf = [&](int e, double* v) {
+                    for (int q = 0; q < 3; ++q) v[q] = dIf[3 * (size_t)h.eIf[e] + q];
+                };
+                std::vector<char> done(C, 0);
+                for (int k = 0; k < nI2; ++k) {
+                    const int P = h.ifCell[k];
+                    if (done[P]) continue;
+                    done[P] = 1;
+                    double inv[6];
+                    subMeshLsInverse(h, P, emptyDir, dOf, inv);
+                    for (int q = 0; q < 6; ++q) h.minv[(size_t)q * C + P] = inv[q];
+                }
+                for (int k = 0; k < nI2; ++k) {  // interface vectors d/|d|^2 (k_grad applies minv, as for bLs)
+                    const double* d = &dIf[3 * (size_t)k];
+                    const double r = 1.0 / dot3(d, d);
+                    for (int q = 0; q < 3; ++q) h.ifLs[(size_t)q * nI2 + k] = d[q] * r;
+                }
+            }
+        }
+        M->minv.upload(h.minv, s);
+        M->zone.upload(h.zone, s);
+        M->ifCell.upload(h.ifCell, s);
+        M->ifN.upload(h.ifN, s);
+        M->ifDc.upload(h.ifDc, s);
+        M->ifLs.upload(h.ifLs, s);
+        M->eIf.upload(h.eIf, s);
+        M->sliceIf.upload(h.sliceIf, s);
+        M->st.DI.alloc(3 * (size_t)h.nIf);
+        M->st.sigI.alloc(12 * (size_t)h.nIf);
+        if (h.nIf > 0) {
+            HIP_CHECK(hipMemsetAsync(M->st.DI.p, 0, M->st.DI.bytes(), s));
+            HIP_CHECK(hipMemsetAsync(M->st.sigI.p, 0, M->st.sigI.bytes(), s));
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+        return S4F_OK;
+    });
+}
+
 s4f_status s4f_mesh_info(const s4f_mesh* m, int64_t out[6]) {
     if (!m || !out) return fail(S4F_ERR_INVALID, "null argument");
     out[0] = m->h.C; out[1] = m->h.F; out[2] = m->h.B; out[3] = (int64_t)m->h.col.size(); out[4] = m->h.nSlices;
     size_t bytes = m->sliceStart.bytes() + m->col.bytes() + m->eS.bytes() + m->eW.bytes() + m->eA.bytes() + m->eC.bytes() +
                    m->eA32.bytes() + m->cC.bytes() + m->minv.bytes() + m->diag.bytes() + m->invDiag.bytes() +
                    m->diag32.bytes() + m->invDiag32.bytes() + m->bN.bytes() + m->bKv.bytes() + m->bLs.bytes() +
-                   m->bDelta.bytes() + m->bVal.bytes() + m->bPres.bytes() + m->bCell.bytes() + m->bType.bytes();
+                   m->bDelta.bytes() + m->bVal.bytes() + m->bPres.bytes() + m->bCell.bytes() + m->bType.bytes() +
+                   m->ifCell.bytes() + m->eIf.bytes() + m->zone.bytes() + m->ifN.bytes() + m->ifDc.bytes() +
+                   m->ifLs.bytes() + m->sliceIf.bytes();
     out[5] = (int64_t)bytes;
     return S4F_OK;
 }
@@ -2293,7 +2493,7 @@ s4f_status s4f_grad(s4f_mesh* M, const double* D, const double* Db, const double
         HIP_CHECK(hipMemcpyAsync(st.ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s));
         const DevMesh dm = M->dm();
         const BlockMap bm = M->bmap();
-        const GradBufs gbuf{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p};
+        const GradBufs gbuf = meshGradBufs(M);
         launchGradStep<double>(dm, bm, (int)M->h.instCell0.size(), M->maxInstNB, gbuf, nullptr, nullptr, st.ctl.p,
                                st.part.p, 2, s);
         HIP_CHECK(hipGetLastError());
@@ -2592,7 +2792,7 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                 break;
             case S4F_KERNEL_GRAD:
                 launchGradStep<T>(dm, bm, 1, M->maxInstNB,
-                                  GradBufs{st.D0.p, st.D1.p, st.G.p, st.Db.p, st.g.p, st.snb.p, st.Gb.p, st.sigb.p},
+                                  meshGradBufs(M),
                                   (const T*)v.x, nullptr, rc.p, st.part.p, 0, s);
                 break;
             case S4F_KERNEL_MG0_DOWN: {
@@ -2769,6 +2969,7 @@ s4f_status s4f_batch_add(s4f_batch* b, s4f_mesh* m, int32_t* index) {
         if (!m->materialSet) throw InvalidError("material not set");
         if (m->ctx != b->ctx) throw InvalidError("mesh belongs to another context");
         if (m->h.instCell0.size() != 1) throw InvalidError("cannot nest batches");
+        if (!m->h.zone.empty()) throw InvalidError("a mesh with material zones (sub-meshes) is solved alone");
         if (m->h.g[0] != 0.0 || m->h.g[1] != 0.0 || m->h.g[2] != 0.0)
             throw InvalidError("batched solves are steady solves without body force (gravity is set on this mesh)");
         b->members.push_back(m);
@@ -3740,6 +3941,7 @@ s4f_status s4f_server_submit(s4f_server* S, const s4f_request* rq, int64_t* id) 
         if (!m->materialSet) throw InvalidError("material not set");
         if (m->ctx != S->ctx) throw InvalidError("mesh belongs to another context");
         if (m->h.instCell0.size() != 1) throw InvalidError("requests take plain meshes");
+        if (!m->h.zone.empty()) throw InvalidError("a mesh with material zones (sub-meshes) is solved alone");
         if (m->inerKey != 0.0 || m->h.g[0] != 0.0 || m->h.g[1] != 0.0 || m->h.g[2] != 0.0)
             throw InvalidError("requests take static meshes without body force (the last solve was transient or "
                                "gravity is set)");

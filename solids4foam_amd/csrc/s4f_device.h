@@ -65,6 +65,16 @@ struct DevMesh {
     // batching: per-instance segment of cells (nullptr for a single mesh)
     const int* cellInst;   // instance of each cell
     int nInst;
+    // material zones of a multi-law case, each solved on its own sub-mesh:
+    // the bi-material interface faces between zones (nIf = 0: one material)
+    int nIf;
+    const int* ifCell;            // 2 nIf: side a (the face owner), side b (the neighbour)
+    const double* ifN;            // 3 nIf (SoA) unit normals from a to b
+    const double* ifDc;           // 2 nIf sub-mesh patch deltaCoeffs 1/|n.(Cf - C)|, side a then b
+    const double* ifLs;           // 3 x 2 nIf (SoA) least-squares boundary vectors of each side
+    const int* eIf;               // per ELL entry: interface side 2 i + (0 a, 1 b), or -1
+    const unsigned char* sliceIf; // per slice: 1 if it holds interface entries
+    const int* zone;              // material zone of each cell (nullptr: one material)
 };
 
 // Solve control block, written by the last workgroup of each reducing kernel

@@ -680,8 +680,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap bm, double* __restrict__ D0,
                                                  const T* __restrict__ x, double* __restrict__ D1,
                                                  double* __restrict__ G, const double* __restrict__ Db,
-                                                 const double* __restrict__ Dold, Ctl* __restrict__ ctlAll,
-                                                 double* __restrict__ part, int mode) {
+                                                 const double* __restrict__ DI, const double* __restrict__ Dold,
+                                                 Ctl* __restrict__ ctlAll, double* __restrict__ part, int mode) {
     const int lb = logicalBlock(bm.nBlocks);
     const int inst = bm.inst[lb];
     Ctl* ctl = ctlAll + inst;
@@ -720,7 +720,13 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         for (int q = 0; q < 9; ++q) A[q] = 0.0;
         bool hasB = false;
         int kInt = width;  // internal slots left to the general loop below
-        if (width == 6) {
+        // a slice holding bi-material interface faces: every slot goes
+        // through the general loop, which gives each side's sub-mesh the
+        // interface displacement as a boundary value
+        const bool ifs = m.sliceIf && m.sliceIf[s];
+        if (ifs) {
+            hasB = true;
+        } else if (width == 6) {
             if (xg) gradInternalW<T, 6, true>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
             else gradInternalW<T, 6, false>(m, P, e0, lane, C, Dg, xg, ra, DnP, CP, A, hasB);
             kInt = 0;
@@ -736,6 +742,18 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
             const int n = m.col[e];
             if (n >= 0) {
                 if (k >= kInt) continue;  // already summed by the fixed-width pass
+                if (ifs && m.eIf[e] >= 0) {
+                    const int si = m.eIf[e], nI2 = 2 * m.nIf;
+                    const double lv[3] = {m.ifLs[si], m.ifLs[(size_t)nI2 + si], m.ifLs[2 * (size_t)nI2 + si]};
+                    double dv[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) dv[q] = DI[(size_t)q * m.nIf + (si >> 1)] - DnP[q];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) A[3 * i + j] += lv[i] * dv[j];
+                    continue;
+                }
                 double d[3], dv[3];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
@@ -793,6 +811,123 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
     }
     if (mode == 2) return;
     storeBlockPartials<6>(acc, 7u, part + (size_t)lb * kMaxPartials);
+}
+
+// ---------------------------------------------------------------------------
+// Sub-meshes of a multi-law case (one law per cellZone): the internal faces
+// between zones are boundary faces of the sub-mesh on either side, holding
+// an interface displacement D_I that is iterated to traction balance.
+// k_iface_pre  solidSubMeshes::interpolateDtoSubMeshD (solidSubMeshes.C:
+//              1550-1950), side a the owner, b the neighbour, n from a to b:
+//                D_I += wab n.(sigma_b - sigma_a) + wa (D_a - Dprev_a) + wb (D_b - Dprev_b)
+//              da = 1/deltaCoeffs_a, db = 1/deltaCoeffs_b, K = impK,
+//              wab = da db/(db Ka + da Kb), wa = db Ka/(db Ka + da Kb), wb = 1 - wa;
+//              sigma_a, sigma_b the sub-meshes' boundary stresses from the
+//              previous correct().  Seen from sub-mesh b the normal, weights
+//              and sides swap and the update is the same, so one D_I serves
+//              both.  Modes 0/1 as k_grad (D = the current field, the new one
+//              D + a x or the Aitken-relaxed one); mode 3: after the loop
+//              (mechanicalModel::interpolate, mechanicalModel.C:851-852),
+//              D the final field and Dprev the one its last iteration began
+//              from.
+// k_iface_post the sub-meshes' boundary gradient at the face,
+//              G_P + n_s ((D_I - D_P) deltaCoeffs - n_s.G_P) (fvc::grad's
+//              boundary value corrected by solidSubMeshes::
+//              correctBoundarySnGrad, solidSubMeshes.C:1952-2031, whose k
+//              vanishes for the normal patch delta), n_s = +n on side a, -n
+//              on side b, and the side's Hooke's law (mechanicalModel.C:
+//              477-498).  A cold start's first stress is zero, like the
+//              cell stress.
+// One instance (sub-meshes are solved alone).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_iface_pre(const DevMesh m, const double* __restrict__ D0,
+                                                      const T* __restrict__ x, const double* __restrict__ D1,
+                                                      double* __restrict__ DI, const double* __restrict__ sigI,
+                                                      const Ctl* __restrict__ ctl, int mode) {
+    if (mode == 0 && ctl->outerDone) return;
+    if (mode != 0 || ctl->firstInner) x = nullptr;
+    const bool post = mode == 3;
+    const double* __restrict__ Dcur = ctl->dpar ? D1 : D0;
+    const double* __restrict__ Doth = ctl->dpar ? D0 : D1;
+    // the old field and the new one: (D, D + a x or the Aitken-relaxed one),
+    // after the loop (Dprev, D)
+    const bool aitken = mode == 0 && ctl->relaxAitken;
+    const double ra = (mode == 0 && !aitken && ctl->relaxD != 1.0) ? ctl->relaxD : 1.0;
+    const double* __restrict__ Dp = post ? Doth : Dcur;
+    const int C = m.nCells, nI = m.nIf, nI2 = 2 * m.nIf;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < nI; i += gridDim.x * kBlock) {
+        const int a = m.ifCell[2 * i], b = m.ifCell[2 * i + 1];
+        const double n[3] = {m.ifN[i], m.ifN[(size_t)nI + i], m.ifN[2 * (size_t)nI + i]};
+        const double da = 1.0 / m.ifDc[2 * i], db = 1.0 / m.ifDc[2 * i + 1];
+        const double Ka = cellImpK(m, a), Kb = cellImpK(m, b);
+        const double wab = da * db / (db * Ka + da * Kb);
+        const double wa = db * Ka / (db * Ka + da * Kb), wb = 1.0 - wa;
+        double sa[6], sb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            sa[q] = sigI[(size_t)q * nI2 + 2 * i];
+            sb[q] = sigI[(size_t)q * nI2 + 2 * i + 1];
+        }
+        const double ta[3] = {sa[0] * n[0] + sa[1] * n[1] + sa[2] * n[2], sa[1] * n[0] + sa[3] * n[1] + sa[4] * n[2],
+                              sa[2] * n[0] + sa[4] * n[1] + sa[5] * n[2]};
+        const double tb[3] = {sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2], sb[1] * n[0] + sb[3] * n[1] + sb[4] * n[2],
+                              sb[2] * n[0] + sb[4] * n[1] + sb[5] * n[2]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double pa = Dp[(size_t)q * C + a], pb = Dp[(size_t)q * C + b];
+            double na, nb;
+            if (post) {
+                na = Dcur[(size_t)q * C + a];
+                nb = Dcur[(size_t)q * C + b];
+            } else if (aitken) {
+                na = Doth[(size_t)q * C + a];
+                nb = Doth[(size_t)q * C + b];
+            } else {
+                na = pa + (x ? ra * (double)x[(size_t)q * C + a] : 0.0);
+                nb = pb + (x ? ra * (double)x[(size_t)q * C + b] : 0.0);
+            }
+            DI[(size_t)q * nI + i] += wab * (tb[q] - ta[q]) + wa * (na - pa) + wb * (nb - pb);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_iface_post(const DevMesh m, const double* __restrict__ D0,
+                                                       const double* __restrict__ D1, const double* __restrict__ G,
+                                                       const double* __restrict__ DI, double* __restrict__ sigI,
+                                                       const Ctl* __restrict__ ctl, int mode) {
+    if (mode == 0 && ctl->outerDone) return;
+    // the gradient's cell values: k_grad left D' in the buffer the flip at
+    // the end of the step makes current (mode 0), else the current field
+    const double* __restrict__ D = (mode == 0) == (ctl->dpar != 0) ? D0 : D1;
+    const bool zero = mode == 1 && ctl->sigma0;
+    const int C = m.nCells, nI = m.nIf, nI2 = 2 * m.nIf;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nI2; k += gridDim.x * kBlock) {
+        const int i = k >> 1, side = k & 1;
+        const int P = m.ifCell[k];
+        const double sg = side ? -1.0 : 1.0;
+        const double n[3] = {sg * m.ifN[i], sg * m.ifN[(size_t)nI + i], sg * m.ifN[2 * (size_t)nI + i]};
+        double GI[9], sI[6];
+        if (zero) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sI[q] = 0.0;
+        } else {
+            double GP[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) GP[q] = G[(size_t)q * C + P];
+            const double dc = m.ifDc[k];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double nG = n[0] * GP[j] + n[1] * GP[3 + j] + n[2] * GP[6 + j];
+                const double sn = (DI[(size_t)j * nI + i] - D[(size_t)j * C + P]) * dc;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) GI[3 * r + j] = GP[3 * r + j] + n[r] * (sn - nG);
+            }
+            hooke(GI, cellMu(m, P), cellK(m, P), sI);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sigI[(size_t)q * nI2 + k] = sI[q];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_grad_bnd_post(const DevMesh m, const BlockMap bm,
@@ -909,23 +1044,52 @@ __global__ __launch_bounds__(kBlock) void k_point_interpolate(const DevMesh m, c
     const int C = m.nCells;
     for (int p = blockIdx.x * kBlock + threadIdx.x; p < nPoints; p += gridDim.x * kBlock) {
         const double x[3] = {pts[3 * (size_t)p], pts[3 * (size_t)p + 1], pts[3 * (size_t)p + 2]};
-        double acc[3] = {0, 0, 0}, sw = 0.0;
-        for (int k = pcOff[p]; k < pcOff[p + 1]; ++k) {
-            const int c = pcCell[k];
-            double d[3];
+        const int k0 = pcOff[p], k1 = pcOff[p + 1];
+        // material zones: each sub-mesh interpolates from its own cells and
+        // the point takes the mean over the materials around it
+        // (mapSubMeshPointFields, solidSubMeshesTemplates.C:271-336), the
+        // materials in the order of their first cell in the list
+        int nz = 1;
+        if (m.zone) {
+            nz = 0;
+            for (int k = k0; k < k1; ++k) {
+                bool first = true;
+                for (int k2 = k0; k2 < k; ++k2) first &= m.zone[pcCell[k2]] != m.zone[pcCell[k]];
+                nz += first ? 1 : 0;
+            }
+        }
+        double res[3] = {0, 0, 0};
+        for (int kz = k0; kz < (m.zone ? k1 : k0 + 1); ++kz) {
+            if (m.zone) {
+                bool first = true;
+                for (int k2 = k0; k2 < kz; ++k2) first &= m.zone[pcCell[k2]] != m.zone[pcCell[kz]];
+                if (!first) continue;
+            }
+            const int z = (m.zone && kz < k1) ? m.zone[pcCell[kz]] : 0;
+            double acc[3] = {0, 0, 0}, sw = 0.0;
+            for (int k = k0; k < k1; ++k) {
+                const int c = pcCell[k];
+                if (m.zone && m.zone[c] != z) continue;
+                double d[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) d[q] = x[q] - m.cC[(size_t)q * C + c];
-            const double w = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                for (int q = 0; q < 3; ++q) d[q] = x[q] - m.cC[(size_t)q * C + c];
+                const double w = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double dG = d[0] * G[(size_t)j * C + c] + d[1] * G[(size_t)(3 + j) * C + c] +
+                                      d[2] * G[(size_t)(6 + j) * C + c];
+                    acc[j] += w * (D[(size_t)j * C + c] + dG);
+                }
+                sw += w;
+            }
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const double dG = d[0] * G[(size_t)j * C + c] + d[1] * G[(size_t)(3 + j) * C + c] +
-                                  d[2] * G[(size_t)(6 + j) * C + c];
-                acc[j] += w * (D[(size_t)j * C + c] + dG);
+                const double v = sw > 1e-300 ? acc[j] / sw : acc[j];
+                res[j] = nz == 1 ? v : res[j] + v / nz;
             }
-            sw += w;
         }
 #pragma unroll
-        for (int j = 0; j < 3; ++j) pD[3 * (size_t)p + j] = sw > 1e-300 ? acc[j] / sw : acc[j];
+        for (int j = 0; j < 3; ++j) pD[3 * (size_t)p + j] = res[j];
     }
 }
 

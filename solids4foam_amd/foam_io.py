@@ -456,7 +456,8 @@ def read_materials(case_dir, nCells):
     """(E, nu, rho) of mechanicalProperties: scalars for one law, per-cell
     arrays when there are several laws, each acting on the cellZone of its
     name (the reference's solidSubMeshes assignment, mechanicalModel.C:195-245;
-    the device path takes them as a per-cell linearElastic material)."""
+    the device path takes them as a per-cell linearElastic material, and
+    the zones of read_material_zones for the sub-mesh interfaces)."""
     d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
     laws = d.get("mechanical")
     if not isinstance(laws, list):
@@ -477,6 +478,27 @@ def read_materials(case_dir, nCells):
     if np.any(np.isnan(E)):
         raise ValueError("some cells belong to no material's cellZone")
     return E, nu, (None if np.all(np.isnan(rho)) else rho)
+
+
+def read_material_zones(case_dir, nCells):
+    """Material zone of every cell for several mechanical laws (the law's
+    index in mechanicalProperties, on the cellZone of its name), None for
+    one law.  The reference solves each law on its own sub-mesh
+    (mechanicalModel.C:571-615, solidSubMeshes); the device path does the
+    same given these zones (s4f_mesh_zones_set)."""
+    d = read_dict(os.path.join(case_dir, "constant", "mechanicalProperties"))
+    laws = d.get("mechanical")
+    if not isinstance(laws, list) or len(laws) == 1:
+        return None
+    zones = read_cell_zones(case_dir)
+    z = np.full(nCells, -1, np.int32)
+    for k, (name, _) in enumerate(laws):
+        if name not in zones:
+            raise ValueError(f"mechanical law {name!r} has no cellZone of that name")
+        z[zones[name]] = k
+    if np.any(z < 0):
+        raise ValueError("some cells belong to no material's cellZone")
+    return z
 
 
 def read_density(case_dir):
@@ -612,6 +634,7 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     Cf = pm.geometry()["Cf"][pm.nInternalFaces:]
     bcs = read_D_boundary(case_dir, bnd, face_centres=Cf)
     E, nu, rho = read_materials(case_dir, pm.nCells)
+    zones = read_material_zones(case_dir, pm.nCells)
     if os.path.exists(os.path.join(case_dir, "0", "sigma0")):  # the initial-stress field (mechanicalLaw.C:157-230)
         raise ValueError("0/sigma0 (initial stress field) is not supported on the device path")
     patches = [dict(type=b["type"], value=b.get("value"), traction=b.get("traction"), pressure=b.get("pressure"))
@@ -636,7 +659,8 @@ def run_case(case_dir, ctx=None, write_time="1", solver_overrides=None):
     m = solid.LinGeomTotalDispSolid(ctx, pm, patches, E, nu, solver=opts,
                                     plane_stress=read_plane_stress(case_dir),
                                     ls_full_delta="extendedLeastSquares" in grad_scheme, rho=rho, g=g,
-                                    d2dt2=scheme, deltaT=tc["deltaT"] if tc else None, predictor=predictor)
+                                    d2dt2=scheme, deltaT=tc["deltaT"] if tc else None, predictor=predictor,
+                                    zones=zones)
     if not transient:
         m.evolve()
         _write_state(m, os.path.join(case_dir, write_time))

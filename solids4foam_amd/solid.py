@@ -190,6 +190,20 @@ class SolverMesh:
         """workgroup tiles: at most max_tiles tiles of at least min_slices 64-cell slices"""
         check(lib().s4f_mesh_tiles_set(self.h, int(max_tiles), int(min_slices)))
 
+    def set_zones(self, zone):
+        """Material zone of every cell of a multi-law case (one law per
+        cellZone), or None for one material: each zone is solved on its own
+        sub-mesh, with interface displacements on the faces between zones
+        (mechanicalModel.C:571-615, solidSubMeshes.C:1550-2031)."""
+        if zone is None:
+            check(lib().s4f_mesh_zones_set(self.h, None, None))
+            self.zones = None
+            return
+        z = i32(np.asarray(zone).reshape(self.nCells))
+        Cf = f64(self.poly.geometry()["Cf"])
+        check(lib().s4f_mesh_zones_set(self.h, ptr(z), ptr(Cf)))
+        self.zones = z
+
     def set_old_times(self, levels):
         """D.oldTime(), D.oldTime().oldTime(), ... (1 to 4 fields of (nCells, 3))"""
         a = f64(np.stack([np.asarray(l, np.float64).reshape(self.nCells, 3) for l in levels]))
@@ -312,12 +326,14 @@ class LinGeomTotalDispSolid:
     typeName = "linearGeometryTotalDisplacement"
 
     def __init__(self, ctx, polymesh, patches, E, nu, coeffs=None, solver=None, ls_full_delta=False, rho=None,
-                 g=None, d2dt2="steadyState", deltaT=None, predictor=False, plane_stress=False):
+                 g=None, d2dt2="steadyState", deltaT=None, predictor=False, plane_stress=False, zones=None):
         # rho: mechanicalProperties density (needed with g or a transient
         # d2dt2); g: constant/g; d2dt2: d2dt2Schemes default (steadyState |
         # Euler | backward); deltaT: the time step of evolve() when transient;
         # predictor: linGeomTotalDispSolid::predict (:47-59), transient only;
-        # plane_stress: mechanicalProperties planeStress (2-D moduli)
+        # plane_stress: mechanicalProperties planeStress (2-D moduli);
+        # zones: the law (cellZone) of every cell when there are several
+        # laws, each solved on its own sub-mesh (None: one law)
         # patches: list of dicts {type: "fixedDisplacement"|"fixedDisplacementZeroShear"|"solidTraction"|
         # "solidSymmetry"|"empty", value|traction, pressure}
         kinds = {"fixedDisplacement": _abi.PATCH_FIXED_DISPLACEMENT, "solidTraction": _abi.PATCH_TRACTION,
@@ -335,6 +351,8 @@ class LinGeomTotalDispSolid:
         if plane_stress:
             self.mesh.set_plane_stress(True)
         self.mesh.set_material(E, nu)
+        if zones is not None:
+            self.mesh.set_zones(zones)
         for i, p in enumerate(patches):
             if p["type"] in ("fixedDisplacement", "fixedDisplacementZeroShear"):
                 self.mesh.set_bc(i, p.get("value", (0.0, 0.0, 0.0)))

@@ -283,15 +283,23 @@ def test_multi_material_cell_zones(tmp_path):
     E, nu, rho = foam_io.read_materials(d, C)
     assert np.all(E[hard] == 200e9) and np.all(E[soft] == 1e9)
     assert np.all(nu[soft] == 0.45) and np.all(rho[hard] == 7800.0)
+    # the law of every cell, for the sub-mesh interfaces
+    z = foam_io.read_material_zones(d, C)
+    assert z.dtype == np.int32 and np.all(z[hard] == 0) and np.all(z[soft] == 1)
     _write(d, "constant/polyMesh/cellZones", _zones_file([("steel", hard[1:]), ("rubber", soft)]))
     with pytest.raises(ValueError, match="no material"):
         foam_io.read_materials(d, C)
+    with pytest.raises(ValueError, match="no material"):
+        foam_io.read_material_zones(d, C)
+    _write(d, "constant/mechanicalProperties", MECH)
+    assert foam_io.read_material_zones(d, C) is None
 
 
 @pytest.mark.gpu
 def test_run_case_multi_material(tmp_path):
-    # two laws on two cellZones: the case runs as a per-cell material and
-    # matches the oracle with the same per-cell E and nu
+    # two laws on two cellZones: the case runs as a per-cell material with
+    # one sub-mesh per law and matches the oracle with the same per-cell E
+    # and nu and the same zones
     import oracle
     from solids4foam_amd import Context
     d = str(tmp_path)
@@ -311,6 +319,10 @@ def test_run_case_multi_material(tmp_path):
     bv = np.zeros((om.nBoundaryFaces, 3))
     bv[om.patch_faces(5)] = (0.0, 0.0, 1e7)
     oc = oracle.Case(om, E, nu, [1, 1, 1, 1, 0, 1], bv, rc_scale=0.2)
+    zones = foam_io.read_material_zones(d, C)
+    assert np.array_equal(zones, np.isin(np.arange(C), soft).astype(np.int32))
+    assert oc.set_zones(zones) == 60
+    oc.init()
     oc.solve(nCorr=50000, solTol=1e-12, altTol=1e-12, pcgTol=1e-14, rcScale=0.2, precond=oracle.PRECOND_DIC)
     Do = oc.get("D").reshape(-1, 3)
     assert np.linalg.norm(m.D - Do) / np.linalg.norm(Do) < 1e-8
