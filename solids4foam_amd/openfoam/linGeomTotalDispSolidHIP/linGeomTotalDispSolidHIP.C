@@ -135,32 +135,69 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
     desc.lsFullDelta = gradScheme.find("extendedLeastSquares") != string::npos;
     check(s4f_mesh_create(ctx_, &desc, &gpuMesh_), "createDeviceMesh()");
 
-    // one linearElastic law (linearElastic.C:62-133)
+    // linearElastic laws (linearElastic.C:62-133): one, or one per cellZone
+    // named like the law, each solved on its own sub-mesh
+    // (mechanicalModel.C:195-245, solidSubMeshes.C:29-44); the device takes
+    // the material per cell and the zones for the bi-material interfaces
     const PtrList<mechanicalLaw>& laws = mechanical();
-    if (laws.size() != 1 || !isA<linearElastic>(laws[0]))
+    std::vector<double> E(nCells, -1.0), nu(nCells, 0.0);
+    std::vector<int32_t> zone(nCells, -1);
+    forAll(laws, lawI)
     {
-        FatalErrorIn(type() + "::createDeviceMesh()")
-            << "the device path takes a single linearElastic law"
-            << abort(FatalError);
+        if (!isA<linearElastic>(laws[lawI]))
+        {
+            FatalErrorIn(type() + "::createDeviceMesh()")
+                << "the device path takes linearElastic laws"
+                << abort(FatalError);
+        }
+        const linearElastic& law = refCast<const linearElastic>(laws[lawI]);
+        // the pressure-equation formulation (mechanicalLaw.C:1374) and an
+        // initial stress (linearElastic.C:158-166) change the reference's
+        // stress; the device path builds neither
+        if (law.solvePressureEqn())
+        {
+            FatalErrorIn(type() + "::createDeviceMesh()")
+                << "solvePressureEqn is not supported by the device path"
+                << abort(FatalError);
+        }
+        if (gMax(mag(law.sigma0())()) > SMALL)
+        {
+            FatalErrorIn(type() + "::createDeviceMesh()")
+                << "a nonzero sigma0 is not supported by the device path"
+                << abort(FatalError);
+        }
+        labelList cells;
+        if (laws.size() == 1)
+        {
+            cells = identity(nCells);
+        }
+        else
+        {
+            const label zoneID = m.cellZones().findZoneID(law.name());
+            if (zoneID < 0)
+            {
+                FatalErrorIn(type() + "::createDeviceMesh()")
+                    << "no cellZone named " << law.name() << " for its law"
+                    << abort(FatalError);
+            }
+            cells = m.cellZones()[zoneID];
+        }
+        forAll(cells, i)
+        {
+            E[cells[i]] = law.E().value();
+            nu[cells[i]] = law.nu().value();
+            zone[cells[i]] = lawI;
+        }
     }
-    const linearElastic& law = refCast<const linearElastic>(laws[0]);
-    // the pressure-equation formulation (mechanicalLaw.C:1374) and an initial
-    // stress (linearElastic.C:158-166) change the reference's stress; the
-    // device path builds neither
-    if (law.solvePressureEqn())
+    for (label cellI = 0; cellI < nCells; ++cellI)
     {
-        FatalErrorIn(type() + "::createDeviceMesh()")
-            << "solvePressureEqn is not supported by the device path"
-            << abort(FatalError);
+        if (zone[cellI] < 0)
+        {
+            FatalErrorIn(type() + "::createDeviceMesh()")
+                << "cell " << cellI << " belongs to no law's cellZone"
+                << abort(FatalError);
+        }
     }
-    if (gMax(mag(law.sigma0())()) > SMALL)
-    {
-        FatalErrorIn(type() + "::createDeviceMesh()")
-            << "a nonzero sigma0 is not supported by the device path"
-            << abort(FatalError);
-    }
-    const double E = law.E().value();
-    const double nu = law.nu().value();
     // planeStress (mechanicalLaw.C:485-497): linearElastic's 2-D lambda and K
     const Switch planeStress
     (
@@ -168,7 +205,11 @@ void linGeomTotalDispSolidHIP::createDeviceMesh()
             .lookupOrDefault<Switch>("planeStress", false)
     );
     check(s4f_plane_stress_set(gpuMesh_, planeStress ? 1 : 0), "createDeviceMesh()");
-    check(s4f_material_set(gpuMesh_, &E, &nu, 0), "createDeviceMesh()");
+    check(s4f_material_set(gpuMesh_, E.data(), nu.data(), laws.size() > 1 ? 1 : 0), "createDeviceMesh()");
+    if (laws.size() > 1)
+    {
+        check(s4f_mesh_zones_set(gpuMesh_, zone.data(), Cf.data()), "createDeviceMesh()");
+    }
 
     // rho (inertia and body force) and g (solidModel::g(), constant/g)
     std::vector<double> rho(nCells);
