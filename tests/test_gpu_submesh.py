@@ -163,3 +163,115 @@ def test_zones_guards_and_clear(ctx, oracle_mod):
     oc2, m2, _, _ = _inclusion(ctx, oracle_mod, n=5, zones=False)
     c = m2.solve(precision=0, nCorrectors=30)
     assert np.array_equal(a["D"], c["D"]) and np.array_equal(a["sigma"], c["sigma"])
+
+
+def _pair(ctx, oracle_mod, n, L, jitter, seed, z_of, E_of, nu, types, bv_of, tets=False, full=False):
+    """oracle case and device mesh of the same box, zones and loads"""
+    from solids4foam_amd.solid import PolyMesh, SolverMesh
+    om = oracle_mod.Mesh.box(*n, L, jitter, seed, tets=tets)
+    z = z_of(om.C).astype(np.int32)
+    E = E_of(z)
+    bv = bv_of(om)
+    oc = oracle_mod.Case(om, E, nu, types, bv, ls_full_delta=full)
+    assert oc.set_zones(z) > 0
+    oc.init()
+    pm = PolyMesh.box(*n, L=L, jitter=jitter, seed=seed, tets=tets)
+    assert np.array_equal(pm.cell_centres(), om.C)
+    m = SolverMesh(ctx, pm, types, ls_full_delta=full)
+    m.set_material(E, nu)
+    for p in range(len(types)):
+        if types[p] != 2:
+            m.set_bc(p, bv[pm.patch_faces(p)])
+    m.set_zones(z)
+    return oc, m
+
+
+def test_submesh_tets_and_2d_match_oracle(ctx, oracle_mod):
+    # tetrahedra (full-delta gradient) with a soft inclusion, and a 2-D
+    # two-layer cantilever (empty faces: the sub-mesh LS inverses keep the
+    # empty axis out)
+    def shear(om):
+        bv = np.zeros((om.nBoundaryFaces, 3))
+        bv[om.patch_faces(1)] = (0.0, 5e6, 0.0)
+        return bv
+
+    def tip(om):
+        bv = np.zeros((om.nBoundaryFaces, 3))
+        bv[om.patch_faces(1)] = (0.0, -1e6, 0.0)
+        return bv
+
+    cases = [("tets", _pair(ctx, oracle_mod, (4, 4, 4), (1.0, 1.0, 1.0), 0.1, 5,
+                            lambda C: np.linalg.norm(C - 0.5, axis=1) < 0.3, lambda z: np.where(z == 0, 200e9, 20e9),
+                            0.3, [FIXED] + [TRACTION] * 5, shear, tets=True, full=True)),
+             ("2-D", _pair(ctx, oracle_mod, (20, 6, 1), (1.0, 0.3, 0.05), 0.1, 3, lambda C: C[:, 1] > 0.15,
+                           lambda z: np.where(z == 0, 200e9, 70e9), 0.3, [FIXED, TRACTION, TRACTION, TRACTION, 2, 2],
+                           tip))]
+    for name, (oc, m) in cases:
+        print(f"[submesh {name}]", flush=True)
+        ro = oc.solve(precond=oracle_mod.PRECOND_DIC, **ORACLE_TIGHT)
+        out = m.solve(precision=0, **TIGHT)
+        eD = rel_l2(out["D"], oc.get("D").reshape(-1, 3))
+        es = rel_l2(out["sigma"], oc.get("sigma").reshape(-1, 6))
+        print(f"  nOuter {out['nOuter']} (oracle {ro['nOuter']}) err D {eD:.2e} sigma {es:.2e}", flush=True)
+        assert ro["converged"] and out["converged"] and eD < 1e-8 and es < 1e-8
+
+
+def _incl6(ctx, oracle_mod):
+    def load(om):
+        bv = np.zeros((om.nBoundaryFaces, 3))
+        bv[om.patch_faces(5)] = (2e6, 0.0, 1e7)
+        return bv
+    return _pair(ctx, oracle_mod, (6, 6, 6), (1.0, 1.0, 1.0), 0.15, 3,
+                 lambda C: np.linalg.norm(C - 0.5, axis=1) < 0.3, lambda z: np.where(z == 0, 200e9, 20e9),
+                 0.3, [TRACTION] * 4 + [FIXED, TRACTION], load)
+
+
+@pytest.mark.parametrize("method", ["aitken", "fixed"])
+def test_submesh_relaxation_follows_oracle(ctx, oracle_mod, method):
+    # field relaxation with sub-meshes: the interface update sees the relaxed
+    # field (Aitken's from its own buffer); the first outer iterations follow
+    # the oracle and the converged field is the oracle's
+    print(f"[submesh relaxation {method}]", flush=True)
+    kw_o = dict(relaxAitken=1, relaxD=0.5) if method == "aitken" else dict(relaxD=0.7)
+    kw_g = dict(relaxationMethod=1, relaxD=0.5) if method == "aitken" else dict(relaxD=0.7)
+    oc, m = _incl6(ctx, oracle_mod)
+    ro = oc.solve(history=10, nCorr=8, precond=oracle_mod.PRECOND_DIAGONAL, **kw_o)
+    m.solve(nCorrectors=8, **kw_g)
+    h = m.history()
+    assert np.allclose(h["initRes"][:8], ro["history"]["initRes"][:8], rtol=1e-8, atol=1e-14)
+    assert np.allclose(h["rvf"][:8], ro["history"]["rvf"][:8], rtol=1e-8)
+    oc, m = _incl6(ctx, oracle_mod)
+    ro = oc.solve(precond=oracle_mod.PRECOND_DIC, **ORACLE_TIGHT, **kw_o)
+    out = m.solve(precision=0, **TIGHT, **kw_g)
+    err = rel_l2(out["D"], oc.get("D").reshape(-1, 3))
+    print(f"  nOuter {out['nOuter']} (oracle {ro['nOuter']}) err D {err:.2e}", flush=True)
+    assert ro["converged"] and out["converged"] and err < 1e-8
+
+
+def test_submesh_transient_steps_match_oracle(ctx, oracle_mod):
+    # Euler d2dt2 with sub-meshes: the interface state carries over between
+    # steps; every converged step matches the oracle's
+    print("[submesh transient]", flush=True)
+
+    def load(om):
+        bv = np.zeros((om.nBoundaryFaces, 3))
+        bv[om.patch_faces(1)] = (0.0, -2e6, 1e6)
+        return bv
+    oc, m = _pair(ctx, oracle_mod, (8, 4, 4), (1.0, 0.25, 0.25), 0.1, 5, lambda C: C[:, 0] > 0.5,
+                  lambda z: np.where(z == 0, 200e9, 70e9), 0.3, [FIXED] + [TRACTION] * 5, load)
+    oc.set("rho", np.full(oc.mesh.nCells, 7800.0))
+    m.set_density(7800.0)
+    kw = dict(nCorr=100000, solTol=1e-12, altTol=1e-12, pcgTol=1e-15, precond=oracle_mod.PRECOND_DIC)
+    C = oc.mesh.nCells
+    Dm, Dmm = np.zeros((C, 3)), np.zeros((C, 3))
+    for i in range(3):
+        if i:
+            oc.advance_time()
+        ro = oc.solve(d2dt2=1, deltaT=2e-5, deltaT0=2e-5, **kw)
+        m.set_old_time(Dm, Dmm)
+        out = m.solve(d2dt2=1, deltaT=2e-5, deltaT0=2e-5, nCorrectors=100000, solutionTolerance=1e-12,
+                      alternativeTolerance=1e-12, pcgTolerance=1e-15)
+        err = rel_l2(out["D"], oc.get("D").reshape(-1, 3))
+        print(f"  step {i}: nOuter {out['nOuter']} (oracle {ro['nOuter']}) err {err:.2e}", flush=True)
+        assert ro["converged"] and out["converged"] and err < 1e-8
+        Dmm, Dm = Dm, out["D"]
