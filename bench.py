@@ -615,6 +615,7 @@ def serve_run(args, ctx, requests, nmax):
     got, ids = {}, {}
     start = time.perf_counter()
     k = 0
+    shown = 0.0
     while len(got) < len(reqs):
         now = time.perf_counter() - start
         while k < len(reqs) and arrivals[k] <= now:
@@ -625,6 +626,9 @@ def serve_run(args, ctx, requests, nmax):
         while rid is not None:
             got[ids[rid]] = srv.result(rid)
             rid = srv.poll(0)
+        if now - shown >= 20.0:  # progress of a long stream
+            shown = now
+            print(f"[bench] serve {now:.0f}s: {k} submitted, {len(got)} done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - start
     srv.close()
     lat = np.array([got[i]["latency"] for i in range(len(reqs))])

@@ -3310,7 +3310,8 @@ void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
         }
     };
     putCell(S->cC.p, h.cC, 3, C, 0.0);
-    putCell(S->minv.p, h.minv, 6, C, 0.0);
+    // the whole-mesh LS inverses (zones set after submission do not reach a request)
+    putCell(S->minv.p, h.minv0.empty() ? h.minv : h.minv0, 6, C, 0.0);
     putCell(S->diag.p, h.diagBase, 1, C, 1.0);
     std::vector<double> inv(C), muv(C), Kv(C), ikv(C);
     std::vector<float> d32(Cp, 1.0f), i32(Cp, 1.0f);
