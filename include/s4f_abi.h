@@ -185,7 +185,8 @@ s4f_status s4f_mesh_tiles_set(s4f_mesh *mesh, int32_t maxTiles, int32_t minSlice
    still set per cell by s4f_material_set), or NULL for one material.
    faceCentres: the mesh's face centres, 3 values per face in face order (at
    least the nInternalFaces internal ones).  The interface state starts at
-   zero; a warm or transient solve continues it, and s4f_grad uses it as
+   zero (and the post-loop fields wait for the next solve); a warm or
+   transient solve continues it, and s4f_grad uses it as
    the last solve left it.  A mesh with zones is solved alone (batches and
    the server refuse it; zones set after a request was submitted do not
    reach that request). */

@@ -2206,6 +2206,7 @@ s4f_status s4f_mesh_zones_set(s4f_mesh* M, const int32_t* cellZone, const double
         const int C = h.C, F = h.F;
         const size_t E = h.col.size();
         if (!h.minv0.empty()) h.minv = h.minv0;  // the whole-mesh inverses back
+        M->st.lastDpar = -1;  // the last solve's fields belong to the old zones
         h.zone.clear();
         h.nIf = 0;
         h.ifCell.clear(); h.ifN.clear(); h.ifDc.clear(); h.ifLs.clear(); h.eIf.clear(); h.sliceIf.clear();
