@@ -10,6 +10,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -907,6 +908,35 @@ void launchPcgChunk(const DevMesh& dm, const BlockMap& bm, int nInst, PcgVecs<T>
 // ---------------------------------------------------------------------------
 // aggregation multigrid: host build
 // ---------------------------------------------------------------------------
+// f(begin, end) over [0, n) split into contiguous ranges on up to 16 host
+// threads (one range below `grain`); every output element is written by one
+// range, so the result does not depend on the thread count
+template <typename F>
+void parallelFor(int n, int grain, F&& f) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({hw, 16, n / std::max(1, grain)}));
+    if (nt <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    std::exception_ptr err;
+    std::mutex em;
+    auto run = [&](int b, int e) {
+        try {
+            f(b, e);
+        } catch (...) {
+            std::lock_guard<std::mutex> g(em);
+            if (!err) err = std::current_exception();
+        }
+    };
+    for (int t = 1; t < nt; ++t) th.emplace_back(run, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+    run(0, (int)((long long)n / nt));
+    for (std::thread& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
 template <typename F>
 inline void forRow(const std::vector<int>& ss, const std::vector<int>& sw, const std::vector<int>& col, int i, F&& f) {
     const int sI = i / kSlice, l = i % kSlice;
@@ -924,8 +954,11 @@ inline void forRow(const std::vector<int>& ss, const std::vector<int>& sw, const
 int aggregate(const AmgHostLevel& L, std::vector<int>& agg, int r0, int r1, int nc0) {
     const double theta = 0.25;
     std::vector<double> amax(r1 - r0, 0.0);
-    for (int i = r0; i < r1; ++i)
-        forRow(L.sliceStart, L.sliceWidth, L.col, i, [&](int e, int) { amax[i - r0] = std::max(amax[i - r0], L.a[e]); });
+    parallelFor(r1 - r0, 65536, [&](int a0, int a1) {
+        for (int i = r0 + a0; i < r0 + a1; ++i)
+            forRow(L.sliceStart, L.sliceWidth, L.col, i,
+                   [&](int e, int) { amax[i - r0] = std::max(amax[i - r0], L.a[e]); });
+    });
     int nc = nc0;
     for (int i = r0; i < r1; ++i) {
         if (agg[i] >= 0) continue;
@@ -974,28 +1007,52 @@ void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHos
             if (agg[i] >= 0) Cl.mem[fill[agg[i]]++] = i;
     }
     Cl.diag.assign(nc, 0.0);
-    std::vector<std::vector<std::pair<int, double>>> rows(nc);
-    for (int I = 0; I < nc; ++I) {
-        std::vector<std::pair<int, double>>& r = rows[I];
-        for (int k = Cl.memOff[I]; k < Cl.memOff[I + 1]; ++k) {
-            const int i = Cl.mem[k];
-            Cl.diag[I] += F.diag[i];
-            forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) {
-                const int J = agg[j];
-                if (J == I) Cl.diag[I] -= F.a[e];
-                else r.push_back({J, F.a[e]});
-            });
+    // coarse rows in contiguous ranges per thread: each range's rows (sorted
+    // by column, duplicates summed) go to that range's own entry list
+    struct Part {
+        int I0 = 0;
+        std::vector<int> off, col;
+        std::vector<double> a;
+    };
+    const int nParts = std::max(1, std::min(64, nc / 4096));
+    std::vector<Part> parts(nParts);
+    std::vector<int> width(nc, 0);
+    parallelFor(nParts, 1, [&](int p0, int p1) {
+        std::vector<std::pair<int, double>> r;
+        for (int p = p0; p < p1; ++p) {
+            Part& P = parts[p];
+            P.I0 = (int)((long long)nc * p / nParts);
+            const int I1 = (int)((long long)nc * (p + 1) / nParts);
+            P.off.assign(I1 - P.I0 + 1, 0);
+            for (int I = P.I0; I < I1; ++I) {
+                r.clear();
+                for (int k = Cl.memOff[I]; k < Cl.memOff[I + 1]; ++k) {
+                    const int i = Cl.mem[k];
+                    Cl.diag[I] += F.diag[i];
+                    forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) {
+                        const int J = agg[j];
+                        if (J == I) Cl.diag[I] -= F.a[e];
+                        else r.push_back({J, F.a[e]});
+                    });
+                }
+                std::sort(r.begin(), r.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+                    return x.first < y.first;
+                });
+                int w = 0;
+                for (size_t k = 0; k < r.size(); ++k) {
+                    if (w > 0 && P.col.back() == r[k].first) {
+                        P.a.back() += r[k].second;
+                    } else {
+                        P.col.push_back(r[k].first);
+                        P.a.push_back(r[k].second);
+                        ++w;
+                    }
+                }
+                width[I] = w;
+                P.off[I - P.I0 + 1] = P.off[I - P.I0] + w;
+            }
         }
-        std::sort(r.begin(), r.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
-            return x.first < y.first;
-        });
-        size_t w = 0;
-        for (size_t k = 0; k < r.size(); ++k) {
-            if (w > 0 && r[w - 1].first == r[k].first) r[w - 1].second += r[k].second;
-            else r[w++] = r[k];
-        }
-        r.resize(w);
-    }
+    });
     Cl.nSlices = (nc + kSlice - 1) / kSlice;
     Cl.sliceStart.resize((size_t)Cl.nSlices + 1);
     Cl.sliceWidth.resize(Cl.nSlices);
@@ -1005,7 +1062,7 @@ void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHos
         int wmax = 0;
         for (int l = 0; l < kSlice; ++l) {
             const int I = sI * kSlice + l;
-            if (I < nc) wmax = std::max(wmax, (int)rows[I].size());
+            if (I < nc) wmax = std::max(wmax, width[I]);
         }
         Cl.sliceWidth[sI] = wmax;
         E += (size_t)wmax * kSlice;
@@ -1013,14 +1070,20 @@ void galerkin(const AmgHostLevel& F, const std::vector<int>& agg, int nc, AmgHos
     Cl.sliceStart[Cl.nSlices] = (int)E;
     Cl.col.assign(E, kPad);
     Cl.a.assign(E, 0.0);
-    for (int I = 0; I < nc; ++I) {
-        const int sI = I / kSlice, l = I % kSlice;
-        for (size_t k = 0; k < rows[I].size(); ++k) {
-            const size_t e = (size_t)Cl.sliceStart[sI] + k * kSlice + l;
-            Cl.col[e] = rows[I][k].first;
-            Cl.a[e] = rows[I][k].second;
+    parallelFor(nParts, 1, [&](int p0, int p1) {
+        for (int p = p0; p < p1; ++p) {
+            const Part& P = parts[p];
+            const int n = (int)P.off.size() - 1;
+            for (int t = 0; t < n; ++t) {
+                const int I = P.I0 + t, sI = I / kSlice, l = I % kSlice;
+                for (int k = P.off[t]; k < P.off[t + 1]; ++k) {
+                    const size_t e = (size_t)Cl.sliceStart[sI] + (size_t)(k - P.off[t]) * kSlice + l;
+                    Cl.col[e] = P.col[k];
+                    Cl.a[e] = P.a[k];
+                }
+            }
         }
-    }
+    });
 }
 
 // Explicit inverse of an SPD matrix (row-major n x n) by Cholesky:
@@ -1081,13 +1144,21 @@ void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<d
     qc.assign(F.col.size(), -1);
     q.assign(F.col.size(), 0.0);
     qw.assign(F.nSlices, 0);
+    parallelFor(F.nSlices, 256, [&](int sA, int sB) {
     std::vector<std::pair<int, double>> row;
-    for (int i = 0; i < F.n; ++i) {
+    for (int i = sA * kSlice; i < std::min(F.n, sB * kSlice); ++i) {
         row.clear();
         forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) { row.push_back({F.agg[j], F.a[e]}); });
-        std::stable_sort(row.begin(), row.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
-            return x.first < y.first;
-        });
+        // stable insertion sort by aggregate (a row has few entries; no allocation)
+        for (size_t u = 1; u < row.size(); ++u) {
+            const std::pair<int, double> t = row[u];
+            size_t v = u;
+            while (v > 0 && row[v - 1].first > t.first) {
+                row[v] = row[v - 1];
+                --v;
+            }
+            row[v] = t;
+        }
         const int sI = i / kSlice, l = i % kSlice;
         int k = -1, last = -1;
         for (const std::pair<int, double>& t : row) {
@@ -1100,6 +1171,7 @@ void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<d
         }
         qw[sI] = std::max(qw[sI], k + 1);
     }
+    });
 }
 
 void buildAmg(s4f_mesh* M) {
@@ -1162,12 +1234,16 @@ void buildAmg(s4f_mesh* M) {
     std::vector<int> invOff(nInst, 0);
     if (A.h.size() > 1) {
         const AmgHostLevel& H = A.h.back();
+        std::vector<std::vector<double>> blks(nInst);
+        std::vector<char> spd(nInst, 1);
+        parallelFor(nInst, 1, [&](int i0, int i1) {
+            for (int i = i0; i < i1; ++i)
+                if (H.instN[i] > 0) spd[i] = spdInverse(H, H.inst0[i], H.instN[i], blks[i]) ? 1 : 0;
+        });
         for (int i = 0; i < nInst; ++i) {
+            if (!spd[i]) throw InvalidError("multigrid coarsest matrix is not positive definite (no fixed boundary?)");
             invOff[i] = (int)inv.size();
-            std::vector<double> blk;
-            if (H.instN[i] > 0 && !spdInverse(H, H.inst0[i], H.instN[i], blk))
-                throw InvalidError("multigrid coarsest matrix is not positive definite (no fixed boundary?)");
-            inv.insert(inv.end(), blk.begin(), blk.end());
+            inv.insert(inv.end(), blks[i].begin(), blks[i].end());
         }
     }
     hipStream_t s = M->ctx->stream;
