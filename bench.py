@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--sample", type=int, default=8, help="batch/serve: boxes solved one at a time for the baseline")
     ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--tets-per-cell", action="store_true",
+                    help="tets: one law with E per cell instead of two laws on sub-meshes")
     ap.add_argument("--repeat", type=float, default=0.2, help="serve: fraction repeating an earlier mesh")
     ap.add_argument("--slots", type=int, default=64, help="serve: in-flight solves")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serve: HBM arena")
@@ -673,13 +675,15 @@ def run_tets(args, ctx):
     import solids4foam_amd as s4f
     from solids4foam_amd import _abi, cases
     t0 = time.time()
-    pm, mesh, frac, nInc, _ = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed)
+    pm, mesh, frac, nInc, _ = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed,
+                                                   zones=not args.tets_per_cell)
     C = mesh.nCells
     print(f"[bench] tets {args.n}^3 x 6: C={C}, {nInc} inclusions, {frac:.3f} of cells soft, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
                 nCorrectors=200000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale, eagerLaunch=int(args.eager),
-                preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
+                preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL,
+                progressSeconds=30)  # minutes-long solves report their progress
     t = time.perf_counter()
     mesh.solve(want_sigma=False, precision=1, **dict(opts, nCorrectors=1))
     print(f"[bench] hierarchy + 1 outer iteration {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
@@ -700,7 +704,9 @@ def run_tets(args, ctx):
                                f"E 200 GPa with {nInc} spherical inclusions (r 0.02-0.06, 10% volume) at 20 GPa; "
                                f"nu 0.3; x-min fixed, x-max traction (0,5e6,0) Pa", "cells": int(C),
                    "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}",
-                   "gradient": "extendedLeastSquares 0 (full delta)"},
+                   "gradient": "extendedLeastSquares 0 (full delta)",
+                   "materials": ("one law, E per cell" if args.tets_per_cell else
+                                 "two laws (matrix, inclusions) on their own sub-meshes")},
         "ms_per_step": round(1e3 * tm, 1), "nOuter": om["nOuter"], "pcg_iterations": om["pcgIterations"],
         "cell_iter_per_s": round(C * om["nOuter"] / tm, 1),
         "fp64_solves_per_s": round(1.0 / tf, 6), "fp64_nOuter": of["nOuter"],

@@ -266,7 +266,9 @@ typedef struct {
        S4F_D2DT2_BACKWARD (backwardD2dt2Scheme.C:227-300: diagonal coefft^2 rho V/dt^2,
        four old-time levels, deltaT0 must be 0 or deltaT) */
     int32_t d2dt2;
-    int32_t reserved2;             /* zero */
+    int32_t progressSeconds;       /* > 0: a progress line on stderr (outer iteration, |initRes|,
+                                      residualvf) at most this often, like the reference's
+                                      per-iteration log; 0 = silent (default) */
     double deltaT;
     double deltaT0;
 } s4f_solve_opts;

@@ -50,7 +50,7 @@ class SolveOpts(ctypes.Structure):
                 ("eagerLaunch", ctypes.c_int32), ("precision", ctypes.c_int32), ("warmStart", ctypes.c_int32),
                 ("mgCoarseScale", ctypes.c_double), ("relaxD", ctypes.c_double),
                 ("relaxationMethod", ctypes.c_int32), ("mixedStallWindow", ctypes.c_int32),
-                ("d2dt2", ctypes.c_int32), ("reserved2", ctypes.c_int32), ("deltaT", ctypes.c_double),
+                ("d2dt2", ctypes.c_int32), ("progressSeconds", ctypes.c_int32), ("deltaT", ctypes.c_double),
                 ("deltaT0", ctypes.c_double)]
 
 

@@ -115,14 +115,17 @@ def inside_spheres_box(C, n, centres, radii, per_hex=1):
 
 
 def tet_inclusions(ctx, n=200, jitter=0.15, seed=2, inc_seed=3, E_matrix=200e9, E_inc=20e9, nu=0.3,
-                   traction=(0.0, 5e6, 0.0), rmin=0.02, rmax=0.06):
+                   traction=(0.0, 5e6, 0.0), rmin=0.02, rmax=0.06, zones=False):
     """The jittered n^3 cube split into 6 tetrahedra per hex (48M cells at
     n = 200), per-cell E: E_inc for cells whose centroid lies in a seeded
     spherical inclusion (10 % volume fraction), E_matrix elsewhere; x-min
     fixed, x-max shear traction, other faces traction-free.  Tetrahedra use
     the full-delta least-squares gradient (extendedLeastSquares 0): with the
-    normal-delta leastSquares the segregated loop diverges on tets.  Returns
-    (PolyMesh, SolverMesh, inclusion cell fraction, inclusion count, E per cell)."""
+    normal-delta leastSquares the segregated loop diverges on tets.  zones:
+    matrix and inclusions as two laws on their own sub-meshes (the
+    reference's route to two materials, linearElastic's E being one value
+    per law); else one per-cell law.  Returns (PolyMesh, SolverMesh,
+    inclusion cell fraction, inclusion count, E per cell)."""
     pm = PolyMesh.box(n, n, n, (1.0, 1.0, 1.0), jitter, seed, True)
     types = [FIX, TRAC, TRAC, TRAC, TRAC, TRAC]
     m = SolverMesh(ctx, pm, types, ls_full_delta=True)
@@ -130,6 +133,8 @@ def tet_inclusions(ctx, n=200, jitter=0.15, seed=2, inc_seed=3, E_matrix=200e9, 
     inc = inside_spheres_box(pm.cell_centres(), n, cen, rad, per_hex=6)
     E = np.where(inc, E_inc, E_matrix)
     m.set_material(E, nu)
+    if zones:
+        m.set_zones(inc.astype(np.int32))
     m.set_bc(0, (0.0, 0.0, 0.0))
     m.set_bc(1, traction)
     for p in range(2, 6):

@@ -1798,6 +1798,8 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
     launchResidual(s);
     HIP_CHECK(hipGetLastError());
     const int stallWindow = o.mixedStallWindow == 0 ? kMixedStallWindow : o.mixedStallWindow;
+    const auto tStart = std::chrono::steady_clock::now();
+    double tShown = 0.0;
     std::vector<double> bestRes(nInst, HUGE_VAL);
     std::vector<int> bestIt(nInst, 0);
     for (int it = 0;; ++it) {
@@ -1825,6 +1827,23 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
             if (total > o.pcgMaxIter + 2) break;  // maxIter deactivates every component before this
         }
         if (allDone) break;
+        if (o.progressSeconds > 0) {  // the per-iteration log, rate-limited
+            const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - tStart).count();
+            if (now - tShown >= o.progressSeconds) {
+                tShown = now;
+                int live = 0, iMax = 0;
+                double rMax = 0.0;
+                for (int i = 0; i < nInst; ++i) {
+                    if (hctl[i].outerDone) continue;
+                    ++live;
+                    iMax = std::max(iMax, hctl[i].iCorr);
+                    rMax = std::max(rMax, hctl[i].initResMag);
+                }
+                fprintf(stderr, "[s4f] %.0f s: outer iteration %d, %d of %d instances iterating, |initRes| %.3e, "
+                        "residualvf %.3e\n", now, iMax, live, nInst, rMax, hctl[0].rvf);
+                fflush(stderr);
+            }
+        }
         int used = 1;
         for (int i = 0; i < nInst; ++i)
             if (!hctl[i].outerDone) used = std::max({used, hctl[i].nIter[0], hctl[i].nIter[1], hctl[i].nIter[2]});

@@ -46,14 +46,18 @@ def _bar(precision):
     return 1e-8 if precision == 0 else 1e-6
 
 
+@pytest.mark.parametrize("zones", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_heterogeneous_tets_match_oracle(ctx, oracle_mod, precision):
+def test_heterogeneous_tets_match_oracle(ctx, oracle_mod, precision, zones):
     from solids4foam_amd import cases
     n = 6
     # inclusion radii scaled to the coarse mesh so that several spheres
-    # capture cell centroids (the full-size case uses 0.02-0.06 at n = 200)
-    pm, m, frac, k, E = cases.tet_inclusions(ctx, n=n, jitter=0.15, seed=2, inc_seed=3, rmin=0.12, rmax=0.22)
-    print(f"[cfg tets] {pm.nCells} tets, {k} inclusions, {frac:.1%} soft cells, precision {precision}", flush=True)
+    # capture cell centroids (the full-size case uses 0.02-0.06 at n = 200);
+    # zones: matrix and inclusions as two laws on their own sub-meshes
+    pm, m, frac, k, E = cases.tet_inclusions(ctx, n=n, jitter=0.15, seed=2, inc_seed=3, rmin=0.12, rmax=0.22,
+                                             zones=zones)
+    print(f"[cfg tets] {pm.nCells} tets, {k} inclusions, {frac:.1%} soft cells, precision {precision}, "
+          f"sub-meshes {zones}", flush=True)
     assert 0.02 < frac < 0.5
     out = m.solve(precision=precision, **TIGHT)
     om = oracle_mod.Mesh.box(n, n, n, (1.0, 1.0, 1.0), 0.15, 2, tets=True)
@@ -61,6 +65,9 @@ def test_heterogeneous_tets_match_oracle(ctx, oracle_mod, precision):
     bv = np.zeros((om.nBoundaryFaces, 3))
     bv[om.patch_faces(1)] = (0.0, 5e6, 0.0)
     oc = oracle_mod.Case(om, E, 0.3, [FIXED] + [TRACTION] * 5, bv, ls_full_delta=True)
+    if zones:
+        assert oc.set_zones((E != E.max()).astype(np.int32)) > 0
+        oc.init()
     ro = oc.solve(**ORACLE_TIGHT)
     Do, so = oc.get("D").reshape(-1, 3), oc.get("sigma").reshape(-1, 6)
     eD, es = rel_l2(out["D"], Do), rel_l2(out["sigma"], so)
