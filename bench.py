@@ -146,8 +146,8 @@ def main():
     ap.add_argument("--sample", type=int, default=8, help="batch/serve: boxes solved one at a time for the baseline")
     ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="serve: Poisson arrival rate (requests/s)")
-    ap.add_argument("--tets-per-cell", action="store_true",
-                    help="tets: one law with E per cell instead of two laws on sub-meshes")
+    ap.add_argument("--tets-zones", action="store_true",
+                    help="tets: matrix and inclusions as two laws on sub-meshes instead of one law with E per cell")
     ap.add_argument("--repeat", type=float, default=0.2, help="serve: fraction repeating an earlier mesh")
     ap.add_argument("--slots", type=int, default=64, help="serve: in-flight solves")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serve: HBM arena")
@@ -676,7 +676,7 @@ def run_tets(args, ctx):
     from solids4foam_amd import _abi, cases
     t0 = time.time()
     pm, mesh, frac, nInc, _ = cases.tet_inclusions(ctx, n=args.n, jitter=args.jitter, seed=args.seed,
-                                                   zones=not args.tets_per_cell)
+                                                   zones=args.tets_zones)
     C = mesh.nCells
     print(f"[bench] tets {args.n}^3 x 6: C={C}, {nInc} inclusions, {frac:.3f} of cells soft, set-up "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -705,8 +705,8 @@ def run_tets(args, ctx):
                                f"nu 0.3; x-min fixed, x-max traction (0,5e6,0) Pa", "cells": int(C),
                    "outer_tolerance": args.tol, "pcg": f"{args.precond}, relTol {args.reltol}",
                    "gradient": "extendedLeastSquares 0 (full delta)",
-                   "materials": ("one law, E per cell" if args.tets_per_cell else
-                                 "two laws (matrix, inclusions) on their own sub-meshes")},
+                   "materials": ("two laws (matrix, inclusions) on their own sub-meshes" if args.tets_zones else
+                                 "one law, E per cell")},
         "ms_per_step": round(1e3 * tm, 1), "nOuter": om["nOuter"], "pcg_iterations": om["pcgIterations"],
         "cell_iter_per_s": round(C * om["nOuter"] / tm, 1),
         "fp64_solves_per_s": round(1.0 / tf, 6), "fp64_nOuter": of["nOuter"],
