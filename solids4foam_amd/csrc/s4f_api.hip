@@ -3225,6 +3225,15 @@ struct Result {
 
 }  // namespace
 
+// host -> device copy on the server's own stream, complete on return (no
+// legacy-stream copy: another server's thread may be capturing a graph)
+template <typename T>
+void hcopy(hipStream_t s, T* dst, const T* src, size_t n) {
+    if (!n) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
 struct s4f_server {
     s4f_ctx* ctx = nullptr;
     s4f_solve_opts opts{};
@@ -3337,7 +3346,7 @@ void serverAllocate(s4f_server* S) {
     std::vector<Ctl> zero(S->maxSlots);
     std::memset(zero.data(), 0, zero.size() * sizeof(Ctl));
     for (Ctl& c : zero) c.outerDone = 1;
-    HIP_CHECK(hipMemcpy(S->ctl.p, zero.data(), zero.size() * sizeof(Ctl), hipMemcpyHostToDevice));
+    hcopy(S->ctx->stream, S->ctl.p, zero.data(), zero.size());
     HIP_CHECK(hipHostMalloc((void**)&S->hctl, S->maxSlots * sizeof(Ctl), hipHostMallocDefault));
     S->hiBlk0.assign(S->maxSlots, 0); S->hiNBlk.assign(S->maxSlots, 0);
     S->hiCell0.assign(S->maxSlots, 0); S->hiNCell.assign(S->maxSlots, 0);
@@ -3364,11 +3373,6 @@ DevMesh serverDm(const s4f_server* S) {
     return m;
 }
 
-template <typename T>
-void hcopy(T* dst, const T* src, size_t n) {
-    if (n) HIP_CHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-}
-
 // writes one mesh's geometry/material into its arena ranges (offsets applied)
 void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
     const size_t Cc = (size_t)S->Ccap, Ec = (size_t)S->Ecap, Bc = (size_t)S->Bcap;
@@ -3376,33 +3380,33 @@ void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
     const int c0 = (int)pl.c0, e0 = (int)pl.e0, b0 = (int)pl.b0;
     std::vector<int> ss(h.nSlices), col(E);
     for (int s = 0; s < h.nSlices; ++s) ss[s] = e0 + h.sliceStart[s];
-    hcopy(S->sliceStart.p + pl.slice0, ss.data(), ss.size());
-    hcopy(S->sliceWidth.p + pl.slice0, h.sliceWidth.data(), h.sliceWidth.size());
+    hcopy(S->ctx->stream, S->sliceStart.p + pl.slice0, ss.data(), ss.size());
+    hcopy(S->ctx->stream, S->sliceWidth.p + pl.slice0, h.sliceWidth.data(), h.sliceWidth.size());
     for (size_t e = 0; e < E; ++e) {
         const int c = h.col[e];
         col[e] = c >= 0 ? c + c0 : (c == kPad ? kPad : -1 - ((-1 - c) + b0));
     }
-    hcopy(S->col.p + e0, col.data(), E);
-    for (int q = 0; q < 3; ++q) hcopy(S->eS.p + (size_t)q * Ec + e0, h.eS.data() + (size_t)q * E, E);
-    hcopy(S->eW.p + e0, h.eW.data(), E);
-    hcopy(S->eA.p + e0, h.eA.data(), E);
+    hcopy(S->ctx->stream, S->col.p + e0, col.data(), E);
+    for (int q = 0; q < 3; ++q) hcopy(S->ctx->stream, S->eS.p + (size_t)q * Ec + e0, h.eS.data() + (size_t)q * E, E);
+    hcopy(S->ctx->stream, S->eW.p + e0, h.eW.data(), E);
+    hcopy(S->ctx->stream, S->eA.p + e0, h.eA.data(), E);
     // Rhie-Chow coefficients of the server's stabilisation scales
     if (S->opts.rhieChowScale == kRhieChowScaleDefault && S->opts.interfaceScale == kInterfaceScaleDefault) {
-        hcopy(S->eC.p + e0, h.eC.data(), E);
+        hcopy(S->ctx->stream, S->eC.p + e0, h.eC.data(), E);
     } else {
         const std::vector<double> eC = stabilisationCoeffs(h, S->opts.rhieChowScale, S->opts.interfaceScale);
-        hcopy(S->eC.p + e0, eC.data(), E);
+        hcopy(S->ctx->stream, S->eC.p + e0, eC.data(), E);
     }
     std::vector<float> a32(E);
     for (size_t e = 0; e < E; ++e) a32[e] = (float)h.eA[e];
-    hcopy(S->eA32.p + e0, a32.data(), E);
+    hcopy(S->ctx->stream, S->eA32.p + e0, a32.data(), E);
     const int C = h.C, Cp = (int)pl.nCellsPad;
     // cell arrays (padding rows: unit diagonal, zero geometry)
     std::vector<double> t(Cp);
     auto putCell = [&](double* base, const std::vector<double>& src, int k, int strideSrc, double pad) {
         for (int q = 0; q < k; ++q) {
             for (int c = 0; c < Cp; ++c) t[c] = c < C ? src[(size_t)q * strideSrc + c] : pad;
-            hcopy(base + (size_t)q * Cc + c0, t.data(), (size_t)Cp);
+            hcopy(S->ctx->stream, base + (size_t)q * Cc + c0, t.data(), (size_t)Cp);
         }
     };
     putCell(S->cC.p, h.cC, 3, C, 0.0);
@@ -3422,26 +3426,26 @@ void placeMesh(s4f_server* S, const HostMesh& h, const Placement& pl) {
     putCell(S->mu.p, muv, 1, C, 1.0);
     putCell(S->K.p, Kv, 1, C, 1.0);
     putCell(S->impK.p, ikv, 1, C, 1.0);
-    hcopy(S->diag32.p + c0, d32.data(), (size_t)Cp);
-    hcopy(S->invDiag32.p + c0, i32.data(), (size_t)Cp);
+    hcopy(S->ctx->stream, S->diag32.p + c0, d32.data(), (size_t)Cp);
+    hcopy(S->ctx->stream, S->invDiag32.p + c0, i32.data(), (size_t)Cp);
     const int B = h.B;
     std::vector<int> bc(B);
     for (int b = 0; b < B; ++b) bc[b] = h.bCell[b] + c0;
-    hcopy(S->bCell.p + b0, bc.data(), (size_t)B);
-    hcopy(S->bType.p + b0, h.bType.data(), (size_t)B);
-    hcopy(S->bDelta.p + b0, h.bDelta.data(), (size_t)B);
+    hcopy(S->ctx->stream, S->bCell.p + b0, bc.data(), (size_t)B);
+    hcopy(S->ctx->stream, S->bType.p + b0, h.bType.data(), (size_t)B);
+    hcopy(S->ctx->stream, S->bDelta.p + b0, h.bDelta.data(), (size_t)B);
     for (int q = 0; q < 3; ++q) {
-        hcopy(S->bN.p + (size_t)q * Bc + b0, h.bN.data() + (size_t)q * B, (size_t)B);
-        hcopy(S->bKv.p + (size_t)q * Bc + b0, h.bKv.data() + (size_t)q * B, (size_t)B);
-        hcopy(S->bLs.p + (size_t)q * Bc + b0, h.bLs.data() + (size_t)q * B, (size_t)B);
+        hcopy(S->ctx->stream, S->bN.p + (size_t)q * Bc + b0, h.bN.data() + (size_t)q * B, (size_t)B);
+        hcopy(S->ctx->stream, S->bKv.p + (size_t)q * Bc + b0, h.bKv.data() + (size_t)q * B, (size_t)B);
+        hcopy(S->ctx->stream, S->bLs.p + (size_t)q * Bc + b0, h.bLs.data() + (size_t)q * B, (size_t)B);
     }
 }
 
 void placeLoads(s4f_server* S, const Request& rq, const Placement& pl) {
     const size_t Bc = (size_t)S->Bcap;
     const int B = (int)pl.nB;
-    for (int q = 0; q < 3; ++q) hcopy(S->bVal.p + (size_t)q * Bc + pl.b0, rq.bVal.data() + (size_t)q * B, (size_t)B);
-    hcopy(S->bPres.p + pl.b0, rq.bPres.data(), (size_t)B);
+    for (int q = 0; q < 3; ++q) hcopy(S->ctx->stream, S->bVal.p + (size_t)q * Bc + pl.b0, rq.bVal.data() + (size_t)q * B, (size_t)B);
+    hcopy(S->ctx->stream, S->bPres.p + pl.b0, rq.bPres.data(), (size_t)B);
 }
 
 void releasePlacement(s4f_server* S, int pi) {
@@ -3586,7 +3590,7 @@ void serverAdmit(s4f_server* S) {
         c.warm = warm;
         c.sigma0 = warm ? 0 : 1;
         HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
-        HIP_CHECK(hipMemcpy(S->ctl.p + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+        hcopy(S->ctx->stream, S->ctl.p + slot, &c, 1);
         const int n = (int)pl.nCells;
         hipLaunchKernelGGL(k_slot_init, dim3(std::max(1, std::min(2048, (n + kBlock - 1) / kBlock))), dim3(kBlock), 0,
                            S->ctx->stream, S->D0.p, S->D1.p, S->Dold.p, S->G.p, S->g.p, (int)S->Ccap, (int)S->Bcap,
