@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
             acc[j] += (double)zv * (double)bv;
         }
     }
-    storeBlockPartials<3>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<3>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_rho(const BlockMap bm, Ctl* _
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     __shared__ double tot[3];
-    finReduce<3>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    finReduce<3>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         for (int j = 0; j < 3; ++j) {
             if (!ctl->active[j]) continue;

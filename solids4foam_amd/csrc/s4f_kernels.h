@@ -68,18 +68,26 @@ __device__ __forceinline__ double waveReduce(double v) {
 
 constexpr int kFinBlock = 256;
 
-// Fixed-order reduction of the instance's partial rows (one workgroup per
+// Block partials are stored value-major: value i of logical block lb sits at
+// part[i * nb + lb] (nb = the launch's logical block count), so a finalize
+// reads each value's partials as one contiguous run instead of one 128-byte
+// row per block.
+//
+// Fixed-order reduction of the instance's partials (one workgroup per
 // instance); bit i of maxMask selects a maximum instead of a sum.  Result in
 // out[0..NV) (shared), valid for every thread after the call.
 template <int NV>
-__device__ void finReduce(const double* __restrict__ part, int blk0, int nblk, unsigned maxMask, double* out) {
+__device__ void finReduce(const double* __restrict__ part, int nb, int blk0, int nblk, unsigned maxMask, double* out) {
     double v[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = 0.0;
     for (int b = threadIdx.x; b < nblk; b += kFinBlock) {
-        const double* row = part + (size_t)(blk0 + b) * kMaxPartials;
+        const double* col = part + blk0 + b;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] = ((maxMask >> i) & 1u) ? nanMax(v[i], row[i]) : v[i] + row[i];
+        for (int i = 0; i < NV; ++i) {
+            const double x = col[(size_t)i * nb];
+            v[i] = ((maxMask >> i) & 1u) ? nanMax(v[i], x) : v[i] + x;
+        }
     }
     __shared__ double sh[NV][kFinBlock / 64];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -98,9 +106,10 @@ __device__ void finReduce(const double* __restrict__ part, int blk0, int nblk, u
     __syncthreads();
 }
 
-// block-level reduction of NV per-thread values; thread 0 stores the row
+// block-level reduction of NV per-thread values; threads 0..NV-1 store them
+// at part[i * nb + lb]
 template <int NV>
-__device__ __forceinline__ void storeBlockPartials(const double* acc, unsigned maxMask, double* row) {
+__device__ __forceinline__ void storeBlockPartials(const double* acc, unsigned maxMask, double* part, int lb, int nb) {
     __shared__ double sh[NV][kBlock / 64];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
@@ -113,7 +122,7 @@ __device__ __forceinline__ void storeBlockPartials(const double* acc, unsigned m
         const int i = threadIdx.x;
         double r = sh[i][0];
         for (int k = 1; k < kBlock / 64; ++k) r = ((maxMask >> i) & 1u) ? nanMax(r, sh[i][k]) : r + sh[i][k];
-        row[i] = r;
+        part[(size_t)i * nb + lb] = r;
     }
 }
 
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const DevMesh m, const Bloc
             acc[9 + j] += (double)z * (double)rT;
         }
     }
-    storeBlockPartials<12>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<12>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // Row of the fused direction update + SpMV with all three components active
@@ -403,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_spmvp(const DevMesh m, const BlockMa
         }
     }
     if (!reduce) return;
-    storeBlockPartials<3>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<3>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const DevMesh m, const BlockM
             acc[3 + j] += fabs((double)rv);
         }
     }
-    storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<6>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // Internal-face slots of the least-squares gradient sum with compile-time
@@ -810,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const BlockMap
         }
     }
     if (mode == 2) return;
-    storeBlockPartials<6>(acc, 7u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<6>(acc, 7u, part, lb, bm.nBlocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,7 +1165,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const DevMesh m, const Blo
             acc[3 + j] += (double)z * (double)rT;
         }
     }
-    storeBlockPartials<6>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<6>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // lduMatrix::solver start for one component (component 0) of a caller's LDU
@@ -1204,7 +1213,7 @@ __global__ __launch_bounds__(kBlock) void k_ldu_start(const DevMesh m, const Blo
         acc[6] += fabs(src[P] - t);
         acc[9] += (double)(invD[P] * rT) * (double)rT;
     }
-    storeBlockPartials<12>(acc, 0u, part + (size_t)lb * kMaxPartials);
+    storeBlockPartials<12>(acc, 0u, part, lb, bm.nBlocks);
 }
 
 // ===========================================================================
@@ -1219,7 +1228,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_residual(const BlockMap bm, C
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone) return;
     __shared__ double tot[12];
-    finReduce<12>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    finReduce<12>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         double mag2 = 0.0;
         int any = 0;
@@ -1251,7 +1260,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_pcg_start(const BlockMap bm, 
     const int inst = bm.finInst ? bm.finInst[blockIdx.x] : (int)blockIdx.x;
     Ctl* ctl = ctlAll + inst;
     __shared__ double tot[6];
-    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    finReduce<6>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         int any = 0;
         for (int j = 0; j < 3; ++j) {
@@ -1276,7 +1285,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_spmv(const BlockMap bm, Ctl* 
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     __shared__ double tot[3];
-    finReduce<3>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    finReduce<3>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         int any = 0;
         for (int j = 0; j < 3; ++j) {
@@ -1300,7 +1309,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_update(const BlockMap bm, Ctl
     Ctl* ctl = ctlAll + inst;
     if (ctl->outerDone || ctl->innerDone) return;
     __shared__ double tot[6];
-    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
+    finReduce<6>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 0u, tot);
     if (threadIdx.x == 0) {
         int any = 0;
         for (int j = 0; j < 3; ++j) {
@@ -1332,7 +1341,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_grad(const BlockMap bm, Ctl* 
     Ctl* ctl = ctlAll + inst;
     if (mode == 0 && ctl->outerDone) return;
     __shared__ double tot[6];
-    finReduce<6>(part, bm.iBlk0[inst], bm.iNBlk[inst], 7u, tot);
+    finReduce<6>(part, bm.nBlocks, bm.iBlk0[inst], bm.iNBlk[inst], 7u, tot);
     if (threadIdx.x != 0) return;
     const double nc = (double)bm.iNCell[inst];
     for (int j = 0; j < 3; ++j) ctl->xbar[j] = tot[3 + j] / nc;
