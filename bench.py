@@ -6,8 +6,8 @@ z-max normal traction 1e7 Pa, other faces traction-free.
 
 A step is one converged solve (outer defect-correction loop of
 linearGeometryTotalDisplacement to solution/alternative tolerance 1e-8; inner
-PCG with the aggregation-multigrid V-cycle preconditioner, tolerance 1e-9,
-relTol 0.9).  The converged field is the fixed point of the FP64 residual and
+PCG with the aggregation-multigrid V-cycle preconditioner, smoothed first
+prolongation, tolerance 1e-9, relTol 0.9).  The converged field is the fixed point of the FP64 residual and
 does not depend on the inner solver settings.  `value` is the mixed-precision
 path (FP32 PCG and V-cycle inside the FP64 defect loop); the plain FP64 path
 is timed beside it and serves as the FP64 reference for the displacement /
@@ -27,12 +27,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 
-def algorithmic_bytes(kernel, C, F, B, T, q_entries=None):
+def algorithmic_bytes(kernel, C, F, B, T, q_entries=None, smoothed=False):
     """Compulsory HBM bytes per launch: each array a kernel touches counted
     once; neighbour gathers assumed cache-resident.  C cells, F internal faces,
     B boundary faces, T = bytes of the PCG scalar (8 FP64, 4 FP32),
-    q_entries = stored entries of the multigrid's level-0 grouped operator
-    (given: the multigrid-preconditioned PCG's kernels)."""
+    q_entries = stored entries of the multigrid's level-0 up-sweep operator
+    (q = offdiag(A) P, or G on a smoothed hierarchy; given: the
+    multigrid-preconditioned PCG's kernels)."""
     if kernel == "spmv" and q_entries is not None:  # z, p old/new, q, diag per cell; a + col per entry
         return C * 13 * T + F * 2 * (T + 4)
     if kernel == "spmv":            # r, 1/diag, p old/new, q, diag per cell; a + col per ELL entry (2 per face)
@@ -43,13 +44,19 @@ def algorithmic_bytes(kernel, C, F, B, T, q_entries=None):
         return 0
     if kernel == "mg0_down":        # b, 1/diag, diag; r write; a + col per ELL entry
         return C * 8 * T + F * 2 * (T + 4)
-    if kernel == "mg0_up":          # b, r, 1/diag, agg; z write; q + column per entry (coarse e cache-resident)
-        return C * (10 * T + 4) + q_entries * (T + 4)
+    if kernel == "mg0_up":          # b, r, 1/diag, agg (not with G); z write; q or G + column per entry
+        return C * (10 * T + (0 if smoothed else 4)) + q_entries * (T + 4)
     if kernel == "residual":        # D, gradD, C, diag + r, p writes; per entry col, S, w, a, c; boundary S, sigma_b
         return C * (24 + 72 + 24 + 8 + 6 * T) + F * 2 * (4 + 24 + 8 + 8 + 8) + B * (4 + 24 + 48)
     if kernel == "grad":            # D, x, C, Minv, gradD w, D' w; col per entry; boundary data r/w
         return C * (24 + 3 * T + 24 + 48 + 72 + 24) + F * 2 * 4 + B * (4 + 24 + 8 + 24 + 24 + 24 + 8 + 24 * 6 + 72 + 48)
     raise KeyError(kernel)
+
+
+def precond_id(name):
+    from solids4foam_amd import _abi
+    return {"diagonal": _abi.PRECOND_DIAGONAL, "gamg": _abi.PRECOND_GAMG,
+            "gamg-sa": _abi.PRECOND_GAMG_SMOOTHED}[name]
 
 
 def rel_l2(a, b):
@@ -106,8 +113,9 @@ def main():
     ap.add_argument("--reltol", type=float, default=0.9)
     ap.add_argument("--pcg-maxiter", type=int, default=1000, help="PCG iteration cap per outer iteration")
     ap.add_argument("--mg-scale", type=float, default=0.0, help="GAMG coarse-correction scale (0: library default)")
-    ap.add_argument("--precond", choices=["diagonal", "gamg"], default="gamg",
-                    help="PCG preconditioner of the device solves")
+    ap.add_argument("--precond", choices=["diagonal", "gamg", "gamg-sa"], default=None,
+                    help="PCG preconditioner of the device solves (gamg-sa: aggregation multigrid with a smoothed "
+                         "first prolongation; default gamg-sa, gamg for --config tets; the serving stream keeps gamg)")
     ap.add_argument("--replay", type=int, default=40, help="back-to-back launches per kernel for the roofline")
     ap.add_argument("--cpu-iters", type=int, default=5, help="CPU baseline: timed outer iterations")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: OMP_NUM_THREADS or nproc)")
@@ -158,6 +166,8 @@ def main():
                     help="batch/serve: fixed under-relaxation of D (the sweep reaches nu = 0.45, where the "
                          "unrelaxed segregated loop oscillates)")
     args = ap.parse_args()
+    if args.precond is None:
+        args.precond = "gamg" if args.config == "tets" else "gamg-sa"
     if args.trace_summary:
         trace_summary(args.trace_summary)
         return
@@ -192,7 +202,7 @@ def main():
           f"device={info['deviceBytes'] / 2**30:.2f} GiB setup={setup_s:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
                 nCorrectors=100000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale, eagerLaunch=int(args.eager),
-                preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
+                preconditioner=precond_id(args.precond))
     L = _abi.lib()
 
     def run(precision, steps, profile):
@@ -218,7 +228,7 @@ def main():
         if args.replay <= 0:
             return prof
         ks = dict(_abi.KERNELS)
-        if args.precond == "gamg":
+        if args.precond != "diagonal":
             ks.update(_abi.MG_KERNELS)
         for k, kid in ks.items():
             n = args.replay if k not in ("residual", "grad") else max(4, args.replay // 4)
@@ -235,16 +245,17 @@ def main():
         mesh.solve(want_sigma=False, precision=precision, **dict(opts, nCorrectors=1, pcgMaxIter=2))
         prof = replay_all(precision)
         T = 8 if precision == 0 else 4
-        qn = mesh.mg_info()["qEntries"] if args.precond == "gamg" else None
+        qn = mesh.mg_info()["qEntries"] if args.precond != "diagonal" else None
         out = {k: dict(avg_ms=round(p["avg_ms"], 5), launches=p["launches"],
-                       gbps=(round(algorithmic_bytes(k, C, F, B, T, qn) / (p["avg_ms"] * 1e-3) / 1e9, 1)
+                       gbps=(round(algorithmic_bytes(k, C, F, B, T, qn, args.precond == "gamg-sa") /
+                                   (p["avg_ms"] * 1e-3) / 1e9, 1)
                              if k not in ("finalize", "vcycle") else None)) for k, p in prof.items()}
         print(json.dumps(dict(mode="kernels-only", cells=C, precision="fp64" if precision == 0 else "mixed",
                               kernels=out)))
         return
 
     amg_setup_s = None
-    if args.precond == "gamg":
+    if args.precond != "diagonal":
         # the multigrid hierarchy is built once per matrix (like OpenFOAM's
         # cached GAMG agglomeration): build it outside the timed region
         t = time.perf_counter()
@@ -267,7 +278,7 @@ def main():
     nOuter = outs[-1]["nOuter"]
     kern = {}
     pcg_it = outs[-1]["pcgIterations"]
-    mg = args.precond == "gamg"
+    mg = args.precond != "diagonal"
     per_solve = {"spmv": pcg_it, "pcg_update": pcg_it, "finalize": (3 if mg else 2) * pcg_it, "residual": nOuter,
                  "grad": nOuter, "mg0_down": pcg_it + nOuter, "mg0_up": pcg_it + nOuter, "vcycle": pcg_it + nOuter}
     mgi = mesh.mg_info() if mg else None
@@ -275,7 +286,7 @@ def main():
         e = dict(avg_ms=round(p["avg_ms"], 5), replay_launches=p["launches"], launches_per_solve=per_solve[k],
                  ms_per_solve_est=round(p["avg_ms"] * per_solve[k], 1))
         if k not in ("finalize", "vcycle"):
-            b = algorithmic_bytes(k, C, F, B, T, mgi["qEntries"] if mgi else None)
+            b = algorithmic_bytes(k, C, F, B, T, mgi["qEntries"] if mgi else None, args.precond == "gamg-sa")
             e.update(alg_bytes=int(b), gbps=round(b / (p["avg_ms"] * 1e-3) / 1e9, 1))
         kern[k] = e
     dom = max((k for k in kern if k not in ("finalize", "vcycle")), key=lambda k: kern[k]["ms_per_solve_est"]) \
@@ -498,18 +509,20 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
                       indent=1, sort_keys=True)
 
 
-def _sequential(ctx, cases_, specs, opts, label, tune=True):
-    """Plain FP64 solves of `specs`, one at a time (GAMG preconditioner, the
-    fastest single-mesh path): per-solve seconds and fields."""
+def _sequential(ctx, cases_, specs, opts, label, tune=True, pc=None):
+    """Plain FP64 solves of `specs`, one at a time (the multigrid
+    preconditioner of the run, the fastest single-mesh path): per-solve
+    seconds and fields."""
     from solids4foam_amd import _abi, autotune
+    pc = _abi.PRECOND_GAMG if pc is None else pc
     table = autotune.load_table() if tune else {}
     secs, Ds = [], []
     for sp in specs:
         _, m, _ = cases_.box_case(ctx, sp)
         autotune.apply(m, table)
-        m.solve(want_sigma=False, precision=0, preconditioner=_abi.PRECOND_GAMG, **dict(opts, nCorrectors=1))
+        m.solve(want_sigma=False, precision=0, preconditioner=pc, **dict(opts, nCorrectors=1))
         t = time.perf_counter()
-        out = m.solve(want_sigma=False, precision=0, preconditioner=_abi.PRECOND_GAMG, **opts)
+        out = m.solve(want_sigma=False, precision=0, preconditioner=pc, **opts)
         secs.append(time.perf_counter() - t)
         Ds.append(out["D"])
         print(f"[bench] {label} {sp['n']}: nOuter={out['nOuter']} converged={out['converged']} {secs[-1]:.2f}s",
@@ -549,7 +562,7 @@ def batch_run(args, ctx, count, nmax):
     print(f"[bench] batch of {len(specs)} boxes, {cells} cells, set-up {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
     from solids4foam_amd import _abi
-    bopts = dict(opts, preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL)
+    bopts = dict(opts, preconditioner=precond_id(args.precond))
     # pack, multigrid hierarchy and first launch outside the clock
     b.solve(want_sigma=False, precision=1, **dict(bopts, nCorrectors=1))
     t = time.perf_counter()
@@ -560,7 +573,8 @@ def batch_run(args, ctx, count, nmax):
     print(f"[bench] batched mixed: {T:.2f}s, converged {nconv}/{len(res)}, outer median {np.median(outer):.0f} "
           f"max {outer.max()}", file=sys.stderr, flush=True)
     idx = list(range(min(args.sample, len(specs))))
-    secs, Ds = _sequential(ctx, cases, [specs[i] for i in idx], opts, "sequential fp64", not args.no_tune)
+    seq_pc = None if args.precond != "gamg-sa" else precond_id(args.precond)
+    secs, Ds = _sequential(ctx, cases, [specs[i] for i in idx], opts, "sequential fp64", not args.no_tune, seq_pc)
     errs = [rel_l2(res[i]["D"], D) for i, D in zip(idx, Ds)]
     seq_rate = len(secs) / sum(secs)
     value = len(res) / T
@@ -571,7 +585,8 @@ def batch_run(args, ctx, count, nmax):
         "config": {"workload": f"{len(specs)} boxes n in 8..{nmax} per direction, jitter 0.1, E log-normal "
                                f"(median 100 GPa, sigma 0.5), nu U(0.2,0.45), x-max traction U(1e5,1e7) Pa random "
                                f"direction, seed {args.seed}", "cells": int(cells), "outer_tolerance": args.sweep_tol,
-                   "pcg": f"{args.precond}, relTol {args.reltol} (batched); gamg (sequential)",
+                   "pcg": f"{args.precond}, relTol {args.reltol} (batched); "
+                          f"{'gamg' if args.precond == 'diagonal' else args.precond} (sequential)",
                    "relaxD": args.relax, "relaxation": args.relax_method},
         "batch_seconds": round(T, 3), "converged": int(nconv), "outer_median": float(np.median(outer)),
         "outer_max": int(outer.max()),
@@ -613,7 +628,7 @@ def serve_run(args, ctx, requests, nmax):
     opts = sweep_opts(args)
     from solids4foam_amd import _abi
     srv = Server(ctx, int(args.budget_gib * (1 << 30)), max_slots=args.slots, precision=1,
-                 preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL, **opts)
+                 preconditioner=_abi.PRECOND_DIAGONAL if args.precond == "diagonal" else _abi.PRECOND_GAMG, **opts)
     got, ids = {}, {}
     start = time.perf_counter()
     k = 0
@@ -647,7 +662,9 @@ def serve_run(args, ctx, requests, nmax):
     print(f"[bench] served {len(reqs)} in {wall:.1f}s: {value:.2f} solves/s p50 {np.percentile(lat, 50):.2f}s "
           f"p99 {np.percentile(lat, 99):.2f}s converged {conv} warm {warm}", file=sys.stderr, flush=True)
     idx = [i for i in range(len(reqs)) if not reqs[i][2]][:args.sample]
-    secs, Ds = _sequential(ctx, cases, [specs[reqs[i][0]] for i in idx], opts, "sequential fp64", not args.no_tune)
+    seq_pc = None if args.precond != "gamg-sa" else precond_id(args.precond)
+    secs, Ds = _sequential(ctx, cases, [specs[reqs[i][0]] for i in idx], opts, "sequential fp64", not args.no_tune,
+                           seq_pc)
     errs = [rel_l2(got[i]["D"], D) for i, D in zip(idx, Ds)]
     seq_rate = len(secs) / sum(secs)
     line = {
@@ -657,7 +674,9 @@ def serve_run(args, ctx, requests, nmax):
         "config": {"workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, boxes n in 8..{nmax}, "
                                f"{args.repeat:.0%} repeat an earlier mesh with a new load (warm start), seed "
                                f"{args.seed}", "slots": args.slots, "budget_GiB": args.budget_gib,
-                   "outer_tolerance": args.sweep_tol, "pcg": f"{args.precond}, relTol {args.reltol}",
+                   "outer_tolerance": args.sweep_tol,
+                   "pcg": f"{'diagonal' if args.precond == 'diagonal' else 'gamg'} (served), relTol {args.reltol}; "
+                          f"sequential {'gamg' if args.precond == 'diagonal' else args.precond}",
                    "relaxD": args.relax, "relaxation": args.relax_method},
         "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
@@ -682,7 +701,7 @@ def run_tets(args, ctx):
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     opts = dict(solutionTolerance=args.tol, alternativeTolerance=args.tol, pcgTolerance=1e-9, pcgRelTol=args.reltol,
                 nCorrectors=200000, pcgMaxIter=args.pcg_maxiter, mgCoarseScale=args.mg_scale, eagerLaunch=int(args.eager),
-                preconditioner=_abi.PRECOND_GAMG if args.precond == "gamg" else _abi.PRECOND_DIAGONAL,
+                preconditioner=precond_id(args.precond),
                 progressSeconds=30)  # minutes-long solves report their progress
     t = time.perf_counter()
     mesh.solve(want_sigma=False, precision=1, **dict(opts, nCorrectors=1))

@@ -77,7 +77,10 @@ enum {
 /* inner preconditioner (fvSolution "preconditioner") */
 enum {
     S4F_PRECOND_DIAGONAL = 0,  /* OpenFOAM "diagonal" (Jacobi) */
-    S4F_PRECOND_GAMG = 2       /* aggregation multigrid V(1,1) cycle (PCG with preconditioner GAMG) */
+    S4F_PRECOND_GAMG = 2,      /* aggregation multigrid V(1,1) cycle (PCG with preconditioner GAMG) */
+    S4F_PRECOND_GAMG_SMOOTHED = 3  /* the same cycle with a smoothed prolongation P = (I - 2/3 D^-1 A) P_agg
+                                      at the first coarsening (smoothed aggregation): fewer PCG iterations
+                                      on large meshes; s4f_solve and s4f_batch_solve (the server refuses it) */
 };
 
 /* ------------------------------------------------------------------ */

@@ -31,6 +31,7 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECOND_DIAGONAL = 0
 PRECOND_GAMG = 2
+PRECOND_GAMG_SMOOTHED = 3
 
 _STATUS_NAMES = {0: "OK", 1: "ERR_INVALID", 2: "ERR_HIP", 3: "ERR_OOM", 4: "ERR_DIVERGED", 5: "WARN_MAXITER",
                  6: "ERR_ADMISSION", 7: "ERR_NOT_READY"}

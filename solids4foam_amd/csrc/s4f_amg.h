@@ -31,6 +31,12 @@
 namespace s4f {
 
 constexpr int kCoarseBlock = 1024;  // rows per instance on the coarsest level (explicit inverse)
+constexpr int kMgChunk = 8;         // slots per unconditional load group of the wide (Galerkin / G) rows
+// smoothed hierarchy: relative size below which first-coarse-level entries
+// are lumped into the diagonal (|a_IJ| < f sqrt(|a_II a_JJ|)).  Off: lumping
+// keeps row sums but can make the coarse operator indefinite (it did on a
+// 384-tetrahedron mesh at f = 0.04), and the cycle must stay SPD.
+constexpr double kMgSmoothedFilter = 0.0;
 constexpr double kMgCoarseScaleDefault = 1.8;  // coarse-correction scale when the caller gives 0
 // Jacobi damping of the smoothing sweeps: near the best smoothing factor of
 // damped Jacobi for 7-point-like operators in 3-D (6/7) and below 1, where
@@ -54,6 +60,13 @@ struct MgLevel {
     const int* agg;    // row -> row of the next coarser level (nullptr on the coarsest)
     const int* memOff; // (coarse side) row -> first member of the finer level
     const int* mem;    // members (rows of the finer level)
+    const float* memW32;   // member weights (smoothed prolongation; nullptr: all 1)
+    const double* memW64;
+    const int* rStart;     // the weighted member lists as sliced ELL (smoothed prolongation only)
+    const int* rWidth;
+    const int* rMem;
+    const float* rW32;
+    const double* rW64;
     const int* rowInst;  // row -> instance
     const int* blkRow0;  // coarsest level, per instance: first row of its block
     const int* blkN;     //   rows of its block
@@ -68,12 +81,16 @@ struct MgLevel {
 template <typename T> struct MgSel;
 template <> struct MgSel<double> {
     __device__ static const double* q(const MgLevel& L) { return L.q64; }
+    __device__ static const double* memW(const MgLevel& L) { return L.memW64; }
+    __device__ static const double* rW(const MgLevel& L) { return L.rW64; }
     __device__ static const double* a(const MgLevel& L) { return L.a64; }
     __device__ static const double* d(const MgLevel& L) { return L.diag64; }
     __device__ static const double* iD(const MgLevel& L) { return L.invD64; }
 };
 template <> struct MgSel<float> {
     __device__ static const float* q(const MgLevel& L) { return L.q32; }
+    __device__ static const float* memW(const MgLevel& L) { return L.memW32; }
+    __device__ static const float* rW(const MgLevel& L) { return L.rW32; }
     __device__ static const float* a(const MgLevel& L) { return L.a32; }
     __device__ static const float* d(const MgLevel& L) { return L.diag32; }
     __device__ static const float* iD(const MgLevel& L) { return L.invD32; }
@@ -307,12 +324,76 @@ __global__ __launch_bounds__(kBlock) void k_mg0_up(const DevMesh m, const BlockM
     storeBlockPartials<3>(acc, 0u, part, lb, bm.nBlocks);
 }
 
+// Prolongation + post-smoothing on level 0 of a smoothed hierarchy (the first
+// prolongation P = (I - wp D^-1 A) P_aggregation).  The corrected
+// xf = x + alpha P e and its Jacobi sweep reduce as in k_mg0_up to
+//   z = w D^-1 (b + r0) + alpha w D^-1 sum_J G_J e_J,   G = (D / w) P - A P,
+// with G host-built per row in its own sliced-ELL slots (an empty slot
+// repeats the row's first column with a zero weight, so every load is
+// unconditional and in bounds).  Partial z.b per block.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg0_up_g(const DevMesh m, const BlockMap bm, const T* __restrict__ b,
+                                                     const T* __restrict__ r0, const int* __restrict__ gStart,
+                                                     const int* __restrict__ gWidth, const int* __restrict__ gCol,
+                                                     const T* __restrict__ g, const Row4<T>* __restrict__ e,
+                                                     T* __restrict__ z, double omega, double alpha,
+                                                     const Ctl* __restrict__ ctl, double* __restrict__ part) {
+    const int lb = logicalBlock(bm.nBlocks);
+    const int inst = bm.inst[lb];
+    const Ctl* c = ctl + inst;
+    if (mgSkip(c)) return;
+    const int C = m.nCells;
+    const T* __restrict__ invD = MatSel<T>::invDiag(m);
+    const bool act[3] = {c->active[0] != 0, c->active[1] != 0, c->active[2] != 0};
+    const T w = (T)omega, al = (T)alpha;
+    double acc[3] = {0, 0, 0};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s1 = bm.s1[lb];
+    const int cEnd = bm.iCell0[inst] + bm.iNCell[inst];
+    for (int s = bm.s0[lb] + wave; s < s1; s += kBlock / 64) {
+        const int P = s * kSlice + lane;
+        if (P >= cEnd) continue;
+        const int e0 = gStart[s];
+        const int width = gWidth[s];
+        T sum[3] = {(T)0, (T)0, (T)0};
+        for (int k0 = 0; k0 < width; k0 += kMgChunk) {
+            int J[kMgChunk];
+            T cw[kMgChunk];
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u) {
+                const int k = k0 + u < width ? k0 + u : width - 1;
+                const int en = e0 + k * kSlice + lane;
+                J[u] = ldS(gCol + en);
+                cw[u] = k0 + u < width ? ldS(g + en) : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u) {
+                const Row4<T> ev = e[J[u]];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sum[j] += cw[u] * ev[j];
+            }
+        }
+        const T wi = w * invD[P];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (!act[j]) continue;
+            const size_t i = (size_t)j * C + P;
+            const T bv = b[i];
+            const T zv = wi * (bv + r0[i]) + al * (wi * sum[j]);
+            stS(z + i, zv);
+            acc[j] += (double)zv * (double)bv;
+        }
+    }
+    storeBlockPartials<3>(acc, 0u, part, lb, bm.nBlocks);
+}
+
 // ---------------------------------------------------------------------------
 // coarse levels (grid-stride over the slices of one level)
 // ---------------------------------------------------------------------------
 
 // restriction: bc[I] = sum over members i of I of r[i] (r of the finer
-// level: component-major on level 0 when FINE, rows of four otherwise), and
+// level: component-major on level 0 when FINE, rows of four otherwise; each
+// member weighted when the prolongation into I is smoothed), and
 // the coarse pre-smoothed value xc[I] = w D^-1 bc[I] (the coarse level's
 // smoothing from zero, formed once here instead of at every use)
 template <typename T, bool FINE>
@@ -321,6 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
                                                         Row4<T>* __restrict__ bc, Row4<T>* __restrict__ xc,
                                                         double omega, const Ctl* __restrict__ ctl) {
     const T* __restrict__ iDg = MgSel<T>::iD(Lc);
+    const T* __restrict__ mw = MgSel<T>::memW(Lc);
     const T w = (T)omega;
     for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
         if (Lc.rowInst[I] < 0) continue;  // padding row
@@ -331,8 +413,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
         // member again, weight zero), in member order
         for (int k0 = Lc.memOff[I]; k0 < k1; k0 += 8) {
             int mi[8];
+            T wv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) mi[u] = Lc.mem[k0 + u < k1 ? k0 + u : k1 - 1];
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u < k1 ? k0 + u : k1 - 1;
+                mi[u] = Lc.mem[k];
+                wv[u] = mw ? mw[k] : (T)1;
+            }
             T rv[8][3];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -349,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
             for (int u = 0; u < 8; ++u)
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
-                    if (k0 + u < k1) s[j] += rv[u][j];
+                    if (k0 + u < k1) s[j] += wv[u] * rv[u][j];
         }
         // all three components stored: a converged component's coarse
         // values are never read for an active one
@@ -360,10 +447,52 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
 }
 
 // Coarse-level rows are wider (Galerkin stencils) and of varying width:
-// slots are processed 8 at a time with every load unconditional (slots past
-// the row's width or empty read an in-bounds slot and the row's own value
-// with a zero coefficient).
-constexpr int kMgChunk = 8;
+// slots are processed kMgChunk at a time with every load unconditional (slots
+// past the row's width or empty read an in-bounds slot and the row's own
+// value with a zero coefficient).
+
+// restriction into the first coarse level of a smoothed hierarchy: bc = P^T r0
+// with the weighted member lists in the coarse level's sliced-ELL slots (one
+// wave per 64 coarse rows, each slot one coalesced load of members and
+// weights; members in ascending fine-row order), and xc = w D^-1 bc
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict_w(const MgLevel Lc, const T* __restrict__ r0, int nFine,
+                                                          Row4<T>* __restrict__ bc, Row4<T>* __restrict__ xc,
+                                                          double omega, const Ctl* __restrict__ ctl) {
+    const T* __restrict__ iDg = MgSel<T>::iD(Lc);
+    const T* __restrict__ rw = MgSel<T>::rW(Lc);
+    const T w = (T)omega;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wavesTotal = gridDim.x * (kBlock / 64);
+    for (int s = blockIdx.x * (kBlock / 64) + wave; s < Lc.nSlices; s += wavesTotal) {
+        const int I = s * kSlice + lane;
+        if (I >= Lc.n) continue;
+        const int inst = Lc.rowInst[I];
+        if (inst < 0) continue;  // padding row
+        if (mgSkip(ctl + inst)) continue;
+        const int e0 = Lc.rStart[s];
+        const int width = Lc.rWidth[s];
+        T sum[3] = {(T)0, (T)0, (T)0};
+        for (int k0 = 0; k0 < width; k0 += kMgChunk) {
+            int mi[kMgChunk];
+            T wv[kMgChunk];
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u) {
+                const int k = k0 + u < width ? k0 + u : width - 1;
+                const int en = e0 + k * kSlice + lane;
+                mi[u] = ldS(Lc.rMem + en);
+                wv[u] = k0 + u < width ? ldS(rw + en) : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kMgChunk; ++u)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sum[j] += wv[u] * r0[(size_t)j * nFine + mi[u]];
+        }
+        const T wi = w * iDg[I];
+        bc[I] = row4(sum[0], sum[1], sum[2]);
+        xc[I] = row4(wi * sum[0], wi * sum[1], wi * sum[2]);
+    }
+}
 
 // down on a coarse level: r = b - A x with x = w D^-1 b from the restriction
 template <typename T>

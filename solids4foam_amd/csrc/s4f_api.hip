@@ -231,6 +231,16 @@ struct AmgHostLevel {
     std::vector<int> inst0, instN; // per instance: first row and row count (instances are contiguous)
     std::vector<int> rowInst;      // row -> instance (-1: padding row)
     std::vector<int> instPass;     // per instance: 1 if it passes unchanged to the next level (already coarsest)
+    std::vector<double> memW;      // member weights (smoothed prolongation into this level; empty: all 1)
+    // the weighted member lists again as sliced ELL over this level's rows
+    // (slot k of a slice: member k of each row; an empty slot repeats the
+    // row's first member with a zero weight): the restriction's coalesced form
+    std::vector<int> rStart, rWidth, rMem;
+    std::vector<double> rW;
+    // level 0 of a smoothed hierarchy: the up-sweep operator G = (D/w) P - A P
+    // in its own sliced-ELL layout (per slice: first entry and width)
+    std::vector<int> gStart, gWidth, gCol;
+    std::vector<double> g;
 };
 struct AmgDevLevel {
     int n = 0, nSlices = 0;
@@ -246,12 +256,21 @@ struct AmgDevLevel {
     DArr<int> qCol, qWidth;  // qWidth: per slice, the filled q slots (coarse levels)
     DArr<float> q32;
     DArr<double> q64;
+    DArr<float> memW32;      // restriction weights into this level (smoothed prolongation)
+    DArr<double> memW64;
+    DArr<int> rStart, rWidth, rMem;  // the same lists as sliced ELL
+    DArr<float> rW32;
+    DArr<double> rW64;
+    DArr<int> gStart, gWidth, gCol;  // level 0 of a smoothed hierarchy: G = (D/w) P - A P
+    DArr<float> g32;
+    DArr<double> g64;
     MgLevel view() const {
         MgLevel L{};
         L.n = n; L.nSlices = nSlices;
         L.sliceStart = sliceStart.p; L.sliceWidth = sliceWidth.p; L.col = col.p;
         L.a32 = a32.p; L.a64 = a64.p; L.diag32 = d32.p; L.diag64 = d64.p; L.invD32 = iD32.p; L.invD64 = iD64.p;
-        L.agg = agg.p; L.memOff = memOff.p; L.mem = mem.p;
+        L.agg = agg.p; L.memOff = memOff.p; L.mem = mem.p; L.memW32 = memW32.p; L.memW64 = memW64.p;
+        L.rStart = rStart.p; L.rWidth = rWidth.p; L.rMem = rMem.p; L.rW32 = rW32.p; L.rW64 = rW64.p;
         L.rowInst = rowInst.p; L.blkRow0 = blkRow0.p; L.blkN = blkN.p; L.invOff = invOff.p; L.pass = instPass.p;
         L.qCol = qCol.p; L.q32 = q32.p; L.q64 = q64.p; L.qWidth = qWidth.p;
         return L;
@@ -259,6 +278,7 @@ struct AmgDevLevel {
 };
 struct Amg {
     unsigned gen = ~0u;  // material generation the hierarchy was built for
+    bool smoothed = false;  // smoothed prolongation at the first coarsening
     std::vector<AmgHostLevel> h;
     std::vector<std::unique_ptr<AmgDevLevel>> d;
     double omega = kMgOmega;
@@ -270,7 +290,9 @@ struct Amg {
 struct s4f_mesh {
     s4f_ctx* ctx = nullptr;
     HostMesh h;
-    Amg amg;
+    Amg amg;     // plain aggregation (S4F_PRECOND_GAMG; the server packs these)
+    Amg amgSm;   // smoothed first prolongation (S4F_PRECOND_GAMG_SMOOTHED)
+    Amg* amgLast = nullptr;  // hierarchy of the last multigrid solve
     // device geometry / addressing
     DArr<int> sliceStart, sliceWidth, col, bCell, bType;
     DArr<double> eS, eW, eA, eC, cC, minv, diag, invDiag, mu, K, impK, bN, bKv, bDelta, bLs, bVal, bPres;
@@ -825,7 +847,7 @@ Ctl makeCtl(const s4f_solve_opts& o, int histCap) {
 }
 
 void singleMeshOnly(const s4f_solve_opts& o, const char* entry) {
-    if (o.preconditioner == S4F_PRECOND_GAMG)
+    if (o.preconditioner == S4F_PRECOND_GAMG || o.preconditioner == S4F_PRECOND_GAMG_SMOOTHED)
         throw InvalidError(std::string(entry) + ": the multigrid preconditioner is not available here "
                            "(s4f_solve, s4f_batch_solve and the server have it)");
 }
@@ -834,7 +856,8 @@ void checkOpts(const s4f_solve_opts& o) {
     if (o.nCorrectors < 1) throw InvalidError("nCorrectors must be >= 1");
     if (o.pcgMaxIter < 0) throw InvalidError("maxIter must be >= 0");
     if (o.precision != S4F_PRECISION_FP64 && o.precision != S4F_PRECISION_MIXED) throw InvalidError("unknown precision");
-    if (o.preconditioner != S4F_PRECOND_DIAGONAL && o.preconditioner != S4F_PRECOND_GAMG)
+    if (o.preconditioner != S4F_PRECOND_DIAGONAL && o.preconditioner != S4F_PRECOND_GAMG &&
+        o.preconditioner != S4F_PRECOND_GAMG_SMOOTHED)
         throw InvalidError("unknown preconditioner");
     if (!(o.mgCoarseScale >= 0.0 && o.mgCoarseScale <= 2.0)) throw InvalidError("mgCoarseScale must be in [0, 2]");
     if (!(o.relaxD >= 0.0 && o.relaxD <= 1.0)) throw InvalidError("relaxD must be in [0, 1]");
@@ -1174,9 +1197,324 @@ void groupByAggregate(const AmgHostLevel& F, std::vector<int>& qc, std::vector<d
     });
 }
 
-void buildAmg(s4f_mesh* M) {
-    Amg& A = M->amg;
+// Row-wise sparse vectors of the smoothed first coarsening (CSR by row).
+struct SparseRows {
+    std::vector<long long> off;
+    std::vector<int> col;
+    std::vector<double> val;
+};
+
+// entries of one row sorted by column (stable: equal columns keep their
+// order) and equal columns summed in that order, so the result depends only
+// on the row
+inline void sortMergeRow(std::vector<std::pair<int, double>>& r) {
+    for (size_t u = 1; u < r.size(); ++u) {
+        const std::pair<int, double> t = r[u];
+        size_t v = u;
+        while (v > 0 && r[v - 1].first > t.first) {
+            r[v] = r[v - 1];
+            --v;
+        }
+        r[v] = t;
+    }
+    size_t w = 0;
+    for (size_t u = 0; u < r.size(); ++u) {
+        if (w > 0 && r[w - 1].first == r[u].first) r[w - 1].second += r[u].second;
+        else r[w++] = r[u];
+    }
+    r.resize(w);
+}
+
+// rows [0, n) of a sparse operator from a row builder, in two threaded passes
+// (sizes, then entries)
+template <typename F>
+SparseRows buildRows(int n, F&& row) {
+    SparseRows S;
+    S.off.assign((size_t)n + 1, 0);
+    parallelFor(n, 4096, [&](int a, int b) {
+        std::vector<std::pair<int, double>> r;
+        for (int i = a; i < b; ++i) {
+            row(i, r);
+            S.off[(size_t)i + 1] = (long long)r.size();
+        }
+    });
+    for (int i = 0; i < n; ++i) S.off[(size_t)i + 1] += S.off[i];
+    S.col.resize((size_t)S.off[n]);
+    S.val.resize((size_t)S.off[n]);
+    parallelFor(n, 4096, [&](int a, int b) {
+        std::vector<std::pair<int, double>> r;
+        for (int i = a; i < b; ++i) {
+            row(i, r);
+            for (size_t k = 0; k < r.size(); ++k) {
+                S.col[(size_t)S.off[i] + k] = r[k].first;
+                S.val[(size_t)S.off[i] + k] = r[k].second;
+            }
+        }
+    });
+    return S;
+}
+
+// Smoothed prolongation of the first coarsening (level 0 -> 1), the
+// smoothed-aggregation construction: P = (I - wp D^-1 A) P_t with P_t the
+// aggregation and wp = 2/3 (4/3 over 2, the bound of the Jacobi-scaled
+// spectrum of a diagonally dominant M-matrix), so row i of P is
+// (1 - wp) e_agg(i) + (wp / D_i) q_i with q = offdiag(A) P_t.  Builds the
+// coarse operator A_c = P^T (A P) in Cl (sliced ELL, off-diagonal entries as
+// magnitudes, the restriction P^T as weighted member lists in fine-row order)
+// and level 0's up-sweep operator G = (D / w) P - A P in F: with it the
+// prolongation fused into the post-smoothing sweep is
+//   z = w D^-1 (b + r0) + alpha w D^-1 G e.
+// Off-diagonal entries of A_c below filter * sqrt(|a_II a_JJ|) are dropped
+// and added to the row's diagonal (row sums kept; the criterion is symmetric,
+// so is the filtered operator): the smoothed stencil's many small far entries
+// cost coarse-level bandwidth and barely help the cycle.
+// F.agg must hold the aggregation.  Every sum runs in an order fixed by the
+// rows alone, so an instance's hierarchy does not depend on its batch.
+void galerkinSmoothed(AmgHostLevel& F, int nc, double omega, AmgHostLevel& Cl, double filter) {
+    const double wp = 2.0 / 3.0;
+    const std::vector<int>& agg = F.agg;
+    const int N = F.n;
+    std::vector<int> qc, qw;
+    std::vector<double> q;
+    groupByAggregate(F, qc, q, qw);
+    const SparseRows P = buildRows(N, [&](int i, std::vector<std::pair<int, double>>& r) {
+        r.clear();
+        if (agg[i] < 0) return;
+        const double d = F.diag[i];
+        r.push_back({agg[i], 1.0 - wp});
+        const int sI = i / kSlice, l = i % kSlice;
+        for (int k = 0; k < F.sliceWidth[sI]; ++k) {
+            const size_t e = (size_t)F.sliceStart[sI] + (size_t)k * kSlice + l;
+            if (qc[e] >= 0) r.push_back({qc[e], wp * q[e] / d});
+        }
+        sortMergeRow(r);
+    });
+    // A P: row i = D_i P_i - sum over neighbours j of a_ij P_j
+    const SparseRows AP = buildRows(N, [&](int i, std::vector<std::pair<int, double>>& r) {
+        r.clear();
+        if (agg[i] < 0) return;
+        for (long long k = P.off[i]; k < P.off[i + 1]; ++k) r.push_back({P.col[k], F.diag[i] * P.val[k]});
+        forRow(F.sliceStart, F.sliceWidth, F.col, i, [&](int e, int j) {
+            for (long long k = P.off[j]; k < P.off[j + 1]; ++k) r.push_back({P.col[k], -F.a[e] * P.val[k]});
+        });
+        sortMergeRow(r);
+    });
+    // G = (D / w) P - A P on level 0, sliced ELL of its own; an empty slot
+    // repeats the row's first column with a zero weight (every load in bounds
+    // and within the row's instance)
+    const SparseRows Gr = buildRows(N, [&](int i, std::vector<std::pair<int, double>>& r) {
+        r.clear();
+        if (agg[i] < 0) return;
+        const double s = F.diag[i] / omega;
+        for (long long k = P.off[i]; k < P.off[i + 1]; ++k) r.push_back({P.col[k], s * P.val[k]});
+        for (long long k = AP.off[i]; k < AP.off[i + 1]; ++k) r.push_back({AP.col[k], -AP.val[k]});
+        sortMergeRow(r);
+    });
+    {
+        const int nS = F.nSlices;
+        F.gStart.assign((size_t)nS + 1, 0);
+        F.gWidth.assign(nS, 0);
+        long long E = 0;
+        for (int sI = 0; sI < nS; ++sI) {
+            int wmax = 0;
+            for (int l = 0; l < kSlice; ++l) {
+                const int i = sI * kSlice + l;
+                if (i < N) wmax = std::max(wmax, (int)(Gr.off[i + 1] - Gr.off[i]));
+            }
+            if (E > INT32_MAX) throw InvalidError("smoothed multigrid operator exceeds 2^31 entries");
+            F.gStart[sI] = (int)E;
+            F.gWidth[sI] = wmax;
+            E += (long long)wmax * kSlice;
+        }
+        if (E > INT32_MAX) throw InvalidError("smoothed multigrid operator exceeds 2^31 entries");
+        F.gStart[nS] = (int)E;
+        F.gCol.assign((size_t)E, 0);
+        F.g.assign((size_t)E, 0.0);
+        parallelFor(nS, 256, [&](int s0, int s1) {
+            for (int sI = s0; sI < s1; ++sI)
+                for (int l = 0; l < kSlice; ++l) {
+                    const int i = sI * kSlice + l;
+                    if (i >= N || Gr.off[i + 1] == Gr.off[i]) continue;
+                    const int w = (int)(Gr.off[i + 1] - Gr.off[i]);
+                    for (int k = 0; k < F.gWidth[sI]; ++k) {
+                        const size_t e = (size_t)F.gStart[sI] + (size_t)k * kSlice + l;
+                        F.gCol[e] = Gr.col[(size_t)Gr.off[i] + (k < w ? k : 0)];
+                        F.g[e] = k < w ? Gr.val[(size_t)Gr.off[i] + k] : 0.0;
+                    }
+                }
+        });
+    }
+    // restriction P^T: per coarse row its fine rows (ascending) and weights
+    Cl = AmgHostLevel();
+    Cl.n = nc;
+    Cl.memOff.assign((size_t)nc + 1, 0);
+    for (long long k = 0; k < P.off[N]; ++k) Cl.memOff[(size_t)P.col[k] + 1]++;
+    for (int I = 0; I < nc; ++I) Cl.memOff[I + 1] += Cl.memOff[I];
+    if ((long long)Cl.memOff[nc] != P.off[N]) throw InvalidError("smoothed prolongation: inconsistent transpose");
+    Cl.mem.resize((size_t)P.off[N]);
+    Cl.memW.resize((size_t)P.off[N]);
+    {
+        std::vector<int> fill(Cl.memOff.begin(), Cl.memOff.end() - 1);
+        for (int i = 0; i < N; ++i)
+            for (long long k = P.off[i]; k < P.off[i + 1]; ++k) {
+                const int I = P.col[k];
+                Cl.mem[fill[I]] = i;
+                Cl.memW[fill[I]] = P.val[k];
+                ++fill[I];
+            }
+    }
+    {
+        const int nS = (nc + kSlice - 1) / kSlice;
+        Cl.rStart.assign((size_t)nS + 1, 0);
+        Cl.rWidth.assign(nS, 0);
+        long long E = 0;
+        for (int sI = 0; sI < nS; ++sI) {
+            int wmax = 0;
+            for (int l = 0; l < kSlice; ++l) {
+                const int I = sI * kSlice + l;
+                if (I < nc) wmax = std::max(wmax, Cl.memOff[I + 1] - Cl.memOff[I]);
+            }
+            if (E > INT32_MAX) throw InvalidError("smoothed multigrid restriction exceeds 2^31 entries");
+            Cl.rStart[sI] = (int)E;
+            Cl.rWidth[sI] = wmax;
+            E += (long long)wmax * kSlice;
+        }
+        if (E > INT32_MAX) throw InvalidError("smoothed multigrid restriction exceeds 2^31 entries");
+        Cl.rStart[nS] = (int)E;
+        Cl.rMem.assign((size_t)E, 0);
+        Cl.rW.assign((size_t)E, 0.0);
+        parallelFor(nS, 256, [&](int s0, int s1) {
+            for (int sI = s0; sI < s1; ++sI)
+                for (int l = 0; l < kSlice; ++l) {
+                    const int I = sI * kSlice + l;
+                    if (I >= nc) continue;
+                    const int k0 = Cl.memOff[I], w = Cl.memOff[I + 1] - k0;
+                    for (int k = 0; k < Cl.rWidth[sI]; ++k) {
+                        const size_t e = (size_t)Cl.rStart[sI] + (size_t)k * kSlice + l;
+                        Cl.rMem[e] = w > 0 ? Cl.mem[k0 + (k < w ? k : 0)] : 0;
+                        Cl.rW[e] = k < w ? Cl.memW[k0 + k] : 0.0;
+                    }
+                }
+        });
+    }
+    // A_c = P^T (A P), coarse rows in contiguous ranges per thread (a dense
+    // accumulator per thread; contributions summed in member, then column order)
+    Cl.diag.assign(nc, 0.0);
+    std::vector<int> width(nc, 0);
+    struct Part {
+        int I0 = 0;
+        std::vector<int> off, col;
+        std::vector<double> a;
+    };
+    const int nParts = std::max(1, std::min(64, nc / 4096));
+    std::vector<Part> parts(nParts);
+    parallelFor(nParts, 1, [&](int p0, int p1) {
+        std::vector<double> acc(nc, 0.0);
+        std::vector<char> mark(nc, 0);
+        std::vector<int> touched;
+        for (int p = p0; p < p1; ++p) {
+            Part& Pt = parts[p];
+            Pt.I0 = (int)((long long)nc * p / nParts);
+            const int I1 = (int)((long long)nc * (p + 1) / nParts);
+            Pt.off.assign(I1 - Pt.I0 + 1, 0);
+            for (int I = Pt.I0; I < I1; ++I) {
+                touched.clear();
+                for (int k = Cl.memOff[I]; k < Cl.memOff[I + 1]; ++k) {
+                    const int i = Cl.mem[k];
+                    const double wgt = Cl.memW[k];
+                    for (long long m = AP.off[i]; m < AP.off[i + 1]; ++m) {
+                        const int J = AP.col[m];
+                        if (!mark[J]) {
+                            mark[J] = 1;
+                            acc[J] = 0.0;
+                            touched.push_back(J);
+                        }
+                        acc[J] += wgt * AP.val[m];
+                    }
+                }
+                std::sort(touched.begin(), touched.end());
+                int w = 0;
+                for (int J : touched) {
+                    mark[J] = 0;
+                    if (J == I) {
+                        Cl.diag[I] = acc[J];
+                        continue;
+                    }
+                    Pt.col.push_back(J);
+                    Pt.a.push_back(-acc[J]);  // stored as the magnitude of an M-matrix entry (entry = -a)
+                    ++w;
+                }
+                width[I] = w;
+                Pt.off[I - Pt.I0 + 1] = Pt.off[I - Pt.I0] + w;
+            }
+        }
+    });
+    // filter against the unfiltered diagonals, then lump
+    std::vector<double> lump(nc, 0.0);
+    std::vector<std::vector<char>> keep(nParts);
+    parallelFor(nParts, 1, [&](int p0, int p1) {
+        for (int p = p0; p < p1; ++p) {
+            const Part& Pt = parts[p];
+            keep[p].assign(Pt.col.size(), 1);
+            const int n = (int)Pt.off.size() - 1;
+            for (int t = 0; t < n; ++t) {
+                const int I = Pt.I0 + t;
+                int w = 0;
+                for (int k = Pt.off[t]; k < Pt.off[t + 1]; ++k) {
+                    const int J = Pt.col[k];
+                    if (std::fabs(Pt.a[k]) < filter * std::sqrt(std::fabs(Cl.diag[I] * Cl.diag[J]))) {
+                        keep[p][k] = 0;
+                        lump[I] -= Pt.a[k];  // the entry (-a) joins the diagonal
+                    } else {
+                        ++w;
+                    }
+                }
+                width[I] = w;
+            }
+        }
+    });
+    for (int I = 0; I < nc; ++I) Cl.diag[I] += lump[I];
+    Cl.nSlices = (nc + kSlice - 1) / kSlice;
+    Cl.sliceStart.resize((size_t)Cl.nSlices + 1);
+    Cl.sliceWidth.resize(Cl.nSlices);
+    size_t E = 0;
+    for (int sI = 0; sI < Cl.nSlices; ++sI) {
+        Cl.sliceStart[sI] = (int)E;
+        int wmax = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            const int I = sI * kSlice + l;
+            if (I < nc) wmax = std::max(wmax, width[I]);
+        }
+        Cl.sliceWidth[sI] = wmax;
+        E += (size_t)wmax * kSlice;
+    }
+    if (E > (size_t)INT32_MAX) throw InvalidError("smoothed multigrid coarse operator exceeds 2^31 entries");
+    Cl.sliceStart[Cl.nSlices] = (int)E;
+    Cl.col.assign(E, kPad);
+    Cl.a.assign(E, 0.0);
+    parallelFor(nParts, 1, [&](int p0, int p1) {
+        for (int p = p0; p < p1; ++p) {
+            const Part& Pt = parts[p];
+            const int n = (int)Pt.off.size() - 1;
+            for (int t = 0; t < n; ++t) {
+                const int I = Pt.I0 + t, sI = I / kSlice, l = I % kSlice;
+                int slot = 0;
+                for (int k = Pt.off[t]; k < Pt.off[t + 1]; ++k) {
+                    if (!keep[p][k]) continue;
+                    const size_t e = (size_t)Cl.sliceStart[sI] + (size_t)slot * kSlice + l;
+                    Cl.col[e] = Pt.col[k];
+                    Cl.a[e] = Pt.a[k];
+                    ++slot;
+                }
+            }
+        }
+    });
+}
+
+void buildAmg(s4f_mesh* M, bool smoothed) {
+    Amg& A = smoothed ? M->amgSm : M->amg;
     if (A.gen == M->generation && !A.d.empty()) return;
+    A.smoothed = smoothed;
     const auto t0 = std::chrono::steady_clock::now();
     const HostMesh& h = M->h;
     const int nInst = (int)h.instCell0.size();
@@ -1220,7 +1558,12 @@ void buildAmg(s4f_mesh* M) {
             nc += k;
         }
         AmgHostLevel G;
-        galerkin(F, agg, nc, G);
+        if (smoothed && A.h.size() == 1) {
+            F.agg = agg;
+            galerkinSmoothed(F, nc, A.omega, G, kMgSmoothedFilter);
+        } else {
+            galerkin(F, agg, nc, G);
+        }
         G.inst0 = std::move(Cl.inst0);
         G.instN = std::move(Cl.instN);
         G.rowInst.assign(nc, -1);
@@ -1253,7 +1596,23 @@ void buildAmg(s4f_mesh* M) {
         D->n = H.n;
         D->nSlices = H.nSlices;
         if (!H.agg.empty()) D->agg.upload(H.agg, s);
-        if (!H.agg.empty()) {  // every level with a coarser one: q = offdiag(A) P
+        if (!H.gCol.empty()) {  // level 0 of a smoothed hierarchy: G replaces q
+            std::vector<float> g32(H.g.begin(), H.g.end());
+            D->gStart.upload(H.gStart, s);
+            D->gWidth.upload(H.gWidth, s);
+            D->gCol.upload(H.gCol, s);
+            D->g64.upload(H.g, s);
+            D->g32.upload(g32, s);
+            long long cnt = 0;  // stored (non-padding) entries
+            for (int sI = 0; sI < H.nSlices; ++sI)
+                for (int lane = 0; lane < kSlice; ++lane) {
+                    const int i = sI * kSlice + lane;
+                    if (i >= H.n || H.agg[i] < 0) continue;
+                    for (int k = 0; k < H.gWidth[sI]; ++k)
+                        if (H.g[(size_t)H.gStart[sI] + (size_t)k * kSlice + lane] != 0.0) ++cnt;
+                }
+            A.qEntries = cnt;
+        } else if (!H.agg.empty()) {  // every level with a coarser one: q = offdiag(A) P
             std::vector<int> qc, qw;
             std::vector<double> q;
             groupByAggregate(H, qc, q, qw);
@@ -1263,6 +1622,19 @@ void buildAmg(s4f_mesh* M) {
             D->q64.upload(q, s);
             D->q32.upload(q32, s);
             D->qWidth.upload(qw, s);
+        }
+        if (!H.memW.empty()) {
+            std::vector<float> w32(H.memW.begin(), H.memW.end());
+            D->memW64.upload(H.memW, s);
+            D->memW32.upload(w32, s);
+        }
+        if (!H.rMem.empty()) {
+            std::vector<float> w32(H.rW.begin(), H.rW.end());
+            D->rStart.upload(H.rStart, s);
+            D->rWidth.upload(H.rWidth, s);
+            D->rMem.upload(H.rMem, s);
+            D->rW64.upload(H.rW, s);
+            D->rW32.upload(w32, s);
         }
         if (l > 0) {
             D->sliceStart.upload(H.sliceStart, s);
@@ -1312,6 +1684,7 @@ struct MgVec<double> {
     static Row4<double>* xs(AmgDevLevel& L) { return v4(L.xs64, L.n); }
     static const double* inv(AmgDevLevel& L) { return L.inv64.p; }
     static const double* q(AmgDevLevel& L) { return L.q64.p; }
+    static const double* g(AmgDevLevel& L) { return L.g64.p; }
     static Row4<double>* v4(DArr<double>& a, int n) {
         a.alloc(4 * (size_t)n);
         return reinterpret_cast<Row4<double>*>(a.p);
@@ -1326,6 +1699,7 @@ struct MgVec<float> {
     static Row4<float>* xs(AmgDevLevel& L) { return v4(L.xs32, L.n); }
     static const float* inv(AmgDevLevel& L) { return L.inv32.p; }
     static const float* q(AmgDevLevel& L) { return L.q32.p; }
+    static const float* g(AmgDevLevel& L) { return L.g32.p; }
     static Row4<float>* v4(DArr<float>& a, int n) {
         a.alloc(4 * (size_t)n);
         return reinterpret_cast<Row4<float>*>(a.p);
@@ -1343,8 +1717,26 @@ struct MgRun {
     int nFin;  // finalize workgroups (instances)
 };
 
-MgRun meshRun(s4f_mesh* M) {
-    return MgRun{&M->amg.d, M->amg.omega, M->amg.alpha, (int)M->h.instCell0.size()};
+MgRun meshRun(s4f_mesh* M, bool smoothed) {
+    Amg& A = smoothed ? M->amgSm : M->amg;
+    return MgRun{&A.d, A.omega, A.alpha, (int)M->h.instCell0.size()};
+}
+
+// level-0 prolongation + post-smoothing: the aggregation form (q) or, on a
+// smoothed hierarchy, the G form
+template <typename T>
+void launchMg0Up(std::vector<std::unique_ptr<AmgDevLevel>>& d, const DevMesh& dm, const BlockMap& bm, PcgVecs<T>& v,
+                 double omega, double alpha, Ctl* ctl, double* part, hipStream_t s) {
+    AmgDevLevel& D0 = *d[0];
+    const Row4<T>* e1 = d.size() > 1 ? (const Row4<T>*)MgVec<T>::xs(*d[1]) : nullptr;
+    if (D0.gCol.p && e1)
+        hipLaunchKernelGGL((k_mg0_up_g<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                           (const T*)MgVec<T>::r0(D0), (const int*)D0.gStart.p, (const int*)D0.gWidth.p,
+                           (const int*)D0.gCol.p, MgVec<T>::g(D0), e1, v.z, omega, alpha, (const Ctl*)ctl, part);
+    else
+        hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
+                           (const T*)MgVec<T>::r0(D0), (const int*)D0.agg.p, (const int*)D0.qCol.p, MgVec<T>::q(D0),
+                           e1, v.z, omega, alpha, (const Ctl*)ctl, part);
 }
 
 // z = V(r): one V(1,1) cycle on the 3-RHS residual, then k_fin_rho
@@ -1370,7 +1762,10 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
             AmgDevLevel& Lf = *A.d[l - 1];
             Row4<T>* b = MgVec<T>::b(L);
             Row4<T>* x = MgVec<T>::x(L);
-            if (l == 1)
+            if (l == 1 && L.rMem.p)  // smoothed prolongation: weighted lists in sliced ELL
+                hipLaunchKernelGGL((k_mg_restrict_w<T>), dim3(gridSlices(L.nSlices)), dim3(kBlock), 0, s, L.view(),
+                                   (const T*)r0, Lf.n, b, x, w, (const Ctl*)ctl);
+            else if (l == 1)
                 hipLaunchKernelGGL((k_mg_restrict<T, true>), dim3(gridRows(L.n)), dim3(kBlock), 0, s, L.view(),
                                    (const T*)r0, (const Row4<T>*)nullptr, Lf.n, b, x, w, (const Ctl*)ctl);
             else
@@ -1400,10 +1795,7 @@ void launchVcycle(const MgRun& R, const DevMesh& dm, const BlockMap& bm, PcgVecs
                                (const Row4<T>*)MgVec<T>::xs(Lc), MgVec<T>::xs(L), w, A.alpha, (const Ctl*)ctl);
         }
     }
-    const Row4<T>* e1 = nl > 1 ? (const Row4<T>*)MgVec<T>::xs(*A.d[1]) : nullptr;
-    hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r, (const T*)r0,
-                       (const int*)D0.agg.p, (const int*)D0.qCol.p, MgVec<T>::q(D0), e1, v.z, w, A.alpha,
-                       (const Ctl*)ctl, part);
+    launchMg0Up<T>(A.d, dm, bm, v, w, A.alpha, ctl, part, s);
     hipLaunchKernelGGL(k_fin_rho, dim3(R.nFin), dim3(kFinBlock), 0, s, bm, ctl, (const double*)part);
     HIP_CHECK(hipGetLastError());
 }
@@ -1672,19 +2064,22 @@ void solveT(s4f_mesh* M, const s4f_solve_opts& o, std::vector<InstanceIO>& io, F
         ~SourceOff() { m->srcOn = m->inerOn = false; }
     } sourceOff{M};
     const bool transient = o.d2dt2 != S4F_D2DT2_STEADY;
-    const bool mg = o.preconditioner == S4F_PRECOND_GAMG;
+    const bool mg = o.preconditioner == S4F_PRECOND_GAMG || o.preconditioner == S4F_PRECOND_GAMG_SMOOTHED;
+    const bool mgSmoothed = o.preconditioner == S4F_PRECOND_GAMG_SMOOTHED;
     if (mg) {
-        buildAmg(M);
-        // over-correction of the piecewise-constant coarse correction (plain
-        // aggregation under-represents smooth modes); any scale in (0, 2]
-        // keeps the cycle symmetric positive definite
-        M->amg.alpha = o.mgCoarseScale > 0.0 ? o.mgCoarseScale : kMgCoarseScaleDefault;
+        buildAmg(M, mgSmoothed);
+        Amg& A = mgSmoothed ? M->amgSm : M->amg;
+        // over-correction of the coarse correction (plain aggregation
+        // under-represents smooth modes); any scale in (0, 2) keeps the
+        // cycle symmetric positive definite
+        A.alpha = o.mgCoarseScale > 0.0 ? o.mgCoarseScale : kMgCoarseScaleDefault;
+        M->amgLast = &A;
     }
     const DevMesh dm = M->dm();
     const BlockMap bmFull = M->bmap();
     BlockMap bm = bmFull;  // narrowed to the iterating instances as a batch's instances finish
     int nFin = nInst;
-    MgRun mgRun = meshRun(M);
+    MgRun mgRun = meshRun(M, mgSmoothed);
     // initial and old-time fields (a resumed run keeps the device state)
     std::vector<double> D0(resume ? 0 : 3 * (size_t)C, 0.0);
     bool warm = false;
@@ -2400,7 +2795,7 @@ s4f_status s4f_mesh_mg_info(const s4f_mesh* m, int32_t maxLevels, int32_t* nLeve
                             int64_t* qEntries) {
     if (!m || !nLevels || maxLevels < 0 || (maxLevels > 0 && (!rows || !entries)))
         return fail(S4F_ERR_INVALID, "null argument");
-    const Amg& A = m->amg;
+    const Amg& A = m->amgLast ? *m->amgLast : m->amg;
     if (A.d.empty() || A.gen != m->generation)
         return fail(S4F_ERR_INVALID, "no multigrid hierarchy for the current matrix (run a GAMG solve first)");
     *nLevels = (int32_t)A.h.size();
@@ -2863,6 +3258,9 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
     DArr<Ctl> rc;
     rc.alloc(1);
     HIP_CHECK(hipMemcpyAsync(rc.p, &c, sizeof(Ctl), hipMemcpyHostToDevice, s));
+    // the multigrid kernels replay the hierarchy of the mesh's last multigrid solve
+    const bool smoothedK = M->amgLast == &M->amgSm;
+    Amg& AK = smoothedK ? M->amgSm : M->amg;
     auto launch = [&]() {
         switch (kernel) {
             case S4F_KERNEL_RESIDUAL:
@@ -2892,32 +3290,25 @@ double replayT(s4f_mesh* M, int kernel, int launches) {
                                   (const T*)v.x, nullptr, rc.p, st.part.p, 0, s);
                 break;
             case S4F_KERNEL_MG0_DOWN: {
-                AmgDevLevel& D0 = *M->amg.d[0];
+                AmgDevLevel& D0 = *AK.d[0];
                 hipLaunchKernelGGL((k_mg0_down<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   MgVec<T>::r0(D0), M->amg.omega, (const Ctl*)rc.p);
+                                   MgVec<T>::r0(D0), AK.omega, (const Ctl*)rc.p);
                 break;
             }
-            case S4F_KERNEL_MG0_UP: {
-                AmgDevLevel& D0 = *M->amg.d[0];
-                const bool two = M->amg.d.size() > 1;
-                hipLaunchKernelGGL((k_mg0_up<T>), dim3(bm.nBlocks), dim3(kBlock), 0, s, dm, bm, (const T*)v.r,
-                                   (const T*)MgVec<T>::r0(D0), (const int*)D0.agg.p, (const int*)D0.qCol.p,
-                                   MgVec<T>::q(D0),
-                                   two ? (const Row4<T>*)MgVec<T>::xs(*M->amg.d[1]) : (const Row4<T>*)nullptr, v.z,
-                                   M->amg.omega, M->amg.alpha, (const Ctl*)rc.p, st.part.p);
+            case S4F_KERNEL_MG0_UP:
+                launchMg0Up<T>(AK.d, dm, bm, v, AK.omega, AK.alpha, rc.p, st.part.p, s);
                 break;
-            }
             case S4F_KERNEL_VCYCLE:
-                launchVcycle<T>(meshRun(M), dm, bm, v, rc.p, st.part.p, s);
+                launchVcycle<T>(meshRun(M, smoothedK), dm, bm, v, rc.p, st.part.p, s);
                 break;
             default:
                 throw InvalidError("unknown kernel id");
         }
     };
     if (kernel >= S4F_KERNEL_MG0_DOWN && kernel != S4F_KERNEL_SPMV_MG) {
-        buildAmg(M);
-        for (size_t l = 0; l < M->amg.d.size(); ++l) {  // defined inputs on every level
-            AmgDevLevel& L = *M->amg.d[l];
+        buildAmg(M, smoothedK);
+        for (size_t l = 0; l < AK.d.size(); ++l) {  // defined inputs on every level
+            AmgDevLevel& L = *AK.d[l];
             if (l == 0) {  // the fine level keeps only the residual
                 HIP_CHECK(hipMemsetAsync(MgVec<T>::r0(L), 0, 3 * (size_t)L.n * sizeof(T), s));
                 continue;
@@ -4018,6 +4409,8 @@ s4f_status s4f_server_create(s4f_ctx* ctx, int64_t budgetBytes, int32_t maxSlots
         if (maxSlots < 1 || maxSlots > 4096) throw InvalidError("maxSlots must be in 1..4096");
         checkOpts(*opts);
         if (opts->d2dt2 != S4F_D2DT2_STEADY) throw InvalidError("the server runs steady solves");
+        if (opts->preconditioner == S4F_PRECOND_GAMG_SMOOTHED)
+            throw InvalidError("the server packs plain-aggregation hierarchies: use S4F_PRECOND_GAMG");
         HIP_CHECK(hipSetDevice(ctx->device));
         auto S = std::make_unique<s4f_server>();
         S->ctx = ctx;
@@ -4051,7 +4444,7 @@ s4f_status s4f_server_submit(s4f_server* S, const s4f_request* rq, int64_t* id) 
             return fail(S4F_ERR_ADMISSION, "request footprint exceeds the server's HBM budget");
         if (S->mgOn) {  // the request's own hierarchy (cached on the mesh), packed by the server thread
             HIP_CHECK(hipSetDevice(S->ctx->device));
-            buildAmg(m);
+            buildAmg(m, false);
         }
         Request r;
         r.mesh = m;

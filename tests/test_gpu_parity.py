@@ -129,20 +129,22 @@ def test_mixed_precision_solve_error(ctx, name, n, kw):
     assert eD < 1e-6 and eS < 1e-6
 
 
+@pytest.mark.parametrize("smoothed", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("name,n,kw", [
     ("cube16_jitter", (16, 16, 16), dict(jitter=0.15, seed=2)),
     ("tet4_jitter", (4, 4, 4), dict(jitter=0.15, seed=5, tets=True, ls_full_delta=True)),
     ("cfg1_cantilever", (50, 10, 10), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
 ])
-def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precision):
+def test_multigrid_preconditioned_solve_matches_fixture(ctx, name, n, kw, precision, smoothed):
     # the preconditioner changes the path, not the fixed point: same converged
-    # field as the oracle (which uses DIC), in fewer PCG iterations
-    from solids4foam_amd import PRECOND_GAMG, sigma_eq
-    print(f"[gamg solve] {name} precision={precision}", flush=True)
+    # field as the oracle (which uses DIC), in fewer PCG iterations; plain
+    # aggregation and the smoothed first prolongation alike
+    from solids4foam_amd import PRECOND_GAMG, PRECOND_GAMG_SMOOTHED, sigma_eq
+    print(f"[gamg solve] {name} precision={precision} smoothed={smoothed}", flush=True)
     g = np.load(os.path.join(GOLD, name + ".npz"))
     pm, m, _, _ = _product_mesh(ctx, n, **kw)
-    out = m.solve(precision=precision, preconditioner=PRECOND_GAMG, **TIGHT)
+    out = m.solve(precision=precision, preconditioner=PRECOND_GAMG_SMOOTHED if smoothed else PRECOND_GAMG, **TIGHT)
     jac = m.solve(precision=precision, **TIGHT)
     eD, eS = rel_l2(out["D"], g["D"]), rel_l2(sigma_eq(out["sigma"]), sigma_eq(g["sigma"]))
     print(f"  nOuter={out['nOuter']} pcg={out['pcgIterations']} (diagonal {jac['pcgIterations']}) "
@@ -208,20 +210,44 @@ def test_multigrid_hierarchy_info(ctx):
     assert out["nOuter"] >= 1
 
 
+def test_smoothed_multigrid_fewer_iterations(ctx):
+    # the smoothed first prolongation (P = (I - 2/3 D^-1 A) P_agg) widens the
+    # first coarse operator and its up-sweep operator, keeps the coarse rows,
+    # and needs fewer PCG iterations per outer iteration than plain aggregation
+    # on the same outer path (same outer count, same converged field)
+    from solids4foam_amd import PRECOND_GAMG, PRECOND_GAMG_SMOOTHED
+    pm, m, _, _ = _product_mesh(ctx, (40, 40, 40), jitter=0.15, seed=2)
+    opts = dict(solutionTolerance=1e-8, alternativeTolerance=1e-8, nCorrectors=20000, pcgRelTol=0.9, precision=1)
+    plain = m.solve(preconditioner=PRECOND_GAMG, **opts)
+    ip = m.mg_info()
+    sm = m.solve(preconditioner=PRECOND_GAMG_SMOOTHED, **opts)
+    isd = m.mg_info()
+    print(f"[smoothed mg] plain {plain['nOuter']} outer / {plain['pcgIterations']} PCG, levels {ip['rows']} "
+          f"entries {ip['entries']} q {ip['qEntries']}; smoothed {sm['nOuter']} / {sm['pcgIterations']}, levels "
+          f"{isd['rows']} entries {isd['entries']} G {isd['qEntries']}", flush=True)
+    assert plain["converged"] and sm["converged"]
+    assert isd["rows"][1] == ip["rows"][1] and isd["entries"][1] > ip["entries"][1]
+    assert isd["qEntries"] > ip["qEntries"]
+    assert sm["pcgIterations"] < 0.8 * plain["pcgIterations"]
+    assert abs(sm["nOuter"] - plain["nOuter"]) <= 0.1 * plain["nOuter"]
+    assert rel_l2(sm["D"], plain["D"]) < 1e-6
+
+
 def test_multigrid_batch_equals_alone(ctx):
     # a batch is one block-diagonal matrix: each instance's hierarchy, cycle
     # and reductions are the ones it has alone (an instance already at its
-    # coarsest size passes through the remaining levels untouched)
-    from solids4foam_amd import PRECOND_GAMG
+    # coarsest size passes through the remaining levels untouched), with the
+    # plain and the smoothed first prolongation
+    from solids4foam_amd import PRECOND_GAMG, PRECOND_GAMG_SMOOTHED
     from solids4foam_amd.batch import Batch
     print("[gamg batch]", flush=True)
     specs = [((9, 7, 6), dict(jitter=0.1, seed=1)), ((30, 6, 6), dict(L=(1.0, 0.2, 0.2), bc="cantilever")),
              ((5, 5, 5), dict(jitter=0.1, seed=3)), ((20, 11, 13), dict(jitter=0.1, seed=4)),
              ((24, 24, 12), dict(jitter=0.1, seed=5))]
     meshes = [_product_mesh(ctx, n, **kw) for n, kw in specs]
-    for precision in (0, 1):
+    for precision, pc in ((0, PRECOND_GAMG), (1, PRECOND_GAMG), (1, PRECOND_GAMG_SMOOTHED)):
         opts = dict(solutionTolerance=1e-10, alternativeTolerance=1e-10, nCorrectors=50000,
-                    preconditioner=PRECOND_GAMG, precision=precision)
+                    preconditioner=pc, precision=precision)
         alone = [m.solve(**opts) for _, m, _, _ in meshes]
         b = Batch(ctx)
         for _, m, _, _ in meshes:

@@ -94,6 +94,32 @@ def test_server_multigrid_equals_alone(ctx, precision):
         assert np.array_equal(r["D"], a["D"]) and np.array_equal(r["sigma"], a["sigma"])
 
 
+def test_server_keeps_plain_multigrid(ctx):
+    # the server packs plain-aggregation hierarchies: it refuses the smoothed
+    # preconditioner, and a mesh that also holds a smoothed hierarchy (from a
+    # solve alone) is served with its plain one, bitwise the plain solve alone
+    from solids4foam_amd import PRECOND_GAMG, PRECOND_GAMG_SMOOTHED, S4FError
+    from solids4foam_amd.serve import Server
+    print("[serving] smoothed multigrid refused, plain kept", flush=True)
+    with pytest.raises(S4FError):
+        Server(ctx, 256 << 20, max_slots=2, precision=1, **dict(OPTS, preconditioner=PRECOND_GAMG_SMOOTHED))
+    m, bv = _mesh(ctx, (20, 9, 5), seed=3)
+    sm = m.solve(precision=1, **dict(OPTS, preconditioner=PRECOND_GAMG_SMOOTHED))
+    alone = m.solve(precision=1, **dict(OPTS, preconditioner=PRECOND_GAMG))
+    assert sm["converged"] and alone["converged"]
+    srv = Server(ctx, 256 << 20, max_slots=2, precision=1, **dict(OPTS, preconditioner=PRECOND_GAMG))
+    rid = srv.submit(m, bv)
+    r = None
+    t0 = time.time()
+    while r is None and time.time() - t0 < 60:
+        got = srv.poll(100)
+        if got is not None:
+            r = srv.result(got)
+    srv.close()
+    assert r is not None and r["converged"] and r["nOuter"] == alone["nOuter"]
+    assert np.array_equal(r["D"], alone["D"]) and np.array_equal(r["sigma"], alone["sigma"])
+
+
 def test_server_aitken_equals_alone(ctx):
     # Aitken relaxation state lives per cell and boundary face of each
     # request's arena range: a served request gets the bits it gets alone
