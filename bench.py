@@ -385,7 +385,7 @@ def main():
 # bench kernel name -> kernel tag in the rocprof summaries
 PMC_TAGS = {"residual": ("k_residual",), "spmv": ("k_spmvp",), "pcg_update": ("k_update",),
             "grad": ("k_grad_bnd_pre", "k_grad", "k_grad_bnd_post"),  # the gradient step's three launches
-            "mg0_down": ("k_mg0_down",), "mg0_up": ("k_mg0_up",)}
+            "mg0_down": ("k_mg0_down",), "mg0_up": (("k_mg0_up_g", "k_mg0_up"),)}  # G form on a smoothed hierarchy
 
 
 def load_traffic(path, kernel, precision):
@@ -404,7 +404,9 @@ def load_traffic(path, kernel, precision):
     suffix = "<f>" if precision == "mixed" else "<d>"
     total = 0.0
     for tag in tags:
-        for name in (tag + suffix, tag):
+        alts = tag if isinstance(tag, tuple) else (tag,)
+        names = [a + sfx for a in alts for sfx in (suffix, "")]
+        for name in names:
             e = d.get("kernels", {}).get(name)
             if e and e.get("FETCH_SIZE") is not None and e.get("WRITE_SIZE") is not None:
                 total += e["FETCH_SIZE"] + e["WRITE_SIZE"]
@@ -483,8 +485,8 @@ def pmc_summary(paths, copy_bytes, json_out=None, meta=None):
             for (k, c), (sv, n, d) in acc.items():
                 short = k.split("(")[0]
                 for tag in ("k_copy", "k_spmvp", "k_update", "k_residual", "k_grad_bnd_pre", "k_grad_bnd_post",
-                            "k_aitken", "k_grad", "k_mg0_down", "k_mg0_up", "k_mg_down", "k_mg_up", "k_mg_restrict",
-                            "k_mg_dense", "k_fin"):
+                            "k_aitken", "k_grad", "k_mg0_down", "k_mg0_up_g", "k_mg0_up", "k_mg_down", "k_mg_up",
+                            "k_mg_restrict_w", "k_mg_restrict", "k_mg_dense", "k_fin"):
                     if tag in short:
                         short = tag + ("<f>" if "IfE" in k else "<d>" if "IdE" in k else "")
                         break

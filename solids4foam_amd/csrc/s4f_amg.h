@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
     const T* __restrict__ iDg = MgSel<T>::iD(Lc);
     const T* __restrict__ mw = MgSel<T>::memW(Lc);
     const T w = (T)omega;
-    for (int I = blockIdx.x * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
+    for (int I = logicalBlock(gridDim.x) * kBlock + threadIdx.x; I < Lc.n; I += gridDim.x * kBlock) {
         if (Lc.rowInst[I] < 0) continue;  // padding row
         if (mgSkip(ctl + Lc.rowInst[I])) continue;
         T s[3] = {(T)0, (T)0, (T)0};
@@ -446,6 +446,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const MgLevel Lc, const 
     }
 }
 
+// Coarse-level kernels take their rows in the XCD-aware order of the level-0
+// kernels (logicalBlock: each XCD a contiguous range of rows, so gathers of
+// neighbouring rows and of a restriction's fine members stay in its L2).
 // Coarse-level rows are wider (Galerkin stencils) and of varying width:
 // slots are processed kMgChunk at a time with every load unconditional (slots
 // past the row's width or empty read an in-bounds slot and the row's own
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict_w(const MgLevel Lc, cons
     const T w = (T)omega;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wavesTotal = gridDim.x * (kBlock / 64);
-    for (int s = blockIdx.x * (kBlock / 64) + wave; s < Lc.nSlices; s += wavesTotal) {
+    for (int s = logicalBlock(gridDim.x) * (kBlock / 64) + wave; s < Lc.nSlices; s += wavesTotal) {
         const int I = s * kSlice + lane;
         if (I >= Lc.n) continue;
         const int inst = Lc.rowInst[I];
@@ -504,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(const MgLevel L, const Row4<
     const int n = L.n;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wavesTotal = gridDim.x * (kBlock / 64);
-    for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
+    for (int s = logicalBlock(gridDim.x) * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
         const int P = s * kSlice + lane;
         if (P >= n) continue;
         const int inst = L.rowInst[P];
@@ -561,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(const MgLevel L, const Row4<T>
     const int* __restrict__ agg = L.agg;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wavesTotal = gridDim.x * (kBlock / 64);
-    for (int s = blockIdx.x * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
+    for (int s = logicalBlock(gridDim.x) * (kBlock / 64) + wave; s < L.nSlices; s += wavesTotal) {
         const int P = s * kSlice + lane;
         if (P >= n) continue;
         const int inst = L.rowInst[P];
